@@ -1,0 +1,114 @@
+// Per-lane accumulators for the SpMV kernels.
+//
+// Acc<T, false>: plain FMA accumulation (the default precision mode).
+// Acc<T, true>:  compensated accumulation for clsparseEnableExtendedPrecision:
+//   error-free TwoProduct (fma residual) and branch-free TwoSum (Knuth), with
+//   the running error kept in a second register -- SumK with K = 2 of Ogita,
+//   Rump and Oishi, "Accurate sum and dot product", SISC 26(6), 2005. The
+//   reference's extended mode (src/library/kernels/csrmv_adaptive.cl L167-308)
+//   drops the product error; this one keeps it.
+#pragma once
+
+#include "common.hpp"
+
+template <typename T, bool EXT>
+struct Acc;
+
+template <typename T>
+struct Acc<T, false>
+{
+    T s = T(0);
+    __device__ __forceinline__ void add(T v) { s += v; }
+    __device__ __forceinline__ void mul_add(T a, T b) { s = fma(a, b, s); }
+    __device__ __forceinline__ void merge(const Acc& o) { s += o.s; }
+    __device__ __forceinline__ T value() const { return s; }
+    template <int CTRL>
+    __device__ __forceinline__ Acc dpp_move() const
+    {
+        Acc r;
+        r.s = dpp<CTRL>(s);
+        return r;
+    }
+    __device__ __forceinline__ Acc xor_move(int m) const
+    {
+        Acc r;
+        r.s = shfl_xor_t(s, m);
+        return r;
+    }
+    // y_out = beta * y_old + sum (fma, as csrmv_adaptive.cl L481-483)
+    __device__ __forceinline__ T with_beta(T beta, T y_old) const { return fma(beta, y_old, s); }
+};
+
+template <typename T>
+struct Acc<T, true>
+{
+    T s = T(0);
+    T c = T(0);
+    __device__ __forceinline__ void add(T v)
+    {
+        const T t = s + v;
+        const T bp = t - s;
+        c += (s - (t - bp)) + (v - bp);
+        s = t;
+    }
+    __device__ __forceinline__ void mul_add(T a, T b)
+    {
+        const T p = a * b;
+        const T e = fma(a, b, -p);
+        add(p);
+        c += e;
+    }
+    __device__ __forceinline__ void merge(const Acc& o)
+    {
+        add(o.s);
+        c += o.c;
+    }
+    __device__ __forceinline__ T value() const { return s + c; }
+    template <int CTRL>
+    __device__ __forceinline__ Acc dpp_move() const
+    {
+        Acc r;
+        r.s = dpp<CTRL>(s);
+        r.c = dpp<CTRL>(c);
+        return r;
+    }
+    __device__ __forceinline__ Acc xor_move(int m) const
+    {
+        Acc r;
+        r.s = shfl_xor_t(s, m);
+        r.c = shfl_xor_t(c, m);
+        return r;
+    }
+    __device__ __forceinline__ T with_beta(T beta, T y_old) const
+    {
+        Acc t = *this;
+        t.mul_add(beta, y_old);
+        return t.value();
+    }
+};
+
+// Butterfly reduction over aligned groups of G lanes (G = 1..64, power of
+// two): DPP inside a row of 16 lanes, cross-lane permutes above that.
+template <typename A>
+__device__ __forceinline__ A group_reduce(A a, unsigned G)
+{
+    if (G >= 2) a.merge(a.template dpp_move<kDppQuadSwap1>());
+    if (G >= 4) a.merge(a.template dpp_move<kDppQuadSwap2>());
+    if (G >= 8) a.merge(a.template dpp_move<kDppRowHalfMirror>());
+    if (G >= 16) a.merge(a.template dpp_move<kDppRowMirror>());
+    if (G >= 32) a.merge(a.xor_move(16));
+    if (G >= 64) a.merge(a.xor_move(32));
+    return a;
+}
+
+template <typename A>
+__device__ __forceinline__ A wave_reduce(A a)
+{
+    a.merge(a.template dpp_move<kDppQuadSwap1>());
+    a.merge(a.template dpp_move<kDppQuadSwap2>());
+    a.merge(a.template dpp_move<kDppRowHalfMirror>());
+    a.merge(a.template dpp_move<kDppRowMirror>());
+    a.merge(a.xor_move(16));
+    a.merge(a.xor_move(32));
+    return a;
+}

@@ -1,0 +1,731 @@
+// Conjugate Gradients on gfx950 with a device-resident iteration loop.
+//
+// Algorithm and stopping rule follow the reference exactly
+// (src/library/solvers/conjugate-gradients.hpp L47-229,
+//  src/library/solvers/solver-control.hpp L47-64):
+//   r = b - A x;  residuum = ||r||_1 / ||b||_1;  stop if finished(residuum);
+//   z = M r; p = z; rz = <r, z>
+//   loop: y = A p; alpha = rz / <y, p>; x += alpha p; r -= alpha y; z = M r;
+//         beta = <r, z>_new / rz; p = z + beta p; residuum = ||r||_1/||b||_1;
+//         finished = residuum <= relTol || residuum <= absTol * initial
+//                    || nIters >= maxIters   (nIters = index of this iteration)
+// The reference drives ~17 launches and 3 blocking host reads per iteration.
+// Here one iteration is four kernels (SpMV, <y,p>, fused x/r/z update with
+// <r,z> and ||r|| partials, fused p update with the convergence test) whose
+// scalars never leave the device; chunks of iterations are captured once in a
+// hipGraph and replayed, and the host reads a small state block after each
+// chunk. After convergence the remaining kernels of a chunk return at once.
+//
+// Reductions are two-stage and fixed-order (per-workgroup partials; every
+// consumer re-adds the same partials in the same order), so a solve is
+// bit-reproducible. Dots and norms accumulate in FP64 for both precisions.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+#include "common.hpp"
+
+namespace {
+
+constexpr int kHist = 64;   // residual history ring (VERBOSE printing)
+constexpr int kChunk = 32;  // iterations per captured graph (even: both parities)
+
+enum Crit : int { kCritL1Ref = 0, kCritL2 = 1 };
+enum Prec : int { kPrecNone = 0, kPrecJacobi = 1 };
+
+struct CgDevState
+{
+    double norm_ref;   // denominator of the residual (||b||_1 or ||b||_2)
+    double init_res;
+    double rel_tol;
+    double abs_tol;
+    double cur_res;
+    int max_iters;
+    int done;
+    int n_iters;
+    int iter;
+    int converged;
+    int crit;
+    double hist[kHist];
+};
+
+// Fixed-order sum of g partials; every thread of the block gets the result.
+__device__ __forceinline__ double sum_partials(const double* __restrict__ part, uint32_t g,
+                                               double* sh)
+{
+    double a = 0.0;
+    for (uint32_t i = threadIdx.x; i < g; i += 256) a += part[i];
+    a = wave_sum(a);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = a;
+    __syncthreads();
+    const double r = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ double norm_term(int crit, double r)
+{
+    return crit == kCritL2 ? r * r : fabs(r);
+}
+
+// r = b - y (or b when x == 0); z = M r; p = z; partials of ||r|| and <r, z>.
+template <typename T>
+__global__ __launch_bounds__(256) void cg_init_kernel(size_t n, const T* __restrict__ b,
+                                                      const T* __restrict__ y, T* __restrict__ r,
+                                                      T* __restrict__ z, T* __restrict__ p,
+                                                      const T* __restrict__ invD, int prec,
+                                                      int x_zero, int crit,
+                                                      double* __restrict__ part_nr,
+                                                      double* __restrict__ part_rz)
+{
+    __shared__ double sh[4];
+    double nr = 0.0, rz = 0.0;
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
+        const T ri = x_zero ? b[i] : b[i] - y[i];
+        const T zi = (prec == kPrecJacobi) ? ri * invD[i] : ri;
+        r[i] = ri;
+        z[i] = zi;
+        p[i] = zi;
+        nr += norm_term(crit, static_cast<double>(ri));
+        rz += static_cast<double>(ri) * static_cast<double>(zi);
+    }
+    nr = block_sum256(nr, sh);
+    rz = block_sum256(rz, sh);
+    if (threadIdx.x == 0) {
+        part_nr[blockIdx.x] = nr;
+        part_rz[blockIdx.x] = rz;
+    }
+}
+
+__global__ __launch_bounds__(256) void cg_init_finalize(CgDevState* __restrict__ st,
+                                                        const double* __restrict__ part_nr,
+                                                        uint32_t g)
+{
+    __shared__ double sh[4];
+    const double s = sum_partials(part_nr, g, sh);
+    if (threadIdx.x == 0) {
+        const double norm = st->crit == kCritL2 ? sqrt(s) : s;
+        st->init_res = norm / st->norm_ref;
+        st->cur_res = st->init_res;
+    }
+}
+
+// part_yp[block] = sum y_i p_i
+template <typename T>
+__global__ __launch_bounds__(256) void cg_dot_kernel(size_t n, const T* __restrict__ y,
+                                                     const T* __restrict__ p,
+                                                     double* __restrict__ part_yp,
+                                                     const CgDevState* __restrict__ st)
+{
+    __shared__ double sh[4];
+    if (st->done) return;
+    double a = 0.0;
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
+        a += static_cast<double>(y[i]) * static_cast<double>(p[i]);
+    a = block_sum256(a, sh);
+    if (threadIdx.x == 0) part_yp[blockIdx.x] = a;
+}
+
+// alpha = rz / yp; x += alpha p; r -= alpha y; z = M r; partials <r,z>, ||r||.
+template <typename T>
+__global__ __launch_bounds__(256) void cg_update_kernel(
+    size_t n, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z, const T* __restrict__ p,
+    const T* __restrict__ y, const T* __restrict__ invD, int prec,
+    const double* __restrict__ part_yp, const double* __restrict__ part_rz_prev,
+    double* __restrict__ part_rz, double* __restrict__ part_nr, uint32_t g,
+    const CgDevState* __restrict__ st)
+{
+    __shared__ double sh[4];
+    if (st->done) return;
+    const double yp = sum_partials(part_yp, g, sh);
+    const double rz = sum_partials(part_rz_prev, g, sh);
+    const T alpha = static_cast<T>(rz / yp);
+    const int crit = st->crit;
+    double nr = 0.0, rzn = 0.0;
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
+        const T pi = p[i];
+        x[i] = x[i] + alpha * pi;
+        const T ri = r[i] - alpha * y[i];
+        r[i] = ri;
+        const T zi = (prec == kPrecJacobi) ? ri * invD[i] : ri;
+        z[i] = zi;
+        nr += norm_term(crit, static_cast<double>(ri));
+        rzn += static_cast<double>(ri) * static_cast<double>(zi);
+    }
+    nr = block_sum256(nr, sh);
+    rzn = block_sum256(rzn, sh);
+    if (threadIdx.x == 0) {
+        part_nr[blockIdx.x] = nr;
+        part_rz[blockIdx.x] = rzn;
+    }
+}
+
+// beta = rz_new / rz_old; p = z + beta p; block 0 evaluates finished().
+template <typename T>
+__global__ __launch_bounds__(256) void cg_pupdate_kernel(size_t n, T* __restrict__ p,
+                                                         const T* __restrict__ z,
+                                                         const double* __restrict__ part_rz,
+                                                         const double* __restrict__ part_rz_prev,
+                                                         const double* __restrict__ part_nr,
+                                                         uint32_t g, CgDevState* __restrict__ st)
+{
+    __shared__ double sh[4];
+    if (st->done) return;
+    const double rz_new = sum_partials(part_rz, g, sh);
+    const double rz_old = sum_partials(part_rz_prev, g, sh);
+    const T beta = static_cast<T>(rz_new / rz_old);
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
+        p[i] = z[i] + beta * p[i];
+
+    if (blockIdx.x == 0) {
+        const double s = sum_partials(part_nr, g, sh);
+        if (threadIdx.x == 0) {
+            const double norm = st->crit == kCritL2 ? sqrt(s) : s;
+            const double res = norm / st->norm_ref;
+            const int it = st->iter;
+            st->cur_res = res;
+            st->hist[it % kHist] = res;
+            const bool conv = (res <= st->rel_tol) || (res <= st->abs_tol * st->init_res);
+            if (conv || it >= st->max_iters) {
+                st->done = 1;
+                st->n_iters = it;
+                st->converged = conv ? 1 : 0;
+            }
+            st->iter = it + 1;
+        }
+    }
+}
+
+template <typename TA, typename TO>
+__global__ __launch_bounds__(256) void extract_inv_diag(uint32_t n, const uint32_t* __restrict__ rp,
+                                                        const uint32_t* __restrict__ ci,
+                                                        const TA* __restrict__ v,
+                                                        TO* __restrict__ invD)
+{
+    const uint32_t row = blockIdx.x * 256 + threadIdx.x;
+    if (row >= n) return;
+    // Rows without a stored diagonal keep 1 (no scaling); the reference leaves
+    // them uninitialised (src/library/solvers/preconditioners/diagonal.hpp L51).
+    TO d = TO(1);
+    for (uint32_t j = rp[row]; j < rp[row + 1]; ++j)
+        if (ci[j] == row) {
+            d = static_cast<TO>(TA(1) / v[j]);
+            break;
+        }
+    invD[row] = d;
+}
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void cast_kernel(size_t n, const TI* __restrict__ in,
+                                                   TO* __restrict__ out)
+{
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
+        out[i] = static_cast<TO>(in[i]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void fill_kernel(size_t n, T* __restrict__ out, T v)
+{
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
+        out[i] = v;
+}
+
+// Mixed-precision refinement helpers.
+__global__ __launch_bounds__(256) void ir_scale_residual(size_t n, const double* __restrict__ r,
+                                                         const double* __restrict__ rnorm,
+                                                         float* __restrict__ b32)
+{
+    const double s = 1.0 / *rnorm;
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
+        b32[i] = static_cast<float>(r[i] * s);
+}
+
+__global__ __launch_bounds__(256) void ir_update(size_t n, double* __restrict__ x,
+                                                 const float* __restrict__ d,
+                                                 const double* __restrict__ rnorm)
+{
+    const double s = *rnorm;
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
+        x[i] = fma(s, static_cast<double>(d[i]), x[i]);
+}
+
+uint32_t grid_for(size_t n)
+{
+    size_t g = (n + 2047) / 2048;
+    if (g < 1) g = 1;
+    if (g > static_cast<size_t>(kReducePartialsMax)) g = kReducePartialsMax;
+    return static_cast<uint32_t>(g);
+}
+
+uint32_t ew_grid(size_t n)
+{
+    size_t g = (n + 1023) / 1024;
+    if (g < 1) g = 1;
+    if (g > 8192) g = 8192;
+    return static_cast<uint32_t>(g);
+}
+
+size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
+
+// Carves named regions out of one device allocation.
+struct Carve
+{
+    char* base;
+    size_t off = 0;
+    template <typename T>
+    T* take(size_t count)
+    {
+        T* p = reinterpret_cast<T*>(base + off);
+        off += align_up(count * sizeof(T));
+        return p;
+    }
+};
+
+template <typename T>
+struct CgBuffers
+{
+    T* r;
+    T* z;
+    T* p;
+    T* y;
+    T* invD;
+    T* one;
+    T* zero;
+    double* part_yp;
+    double* part_rz[2];
+    double* part_nr[2];
+    CgDevState* st;
+
+    static size_t bytes(size_t n)
+    {
+        return 5 * align_up(n * sizeof(T)) + 2 * align_up(sizeof(T)) +
+               5 * align_up(kReducePartialsMax * sizeof(double)) + align_up(sizeof(CgDevState));
+    }
+    void carve(Carve& c, size_t n)
+    {
+        r = c.take<T>(n);
+        z = c.take<T>(n);
+        p = c.take<T>(n);
+        y = c.take<T>(n);
+        invD = c.take<T>(n);
+        one = c.take<T>(1);
+        zero = c.take<T>(1);
+        part_yp = c.take<double>(kReducePartialsMax);
+        part_rz[0] = c.take<double>(kReducePartialsMax);
+        part_rz[1] = c.take<double>(kReducePartialsMax);
+        part_nr[0] = c.take<double>(kReducePartialsMax);
+        part_nr[1] = c.take<double>(kReducePartialsMax);
+        st = c.take<CgDevState>(1);
+    }
+};
+
+bool graphs_enabled()
+{
+    const char* e = std::getenv("CLSPARSE_AMD_NO_GRAPH");
+    return !(e && e[0] == '1');
+}
+
+// The device-resident PCG loop on one system. x, b are T vectors of length n.
+template <typename T>
+class CgEngine
+{
+public:
+    CgEngine(const clsparseCsrMatrix* A, T* x, const T* b, size_t n, int prec, int crit,
+             CgBuffers<T>& buf, hipStream_t s, bool ext)
+        : A_(A), x_(x), b_(b), n_(n), prec_(prec), crit_(crit), buf_(buf), s_(s), ext_(ext),
+          g_(grid_for(n))
+    {
+    }
+    ~CgEngine()
+    {
+        if (exec_) (void)hipGraphExecDestroy(exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+    }
+
+    // Residual of the starting point: r = b - A x (skipped when x == 0),
+    // z = M r, p = z, <r, z>; writes init_res into the device state.
+    clsparseStatus setup(bool x_zero)
+    {
+        if (!x_zero) CLS_TRY(launch_csrmv<T>(buf_.one, A_, x_, buf_.zero, buf_.y, s_, ext_, nullptr));
+        hipLaunchKernelGGL((cg_init_kernel<T>), dim3(g_), dim3(256), 0, s_, n_, b_, buf_.y, buf_.r,
+                           buf_.z, buf_.p, buf_.invD, prec_, x_zero ? 1 : 0, crit_, buf_.part_nr[1],
+                           buf_.part_rz[1]);
+        CLS_HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(cg_init_finalize, dim3(1), dim3(256), 0, s_, buf_.st, buf_.part_nr[1], g_);
+        CLS_HIP_TRY(hipGetLastError());
+        return clsparseSuccess;
+    }
+
+    // Runs iterations until the device state says done. `on_chunk` sees the
+    // host copy of the state after every chunk.
+    template <typename F>
+    clsparseStatus run(CgDevState* host_state, F&& on_chunk)
+    {
+        const bool use_graph = graphs_enabled();
+        if (use_graph) {
+            CLS_HIP_TRY(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+            clsparseStatus st = clsparseSuccess;
+            for (int k = 0; k < kChunk && st == clsparseSuccess; ++k) st = enqueue_iteration(k & 1);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(s_, &g);
+            if (st != clsparseSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return st;
+            }
+            CLS_HIP_TRY(e);
+            graph_ = g;
+            CLS_HIP_TRY(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+        }
+        for (;;) {
+            if (use_graph) {
+                CLS_HIP_TRY(hipGraphLaunch(exec_, s_));
+            } else {
+                for (int k = 0; k < kChunk; ++k) CLS_TRY(enqueue_iteration(k & 1));
+            }
+            CLS_HIP_TRY(hipMemcpyAsync(host_state, buf_.st, sizeof(CgDevState),
+                                       hipMemcpyDeviceToHost, s_));
+            CLS_HIP_TRY(hipStreamSynchronize(s_));
+            on_chunk(*host_state);
+            if (host_state->done) break;
+        }
+        return clsparseSuccess;
+    }
+
+private:
+    clsparseStatus enqueue_iteration(int par)
+    {
+        const int* done = &buf_.st->done;
+        CLS_TRY(launch_csrmv<T>(buf_.one, A_, buf_.p, buf_.zero, buf_.y, s_, ext_, done));
+        hipLaunchKernelGGL((cg_dot_kernel<T>), dim3(g_), dim3(256), 0, s_, n_, buf_.y, buf_.p,
+                           buf_.part_yp, buf_.st);
+        hipLaunchKernelGGL((cg_update_kernel<T>), dim3(g_), dim3(256), 0, s_, n_, x_, buf_.r,
+                           buf_.z, buf_.p, buf_.y, buf_.invD, prec_, buf_.part_yp,
+                           buf_.part_rz[par ^ 1], buf_.part_rz[par], buf_.part_nr[par], g_,
+                           buf_.st);
+        hipLaunchKernelGGL((cg_pupdate_kernel<T>), dim3(g_), dim3(256), 0, s_, n_, buf_.p, buf_.z,
+                           buf_.part_rz[par], buf_.part_rz[par ^ 1], buf_.part_nr[par], g_,
+                           buf_.st);
+        CLS_HIP_TRY(hipGetLastError());
+        return clsparseSuccess;
+    }
+
+    const clsparseCsrMatrix* A_;
+    T* x_;
+    const T* b_;
+    size_t n_;
+    int prec_;
+    int crit_;
+    CgBuffers<T>& buf_;
+    hipStream_t s_;
+    bool ext_;
+    uint32_t g_;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+};
+
+// Orders the control's solver stream after the caller's stream.
+clsparseStatus enter_solver_stream(clsparseControl control)
+{
+    CLS_HIP_TRY(hipEventRecord(control->event, control->stream));
+    CLS_HIP_TRY(hipStreamWaitEvent(control->solver_stream, control->event, 0));
+    return clsparseSuccess;
+}
+
+// Orders the caller's stream after the solver stream and records the event.
+clsparseStatus leave_solver_stream(clsparseControl control)
+{
+    CLS_HIP_TRY(hipEventRecord(control->event, control->solver_stream));
+    CLS_HIP_TRY(hipStreamWaitEvent(control->stream, control->event, 0));
+    CLS_HIP_TRY(hipStreamSynchronize(control->solver_stream));
+    return clsparseSuccess;
+}
+
+template <typename T>
+clsparseStatus cg_solve(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
+                        clSParseSolverControl sc, clsparseControl control)
+{
+    // Size rule of conjugate-gradients.hpp L57-62; CG also needs a square A
+    // (the reference would overrun its length-N temporaries otherwise).
+    if (A->num_cols != b->num_values || A->num_rows != x->num_values ||
+        A->num_rows != A->num_cols)
+        return clsparseInvalidSystemSize;
+    const size_t n = A->num_cols;
+    hipStream_t s = control->solver_stream;
+    T* xp = dev_ptr<T>(x->values, x->off_values);
+    const T* bp = dev_ptr<T>(b->values, b->off_values);
+    if (n > 0 && (xp == nullptr || bp == nullptr)) return clsparseInvalidMemObject;
+
+    void* work = nullptr;
+    CLS_TRY(control_workspace(control, CgBuffers<T>::bytes(n), &work));
+    void* pinned = nullptr;
+    CLS_TRY(control_pinned(control, sizeof(CgDevState) + 64, &pinned));
+    CgDevState* hs = static_cast<CgDevState*>(pinned);
+    Carve c{static_cast<char*>(work)};
+    CgBuffers<T> buf;
+    buf.carve(c, n);
+
+    CLS_TRY(enter_solver_stream(control));
+
+    // ||b||_1 (conjugate-gradients.hpp L77-81)
+    CLS_TRY(launch_reduce<T, double>(RedOp::Abs, bp, nullptr, n, &buf.st->norm_ref, false,
+                                     control->red_partials, s));
+    CLS_HIP_TRY(hipMemcpyAsync(hs, buf.st, sizeof(CgDevState), hipMemcpyDeviceToHost, s));
+    CLS_HIP_TRY(hipStreamSynchronize(s));
+    const double norm_b = hs->norm_ref;
+
+    if (norm_b == 0) {
+        // b == 0: x = b and zero tolerances (L87-95).
+        sc->nIters = 0;
+        sc->absoluteTolerance = 0.0;
+        sc->relativeTolerance = 0.0;
+        if (n > 0)
+            CLS_HIP_TRY(hipMemcpyAsync(xp, bp, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        sc->stats = clsparseAmdSolverStats{};
+        CLS_TRY(leave_solver_stream(control));
+        return clsparseSuccess;
+    }
+
+    const auto t0 = std::chrono::steady_clock::now();
+    const int prec = (sc->preconditioner == DIAGONAL) ? kPrecJacobi : kPrecNone;
+    if (prec == kPrecJacobi) {
+        hipLaunchKernelGGL((extract_inv_diag<T, T>), dim3((A->num_rows + 255) / 256), dim3(256), 0,
+                           s, A->num_rows, dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer),
+                           dev_ptr<uint32_t>(A->col_indices, A->off_col_indices),
+                           dev_ptr<T>(A->values, A->off_values), buf.invD);
+        CLS_HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL((fill_kernel<T>), dim3(1), dim3(256), 0, s, size_t(1), buf.one, T(1));
+    hipLaunchKernelGGL((fill_kernel<T>), dim3(1), dim3(256), 0, s, size_t(1), buf.zero, T(0));
+
+    // Device state: tolerances of the solver control (solver-control.hpp L52-64).
+    CgDevState init{};
+    init.norm_ref = norm_b;
+    init.rel_tol = sc->relativeTolerance;
+    init.abs_tol = sc->absoluteTolerance;
+    init.max_iters = sc->maxIters;
+    init.crit = kCritL1Ref;
+    *hs = init;
+    CLS_HIP_TRY(hipMemcpyAsync(buf.st, hs, sizeof(CgDevState), hipMemcpyHostToDevice, s));
+
+    CgEngine<T> eng(A, xp, bp, n, prec, kCritL1Ref, buf, s, control->extended_precision);
+    CLS_TRY(eng.setup(false));
+    CLS_HIP_TRY(hipMemcpyAsync(hs, buf.st, sizeof(CgDevState), hipMemcpyDeviceToHost, s));
+    CLS_HIP_TRY(hipStreamSynchronize(s));
+
+    sc->initialResidual = hs->init_res;  // L127-129
+    sc->stats = clsparseAmdSolverStats{};
+    if (sc->finished(sc->initialResidual)) {  // L134-138 (uses the stored nIters)
+        sc->nIters = 0;
+        sc->stats.l1_residual = sc->initialResidual;
+        CLS_TRY(leave_solver_stream(control));
+        return clsparseSuccess;
+    }
+
+    int printed = 0;
+    CLS_TRY(eng.run(hs, [&](const CgDevState& st) {
+        const int upto = st.done ? st.n_iters + 1 : st.iter;
+        if (sc->printMode == VERBOSE) {
+            for (int k = printed; k < upto; ++k) {
+                sc->nIters = k;
+                sc->currentResidual = st.hist[k % kHist];
+                sc->print();
+            }
+        }
+        printed = upto;
+    }));
+    sc->nIters = hs->n_iters;
+    sc->currentResidual = hs->cur_res;
+    CLS_TRY(leave_solver_stream(control));
+
+    const auto t1 = std::chrono::steady_clock::now();
+    sc->stats.iterations = hs->n_iters + 1;
+    sc->stats.l1_residual = hs->cur_res;
+    sc->stats.seconds = std::chrono::duration<double>(t1 - t0).count();
+    sc->stats.status_detail = hs->converged ? 0 : 1;
+    return clsparseSuccess;
+}
+
+template <typename T>
+clsparseStatus cg_entry(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
+                        clSParseSolverControl solverControl, clsparseControl control)
+{
+    // Check order of conjugate-gradients.cpp L28-41.
+    if (!clsparseInitialized) return clsparseNotInitialized;
+    if (control == nullptr) return clsparseInvalidControlObject;
+    if (solverControl == nullptr) return clsparseInvalidSolverControlObject;
+    if (x == nullptr || A == nullptr || b == nullptr) return clsparseInvalidValue;
+    const clsparseStatus status = cg_solve<T>(x, A, b, solverControl, control);
+    solverControl->printSummary(status);  // L59
+    return status;
+}
+
+}  // namespace
+
+// src/library/solvers/conjugate-gradients.cpp L18-64
+extern "C" clsparseStatus clsparseScsrcg(cldenseVector* x, const clsparseCsrMatrix* A,
+                                         const cldenseVector* b,
+                                         clSParseSolverControl solverControl,
+                                         clsparseControl control)
+{
+    return cg_entry<float>(x, A, b, solverControl, control);
+}
+
+// src/library/solvers/conjugate-gradients.cpp L66-116
+extern "C" clsparseStatus clsparseDcsrcg(cldenseVector* x, const clsparseCsrMatrix* A,
+                                         const cldenseVector* b,
+                                         clSParseSolverControl solverControl,
+                                         clsparseControl control)
+{
+    return cg_entry<double>(x, A, b, solverControl, control);
+}
+
+// ---- mixed-precision iterative refinement (no reference counterpart) --------
+//
+// Outer loop in FP64: r = b - A x (FP64 CSR-Adaptive SpMV), ||r||_2 decides
+// convergence, x += ||r|| d in FP64. Inner solve in FP32: A d = r / ||r|| by
+// the same device-resident PCG (FP32 values, FP64 dots/norms), stopped on its
+// own L2 residual at `inner_tol` (or when the overall iteration budget ends).
+extern "C" clsparseStatus clsparseAmdDcsrcgMixed(cldenseVector* x, const clsparseCsrMatrix* A,
+                                                 const cldenseVector* b,
+                                                 clSParseSolverControl sc,
+                                                 clsparseControl control)
+{
+    if (!clsparseInitialized) return clsparseNotInitialized;
+    if (control == nullptr) return clsparseInvalidControlObject;
+    if (sc == nullptr) return clsparseInvalidSolverControlObject;
+    if (x == nullptr || A == nullptr || b == nullptr) return clsparseInvalidValue;
+    if (A->num_cols != b->num_values || A->num_rows != x->num_values ||
+        A->num_rows != A->num_cols)
+        return clsparseInvalidSystemSize;
+
+    const size_t n = A->num_rows;
+    const size_t nnz = A->num_nonzeros;
+    hipStream_t s = control->solver_stream;
+    double* xp = dev_ptr<double>(x->values, x->off_values);
+    const double* bp = dev_ptr<double>(b->values, b->off_values);
+    if (n > 0 && (xp == nullptr || bp == nullptr)) return clsparseInvalidMemObject;
+
+    double inner_tol = 1e-4;
+    if (const char* e = std::getenv("CLSPARSE_AMD_IR_INNER_TOL")) inner_tol = std::atof(e);
+
+    // Workspace: FP32 values of A, FP64 residual and scalars, FP32 rhs and
+    // correction, then the FP32 CG buffers.
+    const size_t bytes = align_up(nnz * sizeof(float)) + align_up(n * sizeof(double)) +
+                         align_up(8 * sizeof(double)) + 2 * align_up(n * sizeof(float)) +
+                         CgBuffers<float>::bytes(n);
+    void* work = nullptr;
+    CLS_TRY(control_workspace(control, bytes, &work));
+    void* pinned = nullptr;
+    CLS_TRY(control_pinned(control, sizeof(CgDevState) + 64, &pinned));
+    CgDevState* hs = static_cast<CgDevState*>(pinned);
+    Carve c{static_cast<char*>(work)};
+    float* vals32 = c.take<float>(nnz);
+    double* r64 = c.take<double>(n);
+    double* sc64 = c.take<double>(8);  // [0]=||r||_2, [1]=||b||_2, [2]=-1, [3]=+1
+    float* b32 = c.take<float>(n);
+    float* d32 = c.take<float>(n);
+    CgBuffers<float> buf;
+    buf.carve(c, n);
+
+    CLS_TRY(enter_solver_stream(control));
+    const auto t0 = std::chrono::steady_clock::now();
+
+    CLS_TRY(launch_reduce<double, double>(RedOp::Sqr, bp, nullptr, n, sc64 + 1, true,
+                                          control->red_partials, s));
+    hipLaunchKernelGGL((fill_kernel<double>), dim3(1), dim3(256), 0, s, size_t(1), sc64 + 2, -1.0);
+    hipLaunchKernelGGL((fill_kernel<double>), dim3(1), dim3(256), 0, s, size_t(1), sc64 + 3, 1.0);
+    double hb[8];
+    CLS_HIP_TRY(hipMemcpyAsync(hb, sc64, sizeof(hb), hipMemcpyDeviceToHost, s));
+    CLS_HIP_TRY(hipStreamSynchronize(s));
+    const double norm_b = hb[1];
+    sc->stats = clsparseAmdSolverStats{};
+    if (norm_b == 0) {
+        sc->nIters = 0;
+        if (n > 0) CLS_HIP_TRY(hipMemcpyAsync(xp, bp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        CLS_TRY(leave_solver_stream(control));
+        return clsparseSuccess;
+    }
+
+    // FP32 copy of A on the same pattern (shares col_indices, row_pointer, meta).
+    hipLaunchKernelGGL((cast_kernel<double, float>), dim3(ew_grid(nnz)), dim3(256), 0, s, nnz,
+                       dev_ptr<double>(A->values, A->off_values), vals32);
+    clsparseCsrMatrix A32 = *A;
+    A32.values = vals32;
+    A32.off_values = 0;
+    const int prec = (sc->preconditioner == DIAGONAL) ? kPrecJacobi : kPrecNone;
+    if (prec == kPrecJacobi) {
+        hipLaunchKernelGGL((extract_inv_diag<double, float>), dim3((n + 255) / 256), dim3(256), 0, s,
+                           static_cast<uint32_t>(n),
+                           dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer),
+                           dev_ptr<uint32_t>(A->col_indices, A->off_col_indices),
+                           dev_ptr<double>(A->values, A->off_values), buf.invD);
+    }
+    hipLaunchKernelGGL((fill_kernel<float>), dim3(1), dim3(256), 0, s, size_t(1), buf.one, 1.0f);
+    hipLaunchKernelGGL((fill_kernel<float>), dim3(1), dim3(256), 0, s, size_t(1), buf.zero, 0.0f);
+    CLS_HIP_TRY(hipGetLastError());
+
+    int total_inner = 0, outer = 0;
+    double rel = 0.0, first_rel = -1.0;
+    const int max_outer = 100;
+    int detail = 1;
+    for (;;) {
+        // r = b - A x in FP64
+        CLS_HIP_TRY(hipMemcpyAsync(r64, bp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        CLS_TRY(launch_csrmv<double>(sc64 + 2, A, xp, sc64 + 3, r64, s, control->extended_precision,
+                                     nullptr));
+        CLS_TRY(launch_reduce<double, double>(RedOp::Sqr, r64, nullptr, n, sc64, true,
+                                              control->red_partials, s));
+        CLS_HIP_TRY(hipMemcpyAsync(hb, sc64, sizeof(double), hipMemcpyDeviceToHost, s));
+        CLS_HIP_TRY(hipStreamSynchronize(s));
+        rel = hb[0] / norm_b;
+        if (first_rel < 0) first_rel = rel;
+        if (sc->printMode == VERBOSE)
+            std::printf("Refinement %d: true relative residual %.6e (inner iterations %d)\n", outer,
+                        rel, total_inner);
+        if (rel <= sc->relativeTolerance) {
+            detail = 0;
+            break;
+        }
+        if (total_inner >= sc->maxIters || outer >= max_outer) break;
+
+        // Inner FP32 solve of A d = r / ||r||, d0 = 0.
+        hipLaunchKernelGGL(ir_scale_residual, dim3(ew_grid(n)), dim3(256), 0, s, n, r64, sc64, b32);
+        hipLaunchKernelGGL((fill_kernel<float>), dim3(ew_grid(n)), dim3(256), 0, s, n, d32, 0.0f);
+        CgDevState init{};
+        init.norm_ref = 1.0;
+        init.rel_tol = std::max(inner_tol, 0.5 * sc->relativeTolerance / rel);
+        init.abs_tol = 0.0;
+        init.max_iters = sc->maxIters - total_inner - 1;
+        init.crit = kCritL2;
+        *hs = init;
+        CLS_HIP_TRY(hipMemcpyAsync(buf.st, hs, sizeof(CgDevState), hipMemcpyHostToDevice, s));
+        CgEngine<float> eng(&A32, d32, b32, n, prec, kCritL2, buf, s, false);
+        CLS_TRY(eng.setup(true));
+        CLS_TRY(eng.run(hs, [](const CgDevState&) {}));
+        total_inner += hs->n_iters + 1;
+        ++outer;
+        hipLaunchKernelGGL(ir_update, dim3(ew_grid(n)), dim3(256), 0, s, n, xp, d32, sc64);
+        CLS_HIP_TRY(hipGetLastError());
+    }
+    CLS_TRY(leave_solver_stream(control));
+    const auto t1 = std::chrono::steady_clock::now();
+    sc->nIters = total_inner;
+    sc->initialResidual = first_rel;
+    sc->currentResidual = rel;
+    sc->stats.iterations = total_inner;
+    sc->stats.outer_iterations = outer;
+    sc->stats.true_residual = rel;
+    sc->stats.seconds = std::chrono::duration<double>(t1 - t0).count();
+    sc->stats.status_detail = detail;
+    return clsparseSuccess;
+}

@@ -1,0 +1,384 @@
+// CSR SpMV for gfx950: y = alpha * A * x + beta * y.
+//
+// Two drivers, selected as the reference selects them
+// (src/library/blas2/clsparse-csrmv.hpp L27-56):
+//   * CSR-Adaptive (meta present): one wave64 per row-block entry of the
+//     reference metadata (src/library/internal/data-types/csr-meta.hpp
+//     L93-261), four independent waves per 256-lane workgroup. Each entry is
+//     classified exactly as csrmv_adaptive.cl L349-359, L382, L515, L564:
+//       - stream   (>= 2 rows, <= 1024 nnz): coalesced val/col loads, products
+//                   staged in LDS, per-row reduction by a power-of-two lane
+//                   group (DPP butterflies) or one lane per row;
+//       - vector   (1 row, segment id 0): lane-strided FMA + wave reduction;
+//       - long row (segment id != 0 or empty range): one 3072-nnz segment per
+//                   wave, partial sum to a per-entry slot; a second kernel
+//                   adds the segments of each long row in segment order.
+//     The long-row combine replaces the reference's float atomics and
+//     inter-workgroup spin flag (csrmv_adaptive.cl L564-686): results are
+//     deterministic and the metadata is never written.
+//   * CSR-Vector (meta == NULL): one sub-wave of 2..64 lanes per row, the
+//     sub-wave width chosen from nnz/row as csrmv-vector.hpp L35-49.
+#include "accum.hpp"
+#include "common.hpp"
+
+namespace {
+
+constexpr int kWavesPerWg = 4;
+constexpr int kStreamPerLane = kBlkSize / 64;  // 16 nnz per lane per stream block
+
+template <typename T, bool EXT>
+struct StreamLds
+{
+    T p[kWavesPerWg][kBlkSize];
+};
+template <typename T>
+struct StreamLds<T, true>
+{
+    T p[kWavesPerWg][kBlkSize];
+    T e[kWavesPerWg][kBlkSize];
+};
+
+template <typename T, bool EXT>
+__device__ __forceinline__ void write_row(T* __restrict__ y, uint32_t row, const Acc<T, EXT>& a,
+                                          T beta)
+{
+    // beta == 0 never reads y (csrmv_adaptive.cl L481-483, L505-507, L557-559).
+    y[row] = (beta != T(0)) ? a.with_beta(beta, y[row]) : a.value();
+}
+
+// Sum over [beg, end) of (alpha * val) * x[col], lanes strided by 64, four
+// loads in flight per lane per step.
+template <typename T, bool EXT>
+__device__ __forceinline__ Acc<T, EXT> strided_dot(const T* __restrict__ vals,
+                                                   const uint32_t* __restrict__ cols,
+                                                   const T* __restrict__ x, T alpha,
+                                                   uint32_t beg, uint32_t end, uint32_t lane)
+{
+    Acc<T, EXT> acc;
+    uint32_t j = beg + lane;
+    for (; j + 192 < end; j += 256) {
+        T v[4];
+        uint32_t c[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            v[u] = __builtin_nontemporal_load(vals + j + 64 * u);
+            c[u] = __builtin_nontemporal_load(cols + j + 64 * u);
+        }
+        T xv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xv[u] = x[c[u]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc.mul_add(alpha * v[u], xv[u]);
+    }
+    for (; j < end; j += 64) {
+        const T v = __builtin_nontemporal_load(vals + j);
+        const uint32_t c = __builtin_nontemporal_load(cols + j);
+        acc.mul_add(alpha * v, x[c]);
+    }
+    return acc;
+}
+
+template <typename T, bool EXT>
+__global__ __launch_bounds__(256) void csrmv_adaptive_kernel(
+    const T* __restrict__ vals, const uint32_t* __restrict__ cols,
+    const uint32_t* __restrict__ rowPtr, const T* __restrict__ x, T* __restrict__ y,
+    const uint64_t* __restrict__ rowBlocks, uint32_t numBlocks, const T* __restrict__ pAlpha,
+    const T* __restrict__ pBeta, double* __restrict__ partials, const int* __restrict__ done)
+{
+    __shared__ StreamLds<T, EXT> lds;
+
+    if (done != nullptr && *done != 0) return;
+
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t blk = wg * kWavesPerWg + wave;
+    if (blk >= numBlocks) return;  // wave-uniform; no workgroup barrier below
+
+    const uint64_t e0 = rowBlocks[blk];
+    const uint64_t e1 = rowBlocks[blk + 1];
+    const uint32_t row = static_cast<uint32_t>(e0 >> (64 - kRowBits));
+    const uint32_t stop = static_cast<uint32_t>(e1 >> (64 - kRowBits));
+    const uint32_t seg = static_cast<uint32_t>(e0 & ((1u << kWgBits) - 1u));
+    const uint32_t nrows = stop - row;
+    const T alpha = *pAlpha;
+    const T beta = *pBeta;
+
+    if (nrows > static_cast<uint32_t>(kRowsForVector)) {
+        // ---- CSR-Stream -------------------------------------------------
+        const uint32_t base = rowPtr[row];
+        uint32_t cnt = rowPtr[stop] - base;  // <= 1024 by construction
+        cnt = cnt < static_cast<uint32_t>(kBlkSize) ? cnt : static_cast<uint32_t>(kBlkSize);
+        T* buf = lds.p[wave];
+        {
+            T v[kStreamPerLane];
+            uint32_t c[kStreamPerLane];
+#pragma unroll
+            for (int k = 0; k < kStreamPerLane; ++k) {
+                const uint32_t i = k * 64 + lane;
+                if (i < cnt) {
+                    v[k] = __builtin_nontemporal_load(vals + base + i);
+                    c[k] = __builtin_nontemporal_load(cols + base + i);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kStreamPerLane; ++k) {
+                const uint32_t i = k * 64 + lane;
+                if (i < cnt) {
+                    const T a = alpha * v[k];
+                    const T xv = x[c[k]];
+                    const T p = a * xv;
+                    buf[i] = p;
+                    if constexpr (EXT) lds.e[wave][i] = fma(a, xv, -p);
+                }
+            }
+        }
+        wave_sync_lds();
+
+        if (nrows <= 64) {
+            // Lane groups of 64 / 2^ceil(log2(nrows)) lanes per row.
+            const uint32_t lg = 32 - __clz(nrows - 1);
+            const uint32_t tpr = 64u >> lg;
+            const uint32_t lr = lane >> (6 - lg);
+            const uint32_t t = lane & (tpr - 1);
+            Acc<T, EXT> acc;
+            if (lr < nrows) {
+                const uint32_t s = rowPtr[row + lr] - base;
+                const uint32_t e = rowPtr[row + lr + 1] - base;
+                for (uint32_t j = s + t; j < e; j += tpr) {
+                    acc.add(buf[j]);
+                    if constexpr (EXT) acc.c += lds.e[wave][j];
+                }
+            }
+            acc = group_reduce(acc, tpr);
+            if (t == 0 && lr < nrows) write_row<T, EXT>(y, row + lr, acc, beta);
+        } else {
+            // More rows than lanes: each lane reduces whole rows from LDS.
+            for (uint32_t lr = lane; lr < nrows; lr += 64) {
+                const uint32_t s = rowPtr[row + lr] - base;
+                const uint32_t e = rowPtr[row + lr + 1] - base;
+                Acc<T, EXT> acc;
+                for (uint32_t j = s; j < e; ++j) {
+                    acc.add(buf[j]);
+                    if constexpr (EXT) acc.c += lds.e[wave][j];
+                }
+                write_row<T, EXT>(y, row + lr, acc, beta);
+            }
+        }
+    } else if (nrows == 1 && seg == 0) {
+        // ---- CSR-Vector: one row, <= 3072 nnz --------------------------
+        const uint32_t s = rowPtr[row];
+        const uint32_t e = rowPtr[row + 1];
+        Acc<T, EXT> acc = strided_dot<T, EXT>(vals, cols, x, alpha, s, e, lane);
+        acc = wave_reduce(acc);
+        if (lane == 0) write_row<T, EXT>(y, row, acc, beta);
+    } else {
+        // ---- CSR-LongRows segment --------------------------------------
+        const uint32_t rs = rowPtr[row];
+        const uint32_t re = rowPtr[row + 1];
+        const uint64_t vs64 = static_cast<uint64_t>(rs) + static_cast<uint64_t>(seg) * kLongSeg;
+        const uint32_t vs = vs64 < re ? static_cast<uint32_t>(vs64) : re;
+        const uint32_t ve = (re - vs > static_cast<uint32_t>(kLongSeg)) ? vs + kLongSeg : re;
+        Acc<T, EXT> acc = strided_dot<T, EXT>(vals, cols, x, alpha, vs, ve, lane);
+        acc = wave_reduce(acc);
+        if (lane == 0) {
+            // Segment partial (sum, and its running error in extended mode).
+            partials[2 * blk] = static_cast<double>(acc.s);
+            if constexpr (EXT) partials[2 * blk + 1] = static_cast<double>(acc.c);
+        }
+    }
+}
+
+// y[row] = beta * y[row] + sum of the row's segment partials, in segment order.
+template <typename T, bool EXT>
+__global__ __launch_bounds__(64) void csrmv_longrow_combine(const LongRow* __restrict__ lr,
+                                                           uint32_t numLong,
+                                                           const double* __restrict__ partials,
+                                                           T* __restrict__ y,
+                                                           const T* __restrict__ pBeta,
+                                                           const int* __restrict__ done)
+{
+    if (done != nullptr && *done != 0) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= numLong) return;
+    const LongRow L = lr[i];
+    Acc<double, EXT> acc;
+    for (uint32_t w = 0; w < L.num_segments; ++w) {
+        acc.add(partials[2 * (L.first_entry + w)]);
+        if constexpr (EXT) acc.c += partials[2 * (L.first_entry + w) + 1];
+    }
+    const T beta = *pBeta;
+    if (beta == T(0)) {
+        y[L.row] = static_cast<T>(acc.value());
+    } else if constexpr (EXT) {
+        acc.mul_add(static_cast<double>(beta), static_cast<double>(y[L.row]));
+        y[L.row] = static_cast<T>(acc.value());
+    } else {
+        y[L.row] = fma(beta, y[L.row], static_cast<T>(acc.value()));
+    }
+}
+
+// CSR-Vector fallback (csrmv_general.cl L189-259): SW lanes per row.
+template <typename T, bool EXT, unsigned SW>
+__global__ __launch_bounds__(256) void csrmv_vector_kernel(
+    uint32_t numRows, const T* __restrict__ pAlpha, const uint32_t* __restrict__ rowPtr,
+    const uint32_t* __restrict__ cols, const T* __restrict__ vals, const T* __restrict__ x,
+    const T* __restrict__ pBeta, T* __restrict__ y, const int* __restrict__ done)
+{
+    if (done != nullptr && *done != 0) return;
+    const uint64_t gid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & (SW - 1);
+    const uint64_t nvec = static_cast<uint64_t>(gridDim.x) * blockDim.x / SW;
+    const T alpha = *pAlpha;
+    const T beta = *pBeta;
+    // Every lane of a wave runs the same number of trips so the cross-lane
+    // reduction always sees a full wave.
+    const uint64_t first = gid / SW;
+    const uint64_t wave_first = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + (threadIdx.x & ~63u)) / SW;
+    for (uint64_t base = wave_first; base < numRows; base += nvec) {
+        const uint64_t row = base + (first - wave_first);
+        Acc<T, EXT> acc;
+        if (row < numRows) {
+            const uint32_t s = rowPtr[row];
+            const uint32_t e = rowPtr[row + 1];
+            for (uint32_t j = s + lane; j < e; j += SW) acc.mul_add(alpha * vals[j], x[cols[j]]);
+        }
+        acc = group_reduce(acc, SW);
+        if (lane == 0 && row < numRows) write_row<T, EXT>(y, static_cast<uint32_t>(row), acc, beta);
+    }
+}
+
+template <typename T, bool EXT, unsigned SW>
+void launch_vector_sw(uint32_t n, const T* alpha, const uint32_t* rp, const uint32_t* ci,
+                      const T* v, const T* x, const T* beta, T* y, const int* done,
+                      hipStream_t s)
+{
+    const uint64_t threads = static_cast<uint64_t>(n) * SW;
+    uint64_t wgs = (threads + 255) / 256;
+    if (wgs > (1u << 20)) wgs = (1u << 20);
+    if (wgs == 0) wgs = 1;
+    hipLaunchKernelGGL((csrmv_vector_kernel<T, EXT, SW>), dim3(static_cast<uint32_t>(wgs)),
+                       dim3(256), 0, s, n, alpha, rp, ci, v, x, beta, y, done);
+}
+
+template <typename T, bool EXT>
+void launch_vector(uint32_t n, uint32_t nnz, const T* alpha, const uint32_t* rp,
+                   const uint32_t* ci, const T* v, const T* x, const T* beta, T* y,
+                   const int* done, hipStream_t s)
+{
+    // Sub-wave width from the average row length (csrmv-vector.hpp L35-49, wave 64).
+    const uint32_t npr = nnz / n;
+    if (npr < 4) launch_vector_sw<T, EXT, 2>(n, alpha, rp, ci, v, x, beta, y, done, s);
+    else if (npr < 8) launch_vector_sw<T, EXT, 4>(n, alpha, rp, ci, v, x, beta, y, done, s);
+    else if (npr < 16) launch_vector_sw<T, EXT, 8>(n, alpha, rp, ci, v, x, beta, y, done, s);
+    else if (npr < 32) launch_vector_sw<T, EXT, 16>(n, alpha, rp, ci, v, x, beta, y, done, s);
+    else if (npr < 64) launch_vector_sw<T, EXT, 32>(n, alpha, rp, ci, v, x, beta, y, done, s);
+    else launch_vector_sw<T, EXT, 64>(n, alpha, rp, ci, v, x, beta, y, done, s);
+}
+
+}  // namespace
+
+template <typename T>
+clsparseStatus launch_csrmv(const T* alpha, const clsparseCsrMatrix* A, const T* x,
+                            const T* beta, T* y, hipStream_t s, bool ext, const int* done)
+{
+    const T* vals = dev_ptr<T>(A->values, A->off_values);
+    const uint32_t* cols = dev_ptr<uint32_t>(A->col_indices, A->off_col_indices);
+    const uint32_t* rp = dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer);
+
+    if (A->num_rows == 0) return clsparseSuccess;
+
+    if (A->meta == nullptr) {
+        if (ext)
+            launch_vector<T, true>(A->num_rows, A->num_nonzeros, alpha, rp, cols, vals, x, beta,
+                                   y, done, s);
+        else
+            launch_vector<T, false>(A->num_rows, A->num_nonzeros, alpha, rp, cols, vals, x,
+                                    beta, y, done, s);
+        CLS_HIP_TRY(hipGetLastError());
+        return clsparseSuccess;
+    }
+
+    const matrix_meta* m = get_meta(A);
+    if (m->magic != 0x4d455441u || m->rowBlockSize == 0) return clsparseStructInvalid;
+    // The metadata describes one sparsity pattern; refuse a matrix of another
+    // shape rather than index past its arrays.
+    if (m->num_rows != A->num_rows || m->num_nonzeros != A->num_nonzeros)
+        return clsparseStructInvalid;
+
+    const uint32_t numBlocks = m->entries - 1;
+    if (numBlocks > 0) {
+        const uint32_t wgs = (numBlocks + kWavesPerWg - 1) / kWavesPerWg;
+        if (ext)
+            hipLaunchKernelGGL((csrmv_adaptive_kernel<T, true>), dim3(wgs), dim3(256), 0, s, vals,
+                               cols, rp, x, y, m->d_rowBlocks, numBlocks, alpha, beta,
+                               m->d_partials, done);
+        else
+            hipLaunchKernelGGL((csrmv_adaptive_kernel<T, false>), dim3(wgs), dim3(256), 0, s,
+                               vals, cols, rp, x, y, m->d_rowBlocks, numBlocks, alpha, beta,
+                               m->d_partials, done);
+        CLS_HIP_TRY(hipGetLastError());
+    }
+    if (m->num_long_rows > 0) {
+        const uint32_t g = (m->num_long_rows + 63) / 64;
+        if (ext)
+            hipLaunchKernelGGL((csrmv_longrow_combine<T, true>), dim3(g), dim3(64), 0, s,
+                               m->d_long, m->num_long_rows, m->d_partials, y, beta, done);
+        else
+            hipLaunchKernelGGL((csrmv_longrow_combine<T, false>), dim3(g), dim3(64), 0, s,
+                               m->d_long, m->num_long_rows, m->d_partials, y, beta, done);
+        CLS_HIP_TRY(hipGetLastError());
+    }
+    return clsparseSuccess;
+}
+
+template clsparseStatus launch_csrmv<float>(const float*, const clsparseCsrMatrix*, const float*,
+                                            const float*, float*, hipStream_t, bool, const int*);
+template clsparseStatus launch_csrmv<double>(const double*, const clsparseCsrMatrix*,
+                                             const double*, const double*, double*, hipStream_t,
+                                             bool, const int*);
+
+namespace {
+
+template <typename T>
+clsparseStatus csrmv_entry(const clsparseScalar* alpha, const clsparseCsrMatrix* matx,
+                           const cldenseVector* x, const clsparseScalar* beta, cldenseVector* y,
+                           const clsparseControl control)
+{
+    CLS_TRY(check_init_control(control));
+    if (alpha == nullptr || matx == nullptr || x == nullptr || beta == nullptr || y == nullptr)
+        return clsparseInvalidValue;
+    if (matx->num_rows > 0 && (y->num_values < matx->num_rows || x->num_values < matx->num_cols))
+        return clsparseInvalidSize;
+    if (matx->num_rows > 0 &&
+        (matx->row_pointer == nullptr || alpha->value == nullptr || beta->value == nullptr ||
+         x->values == nullptr || y->values == nullptr ||
+         (matx->num_nonzeros > 0 && (matx->values == nullptr || matx->col_indices == nullptr))))
+        return clsparseInvalidMemObject;
+    CLS_TRY(launch_csrmv<T>(dev_ptr<T>(alpha->value, alpha->off_value), matx,
+                            dev_ptr<T>(x->values, x->off_values),
+                            dev_ptr<T>(beta->value, beta->off_value),
+                            dev_ptr<T>(y->values, y->off_values), control->stream,
+                            control->extended_precision, nullptr));
+    return finish_op(control);
+}
+
+}  // namespace
+
+// src/library/blas2/clsparse-csrmv.cpp L21-48
+extern "C" clsparseStatus clsparseScsrmv(const clsparseScalar* alpha,
+                                         const clsparseCsrMatrix* matx, const cldenseVector* x,
+                                         const clsparseScalar* beta, cldenseVector* y,
+                                         const clsparseControl control)
+{
+    return csrmv_entry<float>(alpha, matx, x, beta, y, control);
+}
+
+// src/library/blas2/clsparse-csrmv.cpp L50-76
+extern "C" clsparseStatus clsparseDcsrmv(const clsparseScalar* alpha,
+                                         const clsparseCsrMatrix* matx, const cldenseVector* x,
+                                         const clsparseScalar* beta, cldenseVector* y,
+                                         const clsparseControl control)
+{
+    return csrmv_entry<double>(alpha, matx, x, beta, y, control);
+}

@@ -1,0 +1,379 @@
+/*
+ * oracle.c -- CPU restatement of the clSPARSE hot path. TEST INFRASTRUCTURE.
+ *
+ * Used only as the checker by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py. Nothing in the product library links,
+ * loads or calls this file.
+ *
+ * Functions and the reference code they restate:
+ *   orc_row_blocks   src/library/internal/data-types/csr-meta.hpp L93-269
+ *                    (ComputeRowBlocks / ComputeRowBlocksSize), with
+ *                    numThreadsForReduction L52-68 (GCC branch).
+ *   orc_csrmv_*      y = alpha*A*x + beta*y as csrmv_adaptive.cl / csrmv_general.cl
+ *                    define it (beta == 0 never reads y, L481-483), evaluated
+ *                    in long double with Neumaier compensation: an FP64 oracle.
+ *   orc_cg_*         src/library/solvers/conjugate-gradients.hpp L47-229 and
+ *                    solver-control.hpp L47-64 (L1 residual, nIters quirk),
+ *                    Jacobi of preconditioners/diagonal.hpp L45-77; dot and
+ *                    norm reductions in double.
+ *   orc_spmv_omp_d   plain FP64 row-parallel CSR SpMV (OpenMP): the CPU
+ *                    baseline timed beside the GPU.
+ *
+ * Parity pins (which checks fix each function) are listed in the header of
+ * tests/test_oracle.py.
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ---- metadata ------------------------------------------------------------ */
+
+static uint64_t orc_nt(uint64_t num_rows)
+{
+    /* 256 >> (8*sizeof(int) - __builtin_clz(num_rows - 1)); the argument is
+     * truncated to unsigned int as the builtin's parameter type does. */
+    unsigned int a = (unsigned int)(num_rows - 1);
+    int sh = 32 - __builtin_clz(a);
+    return (uint64_t)(256 >> (sh & 31));
+}
+
+uint64_t orc_row_blocks(const uint32_t* rowDelimiters, uint32_t nRows, uint64_t* rowBlocks)
+{
+    const int allocate = rowBlocks != NULL;
+    const uint64_t blkSize = 1024, blkMultiplier = 3, rows_for_vector = 1;
+    uint64_t* base = rowBlocks;
+    uint64_t total_row_blocks = 1;
+    uint64_t sum = 0, i, last_i = 0;
+    uint32_t consecutive_long_rows = 0;
+
+    if (allocate) {
+        *rowBlocks = 0;
+        rowBlocks++;
+    }
+    for (i = 1; i <= nRows; i++) {
+        uint32_t row_length = rowDelimiters[i] - rowDelimiters[i - 1];
+        sum += row_length;
+
+        if (row_length > 128)
+            consecutive_long_rows++;
+        else if (consecutive_long_rows > 0) {
+            if (row_length < 32)
+                consecutive_long_rows = (uint32_t)-1;
+            else
+                consecutive_long_rows++;
+        }
+
+        if (consecutive_long_rows == 1) {
+            if (i - last_i > 1) {
+                if (allocate) {
+                    *rowBlocks = (i - 1) << 32;
+                    if (((i - 1) - last_i) > rows_for_vector)
+                        *(rowBlocks - 1) |= orc_nt((i - 1) - last_i);
+                    rowBlocks++;
+                }
+                total_row_blocks++;
+                last_i = i - 1;
+                sum = row_length;
+            }
+        } else if (consecutive_long_rows == (uint32_t)-1) {
+            if (allocate) {
+                *rowBlocks = (i - 1) << 32;
+                if (((i - 1) - last_i) > rows_for_vector)
+                    *(rowBlocks - 1) |= orc_nt((i - 1) - last_i);
+                rowBlocks++;
+            }
+            total_row_blocks++;
+            last_i = i - 1;
+            sum = row_length;
+            consecutive_long_rows = 0;
+        }
+
+        if ((i - last_i == 1) && sum > blkSize) {
+            int numWGReq = (int)ceil((double)row_length / (double)(blkMultiplier * blkSize));
+            numWGReq = (numWGReq < (1 << 24)) ? numWGReq : (1 << 24);
+            if (allocate) {
+                for (int w = 1; w < numWGReq; w++) {
+                    *rowBlocks = ((i - 1) << 32) | (uint64_t)w;
+                    rowBlocks++;
+                }
+                *rowBlocks = i << 32;
+                rowBlocks++;
+            }
+            total_row_blocks += (uint64_t)numWGReq;
+            last_i = i;
+            sum = 0;
+            consecutive_long_rows = 0;
+        } else if ((i - last_i > 1) && sum > blkSize) {
+            i--;
+            if (allocate) {
+                *rowBlocks = i << 32;
+                if ((i - last_i) > rows_for_vector) *(rowBlocks - 1) |= orc_nt(i - last_i);
+                rowBlocks++;
+            }
+            total_row_blocks++;
+            last_i = i;
+            sum = 0;
+            consecutive_long_rows = 0;
+        } else if (sum == blkSize) {
+            if (allocate) {
+                *rowBlocks = i << 32;
+                if ((i - last_i) > rows_for_vector) *(rowBlocks - 1) |= orc_nt(i - last_i);
+                rowBlocks++;
+            }
+            total_row_blocks++;
+            last_i = i;
+            sum = 0;
+            consecutive_long_rows = 0;
+        }
+    }
+
+    if (allocate && (*(rowBlocks - 1) >> 32) != (uint64_t)nRows) {
+        *rowBlocks = (uint64_t)nRows << 32;
+        if (((uint64_t)nRows - last_i) > rows_for_vector) *(rowBlocks - 1) |= orc_nt(i - last_i);
+        rowBlocks++;
+    }
+    total_row_blocks++;
+
+    if (allocate) return 2 * (uint64_t)(rowBlocks - base);
+    return 2 * total_row_blocks;
+}
+
+/* ---- SpMV ------------------------------------------------------------------ */
+
+/* Compensated (Neumaier) accumulation in long double. */
+typedef struct {
+    long double s, c;
+} ksum;
+
+static void ks_add(ksum* k, long double v)
+{
+    long double t = k->s + v;
+    if (fabsl(k->s) >= fabsl(v))
+        k->c += (k->s - t) + v;
+    else
+        k->c += (v - t) + k->s;
+    k->s = t;
+}
+
+/* y_out[i] = alpha * sum_j a_ij x_j + beta * y_in[i]  (y_in unread if beta == 0).
+ * scale[i] = |alpha| * sum_j |a_ij x_j| + |beta y_in[i]|: the magnitude the
+ * per-row relative error is measured against. */
+void orc_csrmv_d(uint32_t n, const uint32_t* rp, const uint32_t* ci, const double* v,
+                 const double* x, double alpha, double beta, const double* y_in, double* y_out,
+                 double* scale)
+{
+    for (uint32_t r = 0; r < n; ++r) {
+        ksum s = {0, 0};
+        long double mag = 0;
+        for (uint32_t j = rp[r]; j < rp[r + 1]; ++j) {
+            long double p = (long double)v[j] * (long double)x[ci[j]];
+            ks_add(&s, p);
+            mag += fabsl(p);
+        }
+        long double out = (long double)alpha * (s.s + s.c);
+        long double m = fabsl((long double)alpha) * mag;
+        if (beta != 0.0) {
+            out += (long double)beta * (long double)y_in[r];
+            m += fabsl((long double)beta * (long double)y_in[r]);
+        }
+        y_out[r] = (double)out;
+        if (scale) scale[r] = (double)m;
+    }
+}
+
+void orc_csrmv_s(uint32_t n, const uint32_t* rp, const uint32_t* ci, const float* v,
+                 const float* x, float alpha, float beta, const float* y_in, double* y_out,
+                 double* scale)
+{
+    for (uint32_t r = 0; r < n; ++r) {
+        ksum s = {0, 0};
+        long double mag = 0;
+        for (uint32_t j = rp[r]; j < rp[r + 1]; ++j) {
+            long double p = (long double)v[j] * (long double)x[ci[j]];
+            ks_add(&s, p);
+            mag += fabsl(p);
+        }
+        long double out = (long double)alpha * (s.s + s.c);
+        long double m = fabsl((long double)alpha) * mag;
+        if (beta != 0.0f) {
+            out += (long double)beta * (long double)y_in[r];
+            m += fabsl((long double)beta * (long double)y_in[r]);
+        }
+        y_out[r] = (double)out;
+        if (scale) scale[r] = (double)m;
+    }
+}
+
+/* Plain row-parallel FP64 CSR SpMV (CPU baseline). */
+void orc_spmv_omp_d(uint32_t n, const uint32_t* rp, const uint32_t* ci, const double* v,
+                    const double* x, double* y, int threads)
+{
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 512) num_threads(threads)
+#endif
+    for (int64_t r = 0; r < (int64_t)n; ++r) {
+        double s = 0.0;
+        for (uint32_t j = rp[r]; j < rp[r + 1]; ++j) s += v[j] * x[ci[j]];
+        y[r] = s;
+    }
+    (void)threads;
+}
+
+int orc_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* ---- BLAS-1 ---------------------------------------------------------------- */
+
+double orc_dot_d(size_t n, const double* x, const double* y)
+{
+    ksum s = {0, 0};
+    for (size_t i = 0; i < n; ++i) ks_add(&s, (long double)x[i] * (long double)y[i]);
+    return (double)(s.s + s.c);
+}
+
+double orc_nrm1_d(size_t n, const double* x)
+{
+    ksum s = {0, 0};
+    for (size_t i = 0; i < n; ++i) ks_add(&s, fabsl((long double)x[i]));
+    return (double)(s.s + s.c);
+}
+
+double orc_nrm2_d(size_t n, const double* x)
+{
+    ksum s = {0, 0};
+    for (size_t i = 0; i < n; ++i) ks_add(&s, (long double)x[i] * (long double)x[i]);
+    return (double)sqrtl(s.s + s.c);
+}
+
+/* ---- CG ---------------------------------------------------------------------- */
+
+typedef struct {
+    int status;          /* 0 = clsparseSuccess */
+    int n_iters;         /* solverControl->nIters after the call */
+    int executed;        /* loop iterations run */
+    int early_exit;      /* 1: returned before the loop */
+    double initial_residual;
+    double current_residual;
+    double rel_tol_out;  /* tolerances after the call (b == 0 zeroes them) */
+    double abs_tol_out;
+} orc_cg_result;
+
+/* fused multiply-add in the vector's own precision */
+#define ORC_FMA(a, b, c) _Generic((a), float: fmaf, default: fma)((a), (b), (c))
+
+#define ORC_CG_BODY(T)                                                                      \
+    orc_cg_result res;                                                                      \
+    memset(&res, 0, sizeof(res));                                                           \
+    res.rel_tol_out = relTol;                                                               \
+    res.abs_tol_out = absTol;                                                               \
+    res.n_iters = nIters_prior;                                                             \
+    double norm_b = 0;                                                                      \
+    for (uint32_t i = 0; i < n; ++i) norm_b += fabs((double)b[i]);                          \
+    if (norm_b == 0) {                                                                      \
+        res.n_iters = 0;                                                                    \
+        res.rel_tol_out = 0;                                                                \
+        res.abs_tol_out = 0;                                                                \
+        for (uint32_t i = 0; i < n; ++i) x[i] = b[i];                                       \
+        res.early_exit = 1;                                                                 \
+        return res;                                                                         \
+    }                                                                                       \
+    T* y = (T*)calloc(n, sizeof(T));                                                        \
+    T* z = (T*)calloc(n, sizeof(T));                                                        \
+    T* r = (T*)calloc(n, sizeof(T));                                                        \
+    T* p = (T*)calloc(n, sizeof(T));                                                        \
+    T* invD = (T*)malloc(n * sizeof(T));                                                    \
+    for (uint32_t i = 0; i < n; ++i) {                                                      \
+        T d = (T)1;                                                                         \
+        for (uint32_t j = rp[i]; j < rp[i + 1]; ++j)                                        \
+            if (ci[j] == i) {                                                               \
+                d = (T)1 / v[j];                                                            \
+                break;                                                                      \
+            }                                                                               \
+        invD[i] = d;                                                                        \
+    }                                                                                       \
+    /* y = A x; r = b - y */                                                                \
+    for (uint32_t i = 0; i < n; ++i) {                                                      \
+        T s = 0;                                                                            \
+        for (uint32_t j = rp[i]; j < rp[i + 1]; ++j) s = ORC_FMA(v[j], x[ci[j]], s);         \
+        y[i] = s;                                                                           \
+        r[i] = b[i] - y[i];                                                                 \
+    }                                                                                       \
+    double norm_r = 0;                                                                      \
+    for (uint32_t i = 0; i < n; ++i) norm_r += fabs((double)r[i]);                          \
+    double residuum = norm_r / norm_b;                                                      \
+    res.initial_residual = residuum;                                                        \
+    res.current_residual = residuum;                                                        \
+    if (residuum <= relTol || residuum <= absTol * residuum || nIters_prior >= maxIters) {  \
+        res.n_iters = 0;                                                                    \
+        res.early_exit = 1;                                                                 \
+        goto done;                                                                          \
+    }                                                                                       \
+    for (uint32_t i = 0; i < n; ++i) {                                                      \
+        z[i] = precond ? r[i] * invD[i] : r[i];                                             \
+        p[i] = z[i];                                                                        \
+    }                                                                                       \
+    double rz = 0;                                                                          \
+    for (uint32_t i = 0; i < n; ++i) rz += (double)r[i] * (double)z[i];                     \
+    int iteration = 0;                                                                      \
+    for (;;) {                                                                              \
+        res.n_iters = iteration;                                                            \
+        for (uint32_t i = 0; i < n; ++i) {                                                  \
+            T s = 0;                                                                        \
+            for (uint32_t j = rp[i]; j < rp[i + 1]; ++j) s = ORC_FMA(v[j], p[ci[j]], s);     \
+            y[i] = s;                                                                       \
+        }                                                                                   \
+        double yp = 0;                                                                      \
+        for (uint32_t i = 0; i < n; ++i) yp += (double)y[i] * (double)p[i];                 \
+        const T alpha = (T)(rz / yp);                                                       \
+        for (uint32_t i = 0; i < n; ++i) {                                                  \
+            x[i] = x[i] + alpha * p[i];                                                     \
+            r[i] = r[i] - alpha * y[i];                                                     \
+            z[i] = precond ? r[i] * invD[i] : r[i];                                         \
+        }                                                                                   \
+        const double rz_old = rz;                                                           \
+        rz = 0;                                                                             \
+        for (uint32_t i = 0; i < n; ++i) rz += (double)r[i] * (double)z[i];                 \
+        const T beta = (T)(rz / rz_old);                                                    \
+        for (uint32_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];                         \
+        norm_r = 0;                                                                         \
+        for (uint32_t i = 0; i < n; ++i) norm_r += fabs((double)r[i]);                      \
+        residuum = norm_r / norm_b;                                                         \
+        iteration++;                                                                        \
+        res.current_residual = residuum;                                                    \
+        if (residuum <= relTol || residuum <= absTol * res.initial_residual ||              \
+            res.n_iters >= maxIters)                                                        \
+            break;                                                                          \
+    }                                                                                       \
+    res.executed = iteration;                                                               \
+done:                                                                                       \
+    free(y);                                                                                \
+    free(z);                                                                                \
+    free(r);                                                                                \
+    free(p);                                                                                \
+    free(invD);                                                                             \
+    return res;
+
+orc_cg_result orc_cg_d(uint32_t n, const uint32_t* rp, const uint32_t* ci, const double* v,
+                       const double* b, double* x, int precond, int maxIters, double relTol,
+                       double absTol, int nIters_prior)
+{
+    ORC_CG_BODY(double)
+}
+
+orc_cg_result orc_cg_s(uint32_t n, const uint32_t* rp, const uint32_t* ci, const float* v,
+                       const float* b, float* x, int precond, int maxIters, double relTol,
+                       double absTol, int nIters_prior)
+{
+    ORC_CG_BODY(float)
+}

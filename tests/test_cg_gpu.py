@@ -1,0 +1,217 @@
+"""clsparseScsrcg / clsparseDcsrcg on the GPU against the oracle's restatement of
+conjugate-gradients.hpp L47-229 and the closed forms it must reproduce.
+
+Parity bar: FP64 -- nIters equal to the oracle's within 1 and the final L1
+residual below the tolerance; FP32 -- nIters within 3 (reduction order differs).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import clsparse_amd as cls
+import oracle
+from gpu_helpers import cg_call, say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctl():
+    c = cls.Control()
+    yield c
+    c.release()
+
+
+def _solve(ctl, rp, ci, v, m, b, x0, dtype, precond, max_iters, rel, abst, meta=True):
+    dt = torch.float32 if dtype == np.float32 else torch.float64
+    A = cls.CsrMatrix(rp, ci, v, m, dtype=dtype)
+    if meta:
+        A.meta_create(ctl)
+    x = torch.tensor(np.asarray(x0, dtype=dtype), device="cuda")
+    bt = torch.tensor(np.asarray(b, dtype=dtype), device="cuda")
+    sc = cls.SolverControl(precond, max_iters, rel, abst)
+    st = cg_call(A, x, bt, sc, ctl)
+    stats = sc.stats()
+    sc.release()
+    return st, x.cpu().numpy(), stats
+
+
+@pytest.mark.parametrize("meta", [True, False], ids=["adaptive", "vector"])
+@pytest.mark.parametrize("precond", [cls.DIAGONAL, cls.NOPRECOND], ids=["jacobi", "none"])
+def test_poisson_fp64_matches_oracle(ctl, precond, meta):
+    say(f"cg poisson fp64 precond={precond} meta={meta}")
+    rp, ci, v, m = cls.gen_poisson2d(64, 64)
+    b = cls.gen_uniform(m, -1, 1, seed=3)
+    st, x, s = _solve(ctl, rp, ci, v, m, b, np.zeros(m), np.float64, precond, 2000, 1e-10, 0.0,
+                      meta)
+    assert st == 0
+    xr, r = oracle.cg(rp, ci, v, b, np.zeros(m), precond, 2000, 1e-10, 0.0)
+    assert abs(s.n_iters - r.n_iters) <= 1, (s.n_iters, r.n_iters)
+    assert s.current_residual <= 1e-10 and s.iterations == s.n_iters + 1
+    assert s.initial_residual == pytest.approx(r.initial_residual, rel=1e-13)
+    np.testing.assert_allclose(x, xr, atol=1e-8 * np.abs(xr).max())
+
+
+def test_poisson_fp32_matches_oracle(ctl):
+    say("cg poisson fp32")
+    rp, ci, v, m = cls.gen_poisson2d(48, 48)
+    b = np.ones(m)
+    st, x, s = _solve(ctl, rp, ci, v, m, b, np.zeros(m), np.float32, cls.DIAGONAL, 1000, 1e-5,
+                      0.0)
+    assert st == 0
+    xr, r = oracle.cg(rp, ci, v.astype(np.float32), b.astype(np.float32),
+                      np.zeros(m, np.float32), 1, 1000, 1e-5, 0.0)
+    assert abs(s.n_iters - r.n_iters) <= 3, (s.n_iters, r.n_iters)
+    assert s.current_residual <= 1e-5
+    np.testing.assert_allclose(x, xr, rtol=0, atol=2e-4 * np.abs(xr).max())
+
+
+def test_reference_defaults(ctl):
+    """test-solvers.cpp defaults: relTol 1e-2, absTol 1e-5, maxIters 100, Diagonal, x0 = 0, b = 1."""
+    rp, ci, v, m = cls.gen_poisson2d(32, 32)
+    for dtype in (np.float32, np.float64):
+        st, x, s = _solve(ctl, rp, ci, v, m, np.ones(m), np.zeros(m), dtype, cls.DIAGONAL, 100,
+                          1e-2, 1e-5)
+        _, r = oracle.cg(rp, ci, v.astype(dtype), np.ones(m, dtype), np.zeros(m, dtype), 1, 100,
+                         1e-2, 1e-5)
+        assert st == 0 and abs(s.n_iters - r.n_iters) <= 1
+
+
+def test_diagonal_matrix_one_iteration(ctl):
+    n = 1000
+    d = np.linspace(0.5, 20, n)
+    rp, ci = np.arange(n + 1, dtype=np.uint32), np.arange(n, dtype=np.uint32)
+    b = cls.gen_uniform(n, -1, 1, seed=1)
+    st, x, s = _solve(ctl, rp, ci, d, n, b, np.zeros(n), np.float64, cls.DIAGONAL, 50, 1e-12, 0.0)
+    assert st == 0 and s.n_iters == 0 and s.iterations == 1
+    np.testing.assert_allclose(x, b / d, rtol=1e-15)
+
+
+def test_three_distinct_eigenvalues(ctl):
+    n = 48
+    d = np.array([1.0, 3.0, 7.0] * 16)
+    q = np.linalg.qr(np.random.default_rng(0).normal(size=(n, n)))[0]
+    a = (q * d) @ q.T
+    a = 0.5 * (a + a.T)
+    rp = np.arange(0, n * n + 1, n, dtype=np.uint32)
+    ci = np.tile(np.arange(n, dtype=np.uint32), n)
+    b = cls.gen_uniform(n, -1, 1, seed=2)
+    st, x, s = _solve(ctl, rp, ci, a.reshape(-1), n, b, np.zeros(n), np.float64, cls.NOPRECOND,
+                      50, 1e-12, 0.0)
+    assert st == 0 and s.iterations <= 3
+    np.testing.assert_allclose(a @ x, b, atol=1e-10)
+
+
+def test_zero_rhs_returns_b(ctl):
+    rp, ci, v, m = cls.gen_poisson2d(16, 16)
+    st, x, s = _solve(ctl, rp, ci, v, m, np.zeros(m), np.full(m, 5.0), np.float64, cls.DIAGONAL,
+                      10, 1e-2, 1e-5)
+    assert st == 0 and np.all(x == 0) and s.n_iters == 0
+    assert s.relative_tolerance == 0 and s.absolute_tolerance == 0
+
+
+def test_max_iters_and_stored_niters_quirks(ctl):
+    rp, ci, v, m = cls.gen_poisson2d(32, 32)
+    A = cls.CsrMatrix(rp, ci, v, m)
+    A.meta_create(ctl)
+    b = torch.ones(m, dtype=torch.float64, device="cuda")
+    x = torch.zeros(m, dtype=torch.float64, device="cuda")
+    sc = cls.SolverControl(cls.DIAGONAL, 3, 0.0, 0.0)
+    assert cg_call(A, x, b, sc, ctl) == 0
+    s = sc.stats()
+    assert s.n_iters == 3 and s.iterations == 4  # maxIters + 1 iterations run
+    xr, r = oracle.cg(rp, ci, v, np.ones(m), np.zeros(m), 1, 3, 0.0, 0.0)
+    np.testing.assert_allclose(x.cpu().numpy(), xr, rtol=1e-12)
+    # Same solver control again: stored nIters (3) >= maxIters returns at once.
+    x2 = torch.zeros(m, dtype=torch.float64, device="cuda")
+    assert cg_call(A, x2, b, sc, ctl) == 0
+    assert sc.stats().n_iters == 0 and torch.count_nonzero(x2).item() == 0
+    sc.release()
+
+
+def test_error_codes(ctl):
+    lib = cls.load()
+    rp, ci, v, m = cls.gen_poisson2d(8, 8)
+    A = cls.CsrMatrix(rp, ci, v, m)
+    x = cls.DenseVector(torch.zeros(m - 1, dtype=torch.float64, device="cuda"))
+    b = cls.DenseVector(torch.ones(m, dtype=torch.float64, device="cuda"))
+    sc = cls.SolverControl()
+    B = ctypes.byref
+    assert lib.clsparseDcsrcg(B(x.s), B(A.s), B(b.s), None, ctl.handle) == \
+        cls.clsparseInvalidSolverControlObject
+    assert lib.clsparseDcsrcg(B(x.s), B(A.s), B(b.s), sc.handle, None) == \
+        cls.clsparseInvalidControlObject
+    assert lib.clsparseDcsrcg(B(x.s), B(A.s), B(b.s), sc.handle, ctl.handle) == \
+        cls.clsparseInvalidSystemSize
+    sc.release()
+
+
+def test_verbose_prints_every_iteration(ctl, capfd):
+    rp, ci, v, m = cls.gen_poisson2d(16, 16)
+    A = cls.CsrMatrix(rp, ci, v, m)
+    A.meta_create(ctl)
+    x = torch.zeros(m, dtype=torch.float64, device="cuda")
+    b = torch.ones(m, dtype=torch.float64, device="cuda")
+    sc = cls.SolverControl(cls.DIAGONAL, 500, 1e-8, 0.0, print_mode=cls.VERBOSE)
+    assert cg_call(A, x, b, sc, ctl) == 0
+    out = capfd.readouterr().out
+    n_iters = sc.stats().n_iters
+    lines = [ln for ln in out.splitlines() if ln.startswith("Iteration: ")]
+    assert len(lines) == n_iters + 1
+    assert lines[-1].startswith(f"Iteration: {n_iters} ")
+    assert "Solver finished calculations with status 0" in out
+    sc.release()
+
+
+def test_graph_and_stream_paths_identical(ctl, monkeypatch):
+    rp, ci, v, m = cls.gen_diffusion3d(24, 24, 24, sigma=1.0, seed=4)
+    b = cls.gen_uniform(m, -1, 1, seed=5)
+    res = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("CLSPARSE_AMD_NO_GRAPH", flag)
+        st, x, s = _solve(ctl, rp, ci, v, m, b, np.zeros(m), np.float64, cls.DIAGONAL, 3000,
+                          1e-9, 0.0)
+        assert st == 0
+        res.append((x, s.n_iters))
+    assert res[0][1] == res[1][1]
+    assert np.array_equal(res[0][0], res[1][0])
+
+
+def test_diffusion_fp64_true_residual(ctl):
+    say("cg diffusion 32^3")
+    rp, ci, v, m = cls.gen_diffusion3d(32, 32, 32, sigma=1.0, seed=6)
+    xs = cls.gen_uniform(m, -1, 1, seed=7)
+    b, _ = oracle.csrmv(rp, ci, v, xs)
+    st, x, s = _solve(ctl, rp, ci, v, m, b, np.zeros(m), np.float64, cls.DIAGONAL, 5000, 1e-11,
+                      0.0)
+    assert st == 0
+    r = b - oracle.csrmv(rp, ci, v, x)[0]
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-9
+    _, ro = oracle.cg(rp, ci, v, b, np.zeros(m), 1, 5000, 1e-11, 0.0)
+    assert abs(s.n_iters - ro.n_iters) <= 2, (s.n_iters, ro.n_iters)
+
+
+def test_mixed_precision_refinement(ctl):
+    """FP32 inner Jacobi-PCG + FP64 outer residual reaches an FP64 true residual of 1e-10."""
+    say("mixed precision IR")
+    rp, ci, v, m = cls.gen_diffusion3d(32, 32, 32, sigma=1.0, seed=8)
+    b = cls.gen_uniform(m, -1, 1, seed=9)
+    A = cls.CsrMatrix(rp, ci, v, m)
+    A.meta_create(ctl)
+    x = torch.zeros(m, dtype=torch.float64, device="cuda")
+    bt = torch.tensor(b, device="cuda")
+    sc = cls.SolverControl(cls.DIAGONAL, 20000, 1e-10, 0.0)
+    lib = cls.load()
+    xd, bd = cls.DenseVector(x), cls.DenseVector(bt)
+    assert lib.clsparseAmdDcsrcgMixed(ctypes.byref(xd.s), ctypes.byref(A.s), ctypes.byref(bd.s),
+                                      sc.handle, ctl.handle) == 0
+    s = sc.stats()
+    xh = x.cpu().numpy()
+    r = b - oracle.csrmv(rp, ci, v, xh)[0]
+    true_rel = np.linalg.norm(r) / np.linalg.norm(b)
+    assert true_rel <= 1e-10 and s.true_residual <= 1e-10
+    assert s.true_residual == pytest.approx(true_rel, rel=1e-3)
+    assert s.outer_iterations >= 2
+    sc.release()

@@ -1,0 +1,173 @@
+"""CPU-side checks of the product library: ABI, host binning, generators.
+
+The HIP library loads without a GPU; only entry points that never touch the
+device are called here.
+"""
+import ctypes
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import clsparse_amd as cls
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden", "row_blocks_known.json")
+
+
+def _declared_symbols():
+    names = set()
+    for h in ("clSPARSE.h", "clSPARSE-amd-ext.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        for m in re.finditer(r"CLSPARSE_EXPORT\s+[\w\s\*]+?\b(\w+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = cls.load()
+    declared = _declared_symbols()
+    assert len(declared) >= 57
+    for name in sorted(declared):
+        assert hasattr(lib, name), name
+    assert declared == set(cls.EXPORTED)
+
+
+def test_header_struct_layout_matches_python_mirror(tmp_path):
+    src = tmp_path / "layout.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "clSPARSE.h"
+#include "clSPARSE-amd-ext.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(clsparseScalar), sizeof(cldenseVector),
+         sizeof(clsparseCsrMatrix), offsetof(clsparseCsrMatrix, meta),
+         sizeof(clsparseAmdSolverStats), sizeof(clsparseAmdHostCsr));
+  return 0;
+}''')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    got = [int(v) for v in out]
+    want = [ctypes.sizeof(cls.clsparseScalar), ctypes.sizeof(cls.cldenseVector),
+            ctypes.sizeof(cls.clsparseCsrMatrix), cls.clsparseCsrMatrix.meta.offset,
+            ctypes.sizeof(cls.clsparseAmdSolverStats), ctypes.sizeof(cls.clsparseAmdHostCsr)]
+    assert got == want
+    # LP64 layout of the reference structs with cl_mem -> pointer.
+    assert got[:4] == [16, 24, 64, 56]
+
+
+def test_version_and_setup_without_device():
+    lib = cls.load()
+    v = [ctypes.c_uint32() for _ in range(4)]
+    assert lib.clsparseGetVersion(*[ctypes.byref(t) for t in v]) == 0
+    assert [t.value for t in v] == [0, 10, 2, 0]
+    assert lib.clsparseTeardown() == 0
+    # Entry points check the init flag before touching the device.
+    s = cls.clsparseScalar()
+    x = cls.cldenseVector()
+    assert lib.cldenseDnrm1(ctypes.byref(s), ctypes.byref(x), ctypes.c_void_p(1)) == \
+        cls.clsparseNotInitialized
+    assert lib.clsparseSetup() == 0
+    assert lib.cldenseDnrm1(ctypes.byref(s), ctypes.byref(x), None) == \
+        cls.clsparseInvalidControlObject
+    assert lib.clsparseReleaseSolverControl(None) == cls.clsparseInvalidSolverControlObject
+    r = lib.clsparseCreateSolverControl(cls.DIAGONAL, 100, 1e-2, 1e-5)
+    assert r.status == 0
+    st = cls.clsparseAmdSolverStats()
+    assert lib.clsparseAmdGetSolverStats(ctypes.c_void_p(r.control), ctypes.byref(st)) == 0
+    assert (st.max_iters, st.relative_tolerance, st.absolute_tolerance) == (100, 1e-2, 1e-5)
+    assert lib.clsparseReleaseSolverControl(ctypes.c_void_p(r.control)) == 0
+
+
+@pytest.mark.parametrize("case", json.load(open(GOLDEN))["cases"], ids=lambda c: c["name"])
+def test_library_binning_known_answers(case):
+    rp = np.concatenate([[0], np.cumsum(case["row_lengths"])]).astype(np.uint32)
+    entries, size = cls.row_blocks_host(rp)
+    assert [int(e) for e in entries] == case["entries"]
+    assert size == case["row_block_size"]
+    assert cls.row_blocks_host(rp, count_only=True) == case["size_pass"]
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_library_binning_matches_oracle_random(seed):
+    rng = np.random.default_rng(1000 + seed)
+    n = int(rng.integers(1, 50000))
+    mix = [rng.integers(0, 8, n), rng.integers(0, 300, n),
+           np.clip(np.round(np.exp(rng.normal(1.6, 1.5, n))), 0, 200000),
+           rng.choice([0, 1, 2, 31, 32, 33, 127, 128, 129, 1023, 1024, 1025, 3072, 3073, 9000], n)]
+    lens = mix[seed % 4].astype(np.int64)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+    e1, s1 = cls.row_blocks_host(rp)
+    e2, s2 = oracle.row_blocks(rp)
+    assert s1 == s2
+    np.testing.assert_array_equal(e1, e2)
+    assert cls.row_blocks_host(rp, count_only=True) == oracle.row_blocks(rp, count_only=True)
+
+
+def test_library_binning_config_patterns_match_oracle():
+    # The three structured workloads at full size (pattern only).
+    for rp in (cls.gen_poisson2d(256, 256)[0], cls.gen_diffusion3d(64, 64, 64)[0]):
+        e1, s1 = cls.row_blocks_host(rp)
+        e2, s2 = oracle.row_blocks(rp)
+        assert s1 == s2
+        np.testing.assert_array_equal(e1, e2)
+
+
+def test_poisson_generator():
+    rp, ci, v, m = cls.gen_poisson2d(256, 256)
+    n = 256 * 256
+    assert m == n and rp[-1] == 5 * n - 4 * 256 == 326656
+    # sorted unique columns, diagonal 4, symmetric
+    for r in (0, 1, 255, 256, 257, 30000, n - 1):
+        cols = ci[rp[r]:rp[r + 1]]
+        assert np.all(np.diff(cols.astype(np.int64)) > 0)
+        assert v[rp[r] + list(cols).index(r)] == 4.0
+    import scipy.sparse as sp
+    a = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    assert abs(a - a.T).max() == 0
+
+
+def test_diffusion_generator_spd_structure():
+    import scipy.sparse as sp
+    rp, ci, v, m = cls.gen_diffusion3d(12, 10, 8, sigma=1.0, ax=1.0, ay=1.0, az=0.01, seed=5)
+    n = 12 * 10 * 8
+    assert rp[-1] == 7 * n - 2 * (12 * 10 + 10 * 8 + 12 * 8)
+    a = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    assert abs(a - a.T).max() <= 1e-15 * abs(a).max()
+    d = a.diagonal()
+    off = np.asarray(abs(a).sum(axis=1)).ravel() - d
+    # Weakly diagonally dominant; strictly on rows with a Dirichlet face.
+    assert np.all(d >= off * (1 - 1e-14))
+    boundary = np.diff(rp.astype(np.int64)) < 7
+    assert np.all(d[boundary] > off[boundary])
+    assert np.linalg.eigvalsh(a.toarray()).min() > 0
+
+
+def test_powerlaw_generator_statistics():
+    rp, ci, v, m = cls.gen_powerlaw(1 << 16, 16.0, 1.5, 200000, 2, 20000, seed=3)
+    lens = np.diff(rp.astype(np.int64))
+    assert lens.min() >= 1 and (lens == 20000).sum() >= 2
+    body = lens[lens != 20000]
+    assert 14.0 < body.mean() < 18.0
+    for r in range(0, m, 997):
+        c = ci[rp[r]:rp[r + 1]].astype(np.int64)
+        assert np.all(np.diff(c) > 0) and c.min() >= 0 and c.max() < m
+    assert np.all(np.abs(v) <= 1.0)
+    # deterministic in the seed
+    rp2, ci2, v2, _ = cls.gen_powerlaw(1 << 16, 16.0, 1.5, 200000, 2, 20000, seed=3)
+    np.testing.assert_array_equal(ci, ci2)
+    np.testing.assert_array_equal(v, v2)
+
+
+def test_uniform_generator():
+    a = cls.gen_uniform(100000, -1.0, 1.0, seed=11)
+    assert a.min() >= -1.0 and a.max() < 1.0 and abs(a.mean()) < 0.01
+    np.testing.assert_array_equal(a, cls.gen_uniform(100000, -1.0, 1.0, seed=11))
+    assert not np.array_equal(a, cls.gen_uniform(100000, -1.0, 1.0, seed=12))
