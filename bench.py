@@ -5,11 +5,22 @@ Workload: n = 2^21 rows/cols, row lengths clip(round(LogNormal(ln 16 - 1.125, 1.
 values and x uniform[-1, 1] (FP64). One step = one clsparseDcsrmv (y = A x) on
 device-resident inputs. GB/s uses the reference's byte count
 (src/benchmarks/clsparse-bench/functions/clfunc_xSpMdV.hpp L91):
-    bytes = 4 (nnz + n) + 8 (nnz + ncols + n)
+    bytes = 4 (nnz + n) + sizeof(T) (nnz + ncols + n)
 Prints one JSON line. The CPU baseline is the oracle's OpenMP FP64 CSR SpMV on
-the host cores, timed on a bounded number of calls of the same matrix.
+the host cores, timed on a bounded number of calls of the same matrix. The
+line's "extra" block carries the solver workload: 3-D 7-point variable-coefficient
+diffusion on 128^3 (kappa ~ LogNormal(0, 1), harmonic-mean faces), b ~ U[-1, 1],
+solved to an FP64 true relative residual of 1e-10 by mixed-precision refinement
+and by FP64 Jacobi-PCG.
+
+Other modes (diagnostics, not the bench line):
+  --mode copy      plain streaming kernel of known size, to calibrate rocprof counters
+  --mode locality  the same power-law rows with columns folded into smaller windows
+                   (x-gather locality sweep), and the 128^3 stencil SpMV in FP64/FP32
+  --mode cg        only the solver workload
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -20,6 +31,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "SpMV GB/s & % HBM peak; CG s-to-1e-8 resid, learned FSAI vs Jacobi"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def spmv_bytes(n, ncols, nnz, value_bytes=8):
@@ -40,8 +55,7 @@ def cpu_baseline(rp, ci, v, x, budget_s=3.0, max_calls=30):
         oracle.spmv_omp(rp, ci, v, x, y, threads)
         times.append(time.perf_counter() - t0)
     times.sort()
-    med = times[len(times) // 2]
-    return med, threads, len(times)
+    return times[len(times) // 2], threads, len(times)
 
 
 def load_traffic():
@@ -52,12 +66,94 @@ def load_traffic():
         return json.load(f).get("traffic_bytes_per_launch")
 
 
+class SpmvRun:
+    """Device-resident CSR-Adaptive SpMV y = A x of one matrix."""
+
+    def __init__(self, cls, torch, rp, ci, v, m, x, dtype, ctl):
+        import numpy as np
+        tdt = torch.float32 if dtype == np.float32 else torch.float64
+        self.cls, self.torch, self.ctl = cls, torch, ctl
+        self.A = cls.CsrMatrix(rp, ci, v, m, dtype=dtype)
+        self.A.meta_create(ctl)
+        self.x = cls.DenseVector(torch.tensor(np.asarray(x, dtype=dtype), device="cuda"))
+        self.y = cls.DenseVector(torch.zeros(rp.size - 1, dtype=tdt, device="cuda"))
+        self.alpha, self.beta = cls.Scalar(1.0, tdt), cls.Scalar(0.0, tdt)
+        self.nbytes = spmv_bytes(rp.size - 1, m, int(rp[-1]), np.dtype(dtype).itemsize)
+
+    def step(self):
+        st = self.A.spmv(self.alpha, self.x, self.beta, self.y, self.ctl)
+        if st != 0:
+            raise self.cls.ClsparseError("csrmv", st)
+
+    def time(self, steps, warmup, stream):
+        torch = self.torch
+        for _ in range(warmup):
+            self.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            self.step()
+        torch.cuda.synchronize()
+        wall_ms = (time.perf_counter() - t0) * 1e3 / steps
+        per = []
+        for _ in range(min(50, steps)):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            self.step()
+            b.record(stream)
+            per.append((a, b))
+        torch.cuda.synchronize()
+        launch_ms = sorted(a.elapsed_time(b) for a, b in per)[len(per) // 2]
+        return wall_ms, launch_ms
+
+    def close(self):
+        self.A.meta_delete()
+
+
+def solver_workload(cls, torch, ctl, n1d=128, tol=1e-10):
+    """Mixed-precision refinement vs FP64 Jacobi-PCG to an FP64 true relative residual."""
+    import numpy as np
+    rp, ci, v, m = cls.gen_diffusion3d(n1d, n1d, n1d, sigma=1.0, seed=3)
+    b = cls.gen_uniform(m, -1.0, 1.0, seed=4)
+    A = cls.CsrMatrix(rp, ci, v, m)
+    A.meta_create(ctl)
+    bt = torch.tensor(b, device="cuda")
+    lib = cls.load()
+    out = {"workload": f"diffusion3d_{n1d}^3_lognormal1_jacobi", "n": m, "nnz": int(rp[-1]),
+           "tolerance": tol}
+    for name, fn in (("mixed_fp32_inner", lib.clsparseAmdDcsrcgMixed),
+                     ("fp64_jacobi_pcg", lib.clsparseAmdDcsrcgRefined)):
+        best = None
+        for _ in range(2):  # second solve is the timed one (first builds graphs, warms caches)
+            x = torch.zeros(m, dtype=torch.float64, device="cuda")
+            sc = cls.SolverControl(cls.DIAGONAL, 20000, tol, 0.0)
+            xd, bd = cls.DenseVector(x), cls.DenseVector(bt)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st = fn(ctypes.byref(xd.s), ctypes.byref(A.s), ctypes.byref(bd.s), sc.handle,
+                    ctl.handle)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            s = sc.stats()
+            sc.release()
+            cls.check(name, st)
+            best = {"seconds": round(dt, 5), "iterations": s.iterations,
+                    "refinements": s.outer_iterations, "true_rel_residual": s.true_residual}
+        out[name] = best
+    out["speedup_mixed_vs_fp64"] = round(out["fp64_jacobi_pcg"]["seconds"] /
+                                         out["mixed_fp32_inner"]["seconds"], 3)
+    A.meta_delete()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-solver", action="store_true", help="skip the solver workload")
+    ap.add_argument("--mode", default="spmv", choices=["spmv", "copy", "locality", "cg"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -71,70 +167,78 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs the GPU"
     torch.cuda.set_device(0)
+    stream = torch.cuda.current_stream()
+
+    if args.mode == "copy":
+        n = 64 << 20
+        src = torch.rand(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        with cls.Control() as c:
+            r, y, one = cls.DenseVector(dst), cls.DenseVector(src), cls.Scalar(1.0)
+            lib = cls.load()
+            for _ in range(args.warmup + args.steps):
+                cls.check("cldenseDscale", lib.cldenseDscale(ctypes.byref(r.s), ctypes.byref(one.s),
+                                                            ctypes.byref(y.s), c.handle))
+        torch.cuda.synchronize()
+        assert torch.equal(dst, src)
+        print(json.dumps({"mode": "copy", "bytes_read_per_launch": n * 8,
+                          "bytes_written_per_launch": n * 8, "launches": args.warmup + args.steps}))
+        return
+
+    ctl = cls.Control(stream=stream, async_mode=True)
+
+    if args.mode == "cg":
+        print(json.dumps({"mode": "cg", "solver": solver_workload(cls, torch, ctl)}))
+        ctl.release()
+        return
 
     t0 = time.time()
     rp, ci, v, m = cls.gen_powerlaw(1 << 21, 16.0, 1.5, 200000, 8, 100000, seed=1)
     x = cls.gen_uniform(m, -1.0, 1.0, seed=2)
     n, nnz = rp.size - 1, int(rp[-1])
-    gen_s = time.time() - t0
-    print(f"[bench] generated n={n} nnz={nnz} in {gen_s:.1f}s", file=sys.stderr, flush=True)
+    log(f"generated n={n} nnz={nnz} in {time.time() - t0:.1f}s")
 
-    stream = torch.cuda.current_stream()
-    ctl = cls.Control(stream=stream, async_mode=True)
-    A = cls.CsrMatrix(rp, ci, v, m, dtype=np.float64)
-    A.meta_create(ctl)
-    xd = cls.DenseVector(torch.tensor(x, device="cuda"))
-    yd = cls.DenseVector(torch.zeros(n, dtype=torch.float64, device="cuda"))
-    alpha, beta = cls.Scalar(1.0), cls.Scalar(0.0)
+    if args.mode == "locality":
+        res = []
+        for bits in (21, 18, 15, 12):
+            cif = (ci & np.uint32((1 << bits) - 1)) if bits < 21 else ci
+            run = SpmvRun(cls, torch, rp, cif, v, m, x, np.float64, ctl)
+            wall, launch = run.time(args.steps, args.warmup, stream)
+            res.append({"matrix": f"powerlaw cols folded to 2^{bits}",
+                        "x_window_bytes": 8 << bits, "launch_us": round(launch * 1e3, 2),
+                        "GBps": round(run.nbytes / (launch * 1e-3) / 1e9, 1)})
+            run.close()
+            log(str(res[-1]))
+        rp3, ci3, v3, m3 = cls.gen_diffusion3d(128, 128, 128, sigma=1.0, seed=3)
+        x3 = cls.gen_uniform(m3, -1.0, 1.0, seed=5)
+        for dt in (np.float64, np.float32):
+            run = SpmvRun(cls, torch, rp3, ci3, v3, m3, x3, dt, ctl)
+            wall, launch = run.time(args.steps, args.warmup, stream)
+            res.append({"matrix": f"diffusion3d 128^3 {np.dtype(dt).name}",
+                        "launch_us": round(launch * 1e3, 2),
+                        "GBps": round(run.nbytes / (launch * 1e-3) / 1e9, 1)})
+            run.close()
+            log(str(res[-1]))
+        print(json.dumps({"mode": "locality", "results": res}))
+        ctl.release()
+        return
 
-    def step():
-        st = A.spmv(alpha, xd, beta, yd, ctl)
-        if st != 0:
-            raise cls.ClsparseError("clsparseDcsrmv", st)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-
-    # Timed region: K steps, synchronized on both sides.
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        step()
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t_start
-    ev_ms = ev0.elapsed_time(ev1) / args.steps
-
-    # Per-launch duration of the SpMV alone, events around each call.
-    per = []
-    for _ in range(min(50, args.steps)):
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(stream)
-        step()
-        b.record(stream)
-        per.append((a, b))
-    torch.cuda.synchronize()
-    launch_ms = float(np.median([a.elapsed_time(b) for a, b in per]))
-
-    nbytes = spmv_bytes(n, m, nnz, 8)
-    ms_per_step = wall * 1e3 / args.steps
+    run = SpmvRun(cls, torch, rp, ci, v, m, x, np.float64, ctl)
+    ms_per_step, launch_ms = run.time(args.steps, args.warmup, stream)
+    nbytes = run.nbytes
     value = nbytes / (ms_per_step * 1e-3) / 1e9
     achieved = nbytes / (launch_ms * 1e-3) / 1e9
 
     # Parity spot check of the last result against the oracle on a row sample.
     import oracle
-    yg = yd.t.cpu().numpy()
-    rows = np.random.default_rng(0).choice(n, 4096, replace=False)
-    rows.sort()
+    yg = run.y.t.cpu().numpy()
+    rows = np.sort(np.random.default_rng(0).choice(n, 4096, replace=False))
     sub_rp = np.concatenate([[0], np.cumsum(np.diff(rp.astype(np.int64))[rows])]).astype(np.uint32)
     sub_ci = np.concatenate([ci[rp[r]:rp[r + 1]] for r in rows])
     sub_v = np.concatenate([v[rp[r]:rp[r + 1]] for r in rows])
     yref, scale = oracle.csrmv(sub_rp, sub_ci, sub_v, x)
     max_err = float(np.max(np.abs(yg[rows] - yref) / np.maximum(scale, 1e-300)))
+    run.close()
 
     cpu = None
     if not args.no_cpu:
@@ -144,7 +248,10 @@ def main():
                "sample": f"{calls} calls of the same FP64 power-law SpMV (full matrix), median; "
                          f"oracle OpenMP CSR SpMV, {threads} threads"}
 
-    traffic = load_traffic()
+    extra = {}
+    if not args.no_solver:
+        extra["solver"] = solver_workload(cls, torch, ctl)
+
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -164,13 +271,13 @@ def main():
                    "hbm_fraction_of_value": round(value / HBM_PEAK_GBS, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic},
+                     "traffic": load_traffic()},
         "cpu_baseline": cpu,
         "parity": {"max_row_rel_err_sample": max_err, "bar": 1e-13},
-        "timing": {"event_ms_per_step": round(ev_ms, 5), "launch_ms_median": round(launch_ms, 5)},
+        "timing": {"launch_ms_median": round(launch_ms, 5)},
+        "extra": extra,
     }
     print(json.dumps(out), flush=True)
-    A.meta_delete()
     ctl.release()
 
 

@@ -137,6 +137,8 @@ _SIGS = {
     "clsparseAmdGetSolverStats": (c_int32, [_V, POINTER(clsparseAmdSolverStats)]),
     "clsparseAmdDcsrcgMixed": (c_int32, [POINTER(cldenseVector), POINTER(clsparseCsrMatrix),
                                          POINTER(cldenseVector), _V, _V]),
+    "clsparseAmdDcsrcgRefined": (c_int32, [POINTER(cldenseVector), POINTER(clsparseCsrMatrix),
+                                           POINTER(cldenseVector), _V, _V]),
     "clsparseAmdHostCsrFree": (c_int32, [POINTER(clsparseAmdHostCsr)]),
     "clsparseAmdGenPoisson2D": (c_int32, [c_uint32, c_uint32, POINTER(clsparseAmdHostCsr)]),
     "clsparseAmdGenPowerLaw": (c_int32, [c_uint32, c_double, c_double, c_uint32, c_uint32,

@@ -44,8 +44,10 @@ struct Acc<T, true>
 {
     T s = T(0);
     T c = T(0);
+    // Error-free transformations: no FMA contraction inside them.
     __device__ __forceinline__ void add(T v)
     {
+#pragma clang fp contract(off)
         const T t = s + v;
         const T bp = t - s;
         c += (s - (t - bp)) + (v - bp);
@@ -53,6 +55,7 @@ struct Acc<T, true>
     }
     __device__ __forceinline__ void mul_add(T a, T b)
     {
+#pragma clang fp contract(off)
         const T p = a * b;
         const T e = fma(a, b, -p);
         add(p);
@@ -60,6 +63,7 @@ struct Acc<T, true>
     }
     __device__ __forceinline__ void merge(const Acc& o)
     {
+#pragma clang fp contract(off)
         add(o.s);
         c += o.c;
     }

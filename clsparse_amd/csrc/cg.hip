@@ -238,27 +238,6 @@ __global__ __launch_bounds__(256) void fill_kernel(size_t n, T* __restrict__ out
         out[i] = v;
 }
 
-// Mixed-precision refinement helpers.
-__global__ __launch_bounds__(256) void ir_scale_residual(size_t n, const double* __restrict__ r,
-                                                         const double* __restrict__ rnorm,
-                                                         float* __restrict__ b32)
-{
-    const double s = 1.0 / *rnorm;
-    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
-    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
-        b32[i] = static_cast<float>(r[i] * s);
-}
-
-__global__ __launch_bounds__(256) void ir_update(size_t n, double* __restrict__ x,
-                                                 const float* __restrict__ d,
-                                                 const double* __restrict__ rnorm)
-{
-    const double s = *rnorm;
-    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
-    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
-        x[i] = fma(s, static_cast<double>(d[i]), x[i]);
-}
-
 uint32_t grid_for(size_t n)
 {
     size_t g = (n + 2047) / 2048;
@@ -589,16 +568,43 @@ extern "C" clsparseStatus clsparseDcsrcg(cldenseVector* x, const clsparseCsrMatr
     return cg_entry<double>(x, A, b, solverControl, control);
 }
 
-// ---- mixed-precision iterative refinement (no reference counterpart) --------
+// ---- iterative refinement with an FP64 outer loop (no reference counterpart) --
 //
 // Outer loop in FP64: r = b - A x (FP64 CSR-Adaptive SpMV), ||r||_2 decides
-// convergence, x += ||r|| d in FP64. Inner solve in FP32: A d = r / ||r|| by
-// the same device-resident PCG (FP32 values, FP64 dots/norms), stopped on its
+// convergence, x += ||r|| d in FP64. Inner solve in precision TI: A d = r/||r||
+// by the same device-resident PCG (TI values, FP64 dots/norms), stopped on its
 // own L2 residual at `inner_tol` (or when the overall iteration budget ends).
-extern "C" clsparseStatus clsparseAmdDcsrcgMixed(cldenseVector* x, const clsparseCsrMatrix* A,
-                                                 const cldenseVector* b,
-                                                 clSParseSolverControl sc,
-                                                 clsparseControl control)
+// TI = float is the mixed-precision solver; TI = double is plain FP64 PCG
+// judged by the same FP64 true residual (the baseline the mixed one is set
+// beside).
+
+namespace {
+
+template <typename TI>
+__global__ __launch_bounds__(256) void ir_scale_residual(size_t n, const double* __restrict__ r,
+                                                         const double* __restrict__ rnorm,
+                                                         TI* __restrict__ bI)
+{
+    const double s = 1.0 / *rnorm;
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
+        bI[i] = static_cast<TI>(r[i] * s);
+}
+
+template <typename TI>
+__global__ __launch_bounds__(256) void ir_update(size_t n, double* __restrict__ x,
+                                                 const TI* __restrict__ d,
+                                                 const double* __restrict__ rnorm)
+{
+    const double s = *rnorm;
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
+        x[i] = fma(s, static_cast<double>(d[i]), x[i]);
+}
+
+template <typename TI>
+clsparseStatus refine_solve(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
+                            clSParseSolverControl sc, clsparseControl control)
 {
     if (!clsparseInitialized) return clsparseNotInitialized;
     if (control == nullptr) return clsparseInvalidControlObject;
@@ -608,6 +614,7 @@ extern "C" clsparseStatus clsparseAmdDcsrcgMixed(cldenseVector* x, const clspars
         A->num_rows != A->num_cols)
         return clsparseInvalidSystemSize;
 
+    constexpr bool kMixed = sizeof(TI) == sizeof(float);
     const size_t n = A->num_rows;
     const size_t nnz = A->num_nonzeros;
     hipStream_t s = control->solver_stream;
@@ -615,26 +622,27 @@ extern "C" clsparseStatus clsparseAmdDcsrcgMixed(cldenseVector* x, const clspars
     const double* bp = dev_ptr<double>(b->values, b->off_values);
     if (n > 0 && (xp == nullptr || bp == nullptr)) return clsparseInvalidMemObject;
 
-    double inner_tol = 1e-4;
-    if (const char* e = std::getenv("CLSPARSE_AMD_IR_INNER_TOL")) inner_tol = std::atof(e);
+    double inner_tol = kMixed ? 1e-4 : 1e-14;
+    if (const char* e = std::getenv(kMixed ? "CLSPARSE_AMD_IR_INNER_TOL" : "CLSPARSE_AMD_IR64_INNER_TOL"))
+        inner_tol = std::atof(e);
 
-    // Workspace: FP32 values of A, FP64 residual and scalars, FP32 rhs and
-    // correction, then the FP32 CG buffers.
-    const size_t bytes = align_up(nnz * sizeof(float)) + align_up(n * sizeof(double)) +
-                         align_up(8 * sizeof(double)) + 2 * align_up(n * sizeof(float)) +
-                         CgBuffers<float>::bytes(n);
+    // Workspace: inner-precision values of A (mixed only), FP64 residual and
+    // scalars, inner-precision rhs and correction, then the inner CG buffers.
+    const size_t bytes = (kMixed ? align_up(nnz * sizeof(TI)) : 0) + align_up(n * sizeof(double)) +
+                         align_up(8 * sizeof(double)) + 2 * align_up(n * sizeof(TI)) +
+                         CgBuffers<TI>::bytes(n);
     void* work = nullptr;
     CLS_TRY(control_workspace(control, bytes, &work));
     void* pinned = nullptr;
     CLS_TRY(control_pinned(control, sizeof(CgDevState) + 64, &pinned));
     CgDevState* hs = static_cast<CgDevState*>(pinned);
     Carve c{static_cast<char*>(work)};
-    float* vals32 = c.take<float>(nnz);
+    TI* valsI = kMixed ? c.take<TI>(nnz) : nullptr;
     double* r64 = c.take<double>(n);
     double* sc64 = c.take<double>(8);  // [0]=||r||_2, [1]=||b||_2, [2]=-1, [3]=+1
-    float* b32 = c.take<float>(n);
-    float* d32 = c.take<float>(n);
-    CgBuffers<float> buf;
+    TI* bI = c.take<TI>(n);
+    TI* dI = c.take<TI>(n);
+    CgBuffers<TI> buf;
     buf.carve(c, n);
 
     CLS_TRY(enter_solver_stream(control));
@@ -656,22 +664,24 @@ extern "C" clsparseStatus clsparseAmdDcsrcgMixed(cldenseVector* x, const clspars
         return clsparseSuccess;
     }
 
-    // FP32 copy of A on the same pattern (shares col_indices, row_pointer, meta).
-    hipLaunchKernelGGL((cast_kernel<double, float>), dim3(ew_grid(nnz)), dim3(256), 0, s, nnz,
-                       dev_ptr<double>(A->values, A->off_values), vals32);
-    clsparseCsrMatrix A32 = *A;
-    A32.values = vals32;
-    A32.off_values = 0;
+    // Inner-precision A on the same pattern (shares col_indices, row_pointer, meta).
+    clsparseCsrMatrix AI = *A;
+    if constexpr (kMixed) {
+        hipLaunchKernelGGL((cast_kernel<double, TI>), dim3(ew_grid(nnz)), dim3(256), 0, s, nnz,
+                           dev_ptr<double>(A->values, A->off_values), valsI);
+        AI.values = valsI;
+        AI.off_values = 0;
+    }
     const int prec = (sc->preconditioner == DIAGONAL) ? kPrecJacobi : kPrecNone;
     if (prec == kPrecJacobi) {
-        hipLaunchKernelGGL((extract_inv_diag<double, float>), dim3((n + 255) / 256), dim3(256), 0, s,
+        hipLaunchKernelGGL((extract_inv_diag<double, TI>), dim3((n + 255) / 256), dim3(256), 0, s,
                            static_cast<uint32_t>(n),
                            dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer),
                            dev_ptr<uint32_t>(A->col_indices, A->off_col_indices),
                            dev_ptr<double>(A->values, A->off_values), buf.invD);
     }
-    hipLaunchKernelGGL((fill_kernel<float>), dim3(1), dim3(256), 0, s, size_t(1), buf.one, 1.0f);
-    hipLaunchKernelGGL((fill_kernel<float>), dim3(1), dim3(256), 0, s, size_t(1), buf.zero, 0.0f);
+    hipLaunchKernelGGL((fill_kernel<TI>), dim3(1), dim3(256), 0, s, size_t(1), buf.one, TI(1));
+    hipLaunchKernelGGL((fill_kernel<TI>), dim3(1), dim3(256), 0, s, size_t(1), buf.zero, TI(0));
     CLS_HIP_TRY(hipGetLastError());
 
     int total_inner = 0, outer = 0;
@@ -698,9 +708,10 @@ extern "C" clsparseStatus clsparseAmdDcsrcgMixed(cldenseVector* x, const clspars
         }
         if (total_inner >= sc->maxIters || outer >= max_outer) break;
 
-        // Inner FP32 solve of A d = r / ||r||, d0 = 0.
-        hipLaunchKernelGGL(ir_scale_residual, dim3(ew_grid(n)), dim3(256), 0, s, n, r64, sc64, b32);
-        hipLaunchKernelGGL((fill_kernel<float>), dim3(ew_grid(n)), dim3(256), 0, s, n, d32, 0.0f);
+        // Inner solve of A d = r / ||r||, d0 = 0.
+        hipLaunchKernelGGL((ir_scale_residual<TI>), dim3(ew_grid(n)), dim3(256), 0, s, n, r64, sc64,
+                           bI);
+        hipLaunchKernelGGL((fill_kernel<TI>), dim3(ew_grid(n)), dim3(256), 0, s, n, dI, TI(0));
         CgDevState init{};
         init.norm_ref = 1.0;
         init.rel_tol = std::max(inner_tol, 0.5 * sc->relativeTolerance / rel);
@@ -709,12 +720,12 @@ extern "C" clsparseStatus clsparseAmdDcsrcgMixed(cldenseVector* x, const clspars
         init.crit = kCritL2;
         *hs = init;
         CLS_HIP_TRY(hipMemcpyAsync(buf.st, hs, sizeof(CgDevState), hipMemcpyHostToDevice, s));
-        CgEngine<float> eng(&A32, d32, b32, n, prec, kCritL2, buf, s, false);
+        CgEngine<TI> eng(&AI, dI, bI, n, prec, kCritL2, buf, s, false);
         CLS_TRY(eng.setup(true));
         CLS_TRY(eng.run(hs, [](const CgDevState&) {}));
         total_inner += hs->n_iters + 1;
         ++outer;
-        hipLaunchKernelGGL(ir_update, dim3(ew_grid(n)), dim3(256), 0, s, n, xp, d32, sc64);
+        hipLaunchKernelGGL((ir_update<TI>), dim3(ew_grid(n)), dim3(256), 0, s, n, xp, dI, sc64);
         CLS_HIP_TRY(hipGetLastError());
     }
     CLS_TRY(leave_solver_stream(control));
@@ -728,4 +739,22 @@ extern "C" clsparseStatus clsparseAmdDcsrcgMixed(cldenseVector* x, const clspars
     sc->stats.seconds = std::chrono::duration<double>(t1 - t0).count();
     sc->stats.status_detail = detail;
     return clsparseSuccess;
+}
+
+}  // namespace
+
+extern "C" clsparseStatus clsparseAmdDcsrcgMixed(cldenseVector* x, const clsparseCsrMatrix* A,
+                                                 const cldenseVector* b,
+                                                 clSParseSolverControl sc,
+                                                 clsparseControl control)
+{
+    return refine_solve<float>(x, A, b, sc, control);
+}
+
+extern "C" clsparseStatus clsparseAmdDcsrcgRefined(cldenseVector* x, const clsparseCsrMatrix* A,
+                                                   const cldenseVector* b,
+                                                   clSParseSolverControl sc,
+                                                   clsparseControl control)
+{
+    return refine_solve<double>(x, A, b, sc, control);
 }

@@ -65,6 +65,13 @@ CLSPARSE_EXPORT clsparseStatus
     clsparseAmdDcsrcgMixed(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
                            clSParseSolverControl solverControl, clsparseControl control);
 
+/* The same refinement loop with FP64 inner solves: FP64 Jacobi-PCG judged by
+ * the FP64 true relative residual (the baseline the mixed solver is set
+ * beside). */
+CLSPARSE_EXPORT clsparseStatus
+    clsparseAmdDcsrcgRefined(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
+                             clSParseSolverControl solverControl, clsparseControl control);
+
 /* ---- synthetic generators (host memory, seeded, deterministic) ----------- */
 typedef struct clsparseAmdHostCsr_
 {

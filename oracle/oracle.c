@@ -167,12 +167,19 @@ void orc_csrmv_d(uint32_t n, const uint32_t* rp, const uint32_t* ci, const doubl
                  const double* x, double alpha, double beta, const double* y_in, double* y_out,
                  double* scale)
 {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1024)
+#endif
     for (uint32_t r = 0; r < n; ++r) {
         ksum s = {0, 0};
         long double mag = 0;
         for (uint32_t j = rp[r]; j < rp[r + 1]; ++j) {
+            /* A double product needs 106 bits; long double keeps 64, so the
+             * rounding error of the product (exact, via fmal) is added too. */
             long double p = (long double)v[j] * (long double)x[ci[j]];
+            long double e = fmal((long double)v[j], (long double)x[ci[j]], -p);
             ks_add(&s, p);
+            ks_add(&s, e);
             mag += fabsl(p);
         }
         long double out = (long double)alpha * (s.s + s.c);
@@ -190,6 +197,9 @@ void orc_csrmv_s(uint32_t n, const uint32_t* rp, const uint32_t* ci, const float
                  const float* x, float alpha, float beta, const float* y_in, double* y_out,
                  double* scale)
 {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1024)
+#endif
     for (uint32_t r = 0; r < n; ++r) {
         ksum s = {0, 0};
         long double mag = 0;

@@ -80,7 +80,8 @@ def spmv_check(rp, ci, v, ncols, dtype, *, alpha=1.0, beta=0.0, seed=1, adaptive
     err = per_row_error(y, yref, scale)
     if tol is not None:
         tolv = np.broadcast_to(np.asarray(tol, dtype=np.float64), err.shape)
-        bad = int(np.argmax(err / tolv)) if err.size else 0
+        over = err - tolv
+        bad = int(np.argmax(over)) if err.size else 0
         assert np.all(err <= tolv), (float(err[bad]), bad, float(tolv[bad]))
     return err, y
 

@@ -94,30 +94,43 @@ def test_axpy_axpby_scale_special_cases(ctl, prec):
     nan = np.full(n, np.nan, dtype=npdt)
     r = _vec(np.zeros(n), dt)
 
+    keep = []  # device objects must outlive the call that uses their pointers
+
+    def a(v):
+        keep.append(cls.Scalar(v, dt))
+        return B(keep[-1].s)
+
+    def vec(arr):
+        keep.append(_vec(arr, dt))
+        return B(keep[-1].s)
+
     def run(fn, *args):
         assert getattr(lib, f"cldense{prec}{fn}")(*args, ctl.handle) == 0
         return r.t.cpu().numpy()
 
-    a = lambda v: cls.Scalar(v, dt)  # noqa: E731
+    X, Y, NAN = vec(x), vec(y), vec(nan)
     # axpy: r = alpha x + y; alpha == 0 copies y even where x is NaN.
-    got = run("axpy", B(r.s), B(a(0.5).s), B(_vec(x, dt).s), B(_vec(y, dt).s))
-    np.testing.assert_allclose(got, npdt(0.5) * x + y, rtol=2 * np.finfo(npdt).eps)
-    got = run("axpy", B(r.s), B(a(0.0).s), B(_vec(nan, dt).s), B(_vec(y, dt).s))
+    got = run("axpy", B(r.s), a(0.5), X, Y)
+    mag = np.abs(0.5 * x.astype(np.float64)) + np.abs(y.astype(np.float64))
+    assert np.all(np.abs(got.astype(np.float64) - (npdt(0.5) * x + y)) <=
+                  4 * np.finfo(npdt).eps * mag)
+    got = run("axpy", B(r.s), a(0.0), NAN, Y)
     np.testing.assert_array_equal(got, y)
     # scale: alpha == 0 writes zeros even for NaN input.
-    got = run("scale", B(r.s), B(a(-2.0).s), B(_vec(x, dt).s))
+    got = run("scale", B(r.s), a(-2.0), X)
     np.testing.assert_array_equal(got, x * npdt(-2.0))
-    got = run("scale", B(r.s), B(a(0.0).s), B(_vec(nan, dt).s))
+    got = run("scale", B(r.s), a(0.0), NAN)
     np.testing.assert_array_equal(got, np.zeros(n))
     # axpby special cases (cldense-axpby.cpp L27-84).
-    X, Y = _vec(x, dt), _vec(y, dt)
     cases = [(0.0, 0.0, np.zeros(n)), (0.0, 3.0, y * npdt(3.0)), (2.0, 0.0, x * npdt(2.0)),
              (1.0, -0.25, x + npdt(-0.25) * y), (-1.5, 1.0, npdt(-1.5) * x + y),
              (0.7, -0.3, npdt(0.7) * x + npdt(-0.3) * y)]
     for al, be, want in cases:
-        got = run("axpby", B(r.s), B(a(al).s), B(X.s), B(a(be).s), B(Y.s))
-        np.testing.assert_allclose(got, want, rtol=4 * np.finfo(npdt).eps, atol=1e-30)
-    got = run("axpby", B(r.s), B(a(0.0).s), B(_vec(nan, dt).s), B(a(0.0).s), B(_vec(nan, dt).s))
+        got = run("axpby", B(r.s), a(al), X, a(be), Y)
+        # FMA contraction may round once instead of twice: bound by the terms' size.
+        mag = np.abs(npdt(al) * x.astype(np.float64)) + np.abs(npdt(be) * y.astype(np.float64))
+        assert np.all(np.abs(got.astype(np.float64) - want) <= 4 * np.finfo(npdt).eps * mag)
+    got = run("axpby", B(r.s), a(0.0), NAN, a(0.0), NAN)
     np.testing.assert_array_equal(got, np.zeros(n))
 
 
