@@ -154,6 +154,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-solver", action="store_true", help="skip the solver workload")
     ap.add_argument("--mode", default="spmv", choices=["spmv", "copy", "locality", "cg"])
+    ap.add_argument("--set", default="21,18,15,12,s64,s32",
+                    help="locality mode: power-law column windows (bits) and stencils (s64, s32)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -200,7 +202,11 @@ def main():
 
     if args.mode == "locality":
         res = []
-        for bits in (21, 18, 15, 12):
+        items = args.set.split(",")
+        for it in items:
+            if it.startswith("s"):
+                continue
+            bits = int(it)
             cif = (ci & np.uint32((1 << bits) - 1)) if bits < 21 else ci
             run = SpmvRun(cls, torch, rp, cif, v, m, x, np.float64, ctl)
             wall, launch = run.time(args.steps, args.warmup, stream)
@@ -209,9 +215,12 @@ def main():
                         "GBps": round(run.nbytes / (launch * 1e-3) / 1e9, 1)})
             run.close()
             log(str(res[-1]))
-        rp3, ci3, v3, m3 = cls.gen_diffusion3d(128, 128, 128, sigma=1.0, seed=3)
-        x3 = cls.gen_uniform(m3, -1.0, 1.0, seed=5)
-        for dt in (np.float64, np.float32):
+        stencils = [it for it in items if it.startswith("s")]
+        if stencils:
+            rp3, ci3, v3, m3 = cls.gen_diffusion3d(128, 128, 128, sigma=1.0, seed=3)
+            x3 = cls.gen_uniform(m3, -1.0, 1.0, seed=5)
+        for it in stencils:
+            dt = np.float64 if it == "s64" else np.float32
             run = SpmvRun(cls, torch, rp3, ci3, v3, m3, x3, dt, ctl)
             wall, launch = run.time(args.steps, args.warmup, stream)
             res.append({"matrix": f"diffusion3d 128^3 {np.dtype(dt).name}",

@@ -74,6 +74,19 @@ struct _solverControl
     void printSummary(clsparseStatus status) const;
 };
 
+// What one row-block entry asks of the SpMV kernel, derived once from the
+// entry pair and the row pointer when the metadata is created:
+//   nrows >= 2: CSR-Stream block of rows [row, row + nrows), nnz [base, base + cnt)
+//   nrows == 1: CSR-Vector row `row`, nnz [base, base + cnt)
+//   nrows == 0: one CSR-LongRows segment of row `row`, nnz [base, base + cnt)
+struct BlockDesc
+{
+    uint32_t row;
+    uint32_t nrows;
+    uint32_t base;
+    uint32_t cnt;
+};
+
 // One long row split over several row-block entries (CSR-LongRows).
 struct LongRow
 {
@@ -96,6 +109,7 @@ struct matrix_meta
     uint32_t num_nonzeros = 0;
     uint64_t* d_rowBlocks = nullptr;  // rowBlockSize words; entries filled, rest zero
     std::vector<uint64_t> h_rowBlocks;  // host copy of the filled entries
+    BlockDesc* d_desc = nullptr;      // entries - 1 kernel descriptors
 
     // Long-row combine tables (deterministic replacement of the reference's
     // atomic + spin-flag protocol, csrmv_adaptive.cl L564-686).

@@ -11,6 +11,7 @@
 // What this file adds: the table of long rows (rows split over several
 // entries) that the deterministic long-row combine kernel needs, and the
 // per-entry partial-sum slots it writes.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -147,6 +148,7 @@ void free_meta(matrix_meta* m)
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(m->device);
     if (m->d_rowBlocks) (void)hipFree(m->d_rowBlocks);
+    if (m->d_desc) (void)hipFree(m->d_desc);
     if (m->d_long) (void)hipFree(m->d_long);
     if (m->d_partials) (void)hipFree(m->d_partials);
     (void)hipSetDevice(cur);
@@ -241,10 +243,44 @@ extern "C" clsparseStatus clsparseCsrMetaCreate(clsparseCsrMatrix* csrMatx,
     }
     m->num_long_rows = static_cast<uint32_t>(longs.size());
 
+    // Kernel descriptors: the entry classification of csrmv_adaptive.cl
+    // L349-359, L382, L515, L564 resolved once, with each block's nnz range.
+    std::vector<BlockDesc> desc(m->entries > 0 ? m->entries - 1 : 0);
+    for (uint32_t e = 0; e + 1 < m->entries; ++e) {
+        const uint32_t row = static_cast<uint32_t>(host[e] >> 32);
+        const uint32_t stop = static_cast<uint32_t>(host[e + 1] >> 32);
+        const uint32_t seg = static_cast<uint32_t>(host[e] & ((1u << kWgBits) - 1u));
+        const uint32_t nrows = stop - row;
+        BlockDesc d{row, nrows, rp[row], 0};
+        if (nrows > static_cast<uint32_t>(kRowsForVector)) {
+            d.cnt = rp[stop] - rp[row];
+            if (d.cnt > static_cast<uint32_t>(kBlkSize)) {  // impossible for a valid binning
+                delete m;
+                return clsparseStructInvalid;
+            }
+        } else if (nrows == 1 && seg == 0) {
+            d.cnt = rp[row + 1] - rp[row];
+        } else {
+            const uint64_t rs = rp[row], re = rp[row + 1];
+            const uint64_t vs = std::min<uint64_t>(rs + static_cast<uint64_t>(seg) * kLongSeg, re);
+            const uint64_t ve = std::min<uint64_t>(vs + kLongSeg, re);
+            d.nrows = 0;
+            d.base = static_cast<uint32_t>(vs);
+            d.cnt = static_cast<uint32_t>(ve - vs);
+        }
+        desc[e] = d;
+    }
+
     hipError_t err = hipMalloc(&m->d_rowBlocks, size_words * sizeof(uint64_t));
     if (err == hipSuccess)
         err = hipMemcpyAsync(m->d_rowBlocks, host.data(), size_words * sizeof(uint64_t),
                              hipMemcpyHostToDevice, control->stream);
+    if (err == hipSuccess && !desc.empty()) {
+        err = hipMalloc(&m->d_desc, desc.size() * sizeof(BlockDesc));
+        if (err == hipSuccess)
+            err = hipMemcpyAsync(m->d_desc, desc.data(), desc.size() * sizeof(BlockDesc),
+                                 hipMemcpyHostToDevice, control->stream);
+    }
     if (err == hipSuccess && !longs.empty()) {
         err = hipMalloc(&m->d_long, longs.size() * sizeof(LongRow));
         if (err == hipSuccess)
