@@ -13,7 +13,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import (POINTER, Structure, c_double, c_int32, c_size_t, c_uint32, c_uint64,
+from ctypes import (POINTER, Structure, c_double, c_float, c_int32, c_size_t, c_uint32, c_uint64,
                     c_void_p)
 
 import numpy as np
@@ -85,7 +85,7 @@ class clsparseMetaSizeResult(Structure):
 class clsparseAmdSolverStats(Structure):
     _fields_ = [("iterations", c_int32), ("outer_iterations", c_int32),
                 ("l1_residual", c_double), ("true_residual", c_double),
-                ("seconds", c_double), ("status_detail", c_int32), ("reserved", c_int32),
+                ("seconds", c_double), ("status_detail", c_int32), ("preconditioner", c_int32),
                 ("n_iters", c_int32), ("max_iters", c_int32),
                 ("relative_tolerance", c_double), ("absolute_tolerance", c_double),
                 ("initial_residual", c_double), ("current_residual", c_double)]
@@ -146,7 +146,30 @@ _SIGS = {
     "clsparseAmdGenDiffusion3D": (c_int32, [c_uint32, c_uint32, c_uint32, c_double, c_double,
                                             c_double, c_double, c_uint64,
                                             POINTER(clsparseAmdHostCsr)]),
+    "clsparseAmdGenAnisoDiffusion2D": (c_int32, [c_uint32, c_uint32, c_double, c_double, c_double,
+                                                 c_double, c_uint64, POINTER(clsparseAmdHostCsr)]),
     "clsparseAmdGenUniform": (c_int32, [POINTER(c_double), c_size_t, c_double, c_double, c_uint64]),
+    # learned FSAI
+    "clsparseAmdMlpCreate": (c_int32, [POINTER(c_void_p), c_uint64]),
+    "clsparseAmdMlpDestroy": (c_int32, [_V]),
+    "clsparseAmdMlpNumParams": (c_uint32, []),
+    "clsparseAmdMlpGetParams": (c_int32, [_V, POINTER(c_float)]),
+    "clsparseAmdMlpSetParams": (c_int32, [_V, POINTER(c_float)]),
+    "clsparseAmdMlpGetGrad": (c_int32, [_V, POINTER(c_float)]),
+    "clsparseAmdFsaiCreate": (c_int32, [POINTER(c_void_p), POINTER(clsparseCsrMatrix), _V]),
+    "clsparseAmdFsaiUpdateValues": (c_int32, [_V, POINTER(clsparseCsrMatrix), _V]),
+    "clsparseAmdFsaiDestroy": (c_int32, [_V]),
+    "clsparseAmdFsaiApplyNet": (c_int32, [_V, _V, _V]),
+    "clsparseAmdFsaiGetValues": (c_int32, [_V, POINTER(c_uint32), POINTER(c_uint32),
+                                           POINTER(c_double), POINTER(c_uint32)]),
+    "clsparseAmdFsaiGetFeatures": (c_int32, [_V, POINTER(ctypes.c_uint16)]),
+    "clsparseAmdFsaiGetOutput": (c_int32, [_V, POINTER(c_float)]),
+    "clsparseAmdFsaiTrainStep": (c_int32, [_V, _V, POINTER(clsparseCsrMatrix), c_uint32, c_uint64,
+                                           c_float, _V, POINTER(c_double), POINTER(c_double)]),
+    "clsparseAmdDcsrcgFsai": (c_int32, [POINTER(cldenseVector), POINTER(clsparseCsrMatrix),
+                                        POINTER(cldenseVector), _V, _V, _V]),
+    "clsparseAmdDcsrcgFsai64": (c_int32, [POINTER(cldenseVector), POINTER(clsparseCsrMatrix),
+                                          POINTER(cldenseVector), _V, _V, _V]),
 }
 for _s in ("S", "D"):
     _SIGS[f"cldense{_s}scale"] = (c_int32, [POINTER(cldenseVector), POINTER(clsparseScalar),
@@ -230,6 +253,16 @@ def gen_diffusion3d(nx: int, ny: int, nz: int, sigma: float = 1.0, ax: float = 1
     m = clsparseAmdHostCsr()
     check("clsparseAmdGenDiffusion3D",
           load().clsparseAmdGenDiffusion3D(nx, ny, nz, sigma, ax, ay, az, seed, ctypes.byref(m)))
+    return _take_host_csr(m)
+
+
+def gen_aniso2d(nx: int, ny: int, corr_len: float = 16.0, sigma: float = 1.0,
+                ratio: float = 10.0, theta: float = 0.0, seed: int = 1):
+    """2-D anisotropic Q1 diffusion (9-point, SPD) on nx x ny interior nodes."""
+    m = clsparseAmdHostCsr()
+    check("clsparseAmdGenAnisoDiffusion2D",
+          load().clsparseAmdGenAnisoDiffusion2D(nx, ny, corr_len, sigma, ratio, theta, seed,
+                                                ctypes.byref(m)))
     return _take_host_csr(m)
 
 

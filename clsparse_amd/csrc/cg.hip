@@ -33,7 +33,7 @@ constexpr int kHist = 64;   // residual history ring (VERBOSE printing)
 constexpr int kChunk = 32;  // iterations per captured graph (even: both parities)
 
 enum Crit : int { kCritL1Ref = 0, kCritL2 = 1 };
-enum Prec : int { kPrecNone = 0, kPrecJacobi = 1 };
+enum Prec : int { kPrecNone = 0, kPrecJacobi = 1, kPrecFsai = 2 };
 
 struct CgDevState
 {
@@ -85,11 +85,12 @@ __global__ __launch_bounds__(256) void cg_init_kernel(size_t n, const T* __restr
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
     for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
         const T ri = x_zero ? b[i] : b[i] - y[i];
-        const T zi = (prec == kPrecJacobi) ? ri * invD[i] : ri;
         r[i] = ri;
+        nr += norm_term(crit, static_cast<double>(ri));
+        if (prec == kPrecFsai) continue;  // z, p and <r, z> follow the G G^T apply
+        const T zi = (prec == kPrecJacobi) ? ri * invD[i] : ri;
         z[i] = zi;
         p[i] = zi;
-        nr += norm_term(crit, static_cast<double>(ri));
         rz += static_cast<double>(ri) * static_cast<double>(zi);
     }
     nr = block_sum256(nr, sh);
@@ -152,9 +153,10 @@ __global__ __launch_bounds__(256) void cg_update_kernel(
         x[i] = x[i] + alpha * pi;
         const T ri = r[i] - alpha * y[i];
         r[i] = ri;
+        nr += norm_term(crit, static_cast<double>(ri));
+        if (prec == kPrecFsai) continue;  // z and <r, z> follow the G G^T apply
         const T zi = (prec == kPrecJacobi) ? ri * invD[i] : ri;
         z[i] = zi;
-        nr += norm_term(crit, static_cast<double>(ri));
         rzn += static_cast<double>(ri) * static_cast<double>(zi);
     }
     nr = block_sum256(nr, sh);
@@ -319,10 +321,12 @@ template <typename T>
 class CgEngine
 {
 public:
+    // G, GT: the factor of M = G G^T and its transpose (kPrecFsai only).
     CgEngine(const clsparseCsrMatrix* A, T* x, const T* b, size_t n, int prec, int crit,
-             CgBuffers<T>& buf, hipStream_t s, bool ext)
+             CgBuffers<T>& buf, hipStream_t s, bool ext, const clsparseCsrMatrix* G = nullptr,
+             const clsparseCsrMatrix* GT = nullptr)
         : A_(A), x_(x), b_(b), n_(n), prec_(prec), crit_(crit), buf_(buf), s_(s), ext_(ext),
-          g_(grid_for(n))
+          g_(grid_for(n)), G_(G), GT_(GT)
     {
     }
     ~CgEngine()
@@ -342,6 +346,12 @@ public:
         CLS_HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(cg_init_finalize, dim3(1), dim3(256), 0, s_, buf_.st, buf_.part_nr[1], g_);
         CLS_HIP_TRY(hipGetLastError());
+        if (prec_ == kPrecFsai) {
+            CLS_TRY(apply_fsai(buf_.part_rz[1], nullptr));
+            hipLaunchKernelGGL((cast_kernel<T, T>), dim3(ew_grid(n_)), dim3(256), 0, s_, n_, buf_.z,
+                               buf_.p);
+            CLS_HIP_TRY(hipGetLastError());
+        }
         return clsparseSuccess;
     }
 
@@ -391,9 +401,22 @@ private:
                            buf_.z, buf_.p, buf_.y, buf_.invD, prec_, buf_.part_yp,
                            buf_.part_rz[par ^ 1], buf_.part_rz[par], buf_.part_nr[par], g_,
                            buf_.st);
+        if (prec_ == kPrecFsai) CLS_TRY(apply_fsai(buf_.part_rz[par], done));
         hipLaunchKernelGGL((cg_pupdate_kernel<T>), dim3(g_), dim3(256), 0, s_, n_, buf_.p, buf_.z,
                            buf_.part_rz[par], buf_.part_rz[par ^ 1], buf_.part_nr[par], g_,
                            buf_.st);
+        CLS_HIP_TRY(hipGetLastError());
+        return clsparseSuccess;
+    }
+
+    // z = G (G^T r) through y (free between the r update and the next A p);
+    // partials of <r, z>.
+    clsparseStatus apply_fsai(double* part_rz, const int* done)
+    {
+        CLS_TRY(launch_csrmv<T>(buf_.one, GT_, buf_.r, buf_.zero, buf_.y, s_, false, done));
+        CLS_TRY(launch_csrmv<T>(buf_.one, G_, buf_.y, buf_.zero, buf_.z, s_, false, done));
+        hipLaunchKernelGGL((cg_dot_kernel<T>), dim3(g_), dim3(256), 0, s_, n_, buf_.r, buf_.z,
+                           part_rz, buf_.st);
         CLS_HIP_TRY(hipGetLastError());
         return clsparseSuccess;
     }
@@ -408,6 +431,8 @@ private:
     hipStream_t s_;
     bool ext_;
     uint32_t g_;
+    const clsparseCsrMatrix* G_;
+    const clsparseCsrMatrix* GT_;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
 };
@@ -602,9 +627,15 @@ __global__ __launch_bounds__(256) void ir_update(size_t n, double* __restrict__ 
         x[i] = fma(s, static_cast<double>(d[i]), x[i]);
 }
 
+// With `fsai` the inner solves use M = G G^T (its G in precision TI) and fall
+// back to Jacobi for the rest of the solve when the factor holds a non-finite
+// value, or an inner solve returns a non-finite or stagnating residual (less
+// than a factor-2 reduction, or its iteration budget ran out), or the FP64
+// true residual fails to decrease over an outer step.
 template <typename TI>
 clsparseStatus refine_solve(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
-                            clSParseSolverControl sc, clsparseControl control)
+                            clSParseSolverControl sc, clsparseControl control,
+                            clsparseAmdFsai fsai = nullptr)
 {
     if (!clsparseInitialized) return clsparseNotInitialized;
     if (control == nullptr) return clsparseInvalidControlObject;
@@ -616,6 +647,7 @@ clsparseStatus refine_solve(cldenseVector* x, const clsparseCsrMatrix* A, const 
 
     constexpr bool kMixed = sizeof(TI) == sizeof(float);
     const size_t n = A->num_rows;
+    if (fsai && fsai_n(fsai) != n) return clsparseInvalidSystemSize;
     const size_t nnz = A->num_nonzeros;
     hipStream_t s = control->solver_stream;
     double* xp = dev_ptr<double>(x->values, x->off_values);
@@ -672,8 +704,22 @@ clsparseStatus refine_solve(cldenseVector* x, const clsparseCsrMatrix* A, const 
         AI.values = valsI;
         AI.off_values = 0;
     }
-    const int prec = (sc->preconditioner == DIAGONAL) ? kPrecJacobi : kPrecNone;
-    if (prec == kPrecJacobi) {
+    int prec = (sc->preconditioner == DIAGONAL) ? kPrecJacobi : kPrecNone;
+    int used = prec;  // what the statistics report: 0 none, 1 Jacobi, 2 FSAI, 3 FSAI -> Jacobi
+    const clsparseCsrMatrix* G = nullptr;
+    const clsparseCsrMatrix* GT = nullptr;
+    if (fsai) {
+        if constexpr (kMixed) {
+            G = fsai_G32(fsai);
+            GT = fsai_GT32(fsai);
+        } else {
+            G = fsai_G64(fsai);
+            GT = fsai_GT64(fsai);
+        }
+        prec = fsai_valid(fsai) ? kPrecFsai : kPrecJacobi;
+        used = fsai_valid(fsai) ? 2 : 3;
+    }
+    if (prec != kPrecNone) {
         hipLaunchKernelGGL((extract_inv_diag<double, TI>), dim3((n + 255) / 256), dim3(256), 0, s,
                            static_cast<uint32_t>(n),
                            dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer),
@@ -685,7 +731,7 @@ clsparseStatus refine_solve(cldenseVector* x, const clsparseCsrMatrix* A, const 
     CLS_HIP_TRY(hipGetLastError());
 
     int total_inner = 0, outer = 0;
-    double rel = 0.0, first_rel = -1.0;
+    double rel = 0.0, first_rel = -1.0, prev_rel = -1.0;
     const int max_outer = 100;
     int detail = 1;
     for (;;) {
@@ -699,6 +745,11 @@ clsparseStatus refine_solve(cldenseVector* x, const clsparseCsrMatrix* A, const 
         CLS_HIP_TRY(hipStreamSynchronize(s));
         rel = hb[0] / norm_b;
         if (first_rel < 0) first_rel = rel;
+        if (prec == kPrecFsai && prev_rel >= 0 && !(rel < prev_rel)) {
+            prec = kPrecJacobi;
+            used = 3;
+        }
+        prev_rel = rel;
         if (sc->printMode == VERBOSE)
             std::printf("Refinement %d: true relative residual %.6e (inner iterations %d)\n", outer,
                         rel, total_inner);
@@ -720,12 +771,18 @@ clsparseStatus refine_solve(cldenseVector* x, const clsparseCsrMatrix* A, const 
         init.crit = kCritL2;
         *hs = init;
         CLS_HIP_TRY(hipMemcpyAsync(buf.st, hs, sizeof(CgDevState), hipMemcpyHostToDevice, s));
-        CgEngine<TI> eng(&AI, dI, bI, n, prec, kCritL2, buf, s, false);
+        CgEngine<TI> eng(&AI, dI, bI, n, prec, kCritL2, buf, s, false, G, GT);
         CLS_TRY(eng.setup(true));
         CLS_TRY(eng.run(hs, [](const CgDevState&) {}));
         total_inner += hs->n_iters + 1;
         ++outer;
-        hipLaunchKernelGGL((ir_update<TI>), dim3(ew_grid(n)), dim3(256), 0, s, n, xp, dI, sc64);
+        const bool finite = std::isfinite(hs->cur_res);
+        if (prec == kPrecFsai && (!finite || hs->cur_res > 0.5 || !hs->converged)) {
+            prec = kPrecJacobi;
+            used = 3;
+        }
+        if (finite)
+            hipLaunchKernelGGL((ir_update<TI>), dim3(ew_grid(n)), dim3(256), 0, s, n, xp, dI, sc64);
         CLS_HIP_TRY(hipGetLastError());
     }
     CLS_TRY(leave_solver_stream(control));
@@ -738,6 +795,7 @@ clsparseStatus refine_solve(cldenseVector* x, const clsparseCsrMatrix* A, const 
     sc->stats.true_residual = rel;
     sc->stats.seconds = std::chrono::duration<double>(t1 - t0).count();
     sc->stats.status_detail = detail;
+    sc->stats.preconditioner = used;
     return clsparseSuccess;
 }
 
@@ -757,4 +815,20 @@ extern "C" clsparseStatus clsparseAmdDcsrcgRefined(cldenseVector* x, const clspa
                                                    clsparseControl control)
 {
     return refine_solve<double>(x, A, b, sc, control);
+}
+
+extern "C" clsparseStatus clsparseAmdDcsrcgFsai(cldenseVector* x, const clsparseCsrMatrix* A,
+                                                const cldenseVector* b, clsparseAmdFsai fsai,
+                                                clSParseSolverControl sc, clsparseControl control)
+{
+    if (fsai == nullptr) return clsparseInvalidValue;
+    return refine_solve<float>(x, A, b, sc, control, fsai);
+}
+
+extern "C" clsparseStatus clsparseAmdDcsrcgFsai64(cldenseVector* x, const clsparseCsrMatrix* A,
+                                                  const cldenseVector* b, clsparseAmdFsai fsai,
+                                                  clSParseSolverControl sc, clsparseControl control)
+{
+    if (fsai == nullptr) return clsparseInvalidValue;
+    return refine_solve<double>(x, A, b, sc, control, fsai);
 }

@@ -179,6 +179,14 @@ clsparseStatus launch_csrmv(const T* alpha, const clsparseCsrMatrix* A, const T*
                             const T* beta, T* y, hipStream_t stream, bool extended,
                             const int* done_flag);
 
+// Learned FSAI factor (fsai.hip): G and G^T in FP32 / FP64 with metadata.
+const clsparseCsrMatrix* fsai_G32(clsparseAmdFsai f);
+const clsparseCsrMatrix* fsai_GT32(clsparseAmdFsai f);
+const clsparseCsrMatrix* fsai_G64(clsparseAmdFsai f);
+const clsparseCsrMatrix* fsai_GT64(clsparseAmdFsai f);
+bool fsai_valid(clsparseAmdFsai f);
+uint32_t fsai_n(clsparseAmdFsai f);
+
 // Deterministic reductions: result written to a device scalar of type R.
 enum class RedOp { Sum, Abs, Sqr, Dot };
 template <typename T, typename R>

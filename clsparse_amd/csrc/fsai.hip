@@ -1,0 +1,919 @@
+// Learned factorized sparse approximate inverse (no reference counterpart).
+//
+// G is lower triangular on the pattern of tril(A) and parametrised as
+//     G_ij = (delta_ij + o_ij) / sqrt(a_ii)
+// where o_ij is the output of a per-nonzero MLP (16 features -> 64 -> 64 -> 1,
+// ReLU, BF16 weights and activations on MFMA with FP32 accumulation). A zero
+// network output is exactly the Jacobi preconditioner. The preconditioner is
+// M = G G^T (z = G (G^T r): two CSR-Adaptive SpMVs with the CSR of G and of
+// G^T), trained so that G^T A G ~ I with the Hutchinson estimate
+//     L = 1/(k n) sum_z ||(G^T A G - I) z||^2,   z Rademacher,
+// whose gradient w.r.t. G_ij is 2/(k n) sum_z (u_i v_j + w_i z_j) with
+// t = G z, u = A t, v = G^T u - z, w = A G v.
+//
+// MLP kernels compute every layer transposed, Z^T = W H^T (hidden x nnz):
+// the 16x16x16 BF16 MFMA writes its result in exactly the lane layout it
+// reads its B operand in, so each layer's activations feed the next MFMA
+// straight from registers. Only the weight-gradient products (summed over
+// nonzeros) transpose a 64x16 tile through LDS.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.hpp"
+
+namespace {
+
+constexpr int kFeat = 16;
+constexpr int kHid = 64;
+// Parameter vector layout (FP32 master copy).
+constexpr int kOffW1 = 0;                        // W1[64][16]
+constexpr int kOffB1 = kOffW1 + kHid * kFeat;    // b1[64]
+constexpr int kOffW2 = kOffB1 + kHid;            // W2[64][64] (row = output unit)
+constexpr int kOffB2 = kOffW2 + kHid * kHid;     // b2[64]
+constexpr int kOffW3 = kOffB2 + kHid;            // w3[64]
+constexpr int kOffB3 = kOffW3 + kHid;            // b3
+constexpr int kNumParams = kOffB3 + 1;           // 5313
+constexpr int kMlpWaves = 4;
+constexpr int kMlpGrid = 256;                    // workgroups of the MLP kernels
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16, round to nearest even (finite inputs).
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h)
+{
+    return __uint_as_float(static_cast<uint32_t>(h) << 16);
+}
+
+__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x4 load_bf16x4(const uint16_t* p)
+{
+    const uint2 v = *reinterpret_cast<const uint2*>(p);
+    bf16x4 r;
+    r[0] = static_cast<short>(v.x & 0xFFFFu);
+    r[1] = static_cast<short>(v.x >> 16);
+    r[2] = static_cast<short>(v.y & 0xFFFFu);
+    r[3] = static_cast<short>(v.y >> 16);
+    return r;
+}
+
+__device__ __forceinline__ bf16x4 pack_bf16x4(f32x4 v)
+{
+    bf16x4 r;
+    for (int e = 0; e < 4; ++e) r[e] = static_cast<short>(f2bf(v[e]));
+    return r;
+}
+
+// Weights as the MFMA reads them (BF16). W1: [64][16]; W2: [64][64] row =
+// output unit; W2T: its transpose (row = input unit) for the backward pass.
+struct MlpDev
+{
+    const uint16_t* W1;
+    const uint16_t* W2;
+    const uint16_t* W2T;
+    const float* p;  // FP32 master parameters (biases, w3, b3 are read from here)
+};
+
+// Forward for 16 nonzeros [r0, r0+16) held by one wave. Lane l works on
+// nonzero l % 16 and hidden units 16m + 4(l/16) + e of block m (e = 0..3).
+// Returns pre-activations a1, a2 and the BF16 activations h1 (for backward).
+struct TileFwd
+{
+    f32x4 a1[4];
+    f32x4 a2[4];
+    bf16x4 h1[4];
+    bf16x4 x;  // B operand X^T (features 4(l/16)..+3 of nonzero l%16)
+};
+
+__device__ __forceinline__ void tile_forward(const MlpDev& net, const uint16_t* __restrict__ X,
+                                             uint32_t r0, uint32_t nnz, uint32_t lane, TileFwd& t)
+{
+    const uint32_t nz = r0 + (lane & 15);
+    const uint32_t q = lane >> 4;
+    if (nz < nnz)
+        t.x = load_bf16x4(X + static_cast<size_t>(nz) * kFeat + 4 * q);
+    else
+        t.x = bf16x4{0, 0, 0, 0};
+    // Layer 1: A1^T (64 x 16nnz) = W1 (64 x 16) . X^T (16 x 16nnz)
+    for (int m = 0; m < 4; ++m) {
+        const bf16x4 w = load_bf16x4(net.W1 + (16 * m + (lane & 15)) * kFeat + 4 * q);
+        f32x4 c = {0.f, 0.f, 0.f, 0.f};
+        c = mfma16(w, t.x, c);
+        for (int e = 0; e < 4; ++e) c[e] += net.p[kOffB1 + 16 * m + 4 * q + e];
+        t.a1[m] = c;
+        f32x4 h;
+        for (int e = 0; e < 4; ++e) h[e] = c[e] > 0.f ? c[e] : 0.f;
+        t.h1[m] = pack_bf16x4(h);
+    }
+    // Layer 2: A2^T (64 x 16) = W2 (64 x 64) . H1^T (64 x 16)
+    for (int m = 0; m < 4; ++m) {
+        f32x4 c = {0.f, 0.f, 0.f, 0.f};
+        for (int kb = 0; kb < 4; ++kb) {
+            const bf16x4 w = load_bf16x4(net.W2 + (16 * m + (lane & 15)) * kHid + 16 * kb + 4 * q);
+            c = mfma16(w, t.h1[kb], c);
+        }
+        for (int e = 0; e < 4; ++e) c[e] += net.p[kOffB2 + 16 * m + 4 * q + e];
+        t.a2[m] = c;
+    }
+}
+
+// o = w3 . relu(a2) + b3 for the lane's nonzero (all 4 lanes of a nonzero agree).
+__device__ __forceinline__ float tile_output(const MlpDev& net, const TileFwd& t, uint32_t lane)
+{
+    const uint32_t q = lane >> 4;
+    float s = 0.f;
+    for (int m = 0; m < 4; ++m)
+        for (int e = 0; e < 4; ++e) {
+            const float h = t.a2[m][e] > 0.f ? bf2f(f2bf(t.a2[m][e])) : 0.f;
+            s += net.p[kOffW3 + 16 * m + 4 * q + e] * h;
+        }
+    s += __shfl_xor(s, 16, 64);
+    s += __shfl_xor(s, 32, 64);
+    return s + net.p[kOffB3];
+}
+
+__global__ __launch_bounds__(64 * kMlpWaves) void mlp_forward_kernel(MlpDev net,
+                                                                      const uint16_t* __restrict__ X,
+                                                                      uint32_t nnz,
+                                                                      float* __restrict__ out)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t tiles = (nnz + 15) / 16;
+    for (uint32_t tile = blockIdx.x * kMlpWaves + wave; tile < tiles; tile += gridDim.x * kMlpWaves) {
+        TileFwd t;
+        tile_forward(net, X, tile * 16, nnz, lane, t);
+        const float o = tile_output(net, t, lane);
+        const uint32_t nz = tile * 16 + (lane & 15);
+        if (lane < 16 && nz < nnz) out[nz] = o;
+    }
+}
+
+// Backward: recompute the tile's forward pass, then accumulate this wave's
+// parameter gradients; one partial gradient vector per wave, reduced later
+// in a fixed order.
+__global__ __launch_bounds__(64 * kMlpWaves) void mlp_backward_kernel(MlpDev net,
+                                                                       const uint16_t* __restrict__ X,
+                                                                       const float* __restrict__ gout,
+                                                                       uint32_t nnz,
+                                                                       float* __restrict__ partials)
+{
+    __shared__ uint16_t sh_t[kMlpWaves][kHid][16];   // a 64 x 16 tile, nnz contiguous
+    __shared__ uint16_t sh_x[kMlpWaves][kFeat][16];  // X^T tile, nnz contiguous
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t q = lane >> 4, c16 = lane & 15;
+
+    f32x4 dW2[4][4];  // block (m = output unit, n = input unit)
+    f32x4 dW1[4];     // block m (rows = hidden), 16 features
+    float db1[16], db2[16], dw3[16];  // per lane: units 16m + 4q + e
+    float db3 = 0.f;
+    for (int m = 0; m < 4; ++m) {
+        dW1[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int n = 0; n < 4; ++n) dW2[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+    for (int i = 0; i < 16; ++i) db1[i] = db2[i] = dw3[i] = 0.f;
+
+    const uint32_t tiles = (nnz + 15) / 16;
+    for (uint32_t tile = blockIdx.x * kMlpWaves + wave; tile < tiles; tile += gridDim.x * kMlpWaves) {
+        TileFwd t;
+        tile_forward(net, X, tile * 16, nnz, lane, t);
+        const uint32_t nz = tile * 16 + c16;
+        const float g = (nz < nnz) ? gout[nz] : 0.f;
+        if (q == 0) db3 += g;
+
+        // D2^T = g * w3 * relu'(a2); dw3 += g * h2
+        f32x4 d2[4];
+        for (int m = 0; m < 4; ++m)
+            for (int e = 0; e < 4; ++e) {
+                const int u = 16 * m + 4 * q + e;
+                const bool on = t.a2[m][e] > 0.f;
+                const float h2 = on ? bf2f(f2bf(t.a2[m][e])) : 0.f;
+                dw3[4 * m + e] += g * h2;
+                const float d = on ? g * net.p[kOffW3 + u] : 0.f;
+                d2[m][e] = d;
+                db2[4 * m + e] += d;
+            }
+        bf16x4 d2b[4];
+        for (int m = 0; m < 4; ++m) d2b[m] = pack_bf16x4(d2[m]);
+
+        // D1^T = (W2^T D2^T) * relu'(a1): A = W2T rows (input unit), K = output unit.
+        f32x4 d1[4];
+        for (int m = 0; m < 4; ++m) {
+            f32x4 c = {0.f, 0.f, 0.f, 0.f};
+            for (int kb = 0; kb < 4; ++kb) {
+                const bf16x4 w = load_bf16x4(net.W2T + (16 * m + c16) * kHid + 16 * kb + 4 * q);
+                c = mfma16(w, d2b[kb], c);
+            }
+            for (int e = 0; e < 4; ++e) {
+                const float d = t.a1[m][e] > 0.f ? c[e] : 0.f;
+                d1[m][e] = d;
+                db1[4 * m + e] += d;
+            }
+        }
+
+        // dW2 += D2^T H1: both operands with the nonzeros in the K slot,
+        // through LDS ([unit][nnz] BF16 tiles).
+        for (int m = 0; m < 4; ++m)
+            for (int e = 0; e < 4; ++e) sh_t[wave][16 * m + 4 * q + e][c16] = static_cast<uint16_t>(d2b[m][e]);
+        wave_sync_lds();
+        bf16x4 aD2[4];
+        for (int m = 0; m < 4; ++m) aD2[m] = load_bf16x4(&sh_t[wave][16 * m + c16][4 * q]);
+        wave_sync_lds();
+        for (int m = 0; m < 4; ++m)
+            for (int e = 0; e < 4; ++e) sh_t[wave][16 * m + 4 * q + e][c16] = static_cast<uint16_t>(t.h1[m][e]);
+        wave_sync_lds();
+        for (int n = 0; n < 4; ++n) {
+            const bf16x4 bH1 = load_bf16x4(&sh_t[wave][16 * n + c16][4 * q]);
+            for (int m = 0; m < 4; ++m) dW2[m][n] = mfma16(aD2[m], bH1, dW2[m][n]);
+        }
+        wave_sync_lds();
+
+        // dW1 += D1^T X
+        for (int m = 0; m < 4; ++m) {
+            const bf16x4 d1b = pack_bf16x4(d1[m]);
+            for (int e = 0; e < 4; ++e) sh_t[wave][16 * m + 4 * q + e][c16] = static_cast<uint16_t>(d1b[e]);
+        }
+        for (int e = 0; e < 4; ++e) sh_x[wave][4 * q + e][c16] = static_cast<uint16_t>(t.x[e]);
+        wave_sync_lds();
+        const bf16x4 bX = load_bf16x4(&sh_x[wave][c16][4 * q]);
+        for (int m = 0; m < 4; ++m) {
+            const bf16x4 aD1 = load_bf16x4(&sh_t[wave][16 * m + c16][4 * q]);
+            dW1[m] = mfma16(aD1, bX, dW1[m]);
+        }
+        wave_sync_lds();
+    }
+
+    // Bias-like sums: lanes with the same q hold the same units for 16
+    // different nonzeros -> reduce over the 16 lanes of a row (DPP).
+    for (int i = 0; i < 16; ++i) {
+        db1[i] = group_sum(db1[i], 16);
+        db2[i] = group_sum(db2[i], 16);
+        dw3[i] = group_sum(dw3[i], 16);
+    }
+    db3 = wave_sum(db3);
+
+    float* out = partials + static_cast<size_t>(blockIdx.x * kMlpWaves + wave) * kNumParams;
+    // dW2 block (m, n): lane l holds rows 16m + 4q + e, column 16n + c16.
+    for (int m = 0; m < 4; ++m)
+        for (int n = 0; n < 4; ++n)
+            for (int e = 0; e < 4; ++e)
+                out[kOffW2 + (16 * m + 4 * q + e) * kHid + 16 * n + c16] = dW2[m][n][e];
+    for (int m = 0; m < 4; ++m)
+        for (int e = 0; e < 4; ++e) out[kOffW1 + (16 * m + 4 * q + e) * kFeat + c16] = dW1[m][e];
+    if (c16 == 0)
+        for (int m = 0; m < 4; ++m)
+            for (int e = 0; e < 4; ++e) {
+                out[kOffB1 + 16 * m + 4 * q + e] = db1[4 * m + e];
+                out[kOffB2 + 16 * m + 4 * q + e] = db2[4 * m + e];
+                out[kOffW3 + 16 * m + 4 * q + e] = dw3[4 * m + e];
+            }
+    if (lane == 0) out[kOffB3] = db3;
+}
+
+// grad[p] = sum over partial vectors (fixed order).
+__global__ __launch_bounds__(256) void grad_reduce_kernel(const float* __restrict__ partials,
+                                                          uint32_t count, float* __restrict__ grad)
+{
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= static_cast<uint32_t>(kNumParams)) return;
+    double s = 0.0;
+    for (uint32_t k = 0; k < count; ++k) s += partials[static_cast<size_t>(k) * kNumParams + p];
+    grad[p] = static_cast<float>(s);
+}
+
+// Adam step and refresh of the BF16 weight copies.
+__global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ p, float* __restrict__ m1,
+                                                   float* __restrict__ m2,
+                                                   const float* __restrict__ g, float lr,
+                                                   float b1t, float b2t, uint16_t* __restrict__ W1,
+                                                   uint16_t* __restrict__ W2,
+                                                   uint16_t* __restrict__ W2T)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= kNumParams) return;
+    const float gi = g[i];
+    const float a = 0.9f * m1[i] + 0.1f * gi;
+    const float b = 0.999f * m2[i] + 0.001f * gi * gi;
+    m1[i] = a;
+    m2[i] = b;
+    const float mh = a / (1.f - b1t), vh = b / (1.f - b2t);
+    const float v = p[i] - lr * mh / (sqrtf(vh) + 1e-8f);
+    p[i] = v;
+    if (i < kOffB1) {
+        W1[i] = f2bf(v);
+    } else if (i >= kOffW2 && i < kOffB2) {
+        const int k = i - kOffW2, r = k / kHid, c = k % kHid;
+        W2[k] = f2bf(v);
+        W2T[c * kHid + r] = f2bf(v);
+    }
+}
+
+__global__ __launch_bounds__(256) void refresh_bf16_kernel(const float* __restrict__ p,
+                                                           uint16_t* __restrict__ W1,
+                                                           uint16_t* __restrict__ W2,
+                                                           uint16_t* __restrict__ W2T)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < kHid * kFeat) W1[i] = f2bf(p[kOffW1 + i]);
+    if (i < kHid * kHid) {
+        const int r = i / kHid, c = i % kHid;
+        W2[i] = f2bf(p[kOffW2 + i]);
+        W2T[c * kHid + r] = f2bf(p[kOffW2 + i]);
+    }
+}
+
+// ---- features and G assembly --------------------------------------------------
+
+// Per row of A: d = a_ii (1 if no stored diagonal), then with dhat = 1/sqrt(d):
+// r = sum_{k != i} |a_ik| dhat_i dhat_k, s = sum_{k != i} a_ik dhat_i dhat_k.
+__global__ __launch_bounds__(256) void row_diag_kernel(uint32_t n, const uint32_t* __restrict__ rp,
+                                                       const uint32_t* __restrict__ ci,
+                                                       const double* __restrict__ v,
+                                                       double* __restrict__ dhat)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double d = 1.0;
+    for (uint32_t j = rp[i]; j < rp[i + 1]; ++j)
+        if (ci[j] == i) {
+            d = v[j];
+            break;
+        }
+    dhat[i] = d > 0 ? 1.0 / sqrt(d) : 1.0;
+}
+
+__global__ __launch_bounds__(256) void row_stats_kernel(uint32_t n, const uint32_t* __restrict__ rp,
+                                                        const uint32_t* __restrict__ ci,
+                                                        const double* __restrict__ v,
+                                                        const double* __restrict__ dhat,
+                                                        float* __restrict__ stats)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double r = 0.0, s = 0.0;
+    for (uint32_t j = rp[i]; j < rp[i + 1]; ++j) {
+        const uint32_t c = ci[j];
+        if (c == i) continue;
+        const double a = v[j] * dhat[i] * dhat[c];
+        r += fabs(a);
+        s += a;
+    }
+    stats[3 * i + 0] = static_cast<float>(r);
+    stats[3 * i + 1] = static_cast<float>(s);
+    stats[3 * i + 2] = static_cast<float>(rp[i + 1] - rp[i]) * 0.0625f;
+}
+
+// Features of G's nonzero k = (i, j), j <= i, taken from A's nonzero src[k]:
+// [ahat_ij, i == j, r_i, r_j, s_i, s_j, ahat_ij^2, nnz_i / 16, 0 ...].
+__global__ __launch_bounds__(256) void feature_kernel(uint32_t nnzG, const uint32_t* __restrict__ grow,
+                                                      const uint32_t* __restrict__ gcol,
+                                                      const uint32_t* __restrict__ src,
+                                                      const double* __restrict__ v,
+                                                      const double* __restrict__ dhat,
+                                                      const float* __restrict__ stats,
+                                                      uint16_t* __restrict__ X)
+{
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nnzG) return;
+    const uint32_t i = grow[k], j = gcol[k];
+    const float a = static_cast<float>(v[src[k]] * dhat[i] * dhat[j]);
+    float f[kFeat] = {a, i == j ? 1.f : 0.f, stats[3 * i], stats[3 * j], stats[3 * i + 1],
+                      stats[3 * j + 1], a * a, stats[3 * i + 2], 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    uint32_t w[kFeat / 2];
+    for (int e = 0; e < kFeat / 2; ++e)
+        w[e] = static_cast<uint32_t>(f2bf(f[2 * e])) | (static_cast<uint32_t>(f2bf(f[2 * e + 1])) << 16);
+    uint4* dst = reinterpret_cast<uint4*>(X + static_cast<size_t>(k) * kFeat);
+    dst[0] = uint4{w[0], w[1], w[2], w[3]};
+    dst[1] = uint4{w[4], w[5], w[6], w[7]};
+}
+
+// G_k = (delta + o_k) dhat_i, scattered to G (CSR) and G^T (CSR) in both precisions.
+template <typename T>
+__global__ __launch_bounds__(256) void assemble_kernel(uint32_t nnzG, const uint32_t* __restrict__ grow,
+                                                       const uint32_t* __restrict__ gcol,
+                                                       const uint32_t* __restrict__ tpos,
+                                                       const float* __restrict__ o,
+                                                       const double* __restrict__ dhat,
+                                                       T* __restrict__ gval, T* __restrict__ gtval,
+                                                       int* __restrict__ bad)
+{
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nnzG) return;
+    const uint32_t i = grow[k], j = gcol[k];
+    const double g = ((i == j ? 1.0 : 0.0) + static_cast<double>(o ? o[k] : 0.f)) * dhat[i];
+    if (!isfinite(g)) *bad = 1;
+    gval[k] = static_cast<T>(g);
+    gtval[tpos[k]] = static_cast<T>(g);
+}
+
+// Hutchinson gradient for one probe: gG[k] += c (u_i v_j + w_i z_j).
+__global__ __launch_bounds__(256) void hutch_grad_kernel(uint32_t nnzG, const uint32_t* __restrict__ grow,
+                                                         const uint32_t* __restrict__ gcol,
+                                                         const double* __restrict__ u,
+                                                         const double* __restrict__ v,
+                                                         const double* __restrict__ w,
+                                                         const double* __restrict__ z, double c,
+                                                         double* __restrict__ gG)
+{
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nnzG) return;
+    const uint32_t i = grow[k], j = gcol[k];
+    gG[k] += c * (u[i] * v[j] + w[i] * z[j]);
+}
+
+// dL/do_k = dL/dG_k * dhat_i
+__global__ __launch_bounds__(256) void chain_kernel(uint32_t nnzG, const uint32_t* __restrict__ grow,
+                                                    const double* __restrict__ gG,
+                                                    const double* __restrict__ dhat,
+                                                    float* __restrict__ gout)
+{
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nnzG) return;
+    gout[k] = static_cast<float>(gG[k] * dhat[grow[k]]);
+}
+
+// Rademacher probe from a counter-based hash of (seed, i).
+__global__ __launch_bounds__(256) void rademacher_kernel(uint32_t n, uint64_t seed, double* __restrict__ z)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    uint64_t x = seed ^ (0x9E3779B97F4A7C15ull * (static_cast<uint64_t>(i) + 1));
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    z[i] = (x & 1) ? 1.0 : -1.0;
+}
+
+__global__ __launch_bounds__(256) void sub_kernel(uint32_t n, const double* __restrict__ a,
+                                                  const double* __restrict__ b, double* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = a[i] - b[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void set_scalar(T* p, T v)
+{
+    if (threadIdx.x == 0) *p = v;
+}
+
+uint32_t blocks_for(size_t n) { return static_cast<uint32_t>((n + 255) / 256); }
+
+}  // namespace
+
+// ---- host objects -------------------------------------------------------------
+
+struct _clsparseAmdMlp
+{
+    int device = 0;
+    float* p = nullptr;   // kNumParams FP32 master
+    float* m1 = nullptr;  // Adam moments
+    float* m2 = nullptr;
+    float* grad = nullptr;
+    float* partials = nullptr;  // kMlpGrid * kMlpWaves partial gradients
+    uint16_t* W1 = nullptr;
+    uint16_t* W2 = nullptr;
+    uint16_t* W2T = nullptr;
+    int step = 0;
+    MlpDev dev() const { return MlpDev{W1, W2, W2T, p}; }
+};
+
+struct _clsparseAmdFsai
+{
+    int device = 0;
+    uint32_t n = 0, nnzG = 0, nnzA = 0;
+    // Pattern of G (CSR, lower) and of G^T (CSR, upper); per-G-nonzero row,
+    // column, source index in A and position in G^T.
+    uint32_t *g_rp = nullptr, *g_ci = nullptr, *gt_rp = nullptr, *gt_ci = nullptr;
+    uint32_t *g_row = nullptr, *g_src = nullptr, *g_tpos = nullptr;
+    double* dhat = nullptr;
+    float* stats = nullptr;
+    uint16_t* X = nullptr;  // features, BF16 [nnzG][16]
+    float* o = nullptr;     // network output per G nonzero
+    double *g64 = nullptr, *gt64 = nullptr;
+    float *g32 = nullptr, *gt32 = nullptr;
+    int* bad = nullptr;
+    clsparseCsrMatrix G64{}, GT64{}, G32{}, GT32{};
+    bool valid = true;
+    // Training scratch (FP64 vectors of length n, and the G gradient).
+    double *t = nullptr, *u = nullptr, *s = nullptr, *v = nullptr, *t2 = nullptr, *w = nullptr,
+           *z = nullptr, *gG = nullptr, *sc = nullptr;
+    float* gout = nullptr;
+};
+
+namespace {
+
+template <typename T>
+hipError_t dmalloc(T** p, size_t count)
+{
+    return hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T));
+}
+
+clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_t st);
+
+// Diagonal scaling, row statistics and features from A's values.
+clsparseStatus compute_features(_clsparseAmdFsai* f, const clsparseCsrMatrix* A, hipStream_t st)
+{
+    const uint32_t n = f->n;
+    const uint32_t* arp = dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer);
+    const uint32_t* aci = dev_ptr<uint32_t>(A->col_indices, A->off_col_indices);
+    const double* av = dev_ptr<double>(A->values, A->off_values);
+    if (n == 0) return clsparseSuccess;
+    hipLaunchKernelGGL(row_diag_kernel, dim3(blocks_for(n)), dim3(256), 0, st, n, arp, aci, av, f->dhat);
+    hipLaunchKernelGGL(row_stats_kernel, dim3(blocks_for(n)), dim3(256), 0, st, n, arp, aci, av,
+                       f->dhat, f->stats);
+    if (f->nnzG)
+        hipLaunchKernelGGL(feature_kernel, dim3(blocks_for(f->nnzG)), dim3(256), 0, st, f->nnzG,
+                           f->g_row, f->g_ci, f->g_src, av, f->dhat, f->stats, f->X);
+    CLS_HIP_TRY(hipGetLastError());
+    CLS_HIP_TRY(hipStreamSynchronize(st));
+    return clsparseSuccess;
+}
+
+void free_fsai(_clsparseAmdFsai* f)
+{
+    if (!f) return;
+    clsparseCsrMetaDelete(&f->G64);
+    clsparseCsrMetaDelete(&f->GT64);
+    clsparseCsrMetaDelete(&f->G32);
+    clsparseCsrMetaDelete(&f->GT32);
+    void* ptrs[] = {f->g_rp, f->g_ci, f->gt_rp, f->gt_ci, f->g_row, f->g_src, f->g_tpos,
+                    f->dhat, f->stats, f->X, f->o, f->g64, f->gt64, f->g32, f->gt32, f->bad,
+                    f->t, f->u, f->s, f->v, f->t2, f->w, f->z, f->gG, f->sc, f->gout};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete f;
+}
+
+}  // namespace
+
+extern "C" clsparseStatus clsparseAmdMlpCreate(clsparseAmdMlp* out, cl_ulong seed)
+{
+    if (out == nullptr) return clsparseInvalidValue;
+    _clsparseAmdMlp* m = new (std::nothrow) _clsparseAmdMlp;
+    if (!m) return clsparseOutOfHostMemory;
+    (void)hipGetDevice(&m->device);
+    // He-normal W1, W2; w3 = 0 and b3 = 0 so the initial G is Jacobi exactly.
+    std::vector<float> h(kNumParams, 0.f);
+    uint64_t st = seed * 0x9E3779B97F4A7C15ull + 12345;
+    auto nrm = [&]() {
+        auto nx = [&]() {
+            st += 0x9E3779B97F4A7C15ull;
+            uint64_t z = st;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            return static_cast<double>((z ^ (z >> 31)) >> 11) * 0x1.0p-53;
+        };
+        double u1 = nx(), u2 = nx();
+        if (u1 < 1e-300) u1 = 1e-300;
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    };
+    for (int i = 0; i < kHid * kFeat; ++i) h[kOffW1 + i] = static_cast<float>(nrm() * std::sqrt(2.0 / 8));
+    for (int i = 0; i < kHid * kHid; ++i) h[kOffW2 + i] = static_cast<float>(nrm() * std::sqrt(2.0 / kHid));
+    hipError_t e = dmalloc(&m->p, kNumParams);
+    if (e == hipSuccess) e = dmalloc(&m->m1, kNumParams);
+    if (e == hipSuccess) e = dmalloc(&m->m2, kNumParams);
+    if (e == hipSuccess) e = dmalloc(&m->grad, kNumParams);
+    if (e == hipSuccess) e = dmalloc(&m->partials, static_cast<size_t>(kMlpGrid) * kMlpWaves * kNumParams);
+    if (e == hipSuccess) e = dmalloc(&m->W1, kHid * kFeat);
+    if (e == hipSuccess) e = dmalloc(&m->W2, kHid * kHid);
+    if (e == hipSuccess) e = dmalloc(&m->W2T, kHid * kHid);
+    if (e == hipSuccess) e = hipMemcpy(m->p, h.data(), kNumParams * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(m->m1, 0, kNumParams * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(m->m2, 0, kNumParams * sizeof(float));
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(refresh_bf16_kernel, dim3(16), dim3(256), 0, 0, m->p, m->W1, m->W2, m->W2T);
+        e = hipDeviceSynchronize();
+    }
+    if (e != hipSuccess) {
+        clsparseAmdMlpDestroy(m);
+        return clsparseOutOfResources;
+    }
+    *out = m;
+    return clsparseSuccess;
+}
+
+extern "C" clsparseStatus clsparseAmdMlpDestroy(clsparseAmdMlp m)
+{
+    if (m == nullptr) return clsparseInvalidValue;
+    void* ptrs[] = {m->p, m->m1, m->m2, m->grad, m->partials, m->W1, m->W2, m->W2T};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete m;
+    return clsparseSuccess;
+}
+
+extern "C" cl_uint clsparseAmdMlpNumParams(void) { return kNumParams; }
+
+extern "C" clsparseStatus clsparseAmdMlpGetParams(clsparseAmdMlp m, cl_float* host)
+{
+    if (m == nullptr || host == nullptr) return clsparseInvalidValue;
+    CLS_HIP_TRY(hipMemcpy(host, m->p, kNumParams * sizeof(float), hipMemcpyDeviceToHost));
+    return clsparseSuccess;
+}
+
+extern "C" clsparseStatus clsparseAmdMlpSetParams(clsparseAmdMlp m, const cl_float* host)
+{
+    if (m == nullptr || host == nullptr) return clsparseInvalidValue;
+    CLS_HIP_TRY(hipMemcpy(m->p, host, kNumParams * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(refresh_bf16_kernel, dim3(16), dim3(256), 0, 0, m->p, m->W1, m->W2, m->W2T);
+    CLS_HIP_TRY(hipDeviceSynchronize());
+    return clsparseSuccess;
+}
+
+extern "C" clsparseStatus clsparseAmdMlpGetGrad(clsparseAmdMlp m, cl_float* host)
+{
+    if (m == nullptr || host == nullptr) return clsparseInvalidValue;
+    CLS_HIP_TRY(hipMemcpy(host, m->grad, kNumParams * sizeof(float), hipMemcpyDeviceToHost));
+    return clsparseSuccess;
+}
+
+// Pattern of G = tril(A) and of G^T, built on the host from A's pattern.
+extern "C" clsparseStatus clsparseAmdFsaiCreate(clsparseAmdFsai* out, const clsparseCsrMatrix* A,
+                                                clsparseControl control)
+{
+    CLS_TRY(check_init_control(control));
+    if (out == nullptr || A == nullptr) return clsparseInvalidValue;
+    if (A->num_rows != A->num_cols) return clsparseInvalidSystemSize;
+    const uint32_t n = A->num_rows, nnzA = A->num_nonzeros;
+    std::vector<uint32_t> rp(n + 1), ci(nnzA);
+    hipStream_t st = control->stream;
+    CLS_HIP_TRY(hipMemcpyAsync(rp.data(), dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer),
+                               (n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    if (nnzA)
+        CLS_HIP_TRY(hipMemcpyAsync(ci.data(), dev_ptr<uint32_t>(A->col_indices, A->off_col_indices),
+                                   nnzA * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    CLS_HIP_TRY(hipStreamSynchronize(st));
+    for (uint32_t r = 0; r < n; ++r)
+        if (rp[r + 1] < rp[r] || rp[r + 1] > nnzA) return clsparseInvalidValue;
+
+    std::vector<uint32_t> grp(n + 1, 0), gci, grow, gsrc;
+    gci.reserve(nnzA / 2 + n);
+    for (uint32_t r = 0; r < n; ++r) {
+        for (uint32_t k = rp[r]; k < rp[r + 1]; ++k) {
+            const uint32_t c = ci[k];
+            if (c >= n) return clsparseInvalidValue;
+            if (c <= r) {
+                gci.push_back(c);
+                grow.push_back(r);
+                gsrc.push_back(k);
+            }
+        }
+        grp[r + 1] = static_cast<uint32_t>(gci.size());
+    }
+    const uint32_t nnzG = static_cast<uint32_t>(gci.size());
+    // G^T by counting sort on the column; tpos[k] = position of G's k in G^T.
+    std::vector<uint32_t> trp(n + 1, 0), tci(nnzG), tpos(nnzG);
+    for (uint32_t k = 0; k < nnzG; ++k) ++trp[gci[k] + 1];
+    for (uint32_t r = 0; r < n; ++r) trp[r + 1] += trp[r];
+    {
+        std::vector<uint32_t> fill(trp.begin(), trp.end() - 1);
+        for (uint32_t k = 0; k < nnzG; ++k) {
+            const uint32_t p = fill[gci[k]]++;
+            tci[p] = grow[k];
+            tpos[k] = p;
+        }
+    }
+
+    _clsparseAmdFsai* f = new (std::nothrow) _clsparseAmdFsai;
+    if (!f) return clsparseOutOfHostMemory;
+    f->device = control->device;
+    f->n = n;
+    f->nnzG = nnzG;
+    hipError_t e = hipSuccess;
+    auto up = [&](uint32_t** d, const std::vector<uint32_t>& h) {
+        if (e == hipSuccess) e = dmalloc(d, h.size());
+        if (e == hipSuccess && !h.empty())
+            e = hipMemcpyAsync(*d, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st);
+    };
+    up(&f->g_rp, grp);
+    up(&f->g_ci, gci);
+    up(&f->gt_rp, trp);
+    up(&f->gt_ci, tci);
+    up(&f->g_row, grow);
+    up(&f->g_src, gsrc);
+    up(&f->g_tpos, tpos);
+    if (e == hipSuccess) e = dmalloc(&f->dhat, n);
+    if (e == hipSuccess) e = dmalloc(&f->stats, 3 * static_cast<size_t>(n));
+    if (e == hipSuccess) e = dmalloc(&f->X, static_cast<size_t>(nnzG) * kFeat);
+    if (e == hipSuccess) e = dmalloc(&f->o, nnzG);
+    if (e == hipSuccess) e = dmalloc(&f->g64, nnzG);
+    if (e == hipSuccess) e = dmalloc(&f->gt64, nnzG);
+    if (e == hipSuccess) e = dmalloc(&f->g32, nnzG);
+    if (e == hipSuccess) e = dmalloc(&f->gt32, nnzG);
+    if (e == hipSuccess) e = dmalloc(&f->bad, 1);
+    if (e != hipSuccess) {
+        free_fsai(f);
+        return clsparseOutOfResources;
+    }
+    auto csr = [&](clsparseCsrMatrix& M, uint32_t* prp, uint32_t* pci, void* vals) {
+        clsparseInitCsrMatrix(&M);
+        M.num_rows = M.num_cols = n;
+        M.num_nonzeros = nnzG;
+        M.row_pointer = prp;
+        M.col_indices = pci;
+        M.values = vals;
+    };
+    csr(f->G64, f->g_rp, f->g_ci, f->g64);
+    csr(f->GT64, f->gt_rp, f->gt_ci, f->gt64);
+    csr(f->G32, f->g_rp, f->g_ci, f->g32);
+    csr(f->GT32, f->gt_rp, f->gt_ci, f->gt32);
+    clsparseStatus s = clsparseCsrMetaCreate(&f->G64, control);
+    if (s == clsparseSuccess) s = clsparseCsrMetaCreate(&f->GT64, control);
+    if (s == clsparseSuccess) s = clsparseCsrMetaCreate(&f->G32, control);
+    if (s == clsparseSuccess) s = clsparseCsrMetaCreate(&f->GT32, control);
+    if (s != clsparseSuccess) {
+        free_fsai(f);
+        return s;
+    }
+
+    f->nnzA = nnzA;
+    s = compute_features(f, A, st);
+    if (s == clsparseSuccess) s = fsai_assemble(f, nullptr, st);
+    if (s != clsparseSuccess) {
+        free_fsai(f);
+        return s;
+    }
+    *out = f;
+    return clsparseSuccess;
+}
+
+extern "C" clsparseStatus clsparseAmdFsaiUpdateValues(clsparseAmdFsai f, const clsparseCsrMatrix* A,
+                                                      clsparseControl control)
+{
+    CLS_TRY(check_init_control(control));
+    if (f == nullptr || A == nullptr) return clsparseInvalidValue;
+    if (A->num_rows != f->n || A->num_cols != f->n || A->num_nonzeros != f->nnzA)
+        return clsparseInvalidSystemSize;
+    CLS_TRY(compute_features(f, A, control->stream));
+    return fsai_assemble(f, nullptr, control->stream);
+}
+
+extern "C" clsparseStatus clsparseAmdFsaiDestroy(clsparseAmdFsai f)
+{
+    if (f == nullptr) return clsparseInvalidValue;
+    free_fsai(f);
+    return clsparseSuccess;
+}
+
+namespace {
+
+clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_t st)
+{
+    if (net) {
+        hipLaunchKernelGGL(mlp_forward_kernel, dim3(kMlpGrid), dim3(64 * kMlpWaves), 0, st, net->dev(),
+                           f->X, f->nnzG, f->o);
+    }
+    CLS_HIP_TRY(hipMemsetAsync(f->bad, 0, sizeof(int), st));
+    const float* o = net ? f->o : nullptr;
+    hipLaunchKernelGGL((assemble_kernel<double>), dim3(blocks_for(f->nnzG)), dim3(256), 0, st, f->nnzG,
+                       f->g_row, f->g_ci, f->g_tpos, o, f->dhat, f->g64, f->gt64, f->bad);
+    hipLaunchKernelGGL((assemble_kernel<float>), dim3(blocks_for(f->nnzG)), dim3(256), 0, st, f->nnzG,
+                       f->g_row, f->g_ci, f->g_tpos, o, f->dhat, f->g32, f->gt32, f->bad);
+    CLS_HIP_TRY(hipGetLastError());
+    int bad = 0;
+    CLS_HIP_TRY(hipMemcpyAsync(&bad, f->bad, sizeof(int), hipMemcpyDeviceToHost, st));
+    CLS_HIP_TRY(hipStreamSynchronize(st));
+    f->valid = (bad == 0);
+    return clsparseSuccess;
+}
+
+clsparseStatus ensure_train_scratch(_clsparseAmdFsai* f)
+{
+    if (f->t) return clsparseSuccess;
+    const size_t n = f->n;
+    hipError_t e = dmalloc(&f->t, n);
+    if (e == hipSuccess) e = dmalloc(&f->u, n);
+    if (e == hipSuccess) e = dmalloc(&f->s, n);
+    if (e == hipSuccess) e = dmalloc(&f->v, n);
+    if (e == hipSuccess) e = dmalloc(&f->t2, n);
+    if (e == hipSuccess) e = dmalloc(&f->w, n);
+    if (e == hipSuccess) e = dmalloc(&f->z, n);
+    if (e == hipSuccess) e = dmalloc(&f->gG, f->nnzG);
+    if (e == hipSuccess) e = dmalloc(&f->sc, 8);
+    if (e == hipSuccess) e = dmalloc(&f->gout, f->nnzG);
+    return e == hipSuccess ? clsparseSuccess : clsparseOutOfResources;
+}
+
+}  // namespace
+
+extern "C" clsparseStatus clsparseAmdFsaiApplyNet(clsparseAmdFsai f, clsparseAmdMlp net,
+                                                  clsparseControl control)
+{
+    CLS_TRY(check_init_control(control));
+    if (f == nullptr) return clsparseInvalidValue;
+    return fsai_assemble(f, net, control->stream);
+}
+
+extern "C" clsparseStatus clsparseAmdFsaiGetValues(clsparseAmdFsai f, cl_uint* g_row_pointer,
+                                                   cl_uint* g_col_indices, cl_double* g_values,
+                                                   cl_uint* nnz)
+{
+    if (f == nullptr) return clsparseInvalidValue;
+    if (nnz) *nnz = f->nnzG;
+    if (g_row_pointer)
+        CLS_HIP_TRY(hipMemcpy(g_row_pointer, f->g_rp, (f->n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (g_col_indices)
+        CLS_HIP_TRY(hipMemcpy(g_col_indices, f->g_ci, f->nnzG * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (g_values)
+        CLS_HIP_TRY(hipMemcpy(g_values, f->g64, f->nnzG * sizeof(double), hipMemcpyDeviceToHost));
+    return clsparseSuccess;
+}
+
+extern "C" clsparseStatus clsparseAmdFsaiGetFeatures(clsparseAmdFsai f, uint16_t* host_bf16)
+{
+    if (f == nullptr || host_bf16 == nullptr) return clsparseInvalidValue;
+    CLS_HIP_TRY(hipMemcpy(host_bf16, f->X, static_cast<size_t>(f->nnzG) * kFeat * sizeof(uint16_t),
+                          hipMemcpyDeviceToHost));
+    return clsparseSuccess;
+}
+
+extern "C" clsparseStatus clsparseAmdFsaiGetOutput(clsparseAmdFsai f, cl_float* host)
+{
+    if (f == nullptr || host == nullptr) return clsparseInvalidValue;
+    CLS_HIP_TRY(hipMemcpy(host, f->o, f->nnzG * sizeof(float), hipMemcpyDeviceToHost));
+    return clsparseSuccess;
+}
+
+// Loss and parameter gradient for one matrix; with lr > 0 also an Adam step.
+// probes: Rademacher vectors seeded by (seed, probe index).
+extern "C" clsparseStatus clsparseAmdFsaiTrainStep(clsparseAmdFsai f, clsparseAmdMlp net,
+                                                   const clsparseCsrMatrix* A, cl_uint probes,
+                                                   cl_ulong seed, cl_float lr,
+                                                   clsparseControl control, cl_double* loss,
+                                                   cl_double* grad_g_host)
+{
+    CLS_TRY(check_init_control(control));
+    if (f == nullptr || net == nullptr || A == nullptr || probes == 0) return clsparseInvalidValue;
+    if (A->num_rows != f->n) return clsparseInvalidSystemSize;
+    CLS_TRY(ensure_train_scratch(f));
+    hipStream_t st = control->stream;
+    CLS_TRY(fsai_assemble(f, net, st));
+    const uint32_t n = f->n, nnzG = f->nnzG;
+    double* one = f->sc;
+    double* zero = f->sc + 1;
+    hipLaunchKernelGGL((set_scalar<double>), dim3(1), dim3(64), 0, st, one, 1.0);
+    hipLaunchKernelGGL((set_scalar<double>), dim3(1), dim3(64), 0, st, zero, 0.0);
+    CLS_HIP_TRY(hipMemsetAsync(f->gG, 0, nnzG * sizeof(double), st));
+    const double c = 2.0 / (static_cast<double>(probes) * n);
+    double total = 0.0;
+    for (uint32_t pi = 0; pi < probes; ++pi) {
+        hipLaunchKernelGGL(rademacher_kernel, dim3(blocks_for(n)), dim3(256), 0, st, n,
+                           seed * 0x100000001B3ull + pi, f->z);
+        CLS_TRY(launch_csrmv<double>(one, &f->G64, f->z, zero, f->t, st, false, nullptr));   // t = G z
+        CLS_TRY(launch_csrmv<double>(one, A, f->t, zero, f->u, st, false, nullptr));         // u = A t
+        CLS_TRY(launch_csrmv<double>(one, &f->GT64, f->u, zero, f->s, st, false, nullptr));  // s = G^T u
+        hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(n)), dim3(256), 0, st, n, f->s, f->z, f->v);
+        CLS_TRY(launch_csrmv<double>(one, &f->G64, f->v, zero, f->t2, st, false, nullptr));  // t2 = G v
+        CLS_TRY(launch_csrmv<double>(one, A, f->t2, zero, f->w, st, false, nullptr));        // w = A t2
+        hipLaunchKernelGGL(hutch_grad_kernel, dim3(blocks_for(nnzG)), dim3(256), 0, st, nnzG,
+                           f->g_row, f->g_ci, f->u, f->v, f->w, f->z, c, f->gG);
+        CLS_TRY(launch_reduce<double, double>(RedOp::Sqr, f->v, nullptr, n, f->sc + 2, false,
+                                              control->red_partials, st));
+        double vv = 0.0;
+        CLS_HIP_TRY(hipMemcpyAsync(&vv, f->sc + 2, sizeof(double), hipMemcpyDeviceToHost, st));
+        CLS_HIP_TRY(hipStreamSynchronize(st));
+        total += vv;
+    }
+    if (loss) *loss = total / (static_cast<double>(probes) * n);
+    if (grad_g_host)
+        CLS_HIP_TRY(hipMemcpyAsync(grad_g_host, f->gG, nnzG * sizeof(double), hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(chain_kernel, dim3(blocks_for(nnzG)), dim3(256), 0, st, nnzG, f->g_row, f->gG,
+                       f->dhat, f->gout);
+    hipLaunchKernelGGL(mlp_backward_kernel, dim3(kMlpGrid), dim3(64 * kMlpWaves), 0, st, net->dev(), f->X,
+                       f->gout, nnzG, net->partials);
+    hipLaunchKernelGGL(grad_reduce_kernel, dim3(blocks_for(kNumParams)), dim3(256), 0, st, net->partials,
+                       static_cast<uint32_t>(kMlpGrid * kMlpWaves), net->grad);
+    if (lr > 0.f) {
+        net->step += 1;
+        const float b1t = std::pow(0.9f, static_cast<float>(net->step));
+        const float b2t = std::pow(0.999f, static_cast<float>(net->step));
+        hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(kNumParams)), dim3(256), 0, st, net->p, net->m1,
+                           net->m2, net->grad, lr, b1t, b2t, net->W1, net->W2, net->W2T);
+    }
+    CLS_HIP_TRY(hipGetLastError());
+    CLS_HIP_TRY(hipStreamSynchronize(st));
+    return clsparseSuccess;
+}
+
+// Accessors for the solver (cg.hip).
+const clsparseCsrMatrix* fsai_G32(clsparseAmdFsai f) { return &f->G32; }
+const clsparseCsrMatrix* fsai_GT32(clsparseAmdFsai f) { return &f->GT32; }
+const clsparseCsrMatrix* fsai_G64(clsparseAmdFsai f) { return &f->G64; }
+const clsparseCsrMatrix* fsai_GT64(clsparseAmdFsai f) { return &f->GT64; }
+bool fsai_valid(clsparseAmdFsai f) { return f->valid; }
+uint32_t fsai_n(clsparseAmdFsai f) { return f->n; }

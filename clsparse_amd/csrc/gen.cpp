@@ -218,6 +218,105 @@ extern "C" clsparseStatus clsparseAmdGenPowerLaw(cl_uint n, cl_double mean_len, 
     return clsparseSuccess;
 }
 
+// Q1 (bilinear) element stiffness on the unit square for a constant tensor
+// K = [[k11, k12], [k12, k22]]; local nodes (0,0), (1,0), (0,1), (1,1).
+// Exact integrals of grad(phi_a)^T K grad(phi_b): SPSD by construction.
+static void q1_stiffness(double k11, double k12, double k22, double ke[4][4])
+{
+    static const double Sxx[4][4] = {{1 / 3., -1 / 3., 1 / 6., -1 / 6.},
+                                     {-1 / 3., 1 / 3., -1 / 6., 1 / 6.},
+                                     {1 / 6., -1 / 6., 1 / 3., -1 / 3.},
+                                     {-1 / 6., 1 / 6., -1 / 3., 1 / 3.}};
+    static const double Syy[4][4] = {{1 / 3., 1 / 6., -1 / 3., -1 / 6.},
+                                     {1 / 6., 1 / 3., -1 / 6., -1 / 3.},
+                                     {-1 / 3., -1 / 6., 1 / 3., 1 / 6.},
+                                     {-1 / 6., -1 / 3., 1 / 6., 1 / 3.}};
+    // d(phi_a)/dx depends on y only and d(phi_b)/dy on x only, so
+    // int dx(phi_a) dy(phi_b) = (int dx phi_a dy) (int dy phi_b dx).
+    static const double cx[4] = {-0.5, 0.5, -0.5, 0.5};
+    static const double cy[4] = {-0.5, -0.5, 0.5, 0.5};
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b)
+            ke[a][b] = k11 * Sxx[a][b] + k22 * Syy[a][b] + k12 * (cx[a] * cy[b] + cx[b] * cy[a]);
+}
+
+extern "C" clsparseStatus clsparseAmdGenAnisoDiffusion2D(cl_uint nx, cl_uint ny, cl_double corr_len,
+                                                         cl_double sigma, cl_double ratio,
+                                                         cl_double theta, cl_ulong seed,
+                                                         clsparseAmdHostCsr* out)
+{
+    if (out == nullptr || nx == 0 || ny == 0 || ratio <= 0 || corr_len <= 0)
+        return clsparseInvalidValue;
+    const uint64_t n = static_cast<uint64_t>(nx) * ny;
+    if (n > 0xFFFFFFFFull) return clsparseInvalidSize;
+    // Log-coefficient: Gaussian random field with covariance
+    // sigma^2 exp(-r^2 / (2 corr_len^2)), sampled by M random Fourier modes.
+    constexpr int M = 128;
+    double kxm[M], kym[M], phm[M];
+    {
+        Rng r(seed, 0xF00D);
+        for (int m = 0; m < M; ++m) {
+            kxm[m] = r.normal() / corr_len;
+            kym[m] = r.normal() / corr_len;
+            phm[m] = 6.283185307179586 * r.uniform();
+        }
+    }
+    const double amp = sigma * std::sqrt(2.0 / M);
+    const uint32_t ex = nx + 1, ey = ny + 1;  // elements; nodes 0 and n+1 are Dirichlet
+    std::vector<double> kappa(static_cast<size_t>(ex) * ey);
+    parallel_rows(static_cast<uint64_t>(ex) * ey, [&](uint64_t e) {
+        const double cxp = static_cast<double>(e % ex) + 0.5, cyp = static_cast<double>(e / ex) + 0.5;
+        double s = 0.0;
+        for (int m = 0; m < M; ++m) s += std::cos(kxm[m] * cxp + kym[m] * cyp + phm[m]);
+        kappa[e] = std::exp(amp * s);
+    });
+    // Tensor R(theta) diag(1, 1/ratio) R(theta)^T.
+    const double c = std::cos(theta), sn = std::sin(theta), eps = 1.0 / ratio;
+    const double t11 = c * c + eps * sn * sn, t22 = sn * sn + eps * c * c, t12 = (1.0 - eps) * c * sn;
+
+    // Row pointer: interior node couples with its 3x3 neighbourhood.
+    uint64_t nnz = 0;
+    std::vector<uint32_t> rlen(n);
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint32_t ix = static_cast<uint32_t>(i % nx), iy = static_cast<uint32_t>(i / nx);
+        const uint32_t wx = 1 + (ix > 0) + (ix + 1 < nx), wy = 1 + (iy > 0) + (iy + 1 < ny);
+        rlen[i] = wx * wy;
+        nnz += rlen[i];
+    }
+    if (!alloc_csr(out, static_cast<uint32_t>(n), static_cast<uint32_t>(n), nnz))
+        return clsparseOutOfHostMemory;
+    out->row_pointer[0] = 0;
+    for (uint64_t i = 0; i < n; ++i) out->row_pointer[i + 1] = out->row_pointer[i] + rlen[i];
+    parallel_rows(n, [&](uint64_t i) {
+        const int ix = static_cast<int>(i % nx), iy = static_cast<int>(i / nx);
+        double acc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // [dy+1][dx+1]
+        // Node (ix+1, iy+1) in node numbering; the 4 elements around it.
+        for (int ey0 = 0; ey0 < 2; ++ey0)
+            for (int ex0 = 0; ex0 < 2; ++ex0) {
+                const int ecx = ix + ex0, ecy = iy + ey0;  // element index
+                const double k = kappa[static_cast<size_t>(ecy) * ex + ecx];
+                double ke[4][4];
+                q1_stiffness(k * t11, k * t12, k * t22, ke);
+                const int me = (1 - ex0) + 2 * (1 - ey0);  // local index of this node
+                for (int b = 0; b < 4; ++b) {
+                    const int dx = (b & 1) - (1 - ex0), dy = (b >> 1) - (1 - ey0);
+                    acc[dy + 1][dx + 1] += ke[me][b];
+                }
+            }
+        uint64_t j = out->row_pointer[i];
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int jx = ix + dx, jy = iy + dy;
+                if (jx < 0 || jy < 0 || jx >= static_cast<int>(nx) || jy >= static_cast<int>(ny))
+                    continue;
+                out->col_indices[j] = static_cast<uint32_t>(jy) * nx + static_cast<uint32_t>(jx);
+                out->values[j] = acc[dy + 1][dx + 1];
+                ++j;
+            }
+    });
+    return clsparseSuccess;
+}
+
 extern "C" clsparseStatus clsparseAmdGenDiffusion3D(cl_uint nx, cl_uint ny, cl_uint nz,
                                                     cl_double sigma, cl_double ax, cl_double ay,
                                                     cl_double az, cl_ulong seed,

@@ -6,6 +6,7 @@
  *   - metadata read-back (lets callers check the row-block array bit for bit),
  *   - solver statistics (executed iterations, FP64 true residual, timings),
  *   - the mixed-precision (FP32 inner / FP64 outer) Jacobi-PCG solver,
+ *   - the learned factorized sparse approximate inverse preconditioner (FSAI),
  *   - seeded synthetic matrix and vector generators used by tests and benches.
  */
 #ifndef CLSPARSE_AMD_EXT_H_
@@ -40,7 +41,8 @@ typedef struct clsparseAmdSolverStats_
     cl_double true_residual;    /* final FP64 ||b - A x||_2 / ||b||_2 (mixed solver) */
     cl_double seconds;          /* wall time of the solve, host clock */
     cl_int status_detail;       /* 0 converged, 1 maxIters reached, 2 stagnated */
-    cl_int reserved;
+    cl_int preconditioner;      /* refinement solvers: 0 none, 1 Jacobi, 2 learned FSAI,
+                                   3 learned FSAI that fell back to Jacobi */
     /* Snapshot of the solver control's reference fields
      * (src/library/solvers/solver-control.hpp L108-127). */
     cl_int n_iters;
@@ -71,6 +73,74 @@ CLSPARSE_EXPORT clsparseStatus
 CLSPARSE_EXPORT clsparseStatus
     clsparseAmdDcsrcgRefined(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
                              clSParseSolverControl solverControl, clsparseControl control);
+
+/* ---- learned factorized sparse approximate inverse ----------------------
+ * G is lower triangular on the pattern of tril(A), G_ij = (delta_ij + o_ij) /
+ * sqrt(a_ii), where o_ij is the output of a per-nonzero MLP (16 features ->
+ * 64 -> 64 -> 1, ReLU; BF16 weights and activations on MFMA, FP32
+ * accumulation). The preconditioner is M = G G^T. A network with zero output
+ * gives exactly the Jacobi preconditioner; so does a NULL network. */
+typedef struct _clsparseAmdMlp* clsparseAmdMlp;
+typedef struct _clsparseAmdFsai* clsparseAmdFsai;
+
+/* Network on the current device: He-normal hidden layers from `seed`, output
+ * layer zero (so a new network reproduces Jacobi). */
+CLSPARSE_EXPORT clsparseStatus clsparseAmdMlpCreate(clsparseAmdMlp* net, cl_ulong seed);
+CLSPARSE_EXPORT clsparseStatus clsparseAmdMlpDestroy(clsparseAmdMlp net);
+/* Number of FP32 parameters: W1[64][16], b1[64], W2[64][64], b2[64], w3[64], b3. */
+CLSPARSE_EXPORT cl_uint clsparseAmdMlpNumParams(void);
+CLSPARSE_EXPORT clsparseStatus clsparseAmdMlpGetParams(clsparseAmdMlp net, cl_float* host);
+CLSPARSE_EXPORT clsparseStatus clsparseAmdMlpSetParams(clsparseAmdMlp net, const cl_float* host);
+/* Parameter gradient of the last clsparseAmdFsaiTrainStep. */
+CLSPARSE_EXPORT clsparseStatus clsparseAmdMlpGetGrad(clsparseAmdMlp net, cl_float* host);
+
+/* Builds G's pattern (and that of G^T) from a square FP64 matrix A with
+ * metadata-free CSR input, the diagonal scaling and the per-nonzero features
+ * (BF16, 16 per nonzero of G). G starts as Jacobi. */
+CLSPARSE_EXPORT clsparseStatus
+    clsparseAmdFsaiCreate(clsparseAmdFsai* fsai, const clsparseCsrMatrix* A, clsparseControl control);
+/* Recomputes scaling and features for new values of A on the same pattern. */
+CLSPARSE_EXPORT clsparseStatus
+    clsparseAmdFsaiUpdateValues(clsparseAmdFsai fsai, const clsparseCsrMatrix* A, clsparseControl control);
+CLSPARSE_EXPORT clsparseStatus clsparseAmdFsaiDestroy(clsparseAmdFsai fsai);
+/* Evaluates the network (NULL = Jacobi) and assembles G and G^T in FP64 and FP32. */
+CLSPARSE_EXPORT clsparseStatus
+    clsparseAmdFsaiApplyNet(clsparseAmdFsai fsai, clsparseAmdMlp net, clsparseControl control);
+/* Host copies of G (CSR: num_rows + 1 row pointers, *nnz columns and FP64
+ * values); any output may be NULL. */
+CLSPARSE_EXPORT clsparseStatus
+    clsparseAmdFsaiGetValues(clsparseAmdFsai fsai, cl_uint* row_pointer, cl_uint* col_indices,
+                             cl_double* values, cl_uint* nnz);
+CLSPARSE_EXPORT clsparseStatus clsparseAmdFsaiGetFeatures(clsparseAmdFsai fsai, uint16_t* bf16);
+/* Network output o per nonzero of G from the last evaluation. */
+CLSPARSE_EXPORT clsparseStatus clsparseAmdFsaiGetOutput(clsparseAmdFsai fsai, cl_float* out);
+
+/* One training step on A: assembles G from the network, estimates
+ * L = ||(G^T A G - I) Z||_F^2 / (probes * n) with `probes` Rademacher columns
+ * Z drawn from `seed`, back-propagates dL/dG through the network, and, when
+ * lr > 0, applies one Adam step (beta 0.9 / 0.999). loss and grad_g (dL/dG
+ * per nonzero of G, FP64) may be NULL. */
+CLSPARSE_EXPORT clsparseStatus
+    clsparseAmdFsaiTrainStep(clsparseAmdFsai fsai, clsparseAmdMlp net, const clsparseCsrMatrix* A,
+                             cl_uint probes, cl_ulong seed, cl_float lr, clsparseControl control,
+                             cl_double* loss, cl_double* grad_g);
+
+/* Mixed-precision refinement (as clsparseAmdDcsrcgMixed) whose FP32 inner
+ * solves are preconditioned with M = G G^T (two CSR-Adaptive SpMVs per
+ * iteration). Falls back to Jacobi for the rest of the solve when G holds a
+ * non-finite value, when an inner solve returns a non-finite residual, reduces
+ * its residual less than twofold or exhausts its iterations, or when the FP64
+ * true residual does not decrease over an outer step. The FP64 true residual
+ * decides convergence. */
+CLSPARSE_EXPORT clsparseStatus
+    clsparseAmdDcsrcgFsai(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
+                          clsparseAmdFsai fsai, clSParseSolverControl solverControl,
+                          clsparseControl control);
+/* The same with FP64 inner solves. */
+CLSPARSE_EXPORT clsparseStatus
+    clsparseAmdDcsrcgFsai64(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
+                            clsparseAmdFsai fsai, clSParseSolverControl solverControl,
+                            clsparseControl control);
 
 /* ---- synthetic generators (host memory, seeded, deterministic) ----------- */
 typedef struct clsparseAmdHostCsr_
@@ -105,6 +175,17 @@ CLSPARSE_EXPORT clsparseStatus
     clsparseAmdGenDiffusion3D(cl_uint nx, cl_uint ny, cl_uint nz, cl_double sigma,
                               cl_double ax, cl_double ay, cl_double az, cl_ulong seed,
                               clsparseAmdHostCsr* out);
+
+/* 2-D anisotropic diffusion -div(kappa R diag(1, 1/ratio) R^T grad u) on an
+ * nx x ny grid of interior nodes, bilinear (Q1) finite elements, Dirichlet
+ * boundary: a symmetric positive definite 9-point stencil. log(kappa) is a
+ * Gaussian random field of standard deviation sigma and correlation length
+ * corr_len (cells), sampled with 128 seeded random Fourier modes; R rotates
+ * by theta. */
+CLSPARSE_EXPORT clsparseStatus
+    clsparseAmdGenAnisoDiffusion2D(cl_uint nx, cl_uint ny, cl_double corr_len, cl_double sigma,
+                                   cl_double ratio, cl_double theta, cl_ulong seed,
+                                   clsparseAmdHostCsr* out);
 
 /* Fills out[0..n) with U[lo, hi) from a SplitMix64 stream of `seed`. */
 CLSPARSE_EXPORT clsparseStatus

@@ -150,6 +150,62 @@ def test_diffusion_generator_spd_structure():
     assert np.linalg.eigvalsh(a.toarray()).min() > 0
 
 
+def _aniso_stencil(eps):
+    """Closed-form Q1 stencil of -div(diag(1, eps) grad u), unit cells:
+    the x part is (1-D stiffness [-1 2 -1]) x (1-D mass [1/6 2/3 1/6]) and the
+    y part its transpose scaled by eps."""
+    s1, m1 = np.array([-1.0, 2.0, -1.0]), np.array([1 / 6, 2 / 3, 1 / 6])
+    return np.outer(m1, s1) + eps * np.outer(s1, m1)  # [dy][dx]
+
+
+@pytest.mark.parametrize("theta,ratio", [(0.0, 1.0), (0.0, 50.0), (np.pi / 2, 50.0)])
+def test_aniso2d_generator_closed_form_stencil(theta, ratio):
+    import scipy.sparse as sp
+    nx, ny = 7, 6
+    rp, ci, v, m = cls.gen_aniso2d(nx, ny, corr_len=4.0, sigma=0.0, ratio=ratio, theta=theta,
+                                   seed=2)
+    n = nx * ny
+    assert m == n and rp[-1] == 9 * n - 6 * (nx + ny) + 4
+    a = sp.csr_matrix((v, ci, rp), shape=(n, n)).toarray()
+    st = _aniso_stencil(1.0 / ratio)
+    if theta != 0.0:
+        st = st.T  # rotating by 90 degrees swaps the strong and the weak axis
+    # every interior node: the 3x3 stencil, exactly (up to rounding of cos/sin)
+    for y in range(ny):
+        for x in range(nx):
+            i = y * nx + x
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    xx, yy = x + dx, y + dy
+                    if 0 <= xx < nx and 0 <= yy < ny:
+                        assert abs(a[i, yy * nx + xx] - st[dy + 1, dx + 1]) < 1e-12
+    if ratio == 1.0:  # isotropic Q1 Laplacian: 8/3 and -1/3
+        assert abs(st[1, 1] - 8 / 3) < 1e-15 and abs(st[0, 0] + 1 / 3) < 1e-15
+        assert abs(st[0, 1] + 1 / 3) < 1e-15
+
+
+def test_aniso2d_generator_spd_random_field():
+    import scipy.sparse as sp
+    nx, ny = 16, 12
+    rp, ci, v, m = cls.gen_aniso2d(nx, ny, corr_len=4.0, sigma=1.0, ratio=300.0, theta=0.6,
+                                   seed=9)
+    n = nx * ny
+    a = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    assert abs(a - a.T).max() <= 1e-14 * abs(a).max()
+    assert np.linalg.eigvalsh(a.toarray()).min() > 0
+    for r in range(n):
+        c = ci[rp[r]:rp[r + 1]].astype(np.int64)
+        assert np.all(np.diff(c) > 0)
+    # the coefficient field varies (sigma > 0) and the generator is seeded
+    d = a.diagonal()
+    assert d.max() / d.min() > 2.0
+    rp2, ci2, v2, _ = cls.gen_aniso2d(nx, ny, corr_len=4.0, sigma=1.0, ratio=300.0, theta=0.6,
+                                      seed=9)
+    np.testing.assert_array_equal(v, v2)
+    _, _, v3, _ = cls.gen_aniso2d(nx, ny, corr_len=4.0, sigma=1.0, ratio=300.0, theta=0.6, seed=10)
+    assert not np.array_equal(v, v3)
+
+
 def test_powerlaw_generator_statistics():
     rp, ci, v, m = cls.gen_powerlaw(1 << 16, 16.0, 1.5, 200000, 2, 20000, seed=3)
     lens = np.diff(rp.astype(np.int64))
