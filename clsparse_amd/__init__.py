@@ -291,4 +291,4 @@ def row_blocks_host(row_pointer: np.ndarray, count_only: bool = False):
     return out[: created // 2].copy(), int(created)
 
 
-from .device import Control, CsrMatrix, DenseVector, Scalar, SolverControl  # noqa: E402,F401
+from .device import Control, CsrMatrix, DenseVector, Fsai, Mlp, Scalar, SolverControl  # noqa: E402,F401

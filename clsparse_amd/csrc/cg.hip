@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -132,9 +133,10 @@ __global__ __launch_bounds__(256) void cg_dot_kernel(size_t n, const T* __restri
 }
 
 // alpha = rz / yp; x += alpha p; r -= alpha y; z = M r; partials <r,z>, ||r||.
-template <typename T>
+// x may be held in a wider type than the other vectors (FP64 solution update).
+template <typename T, typename TX>
 __global__ __launch_bounds__(256) void cg_update_kernel(
-    size_t n, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z, const T* __restrict__ p,
+    size_t n, TX* __restrict__ x, T* __restrict__ r, T* __restrict__ z, const T* __restrict__ p,
     const T* __restrict__ y, const T* __restrict__ invD, int prec,
     const double* __restrict__ part_yp, const double* __restrict__ part_rz_prev,
     double* __restrict__ part_rz, double* __restrict__ part_nr, uint32_t g,
@@ -145,12 +147,13 @@ __global__ __launch_bounds__(256) void cg_update_kernel(
     const double yp = sum_partials(part_yp, g, sh);
     const double rz = sum_partials(part_rz_prev, g, sh);
     const T alpha = static_cast<T>(rz / yp);
+    const TX alpha_x = static_cast<TX>(rz / yp);
     const int crit = st->crit;
     double nr = 0.0, rzn = 0.0;
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
     for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
         const T pi = p[i];
-        x[i] = x[i] + alpha * pi;
+        x[i] = x[i] + alpha_x * static_cast<TX>(pi);
         const T ri = r[i] - alpha * y[i];
         r[i] = ri;
         nr += norm_term(crit, static_cast<double>(ri));
@@ -240,6 +243,17 @@ __global__ __launch_bounds__(256) void fill_kernel(size_t n, T* __restrict__ out
         out[i] = v;
 }
 
+// out = M r for the elementwise preconditioners.
+template <typename T>
+__global__ __launch_bounds__(256) void precond_kernel(size_t n, const T* __restrict__ r,
+                                                      const T* __restrict__ invD, int prec,
+                                                      T* __restrict__ out)
+{
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
+        out[i] = (prec == kPrecJacobi) ? r[i] * invD[i] : r[i];
+}
+
 uint32_t grid_for(size_t n)
 {
     size_t g = (n + 2047) / 2048;
@@ -317,12 +331,12 @@ bool graphs_enabled()
 }
 
 // The device-resident PCG loop on one system. x, b are T vectors of length n.
-template <typename T>
+template <typename T, typename TX = T>
 class CgEngine
 {
 public:
     // G, GT: the factor of M = G G^T and its transpose (kPrecFsai only).
-    CgEngine(const clsparseCsrMatrix* A, T* x, const T* b, size_t n, int prec, int crit,
+    CgEngine(const clsparseCsrMatrix* A, TX* x, const T* b, size_t n, int prec, int crit,
              CgBuffers<T>& buf, hipStream_t s, bool ext, const clsparseCsrMatrix* G = nullptr,
              const clsparseCsrMatrix* GT = nullptr)
         : A_(A), x_(x), b_(b), n_(n), prec_(prec), crit_(crit), buf_(buf), s_(s), ext_(ext),
@@ -339,7 +353,12 @@ public:
     // z = M r, p = z, <r, z>; writes init_res into the device state.
     clsparseStatus setup(bool x_zero)
     {
-        if (!x_zero) CLS_TRY(launch_csrmv<T>(buf_.one, A_, x_, buf_.zero, buf_.y, s_, ext_, nullptr));
+        if constexpr (std::is_same_v<T, TX>) {
+            if (!x_zero)
+                CLS_TRY(launch_csrmv<T>(buf_.one, A_, x_, buf_.zero, buf_.y, s_, ext_, nullptr));
+        } else if (!x_zero) {
+            return clsparseInvalidValue;  // the wide-x engine starts from a given residual
+        }
         hipLaunchKernelGGL((cg_init_kernel<T>), dim3(g_), dim3(256), 0, s_, n_, b_, buf_.y, buf_.r,
                            buf_.z, buf_.p, buf_.invD, prec_, x_zero ? 1 : 0, crit_, buf_.part_nr[1],
                            buf_.part_rz[1]);
@@ -355,13 +374,13 @@ public:
         return clsparseSuccess;
     }
 
-    // Runs iterations until the device state says done. `on_chunk` sees the
-    // host copy of the state after every chunk.
-    template <typename F>
-    clsparseStatus run(CgDevState* host_state, F&& on_chunk)
+    // One chunk of kChunk iterations (captured once in a graph, then
+    // replayed); afterwards the host copy of the device state is current.
+    // Iterations after the device state says done return at once.
+    clsparseStatus run_chunk(CgDevState* host_state)
     {
         const bool use_graph = graphs_enabled();
-        if (use_graph) {
+        if (use_graph && exec_ == nullptr) {
             CLS_HIP_TRY(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
             clsparseStatus st = clsparseSuccess;
             for (int k = 0; k < kChunk && st == clsparseSuccess; ++k) st = enqueue_iteration(k & 1);
@@ -375,20 +394,45 @@ public:
             graph_ = g;
             CLS_HIP_TRY(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
         }
+        if (use_graph) {
+            CLS_HIP_TRY(hipGraphLaunch(exec_, s_));
+        } else {
+            for (int k = 0; k < kChunk; ++k) CLS_TRY(enqueue_iteration(k & 1));
+        }
+        CLS_HIP_TRY(hipMemcpyAsync(host_state, buf_.st, sizeof(CgDevState), hipMemcpyDeviceToHost,
+                                   s_));
+        CLS_HIP_TRY(hipStreamSynchronize(s_));
+        return clsparseSuccess;
+    }
+
+    // Runs chunks until the device state says done. `on_chunk` sees the host
+    // copy of the state after every chunk.
+    template <typename F>
+    clsparseStatus run(CgDevState* host_state, F&& on_chunk)
+    {
         for (;;) {
-            if (use_graph) {
-                CLS_HIP_TRY(hipGraphLaunch(exec_, s_));
-            } else {
-                for (int k = 0; k < kChunk; ++k) CLS_TRY(enqueue_iteration(k & 1));
-            }
-            CLS_HIP_TRY(hipMemcpyAsync(host_state, buf_.st, sizeof(CgDevState),
-                                       hipMemcpyDeviceToHost, s_));
-            CLS_HIP_TRY(hipStreamSynchronize(s_));
+            CLS_TRY(run_chunk(host_state));
             on_chunk(*host_state);
             if (host_state->done) break;
         }
         return clsparseSuccess;
     }
+
+    // z_new = M r into `out` (Jacobi / none: elementwise; FSAI: G (G^T r)
+    // through `tmp`), for the residual replacement of the reliable solver.
+    clsparseStatus precondition(T* out, T* tmp)
+    {
+        if (prec_ == kPrecFsai) {
+            CLS_TRY(launch_csrmv<T>(buf_.one, GT_, buf_.r, buf_.zero, tmp, s_, false, nullptr));
+            CLS_TRY(launch_csrmv<T>(buf_.one, G_, tmp, buf_.zero, out, s_, false, nullptr));
+        } else {
+            hipLaunchKernelGGL((precond_kernel<T>), dim3(ew_grid(n_)), dim3(256), 0, s_, n_, buf_.r,
+                               buf_.invD, prec_, out);
+            CLS_HIP_TRY(hipGetLastError());
+        }
+        return clsparseSuccess;
+    }
+    int prec() const { return prec_; }
 
 private:
     clsparseStatus enqueue_iteration(int par)
@@ -397,7 +441,7 @@ private:
         CLS_TRY(launch_csrmv<T>(buf_.one, A_, buf_.p, buf_.zero, buf_.y, s_, ext_, done));
         hipLaunchKernelGGL((cg_dot_kernel<T>), dim3(g_), dim3(256), 0, s_, n_, buf_.y, buf_.p,
                            buf_.part_yp, buf_.st);
-        hipLaunchKernelGGL((cg_update_kernel<T>), dim3(g_), dim3(256), 0, s_, n_, x_, buf_.r,
+        hipLaunchKernelGGL((cg_update_kernel<T, TX>), dim3(g_), dim3(256), 0, s_, n_, x_, buf_.r,
                            buf_.z, buf_.p, buf_.y, buf_.invD, prec_, buf_.part_yp,
                            buf_.part_rz[par ^ 1], buf_.part_rz[par], buf_.part_nr[par], g_,
                            buf_.st);
@@ -422,7 +466,7 @@ private:
     }
 
     const clsparseCsrMatrix* A_;
-    T* x_;
+    TX* x_;
     const T* b_;
     size_t n_;
     int prec_;
@@ -593,49 +637,75 @@ extern "C" clsparseStatus clsparseDcsrcg(cldenseVector* x, const clsparseCsrMatr
     return cg_entry<double>(x, A, b, solverControl, control);
 }
 
-// ---- iterative refinement with an FP64 outer loop (no reference counterpart) --
+// ---- mixed-precision CG with reliable updates (no reference counterpart) -----
 //
-// Outer loop in FP64: r = b - A x (FP64 CSR-Adaptive SpMV), ||r||_2 decides
-// convergence, x += ||r|| d in FP64. Inner solve in precision TI: A d = r/||r||
-// by the same device-resident PCG (TI values, FP64 dots/norms), stopped on its
-// own L2 residual at `inner_tol` (or when the overall iteration budget ends).
-// TI = float is the mixed-precision solver; TI = double is plain FP64 PCG
-// judged by the same FP64 true residual (the baseline the mixed one is set
-// beside).
+// One preconditioned CG run in precision TI (values of A, r, z, p, y) with the
+// solution x held and updated in FP64 and every dot product and norm
+// accumulated in FP64. Whenever the recurrence residual has fallen by a factor
+// `delta` since the last replacement, or claims convergence, the residual is
+// replaced by the FP64 true residual b - A x (FP64 CSR-Adaptive SpMV) -- the
+// reliable-update scheme of van der Vorst and Ye ("Residual replacement
+// strategies for Krylov subspace iterative methods", SIAM J. Sci. Comput.
+// 22(3), 2000) -- and the search direction is re-based on the new residual
+// (p' = z' + (<r',z'>/<r,z>) (p - z), z' = M r'), so the Krylov process goes on
+// without a restart. The FP64 true relative residual ||b - A x||_2 / ||b||_2
+// decides convergence. TI = float is the mixed-precision solver; TI = double
+// is FP64 PCG judged by the same FP64 true residual (the baseline).
 
 namespace {
 
-template <typename TI>
-__global__ __launch_bounds__(256) void ir_scale_residual(size_t n, const double* __restrict__ r,
-                                                         const double* __restrict__ rnorm,
-                                                         TI* __restrict__ bI)
+constexpr int kStagnationChunks = 16;  // learned FSAI must halve ||r|| within this many chunks
+
+// r = (T) r64
+template <typename T>
+__global__ __launch_bounds__(256) void rr_cast(size_t n, const double* __restrict__ r64,
+                                               T* __restrict__ r)
 {
-    const double s = 1.0 / *rnorm;
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
     for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
-        bI[i] = static_cast<TI>(r[i] * s);
+        r[i] = static_cast<T>(r64[i]);
+}
+
+// s = <r', z'> / <r, z>;  p = z' + s (p - z);  z = z'.
+template <typename T>
+__global__ __launch_bounds__(256) void rr_rebase(size_t n, T* __restrict__ p, T* __restrict__ z,
+                                                 const T* __restrict__ znew,
+                                                 const double* __restrict__ part_old,
+                                                 const double* __restrict__ part_new, uint32_t g)
+{
+    __shared__ double sh[4];
+    const double rz_old = sum_partials(part_old, g, sh);
+    const double rz_new = sum_partials(part_new, g, sh);
+    const T sc = static_cast<T>(rz_old != 0.0 ? rz_new / rz_old : 0.0);
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
+        const T zn = znew[i];
+        p[i] = zn + sc * (p[i] - z[i]);
+        z[i] = zn;
+    }
+}
+
+__global__ __launch_bounds__(256) void copy_partials(const double* __restrict__ src,
+                                                     double* __restrict__ dst, uint32_t g)
+{
+    for (uint32_t i = threadIdx.x; i < g; i += 256) dst[i] = src[i];
+}
+
+// Lets the loop go on after a residual replacement: clears done / converged
+// and sets the recurrence residual to the replaced one.
+__global__ void state_resume(CgDevState* st, const double* __restrict__ rnorm)
+{
+    if (threadIdx.x == 0) {
+        st->done = 0;
+        st->converged = 0;
+        st->cur_res = *rnorm / st->norm_ref;
+    }
 }
 
 template <typename TI>
-__global__ __launch_bounds__(256) void ir_update(size_t n, double* __restrict__ x,
-                                                 const TI* __restrict__ d,
-                                                 const double* __restrict__ rnorm)
-{
-    const double s = *rnorm;
-    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
-    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
-        x[i] = fma(s, static_cast<double>(d[i]), x[i]);
-}
-
-// With `fsai` the inner solves use M = G G^T (its G in precision TI) and fall
-// back to Jacobi for the rest of the solve when the factor holds a non-finite
-// value, or an inner solve returns a non-finite or stagnating residual (less
-// than a factor-2 reduction, or its iteration budget ran out), or the FP64
-// true residual fails to decrease over an outer step.
-template <typename TI>
-clsparseStatus refine_solve(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
-                            clSParseSolverControl sc, clsparseControl control,
-                            clsparseAmdFsai fsai = nullptr)
+clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
+                              clSParseSolverControl sc, clsparseControl control,
+                              clsparseAmdFsai fsai = nullptr)
 {
     if (!clsparseInitialized) return clsparseNotInitialized;
     if (control == nullptr) return clsparseInvalidControlObject;
@@ -647,21 +717,21 @@ clsparseStatus refine_solve(cldenseVector* x, const clsparseCsrMatrix* A, const 
 
     constexpr bool kMixed = sizeof(TI) == sizeof(float);
     const size_t n = A->num_rows;
-    if (fsai && fsai_n(fsai) != n) return clsparseInvalidSystemSize;
     const size_t nnz = A->num_nonzeros;
+    if (fsai && fsai_n(fsai) != n) return clsparseInvalidSystemSize;
     hipStream_t s = control->solver_stream;
     double* xp = dev_ptr<double>(x->values, x->off_values);
     const double* bp = dev_ptr<double>(b->values, b->off_values);
     if (n > 0 && (xp == nullptr || bp == nullptr)) return clsparseInvalidMemObject;
 
-    double inner_tol = kMixed ? 1e-4 : 1e-14;
-    if (const char* e = std::getenv(kMixed ? "CLSPARSE_AMD_IR_INNER_TOL" : "CLSPARSE_AMD_IR64_INNER_TOL"))
-        inner_tol = std::atof(e);
+    double delta = 0.1;
+    if (const char* e = std::getenv("CLSPARSE_AMD_RELIABLE_DELTA")) delta = std::atof(e);
 
-    // Workspace: inner-precision values of A (mixed only), FP64 residual and
-    // scalars, inner-precision rhs and correction, then the inner CG buffers.
-    const size_t bytes = (kMixed ? align_up(nnz * sizeof(TI)) : 0) + align_up(n * sizeof(double)) +
-                         align_up(8 * sizeof(double)) + 2 * align_up(n * sizeof(TI)) +
+    // Workspace: TI values of A (mixed only), FP64 residual, FP64 copy of x
+    // (restored if a preconditioner breaks down), scalars, TI start residual,
+    // then the CG buffers.
+    const size_t bytes = (kMixed ? align_up(nnz * sizeof(TI)) : 0) + 2 * align_up(n * sizeof(double)) +
+                         align_up(8 * sizeof(double)) + align_up(n * sizeof(TI)) +
                          CgBuffers<TI>::bytes(n);
     void* work = nullptr;
     CLS_TRY(control_workspace(control, bytes, &work));
@@ -671,9 +741,9 @@ clsparseStatus refine_solve(cldenseVector* x, const clsparseCsrMatrix* A, const 
     Carve c{static_cast<char*>(work)};
     TI* valsI = kMixed ? c.take<TI>(nnz) : nullptr;
     double* r64 = c.take<double>(n);
+    double* xsave = c.take<double>(n);
     double* sc64 = c.take<double>(8);  // [0]=||r||_2, [1]=||b||_2, [2]=-1, [3]=+1
-    TI* bI = c.take<TI>(n);
-    TI* dI = c.take<TI>(n);
+    TI* rI = c.take<TI>(n);            // TI start residual; FSAI temporary afterwards
     CgBuffers<TI> buf;
     buf.carve(c, n);
 
@@ -696,7 +766,7 @@ clsparseStatus refine_solve(cldenseVector* x, const clsparseCsrMatrix* A, const 
         return clsparseSuccess;
     }
 
-    // Inner-precision A on the same pattern (shares col_indices, row_pointer, meta).
+    // TI-precision A on the same pattern (shares col_indices, row_pointer, meta).
     clsparseCsrMatrix AI = *A;
     if constexpr (kMixed) {
         hipLaunchKernelGGL((cast_kernel<double, TI>), dim3(ew_grid(nnz)), dim3(256), 0, s, nnz,
@@ -705,7 +775,7 @@ clsparseStatus refine_solve(cldenseVector* x, const clsparseCsrMatrix* A, const 
         AI.off_values = 0;
     }
     int prec = (sc->preconditioner == DIAGONAL) ? kPrecJacobi : kPrecNone;
-    int used = prec;  // what the statistics report: 0 none, 1 Jacobi, 2 FSAI, 3 FSAI -> Jacobi
+    int used = prec;  // statistics: 0 none, 1 Jacobi, 2 FSAI, 3 FSAI -> Jacobi
     const clsparseCsrMatrix* G = nullptr;
     const clsparseCsrMatrix* GT = nullptr;
     if (fsai) {
@@ -730,68 +800,133 @@ clsparseStatus refine_solve(cldenseVector* x, const clsparseCsrMatrix* A, const 
     hipLaunchKernelGGL((fill_kernel<TI>), dim3(1), dim3(256), 0, s, size_t(1), buf.zero, TI(0));
     CLS_HIP_TRY(hipGetLastError());
 
-    int total_inner = 0, outer = 0;
-    double rel = 0.0, first_rel = -1.0, prev_rel = -1.0;
-    const int max_outer = 100;
-    int detail = 1;
-    for (;;) {
-        // r = b - A x in FP64
+    // r64 = b - A x; ||r64||_2 -> sc64[0]; returns the true relative residual.
+    auto true_residual = [&](double* rel) -> clsparseStatus {
         CLS_HIP_TRY(hipMemcpyAsync(r64, bp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         CLS_TRY(launch_csrmv<double>(sc64 + 2, A, xp, sc64 + 3, r64, s, control->extended_precision,
                                      nullptr));
         CLS_TRY(launch_reduce<double, double>(RedOp::Sqr, r64, nullptr, n, sc64, true,
                                               control->red_partials, s));
-        CLS_HIP_TRY(hipMemcpyAsync(hb, sc64, sizeof(double), hipMemcpyDeviceToHost, s));
+        double h = 0.0;
+        CLS_HIP_TRY(hipMemcpyAsync(&h, sc64, sizeof(double), hipMemcpyDeviceToHost, s));
         CLS_HIP_TRY(hipStreamSynchronize(s));
-        rel = hb[0] / norm_b;
-        if (first_rel < 0) first_rel = rel;
-        if (prec == kPrecFsai && prev_rel >= 0 && !(rel < prev_rel)) {
-            prec = kPrecJacobi;
-            used = 3;
-        }
-        prev_rel = rel;
-        if (sc->printMode == VERBOSE)
-            std::printf("Refinement %d: true relative residual %.6e (inner iterations %d)\n", outer,
-                        rel, total_inner);
-        if (rel <= sc->relativeTolerance) {
-            detail = 0;
-            break;
-        }
-        if (total_inner >= sc->maxIters || outer >= max_outer) break;
+        *rel = h / norm_b;
+        return clsparseSuccess;
+    };
 
-        // Inner solve of A d = r / ||r||, d0 = 0.
-        hipLaunchKernelGGL((ir_scale_residual<TI>), dim3(ew_grid(n)), dim3(256), 0, s, n, r64, sc64,
-                           bI);
-        hipLaunchKernelGGL((fill_kernel<TI>), dim3(ew_grid(n)), dim3(256), 0, s, n, dI, TI(0));
-        CgDevState init{};
-        init.norm_ref = 1.0;
-        init.rel_tol = std::max(inner_tol, 0.5 * sc->relativeTolerance / rel);
-        init.abs_tol = 0.0;
-        init.max_iters = sc->maxIters - total_inner - 1;
-        init.crit = kCritL2;
-        *hs = init;
-        CLS_HIP_TRY(hipMemcpyAsync(buf.st, hs, sizeof(CgDevState), hipMemcpyHostToDevice, s));
-        CgEngine<TI> eng(&AI, dI, bI, n, prec, kCritL2, buf, s, false, G, GT);
-        CLS_TRY(eng.setup(true));
-        CLS_TRY(eng.run(hs, [](const CgDevState&) {}));
-        total_inner += hs->n_iters + 1;
-        ++outer;
-        const bool finite = std::isfinite(hs->cur_res);
-        if (prec == kPrecFsai && (!finite || hs->cur_res > 0.5 || !hs->converged)) {
-            prec = kPrecJacobi;
-            used = 3;
-        }
-        if (finite)
-            hipLaunchKernelGGL((ir_update<TI>), dim3(ew_grid(n)), dim3(256), 0, s, n, xp, dI, sc64);
+    // Device state of the TI loop: L2 recurrence residual relative to ||b||.
+    CgDevState init{};
+    init.norm_ref = norm_b;
+    init.rel_tol = sc->relativeTolerance;
+    init.abs_tol = 0.0;
+    init.max_iters = sc->maxIters - 1;
+    init.crit = kCritL2;
+    *hs = init;
+    CLS_HIP_TRY(hipMemcpyAsync(buf.st, hs, sizeof(CgDevState), hipMemcpyHostToDevice, s));
+
+    CgEngine<TI, double> eng_main(&AI, xp, rI, n, prec, kCritL2, buf, s, false, G, GT);
+    CgEngine<TI, double> eng_jac(&AI, xp, rI, n, kPrecJacobi, kCritL2, buf, s, false);
+    CgEngine<TI, double>* eng = &eng_main;
+
+    // (Re)starts the TI loop from the current x: r = (TI)(b - A x), z = M r, p = z.
+    auto restart = [&]() -> clsparseStatus {
+        hipLaunchKernelGGL((rr_cast<TI>), dim3(ew_grid(n)), dim3(256), 0, s, n, r64, rI);
         CLS_HIP_TRY(hipGetLastError());
+        CLS_TRY(eng->setup(true));
+        hipLaunchKernelGGL(state_resume, dim3(1), dim3(64), 0, s, buf.st, sc64);
+        CLS_HIP_TRY(hipGetLastError());
+        return clsparseSuccess;
+    };
+
+    double rel = 0.0, first_rel = 0.0;
+    CLS_TRY(true_residual(&rel));
+    first_rel = rel;
+    int replacements = 0;
+    int detail = 1;
+    int iters = 0;
+    if (rel <= sc->relativeTolerance) {
+        detail = 0;
+    } else {
+        CLS_HIP_TRY(hipMemcpyAsync(xsave, xp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        CLS_TRY(restart());
+        double res_at_replace = rel;
+        double best_res = rel;
+        int chunks_since_best = 0;
+        int iter_at_chunk = 0;
+        for (;;) {
+            CLS_TRY(eng->run_chunk(hs));
+            const int ran = hs->iter - iter_at_chunk;
+            iter_at_chunk = hs->iter;
+            iters = hs->iter;
+            const double cur = hs->cur_res;
+            const bool finite = std::isfinite(cur);
+            // Breakdown or stagnation of the learned preconditioner: back to
+            // the last saved x and on with Jacobi.
+            if (eng->prec() == kPrecFsai) {
+                if (finite && cur < 0.5 * best_res) {
+                    best_res = cur;
+                    chunks_since_best = 0;
+                } else {
+                    ++chunks_since_best;
+                }
+                if (!finite || chunks_since_best >= kStagnationChunks) {
+                    if (!finite)
+                        CLS_HIP_TRY(hipMemcpyAsync(xp, xsave, n * sizeof(double),
+                                                   hipMemcpyDeviceToDevice, s));
+                    eng = &eng_jac;
+                    used = 3;
+                    CLS_TRY(true_residual(&rel));
+                    if (rel <= sc->relativeTolerance) {
+                        detail = 0;
+                        break;
+                    }
+                    if (iters >= sc->maxIters) break;
+                    CLS_TRY(restart());
+                    res_at_replace = rel;
+                    iter_at_chunk = hs->iter;
+                    continue;
+                }
+            }
+            if (!finite) break;  // Jacobi or none broke down: stop with the last true residual
+            const bool out_of_iters = hs->done && !hs->converged;
+            if (!hs->done && cur > delta * res_at_replace) continue;
+
+            // Residual replacement (or the final check).
+            CLS_TRY(true_residual(&rel));
+            ++replacements;
+            if (rel <= sc->relativeTolerance) {
+                detail = 0;
+                break;
+            }
+            if (out_of_iters || iters >= sc->maxIters) break;
+            CLS_HIP_TRY(hipMemcpyAsync(xsave, xp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            // r = (TI) r64; z' = M r into y; <r, z'> partials -> part_yp;
+            // re-base p and z; the new <r, z> becomes the previous one.
+            hipLaunchKernelGGL((rr_cast<TI>), dim3(ew_grid(n)), dim3(256), 0, s, n, r64, buf.r);
+            CLS_HIP_TRY(hipGetLastError());
+            CLS_TRY(eng->precondition(buf.y, rI));
+            const uint32_t g = grid_for(n);
+            // Sets of partials: the last executed iteration of the chunk wrote
+            // <r, z> to part_rz[(ran - 1) & 1].
+            const double* old_part = buf.part_rz[(ran - 1) & 1];
+            // cg_dot_kernel is a no-op once done is set: clear it first.
+            hipLaunchKernelGGL(state_resume, dim3(1), dim3(64), 0, s, buf.st, sc64);
+            hipLaunchKernelGGL((cg_dot_kernel<TI>), dim3(g), dim3(256), 0, s, n, buf.r, buf.y,
+                               buf.part_yp, buf.st);
+            hipLaunchKernelGGL((rr_rebase<TI>), dim3(g), dim3(256), 0, s, n, buf.p, buf.z, buf.y,
+                               old_part, buf.part_yp, g);
+            hipLaunchKernelGGL(copy_partials, dim3(1), dim3(256), 0, s, buf.part_yp, buf.part_rz[1], g);
+            CLS_HIP_TRY(hipGetLastError());
+            res_at_replace = rel;
+        }
     }
     CLS_TRY(leave_solver_stream(control));
     const auto t1 = std::chrono::steady_clock::now();
-    sc->nIters = total_inner;
+    sc->nIters = iters;
     sc->initialResidual = first_rel;
     sc->currentResidual = rel;
-    sc->stats.iterations = total_inner;
-    sc->stats.outer_iterations = outer;
+    sc->stats.iterations = iters;
+    sc->stats.outer_iterations = replacements;
     sc->stats.true_residual = rel;
     sc->stats.seconds = std::chrono::duration<double>(t1 - t0).count();
     sc->stats.status_detail = detail;
@@ -806,7 +941,7 @@ extern "C" clsparseStatus clsparseAmdDcsrcgMixed(cldenseVector* x, const clspars
                                                  clSParseSolverControl sc,
                                                  clsparseControl control)
 {
-    return refine_solve<float>(x, A, b, sc, control);
+    return reliable_solve<float>(x, A, b, sc, control);
 }
 
 extern "C" clsparseStatus clsparseAmdDcsrcgRefined(cldenseVector* x, const clsparseCsrMatrix* A,
@@ -814,7 +949,7 @@ extern "C" clsparseStatus clsparseAmdDcsrcgRefined(cldenseVector* x, const clspa
                                                    clSParseSolverControl sc,
                                                    clsparseControl control)
 {
-    return refine_solve<double>(x, A, b, sc, control);
+    return reliable_solve<double>(x, A, b, sc, control);
 }
 
 extern "C" clsparseStatus clsparseAmdDcsrcgFsai(cldenseVector* x, const clsparseCsrMatrix* A,
@@ -822,7 +957,7 @@ extern "C" clsparseStatus clsparseAmdDcsrcgFsai(cldenseVector* x, const clsparse
                                                 clSParseSolverControl sc, clsparseControl control)
 {
     if (fsai == nullptr) return clsparseInvalidValue;
-    return refine_solve<float>(x, A, b, sc, control, fsai);
+    return reliable_solve<float>(x, A, b, sc, control, fsai);
 }
 
 extern "C" clsparseStatus clsparseAmdDcsrcgFsai64(cldenseVector* x, const clsparseCsrMatrix* A,
@@ -830,5 +965,5 @@ extern "C" clsparseStatus clsparseAmdDcsrcgFsai64(cldenseVector* x, const clspar
                                                   clSParseSolverControl sc, clsparseControl control)
 {
     if (fsai == nullptr) return clsparseInvalidValue;
-    return refine_solve<double>(x, A, b, sc, control, fsai);
+    return reliable_solve<double>(x, A, b, sc, control, fsai);
 }

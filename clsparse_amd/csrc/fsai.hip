@@ -367,7 +367,7 @@ __global__ __launch_bounds__(256) void row_stats_kernel(uint32_t n, const uint32
     double r = 0.0, s = 0.0;
     for (uint32_t j = rp[i]; j < rp[i + 1]; ++j) {
         const uint32_t c = ci[j];
-        if (c == i) continue;
+        if (c == i || c >= n) continue;
         const double a = v[j] * dhat[i] * dhat[c];
         r += fabs(a);
         s += a;
@@ -496,6 +496,7 @@ struct _clsparseAmdFsai
 {
     int device = 0;
     uint32_t n = 0, nnzG = 0, nnzA = 0;
+    bool full_diag = true;  // every row of A stores its diagonal (else G is singular)
     // Pattern of G (CSR, lower) and of G^T (CSR, upper); per-G-nonzero row,
     // column, source index in A and position in G^T.
     uint32_t *g_rp = nullptr, *g_ci = nullptr, *gt_rp = nullptr, *gt_ci = nullptr;
@@ -649,6 +650,7 @@ extern "C" clsparseStatus clsparseAmdFsaiCreate(clsparseAmdFsai* out, const clsp
     CLS_TRY(check_init_control(control));
     if (out == nullptr || A == nullptr) return clsparseInvalidValue;
     if (A->num_rows != A->num_cols) return clsparseInvalidSystemSize;
+    if (A->num_rows == 0) return clsparseInvalidSize;
     const uint32_t n = A->num_rows, nnzA = A->num_nonzeros;
     std::vector<uint32_t> rp(n + 1), ci(nnzA);
     hipStream_t st = control->stream;
@@ -663,10 +665,13 @@ extern "C" clsparseStatus clsparseAmdFsaiCreate(clsparseAmdFsai* out, const clsp
 
     std::vector<uint32_t> grp(n + 1, 0), gci, grow, gsrc;
     gci.reserve(nnzA / 2 + n);
+    bool full_diag = true;
     for (uint32_t r = 0; r < n; ++r) {
+        bool diag = false;
         for (uint32_t k = rp[r]; k < rp[r + 1]; ++k) {
             const uint32_t c = ci[k];
             if (c >= n) return clsparseInvalidValue;
+            diag = diag || (c == r);
             if (c <= r) {
                 gci.push_back(c);
                 grow.push_back(r);
@@ -674,6 +679,7 @@ extern "C" clsparseStatus clsparseAmdFsaiCreate(clsparseAmdFsai* out, const clsp
             }
         }
         grp[r + 1] = static_cast<uint32_t>(gci.size());
+        full_diag = full_diag && diag;
     }
     const uint32_t nnzG = static_cast<uint32_t>(gci.size());
     // G^T by counting sort on the column; tpos[k] = position of G's k in G^T.
@@ -694,6 +700,7 @@ extern "C" clsparseStatus clsparseAmdFsaiCreate(clsparseAmdFsai* out, const clsp
     f->device = control->device;
     f->n = n;
     f->nnzG = nnzG;
+    f->full_diag = full_diag;
     hipError_t e = hipSuccess;
     auto up = [&](uint32_t** d, const std::vector<uint32_t>& h) {
         if (e == hipSuccess) e = dmalloc(d, h.size());
@@ -759,6 +766,13 @@ extern "C" clsparseStatus clsparseAmdFsaiUpdateValues(clsparseAmdFsai f, const c
     if (f == nullptr || A == nullptr) return clsparseInvalidValue;
     if (A->num_rows != f->n || A->num_cols != f->n || A->num_nonzeros != f->nnzA)
         return clsparseInvalidSystemSize;
+    // The kernels walk A's rows: the row pointer must stay inside the arrays.
+    std::vector<uint32_t> rp(f->n + 1);
+    CLS_HIP_TRY(hipMemcpyAsync(rp.data(), dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer),
+                               (f->n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, control->stream));
+    CLS_HIP_TRY(hipStreamSynchronize(control->stream));
+    for (uint32_t r = 0; r < f->n; ++r)
+        if (rp[r + 1] < rp[r] || rp[r + 1] > f->nnzA) return clsparseInvalidValue;
     CLS_TRY(compute_features(f, A, control->stream));
     return fsai_assemble(f, nullptr, control->stream);
 }
@@ -774,6 +788,10 @@ namespace {
 
 clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_t st)
 {
+    if (f->nnzG == 0) {
+        f->valid = false;
+        return clsparseSuccess;
+    }
     if (net) {
         hipLaunchKernelGGL(mlp_forward_kernel, dim3(kMlpGrid), dim3(64 * kMlpWaves), 0, st, net->dev(),
                            f->X, f->nnzG, f->o);
@@ -788,7 +806,7 @@ clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_
     int bad = 0;
     CLS_HIP_TRY(hipMemcpyAsync(&bad, f->bad, sizeof(int), hipMemcpyDeviceToHost, st));
     CLS_HIP_TRY(hipStreamSynchronize(st));
-    f->valid = (bad == 0);
+    f->valid = (bad == 0) && f->full_diag;
     return clsparseSuccess;
 }
 
@@ -859,7 +877,8 @@ extern "C" clsparseStatus clsparseAmdFsaiTrainStep(clsparseAmdFsai f, clsparseAm
 {
     CLS_TRY(check_init_control(control));
     if (f == nullptr || net == nullptr || A == nullptr || probes == 0) return clsparseInvalidValue;
-    if (A->num_rows != f->n) return clsparseInvalidSystemSize;
+    if (A->num_rows != f->n || A->num_cols != f->n) return clsparseInvalidSystemSize;
+    if (f->nnzG == 0) return clsparseInvalidValue;
     CLS_TRY(ensure_train_scratch(f));
     hipStream_t st = control->stream;
     CLS_TRY(fsai_assemble(f, net, st));

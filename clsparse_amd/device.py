@@ -158,5 +158,107 @@ class SolverControl:
         self.handle = None
 
 
+class Mlp:
+    """The per-nonzero network of the learned FSAI (clsparseAmdMlp)."""
+
+    def __init__(self, seed: int = 1):
+        self.handle = ctypes.c_void_p()
+        check("clsparseAmdMlpCreate", load().clsparseAmdMlpCreate(byref(self.handle), seed))
+
+    @staticmethod
+    def num_params() -> int:
+        return int(load().clsparseAmdMlpNumParams())
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.num_params(), dtype=np.float32)
+        check("clsparseAmdMlpGetParams", load().clsparseAmdMlpGetParams(
+            self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.num_params():
+            raise ValueError("wrong parameter count")
+        check("clsparseAmdMlpSetParams", load().clsparseAmdMlpSetParams(
+            self.handle, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+
+    def grad(self) -> np.ndarray:
+        out = np.zeros(self.num_params(), dtype=np.float32)
+        check("clsparseAmdMlpGetGrad", load().clsparseAmdMlpGetGrad(
+            self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def release(self) -> None:
+        if self.handle is not None and self.handle.value:
+            load().clsparseAmdMlpDestroy(self.handle)
+        self.handle = None
+
+
+class Fsai:
+    """Learned FSAI factor G of an FP64 matrix (clsparseAmdFsai)."""
+
+    def __init__(self, A: CsrMatrix, control: Control):
+        self.handle = ctypes.c_void_p()
+        check("clsparseAmdFsaiCreate",
+              load().clsparseAmdFsaiCreate(byref(self.handle), byref(A.s), control.handle))
+        self.n = A.num_rows
+
+    def update_values(self, A: CsrMatrix, control: Control) -> None:
+        check("clsparseAmdFsaiUpdateValues",
+              load().clsparseAmdFsaiUpdateValues(self.handle, byref(A.s), control.handle))
+
+    def apply_net(self, net: Mlp | None, control: Control) -> None:
+        h = net.handle if net is not None else None
+        check("clsparseAmdFsaiApplyNet", load().clsparseAmdFsaiApplyNet(self.handle, h,
+                                                                         control.handle))
+
+    def nnz(self) -> int:
+        k = ctypes.c_uint32(0)
+        check("clsparseAmdFsaiGetValues",
+              load().clsparseAmdFsaiGetValues(self.handle, None, None, None, byref(k)))
+        return int(k.value)
+
+    def values(self):
+        """(row_pointer, col_indices, values) of G on the host."""
+        k = self.nnz()
+        rp = np.zeros(self.n + 1, dtype=np.uint32)
+        ci = np.zeros(max(k, 1), dtype=np.uint32)
+        va = np.zeros(max(k, 1), dtype=np.float64)
+        P = ctypes.POINTER
+        check("clsparseAmdFsaiGetValues", load().clsparseAmdFsaiGetValues(
+            self.handle, rp.ctypes.data_as(P(ctypes.c_uint32)),
+            ci.ctypes.data_as(P(ctypes.c_uint32)), va.ctypes.data_as(P(ctypes.c_double)), None))
+        return rp, ci[:k], va[:k]
+
+    def features(self) -> np.ndarray:
+        """BF16 bit patterns [nnz][16]."""
+        out = np.zeros((max(self.nnz(), 1), 16), dtype=np.uint16)
+        check("clsparseAmdFsaiGetFeatures", load().clsparseAmdFsaiGetFeatures(
+            self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))))
+        return out[: self.nnz()]
+
+    def output(self) -> np.ndarray:
+        out = np.zeros(max(self.nnz(), 1), dtype=np.float32)
+        check("clsparseAmdFsaiGetOutput", load().clsparseAmdFsaiGetOutput(
+            self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out[: self.nnz()]
+
+    def train_step(self, net: Mlp, A: CsrMatrix, control: Control, probes: int = 8,
+                   seed: int = 1, lr: float = 0.0, want_grad: bool = False):
+        """Returns (loss, dL/dG per nonzero of G or None)."""
+        loss = ctypes.c_double(0.0)
+        g = np.zeros(max(self.nnz(), 1), dtype=np.float64) if want_grad else None
+        gp = g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if want_grad else None
+        check("clsparseAmdFsaiTrainStep", load().clsparseAmdFsaiTrainStep(
+            self.handle, net.handle, byref(A.s), probes, seed, lr, control.handle, byref(loss),
+            gp))
+        return loss.value, (g[: self.nnz()] if want_grad else None)
+
+    def release(self) -> None:
+        if self.handle is not None and self.handle.value:
+            load().clsparseAmdFsaiDestroy(self.handle)
+        self.handle = None
+
+
 __all__ = ["Control", "Scalar", "DenseVector", "CsrMatrix", "SolverControl", "ClsparseError",
-           "NOPRECOND", "DIAGONAL"]
+           "NOPRECOND", "DIAGONAL", "Mlp", "Fsai"]

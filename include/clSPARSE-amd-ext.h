@@ -56,20 +56,24 @@ typedef struct clsparseAmdSolverStats_
 CLSPARSE_EXPORT clsparseStatus
     clsparseAmdGetSolverStats(clSParseSolverControl solverControl, clsparseAmdSolverStats* out);
 
-/* Mixed-precision iterative refinement with Jacobi-PCG (no reference
- * counterpart). A, x and b are FP64. The inner solves run in FP32 values with
- * FP64 dot/norm reductions; the outer loop forms r = b - A x with the FP64
- * CSR-Adaptive SpMV and updates x in FP64. Stops when the FP64 true relative
- * residual ||b - A x||_2 / ||b||_2 <= relTol of the solver control, or when
- * maxIters inner iterations in total have run. The preconditioner field of the
- * solver control selects DIAGONAL or NOPRECOND for the inner solves. */
+/* Mixed-precision Jacobi-PCG with FP64 residual replacement (no reference
+ * counterpart). A, x and b are FP64. One CG run keeps A's values and the
+ * vectors r, z, p, A p in FP32 and accumulates every dot product and norm in
+ * FP64; the solution x is updated in FP64. Whenever the recurrence residual
+ * has fallen tenfold since the last replacement, or claims convergence, the
+ * residual is replaced by the FP64 true residual b - A x (FP64 CSR-Adaptive
+ * SpMV) and the search direction is re-based on it, without a restart
+ * (reliable updates; CLSPARSE_AMD_RELIABLE_DELTA sets the factor). Stops when
+ * the FP64 true relative residual ||b - A x||_2 / ||b||_2 <= relTol of the
+ * solver control, or when maxIters iterations have run. The preconditioner
+ * field of the solver control selects DIAGONAL or NOPRECOND. Statistics:
+ * iterations, residual replacements (outer_iterations), true residual. */
 CLSPARSE_EXPORT clsparseStatus
     clsparseAmdDcsrcgMixed(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
                            clSParseSolverControl solverControl, clsparseControl control);
 
-/* The same refinement loop with FP64 inner solves: FP64 Jacobi-PCG judged by
- * the FP64 true relative residual (the baseline the mixed solver is set
- * beside). */
+/* The same solver with FP64 values and vectors: FP64 PCG judged by the FP64
+ * true relative residual (the baseline the mixed solver is set beside). */
 CLSPARSE_EXPORT clsparseStatus
     clsparseAmdDcsrcgRefined(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
                              clSParseSolverControl solverControl, clsparseControl control);
@@ -125,13 +129,13 @@ CLSPARSE_EXPORT clsparseStatus
                              cl_uint probes, cl_ulong seed, cl_float lr, clsparseControl control,
                              cl_double* loss, cl_double* grad_g);
 
-/* Mixed-precision refinement (as clsparseAmdDcsrcgMixed) whose FP32 inner
- * solves are preconditioned with M = G G^T (two CSR-Adaptive SpMVs per
- * iteration). Falls back to Jacobi for the rest of the solve when G holds a
- * non-finite value, when an inner solve returns a non-finite residual, reduces
- * its residual less than twofold or exhausts its iterations, or when the FP64
- * true residual does not decrease over an outer step. The FP64 true residual
- * decides convergence. */
+/* The mixed-precision solver of clsparseAmdDcsrcgMixed with M = G G^T
+ * (two CSR-Adaptive SpMVs per iteration) in place of Jacobi. Falls back to
+ * Jacobi for the rest of the solve when G holds a non-finite value or a
+ * missing diagonal, when the recurrence residual turns non-finite (x is
+ * restored to the last replacement point), or when it fails to halve within
+ * 16 chunks of 32 iterations. The FP64 true residual decides convergence;
+ * stats.preconditioner reports 2 (FSAI throughout) or 3 (fell back). */
 CLSPARSE_EXPORT clsparseStatus
     clsparseAmdDcsrcgFsai(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
                           clsparseAmdFsai fsai, clSParseSolverControl solverControl,

@@ -215,3 +215,30 @@ def test_mixed_precision_refinement(ctl):
     assert s.true_residual == pytest.approx(true_rel, rel=1e-3)
     assert s.outer_iterations >= 2
     sc.release()
+
+
+def test_mixed_reliable_updates_track_fp64_iterations(ctl):
+    """Residual replacement keeps the FP32 Krylov run going: the mixed solver
+    needs about as many iterations as FP64 PCG for the same FP64 true residual."""
+    say("mixed vs FP64 iteration counts, diffusion 40^3")
+    rp, ci, v, m = cls.gen_diffusion3d(40, 40, 40, sigma=1.0, ax=1.0, ay=1.0, az=30.0, seed=11)
+    b = cls.gen_uniform(m, -1, 1, seed=12)
+    A = cls.CsrMatrix(rp, ci, v, m)
+    A.meta_create(ctl)
+    lib = cls.load()
+    bt = torch.tensor(b, device="cuda")
+    res = {}
+    for name, fn in (("mixed", lib.clsparseAmdDcsrcgMixed), ("fp64", lib.clsparseAmdDcsrcgRefined)):
+        x = torch.zeros(m, dtype=torch.float64, device="cuda")
+        sc = cls.SolverControl(cls.DIAGONAL, 20000, 1e-9, 0.0)
+        xd, bd = cls.DenseVector(x), cls.DenseVector(bt)
+        assert fn(ctypes.byref(xd.s), ctypes.byref(A.s), ctypes.byref(bd.s), sc.handle,
+                  ctl.handle) == 0
+        s = sc.stats()
+        r = b - oracle.csrmv(rp, ci, v, x.cpu().numpy())[0]
+        assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-9
+        assert s.status_detail == 0 and s.preconditioner == 1
+        res[name] = s.iterations
+        sc.release()
+    say(f"iterations mixed {res['mixed']} fp64 {res['fp64']}")
+    assert res["mixed"] <= 1.15 * res["fp64"] + 32

@@ -1,0 +1,260 @@
+"""Learned FSAI on the GPU against the CPU oracle (oracle/fsai.py).
+
+Features, G assembly and the Hutchinson gradient are compared to the oracle on
+the same inputs; the MFMA forward and backward kernels to the oracle's
+BF16-emulating network; the solver path to its own Jacobi fallback; and a
+short training run shows what the network gains over the method without it
+(zero network = Jacobi).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+import clsparse_amd as cls
+from oracle import fsai as of
+
+from gpu_helpers import say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctl():
+    c = cls.Control()
+    yield c
+    c.release()
+
+
+def _aniso(nx, ny, seed, ratio=100.0, theta=0.6, sigma=1.0, corr=4.0):
+    rp, ci, v, m = cls.gen_aniso2d(nx, ny, corr_len=corr, sigma=sigma, ratio=ratio, theta=theta,
+                                   seed=seed)
+    return rp, ci, v, m
+
+
+def _random_params(seed, scale=0.2):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(of.N_PARAMS, dtype=np.float32)
+    p[of.OFF_W1:of.OFF_B1] = rng.normal(scale=0.5, size=of.OFF_B1 - of.OFF_W1)
+    p[of.OFF_B1:of.OFF_W2] = rng.normal(scale=0.1, size=of.N_HID)
+    p[of.OFF_W2:of.OFF_B2] = rng.normal(scale=0.18, size=of.N_HID * of.N_HID)
+    p[of.OFF_B2:of.OFF_W3] = rng.normal(scale=0.1, size=of.N_HID)
+    p[of.OFF_W3:of.OFF_B3] = rng.normal(scale=scale / 8, size=of.N_HID)
+    p[of.OFF_B3] = 0.01
+    return p
+
+
+def _sample_matrix(ctl, nx=24, ny=20, seed=3):
+    rp, ci, v, m = _aniso(nx, ny, seed)
+    A = cls.CsrMatrix(rp, ci, v, m)
+    A.meta_create(ctl)
+    return rp, ci, v, m, A
+
+
+def test_features_and_jacobi_factor_match_oracle(ctl):
+    rp, ci, v, m, A = _sample_matrix(ctl)
+    say("fsai: features 24x20 aniso")
+    f = cls.Fsai(A, ctl)
+    try:
+        X = of.bf16_bits_to_f32(f.features()).reshape(-1, 16)
+        Xo = of.bf16_round(of.features(rp, ci, v))
+        assert X.shape == Xo.shape
+        # identical up to one BF16 step where FP64 summation order moves a row sum
+        diff = np.abs(X - Xo)
+        assert np.all(diff <= 2.0 ** -7 * np.maximum(np.abs(Xo), 1e-30) + 1e-30)
+        assert (diff > 0).mean() < 0.01
+        np.testing.assert_array_equal(X[:, 0:2], Xo[:, 0:2])
+        f.apply_net(None, ctl)
+        grp, gci, gv = f.values()
+        G = of.assemble_g(rp, ci, v)
+        np.testing.assert_array_equal(grp, G.indptr)
+        np.testing.assert_array_equal(gci, G.indices)
+        np.testing.assert_array_equal(gv, G.data)  # 1/sqrt(a_ii): correctly rounded both sides
+    finally:
+        f.release()
+
+
+def test_mlp_forward_mfma_matches_oracle(ctl):
+    rp, ci, v, m, A = _sample_matrix(ctl, 33, 17, seed=5)  # nnz not a multiple of 16
+    net = cls.Mlp(seed=7)
+    f = cls.Fsai(A, ctl)
+    try:
+        p = _random_params(11)
+        net.set_params(p)
+        say("fsai: MLP forward vs oracle")
+        f.apply_net(net, ctl)
+        o = f.output().astype(np.float64)
+        X = of.bf16_bits_to_f32(f.features()).reshape(-1, 16)
+        o_ref = of.mlp_forward(p, X)
+        scale = np.abs(o_ref).max()
+        err = np.abs(o - o_ref)
+        assert err.max() <= 2e-2 * scale
+        assert np.median(err) <= 1e-3 * scale
+        # G assembled from the network output
+        _, _, gv = f.values()
+        G = of.assemble_g(rp, ci, v, o)
+        np.testing.assert_allclose(gv, G.data, rtol=1e-15, atol=0)
+    finally:
+        f.release()
+        net.release()
+
+
+def test_new_network_reproduces_jacobi(ctl):
+    rp, ci, v, m, A = _sample_matrix(ctl)
+    net = cls.Mlp(seed=3)
+    f = cls.Fsai(A, ctl)
+    try:
+        say("fsai: new network = Jacobi")
+        f.apply_net(net, ctl)
+        assert np.all(f.output() == 0)
+        _, _, gv = f.values()
+        np.testing.assert_array_equal(gv, of.assemble_g(rp, ci, v).data)
+    finally:
+        f.release()
+        net.release()
+
+
+def test_hutchinson_loss_gradient_and_backward_match_oracle(ctl):
+    rp, ci, v, m, A = _sample_matrix(ctl, 20, 18, seed=8)
+    n = m
+    net = cls.Mlp(seed=2)
+    f = cls.Fsai(A, ctl)
+    try:
+        p = _random_params(5)
+        net.set_params(p)
+        say("fsai: Hutchinson gradient + MLP backward vs oracle")
+        probes, seed = 8, 77
+        loss, gG = f.train_step(net, A, ctl, probes=probes, seed=seed, lr=0.0, want_grad=True)
+        o = f.output().astype(np.float64)
+        X = of.bf16_bits_to_f32(f.features()).reshape(-1, 16)
+        Acpu = sp.csr_matrix((v, ci, rp), shape=(n, n))
+        G = of.assemble_g(rp, ci, v, o)
+        Z = np.stack([of.rademacher(n, of.probe_seed(seed, k)) for k in range(probes)], axis=1)
+        loss_ref, grad_ref = of.hutchinson(Acpu, G, Z)
+        assert loss == pytest.approx(loss_ref, rel=1e-11)
+        gi, gj, _ = of.tril_pattern(rp, ci)
+        gref = np.asarray(grad_ref[gi, gj]).ravel()
+        np.testing.assert_allclose(gG, gref, rtol=1e-9, atol=1e-12 * np.abs(gref).max())
+        # parameter gradient through the network: gout = dL/dG * dhat_i
+        gout = gG * of.dhat(rp, ci, v)[gi]
+        g_ref = of.mlp_backward(p, X, gout.astype(np.float32))
+        g_dev = net.grad().astype(np.float64)
+        for a, b in [(of.OFF_W1, of.OFF_B1), (of.OFF_B1, of.OFF_W2), (of.OFF_W2, of.OFF_B2),
+                     (of.OFF_B2, of.OFF_W3), (of.OFF_W3, of.OFF_B3), (of.OFF_B3, of.N_PARAMS)]:
+            rel = np.linalg.norm(g_dev[a:b] - g_ref[a:b]) / np.linalg.norm(g_ref[a:b])
+            assert rel < 2e-2, (a, rel)
+    finally:
+        f.release()
+        net.release()
+
+
+def _solve_fsai(ctl, A, f, b, tol=1e-8, fp64_inner=False, max_iters=20000):
+    lib = cls.load()
+    n = A.num_rows
+    xd = cls.DenseVector(torch.zeros(n, dtype=torch.float64, device="cuda"))
+    bd = cls.DenseVector(torch.tensor(b, dtype=torch.float64, device="cuda"))
+    sc = cls.SolverControl(cls.DIAGONAL, max_iters, tol, 0.0)
+    try:
+        fn = lib.clsparseAmdDcsrcgFsai64 if fp64_inner else lib.clsparseAmdDcsrcgFsai
+        cls.check("clsparseAmdDcsrcgFsai", fn(ctypes.byref(xd.s), ctypes.byref(A.s),
+                                               ctypes.byref(bd.s), f.handle, sc.handle,
+                                               ctl.handle))
+        return sc.stats(), xd.t.cpu().numpy()
+    finally:
+        sc.release()
+
+
+def _solve_jacobi_mixed(ctl, A, b, tol=1e-8, max_iters=20000):
+    lib = cls.load()
+    n = A.num_rows
+    xd = cls.DenseVector(torch.zeros(n, dtype=torch.float64, device="cuda"))
+    bd = cls.DenseVector(torch.tensor(b, dtype=torch.float64, device="cuda"))
+    sc = cls.SolverControl(cls.DIAGONAL, max_iters, tol, 0.0)
+    try:
+        cls.check("clsparseAmdDcsrcgMixed", lib.clsparseAmdDcsrcgMixed(
+            ctypes.byref(xd.s), ctypes.byref(A.s), ctypes.byref(bd.s), sc.handle, ctl.handle))
+        return sc.stats(), xd.t.cpu().numpy()
+    finally:
+        sc.release()
+
+
+def test_fsai_solver_with_jacobi_factor_matches_jacobi(ctl):
+    rp, ci, v, m, A = _sample_matrix(ctl, 40, 40, seed=12)
+    b = cls.gen_uniform(m, -1, 1, seed=4)
+    f = cls.Fsai(A, ctl)
+    try:
+        say("fsai: PCG with G = D^-1/2 vs Jacobi PCG")
+        st_f, x_f = _solve_fsai(ctl, A, f, b)
+        st_j, x_j = _solve_jacobi_mixed(ctl, A, b)
+        assert st_f.status_detail == 0 and st_j.status_detail == 0
+        assert st_f.preconditioner == 2 and st_j.preconditioner == 1
+        assert abs(st_f.iterations - st_j.iterations) <= max(3, st_j.iterations // 50)
+        Acpu = sp.csr_matrix((v, ci, rp), shape=(m, m))
+        assert np.linalg.norm(b - Acpu @ x_f) / np.linalg.norm(b) <= 1e-8
+    finally:
+        f.release()
+
+
+def test_non_finite_factor_falls_back_to_jacobi(ctl):
+    rp, ci, v, m, A = _sample_matrix(ctl, 30, 30, seed=13)
+    b = cls.gen_uniform(m, -1, 1, seed=5)
+    net = cls.Mlp(seed=1)
+    f = cls.Fsai(A, ctl)
+    try:
+        p = _random_params(3)
+        p[of.OFF_B3] = np.inf
+        net.set_params(p)
+        say("fsai: non-finite factor -> Jacobi")
+        f.apply_net(net, ctl)
+        st, x = _solve_fsai(ctl, A, f, b)
+        assert st.preconditioner == 3 and st.status_detail == 0
+        Acpu = sp.csr_matrix((v, ci, rp), shape=(m, m))
+        assert np.linalg.norm(b - Acpu @ x) / np.linalg.norm(b) <= 1e-8
+    finally:
+        f.release()
+        net.release()
+
+
+def test_training_lowers_loss_and_beats_the_untrained_method(ctl):
+    """Trains on 6 seeded matrices, evaluates on a held-out one: the learned G
+    lowers the Hutchinson loss and the PCG iteration count below the same
+    solver with a zero network (= Jacobi)."""
+    net = cls.Mlp(seed=21)
+    train = []
+    for s in range(6):
+        rp, ci, v, m = _aniso(48, 48, seed=100 + s, ratio=float(10 ** (1 + s % 3)),
+                              theta=0.5 * s)
+        A = cls.CsrMatrix(rp, ci, v, m)
+        A.meta_create(ctl)
+        train.append((A, cls.Fsai(A, ctl)))
+    rp, ci, v, m = _aniso(48, 48, seed=999, ratio=100.0, theta=1.1)
+    Ah = cls.CsrMatrix(rp, ci, v, m)
+    Ah.meta_create(ctl)
+    fh = cls.Fsai(Ah, ctl)
+    try:
+        say("fsai: training 6 matrices x 40 epochs")
+        loss0, _ = fh.train_step(net, Ah, ctl, probes=8, seed=5, lr=0.0)
+        for ep in range(40):
+            for k, (A, f) in enumerate(train):
+                f.train_step(net, A, ctl, probes=8, seed=1000 * ep + k, lr=2e-3)
+        loss1, _ = fh.train_step(net, Ah, ctl, probes=8, seed=5, lr=0.0)
+        say(f"fsai: held-out loss {loss0:.4f} -> {loss1:.4f}")
+        assert loss1 < 0.8 * loss0
+        b = cls.gen_uniform(m, -1, 1, seed=6)
+        fh.apply_net(net, ctl)
+        st_l, x_l = _solve_fsai(ctl, Ah, fh, b)
+        fh.apply_net(None, ctl)
+        st_0, _ = _solve_fsai(ctl, Ah, fh, b)
+        say(f"fsai: held-out iterations learned {st_l.iterations} vs zero network "
+            f"{st_0.iterations}")
+        assert st_l.status_detail == 0 and st_l.preconditioner == 2
+        assert st_l.iterations < 0.9 * st_0.iterations
+        Acpu = sp.csr_matrix((v, ci, rp), shape=(m, m))
+        assert np.linalg.norm(b - Acpu @ x_l) / np.linalg.norm(b) <= 1e-8
+    finally:
+        for _, f in train:
+            f.release()
+        fh.release()
+        net.release()
