@@ -18,6 +18,12 @@ Other modes (diagnostics, not the bench line):
   --mode locality  the same power-law rows with columns folded into smaller windows
                    (x-gather locality sweep), and the 128^3 stencil SpMV in FP64/FP32
   --mode cg        only the solver workload
+  --mode fsai      only the learned-FSAI workload: held-out 3-D anisotropic diffusion
+                   on 160^3 (z faces x100, kappa ~ LogNormal(0, 1)) solved to an FP64 true
+                   relative residual of 1e-8 by learned-FSAI mixed PCG, mixed Jacobi-PCG and
+                   FP64 Jacobi-PCG, with the committed trained network
+  --mode fsai-train  trains the network on the seeded 2-D anisotropic family (512^2,
+                   2000 matrices, the last 200 held out) and writes the weights
 """
 import argparse
 import ctypes
@@ -146,6 +152,103 @@ def solver_workload(cls, torch, ctl, n1d=128, tol=1e-10):
     return out
 
 
+def fsai_workload(cls, torch, ctl, n1d=160, tol=1e-8, weights=None):
+    """Learned-FSAI mixed PCG vs Jacobi (mixed, FP64) on the held-out 3-D problem."""
+    import numpy as np
+
+    from clsparse_amd import fsai as fs
+    path = weights or fs.DEFAULT_WEIGHTS
+    if not os.path.exists(path):
+        return {"error": f"no trained weights at {os.path.relpath(path, ROOT)}"}
+    params, rec = fs.load_weights(path)
+    t0 = time.time()
+    rp, ci, v, m = fs.gen_target3d(n1d)
+    b = cls.gen_uniform(m, -1.0, 1.0, seed=41)
+    log(f"fsai target generated n={m} nnz={int(rp[-1])} in {time.time() - t0:.1f}s")
+    A = cls.CsrMatrix(rp, ci, v, m)
+    A.meta_create(ctl)
+    bt = torch.tensor(b, device="cuda")
+    lib = cls.load()
+    net = cls.Mlp(seed=1)
+    net.set_params(params)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f = cls.Fsai(A, ctl)
+    t1 = time.perf_counter()
+    f.apply_net(net, ctl)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    out = {"workload": f"diffusion3d_{n1d}^3_az100_lognormal1", "n": m, "nnz": int(rp[-1]),
+           "nnz_G": f.nnz(), "tolerance": tol, "weights_sha256": rec["sha256_float32_le"][:16],
+           "fsai_pattern_setup_s": round(t1 - t0, 4), "fsai_network_apply_s": round(t2 - t1, 5)}
+
+    def solve(fn, with_fsai):
+        best = None
+        for _ in range(2):  # the second solve is timed (the first captures graphs)
+            x = torch.zeros(m, dtype=torch.float64, device="cuda")
+            sc = cls.SolverControl(cls.DIAGONAL, 50000, tol, 0.0)
+            xd, bd = cls.DenseVector(x), cls.DenseVector(bt)
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            if with_fsai:
+                st = fn(ctypes.byref(xd.s), ctypes.byref(A.s), ctypes.byref(bd.s), f.handle,
+                        sc.handle, ctl.handle)
+            else:
+                st = fn(ctypes.byref(xd.s), ctypes.byref(A.s), ctypes.byref(bd.s), sc.handle,
+                        ctl.handle)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - ts
+            s = sc.stats()
+            sc.release()
+            cls.check("solve", st)
+            best = {"seconds": round(dt, 5), "iterations": s.iterations,
+                    "replacements": s.outer_iterations, "true_rel_residual": s.true_residual,
+                    "converged": s.status_detail == 0, "preconditioner": s.preconditioner}
+        return best
+
+    out["learned_fsai_mixed"] = solve(lib.clsparseAmdDcsrcgFsai, True)
+    log(f"learned FSAI mixed: {out['learned_fsai_mixed']}")
+    out["jacobi_mixed"] = solve(lib.clsparseAmdDcsrcgMixed, False)
+    log(f"Jacobi mixed: {out['jacobi_mixed']}")
+    out["fp64_jacobi_pcg"] = solve(lib.clsparseAmdDcsrcgRefined, False)
+    log(f"FP64 Jacobi: {out['fp64_jacobi_pcg']}")
+    f.apply_net(None, ctl)  # zero network: the same solver with G = D^-1/2
+    out["untrained_fsai_mixed"] = solve(lib.clsparseAmdDcsrcgFsai, True)
+    log(f"untrained FSAI mixed: {out['untrained_fsai_mixed']}")
+    base = out["fp64_jacobi_pcg"]["seconds"]
+    out["speedup_learned_fsai_vs_fp64_jacobi"] = round(base / out["learned_fsai_mixed"]["seconds"], 3)
+    out["speedup_jacobi_mixed_vs_fp64_jacobi"] = round(base / out["jacobi_mixed"]["seconds"], 3)
+    out["iteration_ratio_learned_vs_untrained"] = round(
+        out["learned_fsai_mixed"]["iterations"] / out["untrained_fsai_mixed"]["iterations"], 3)
+    f.release()
+    net.release()
+    A.meta_delete()
+    return out
+
+
+def fsai_train(cls, torch, ctl, args):
+    from clsparse_amd import fsai as fs
+    n_total, held = 2000, 200
+    train_ids = list(range(min(args.train_matrices, n_total - held)))
+    held_ids = list(range(n_total - held, n_total))[: args.eval_matrices]
+    net = cls.Mlp(seed=args.seed)
+    log(f"held-out Jacobi loss on {len(held_ids)} matrices")
+    loss_jac = fs.evaluate(None, ctl, held_ids)
+    rec = fs.train(net, ctl, train_ids, steps_per_matrix=args.steps_per_matrix, lr=args.lr,
+                   log=log, time_budget_s=args.budget)
+    loss_net = fs.evaluate(net, ctl, held_ids)
+    params = net.params()
+    record = {"family": "aniso2d 512^2, corr 16, sigma 1, log10 ratio U[0,3], theta U[0,pi)",
+              "train": rec, "held_out_ids": [held_ids[0], held_ids[-1]] if held_ids else [],
+              "held_out_loss_jacobi_mean": float(sum(loss_jac) / max(len(loss_jac), 1)),
+              "held_out_loss_learned_mean": float(sum(loss_net) / max(len(loss_net), 1)),
+              "seed": args.seed}
+    digest = fs.save_weights(args.weights or fs.DEFAULT_WEIGHTS, params, record)
+    record["sha256_float32_le"] = digest
+    net.release()
+    return record
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,7 +256,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-solver", action="store_true", help="skip the solver workload")
-    ap.add_argument("--mode", default="spmv", choices=["spmv", "copy", "locality", "cg"])
+    ap.add_argument("--mode", default="spmv",
+                    choices=["spmv", "copy", "locality", "cg", "fsai", "fsai-train"])
+    ap.add_argument("--no-fsai", action="store_true", help="skip the learned-FSAI workload")
+    ap.add_argument("--weights", default=None, help="learned-FSAI weights (.npy)")
+    ap.add_argument("--train-matrices", type=int, default=1800)
+    ap.add_argument("--eval-matrices", type=int, default=200)
+    ap.add_argument("--steps-per-matrix", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--budget", type=float, default=None, help="fsai-train time budget (s)")
     ap.add_argument("--set", default="21,18,15,12,s64,s32",
                     help="locality mode: power-law column windows (bits) and stencils (s64, s32)")
     args = ap.parse_args()
@@ -191,6 +303,15 @@ def main():
 
     if args.mode == "cg":
         print(json.dumps({"mode": "cg", "solver": solver_workload(cls, torch, ctl)}))
+        ctl.release()
+        return
+    if args.mode == "fsai":
+        print(json.dumps({"mode": "fsai", "fsai": fsai_workload(cls, torch, ctl,
+                                                                 weights=args.weights)}))
+        ctl.release()
+        return
+    if args.mode == "fsai-train":
+        print(json.dumps({"mode": "fsai-train", "record": fsai_train(cls, torch, ctl, args)}))
         ctl.release()
         return
 
@@ -260,6 +381,8 @@ def main():
     extra = {}
     if not args.no_solver:
         extra["solver"] = solver_workload(cls, torch, ctl)
+    if not args.no_fsai:
+        extra["fsai"] = fsai_workload(cls, torch, ctl, weights=args.weights)
 
     out = {
         "metric": METRIC,

@@ -88,7 +88,8 @@ class clsparseAmdSolverStats(Structure):
                 ("seconds", c_double), ("status_detail", c_int32), ("preconditioner", c_int32),
                 ("n_iters", c_int32), ("max_iters", c_int32),
                 ("relative_tolerance", c_double), ("absolute_tolerance", c_double),
-                ("initial_residual", c_double), ("current_residual", c_double)]
+                ("initial_residual", c_double), ("current_residual", c_double),
+                ("pilot_rate_fsai", c_double), ("pilot_rate_jacobi", c_double)]
 
 
 class clsparseAmdHostCsr(Structure):

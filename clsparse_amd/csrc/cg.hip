@@ -655,6 +655,15 @@ extern "C" clsparseStatus clsparseDcsrcg(cldenseVector* x, const clsparseCsrMatr
 namespace {
 
 constexpr int kStagnationChunks = 16;  // learned FSAI must halve ||r|| within this many chunks
+constexpr int kPilotChunks = 4;        // chunks per preconditioner in the pilot race
+
+// CLSPARSE_AMD_FSAI_POLICY=always keeps the learned factor unless it breaks
+// down or stagnates; the default ("race") also runs the pilot race.
+bool fsai_race()
+{
+    const char* e = std::getenv("CLSPARSE_AMD_FSAI_POLICY");
+    return !(e && std::strcmp(e, "always") == 0);
+}
 
 // r = (T) r64
 template <typename T>
@@ -844,6 +853,7 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     int replacements = 0;
     int detail = 1;
     int iters = 0;
+    double rate_fsai_out = 0.0, rate_jac_out = 0.0;
     if (rel <= sc->relativeTolerance) {
         detail = 0;
     } else {
@@ -853,6 +863,35 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
         double best_res = rel;
         int chunks_since_best = 0;
         int iter_at_chunk = 0;
+        // Pilot race of the learned factor against Jacobi: kPilotChunks chunks
+        // of each (FSAI first, then Jacobi from where FSAI left x); the rate
+        // -ln(res_end / res_start) / seconds is measured over all chunks but
+        // the first of each (which captures the graph). The faster one goes on.
+        enum Phase { kPilotFsai, kPilotJacobi, kMain };
+        Phase phase = (eng->prec() == kPrecFsai && fsai_race()) ? kPilotFsai : kMain;
+        int phase_chunks = 0;
+        double phase_res0 = 0.0, rate_fsai = 0.0, rate_jac = 0.0;
+        auto phase_t0 = std::chrono::steady_clock::now();
+        // Switches to engine e from the current x (true residual, restart);
+        // *over is set when the solve has ended (converged or out of iterations).
+        auto switch_to = [&](CgEngine<TI, double>* e, bool* over) -> clsparseStatus {
+            eng = e;
+            CLS_TRY(true_residual(&rel));
+            *over = true;
+            if (rel <= sc->relativeTolerance) {
+                detail = 0;
+                return clsparseSuccess;
+            }
+            if (iters >= sc->maxIters) return clsparseSuccess;
+            *over = false;
+            CLS_TRY(restart());
+            res_at_replace = rel;
+            best_res = rel;
+            chunks_since_best = 0;
+            iter_at_chunk = hs->iter;
+            phase_chunks = 0;
+            return clsparseSuccess;
+        };
         for (;;) {
             CLS_TRY(eng->run_chunk(hs));
             const int ran = hs->iter - iter_at_chunk;
@@ -860,6 +899,34 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
             iters = hs->iter;
             const double cur = hs->cur_res;
             const bool finite = std::isfinite(cur);
+            if (phase != kMain && finite && !hs->done) {
+                const auto now = std::chrono::steady_clock::now();
+                if (++phase_chunks == 1) {
+                    phase_res0 = cur;
+                    phase_t0 = now;
+                } else if (phase_chunks == kPilotChunks) {
+                    const double dt = std::chrono::duration<double>(now - phase_t0).count();
+                    const double rate = dt > 0 ? std::log(phase_res0 / cur) / dt : 0.0;
+                    bool over = false;
+                    if (phase == kPilotFsai) {
+                        rate_fsai = rate;
+                        phase = kPilotJacobi;
+                        used = 3;
+                        CLS_TRY(switch_to(&eng_jac, &over));
+                    } else {
+                        rate_jac = rate;
+                        phase = kMain;
+                        if (rate_fsai > rate_jac) {
+                            used = 2;
+                            CLS_TRY(switch_to(&eng_main, &over));
+                        }
+                    }
+                    if (over) break;
+                    continue;
+                }
+            } else if (phase != kMain && !finite) {
+                phase = kMain;  // the stagnation / breakdown rule below takes over
+            }
             // Breakdown or stagnation of the learned preconditioner: back to
             // the last saved x and on with Jacobi.
             if (eng->prec() == kPrecFsai) {
@@ -873,17 +940,11 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
                     if (!finite)
                         CLS_HIP_TRY(hipMemcpyAsync(xp, xsave, n * sizeof(double),
                                                    hipMemcpyDeviceToDevice, s));
-                    eng = &eng_jac;
                     used = 3;
-                    CLS_TRY(true_residual(&rel));
-                    if (rel <= sc->relativeTolerance) {
-                        detail = 0;
-                        break;
-                    }
-                    if (iters >= sc->maxIters) break;
-                    CLS_TRY(restart());
-                    res_at_replace = rel;
-                    iter_at_chunk = hs->iter;
+                    phase = kMain;
+                    bool over = false;
+                    CLS_TRY(switch_to(&eng_jac, &over));
+                    if (over) break;
                     continue;
                 }
             }
@@ -919,6 +980,8 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
             CLS_HIP_TRY(hipGetLastError());
             res_at_replace = rel;
         }
+        rate_fsai_out = rate_fsai;
+        rate_jac_out = rate_jac;
     }
     CLS_TRY(leave_solver_stream(control));
     const auto t1 = std::chrono::steady_clock::now();
@@ -931,6 +994,8 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     sc->stats.seconds = std::chrono::duration<double>(t1 - t0).count();
     sc->stats.status_detail = detail;
     sc->stats.preconditioner = used;
+    sc->stats.pilot_rate_fsai = rate_fsai_out;
+    sc->stats.pilot_rate_jacobi = rate_jac_out;
     return clsparseSuccess;
 }
 

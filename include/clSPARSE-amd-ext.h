@@ -51,6 +51,10 @@ typedef struct clsparseAmdSolverStats_
     cl_double absolute_tolerance;
     cl_double initial_residual;
     cl_double current_residual;
+    /* Learned-FSAI solver: residual reduction rate -ln(r_end / r_start) per
+     * second of the pilot race, for the factor and for Jacobi (0 if not run). */
+    cl_double pilot_rate_fsai;
+    cl_double pilot_rate_jacobi;
 } clsparseAmdSolverStats;
 
 CLSPARSE_EXPORT clsparseStatus

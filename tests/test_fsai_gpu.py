@@ -180,7 +180,8 @@ def _solve_jacobi_mixed(ctl, A, b, tol=1e-8, max_iters=20000):
         sc.release()
 
 
-def test_fsai_solver_with_jacobi_factor_matches_jacobi(ctl):
+def test_fsai_solver_with_jacobi_factor_matches_jacobi(ctl, monkeypatch):
+    monkeypatch.setenv("CLSPARSE_AMD_FSAI_POLICY", "always")
     rp, ci, v, m, A = _sample_matrix(ctl, 40, 40, seed=12)
     b = cls.gen_uniform(m, -1, 1, seed=4)
     f = cls.Fsai(A, ctl)
@@ -217,10 +218,11 @@ def test_non_finite_factor_falls_back_to_jacobi(ctl):
         net.release()
 
 
-def test_training_lowers_loss_and_beats_the_untrained_method(ctl):
+def test_training_lowers_loss_and_beats_the_untrained_method(ctl, monkeypatch):
     """Trains on 6 seeded matrices, evaluates on a held-out one: the learned G
     lowers the Hutchinson loss and the PCG iteration count below the same
     solver with a zero network (= Jacobi)."""
+    monkeypatch.setenv("CLSPARSE_AMD_FSAI_POLICY", "always")
     net = cls.Mlp(seed=21)
     train = []
     for s in range(6):
@@ -258,3 +260,21 @@ def test_training_lowers_loss_and_beats_the_untrained_method(ctl):
             f.release()
         fh.release()
         net.release()
+
+
+def test_pilot_race_picks_a_preconditioner_and_converges(ctl, monkeypatch):
+    monkeypatch.delenv("CLSPARSE_AMD_FSAI_POLICY", raising=False)
+    rp, ci, v, m, A = _sample_matrix(ctl, 64, 64, seed=14)
+    b = cls.gen_uniform(m, -1, 1, seed=7)
+    f = cls.Fsai(A, ctl)
+    try:
+        say("fsai: pilot race")
+        st, x = _solve_fsai(ctl, A, f, b)
+        assert st.status_detail == 0 and st.preconditioner in (2, 3)
+        if st.iterations > 4 * 32:
+            assert st.pilot_rate_fsai > 0 and st.pilot_rate_jacobi > 0
+            assert (st.preconditioner == 2) == (st.pilot_rate_fsai > st.pilot_rate_jacobi)
+        Acpu = sp.csr_matrix((v, ci, rp), shape=(m, m))
+        assert np.linalg.norm(b - Acpu @ x) / np.linalg.norm(b) <= 1e-8
+    finally:
+        f.release()

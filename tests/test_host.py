@@ -227,3 +227,28 @@ def test_uniform_generator():
     assert a.min() >= -1.0 and a.max() < 1.0 and abs(a.mean()) < 0.01
     np.testing.assert_array_equal(a, cls.gen_uniform(100000, -1.0, 1.0, seed=11))
     assert not np.array_equal(a, cls.gen_uniform(100000, -1.0, 1.0, seed=12))
+
+
+def test_fsai_weights_roundtrip_and_checksum(tmp_path):
+    from clsparse_amd import fsai as fs
+    n = cls.load().clsparseAmdMlpNumParams()
+    assert n == 64 * 16 + 64 + 64 * 64 + 64 + 64 + 1
+    p = np.random.default_rng(0).normal(size=n).astype(np.float32)
+    path = str(tmp_path / "w.npy")
+    digest = fs.save_weights(path, p, {"note": "test"})
+    q, rec = fs.load_weights(path)
+    np.testing.assert_array_equal(p, q)
+    assert rec["sha256_float32_le"] == digest and rec["note"] == "test"
+    p[7] += 1.0
+    np.save(path, p, allow_pickle=False)
+    with pytest.raises(ValueError):
+        fs.load_weights(path)
+
+
+def test_fsai_training_family_is_seeded():
+    from clsparse_amd import fsai as fs
+    a, b = fs.aniso2d_params(5), fs.aniso2d_params(5)
+    assert a == b and a != fs.aniso2d_params(6)
+    ratios = [fs.aniso2d_params(i)["ratio"] for i in range(400)]
+    assert 1.0 <= min(ratios) and max(ratios) <= 1000.0
+    assert 0.3 < np.mean(np.log10(ratios)) / 1.5 < 1.7
