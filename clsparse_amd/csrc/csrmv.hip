@@ -220,7 +220,19 @@ __global__ __launch_bounds__(64 * kWavesPerWg) void csrmv_adaptive_kernel(
                     const uint32_t s = ro[k];
                     const uint32_t e = (lane < 63) ? nxt : wrap;
                     Acc<T, EXT> acc;
-                    for (uint32_t j = s; j < e; ++j) {
+                    uint32_t j = s;
+                    if constexpr (!EXT) {
+                        // The first 8 terms: independent LDS reads (index
+                        // clamped inside the tile), then adds in row order.
+                        T t8[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) t8[u] = buf[min(s + u, uint32_t(kBlkSize - 1))];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+                            if (s + u < e) acc.add(t8[u]);
+                        j = s + 8;
+                    }
+                    for (; j < e; ++j) {
                         acc.add(buf[j]);
                         if constexpr (EXT) acc.c += lds.e[wave][j];
                     }
