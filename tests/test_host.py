@@ -252,3 +252,11 @@ def test_fsai_training_family_is_seeded():
     ratios = [fs.aniso2d_params(i)["ratio"] for i in range(400)]
     assert 1.0 <= min(ratios) and max(ratios) <= 1000.0
     assert 0.3 < np.mean(np.log10(ratios)) / 1.5 < 1.7
+
+
+def test_committed_fsai_weights_load_and_checksum():
+    from clsparse_amd import fsai as fs
+    p, rec = fs.load_weights()
+    assert p.shape == (cls.load().clsparseAmdMlpNumParams(),) and np.all(np.isfinite(p))
+    assert rec["held_out_loss_learned_mean"] < rec["held_out_loss_jacobi_mean"]
+    assert rec["train"]["matrices"] == 1800
