@@ -32,6 +32,9 @@ clsparseStructInvalid = -1022
 clsparseInvalidSize = -1021
 clsparseInvalidControlObject = -1018
 clsparseInvalidKernelExecution = -1015
+clsparseInvalidMemObj = -1020
+clsparseInvalidFile = -1017
+clsparseInvalidFileFormat = -1016
 clsparseInvalidSolverControlObject = -2048
 clsparseInvalidSystemSize = -2047
 
@@ -132,6 +135,14 @@ _SIGS = {
     "clsparseDcsrmv": (c_int32, [POINTER(clsparseScalar), POINTER(clsparseCsrMatrix),
                                  POINTER(cldenseVector), POINTER(clsparseScalar),
                                  POINTER(cldenseVector), _V]),
+    "clsparseHeaderfromFile": (c_int32, [POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32),
+                                         ctypes.c_char_p]),
+    "clsparseSCooMatrixfromFile": (c_int32, [POINTER(clsparseCooMatrix), ctypes.c_char_p, _V, c_uint32]),
+    "clsparseDCooMatrixfromFile": (c_int32, [POINTER(clsparseCooMatrix), ctypes.c_char_p, _V, c_uint32]),
+    "clsparseSCsrMatrixfromFile": (c_int32, [POINTER(clsparseCsrMatrix), ctypes.c_char_p, _V, c_uint32]),
+    "clsparseDCsrMatrixfromFile": (c_int32, [POINTER(clsparseCsrMatrix), ctypes.c_char_p, _V, c_uint32]),
+    "clsparseAmdCsrMatrixfromFileHost": (c_int32, [ctypes.c_char_p, c_uint32,
+                                                   POINTER(clsparseAmdHostCsr)]),
     "clsparseAmdCsrMetaGetRowBlocks": (c_int32, [POINTER(clsparseCsrMatrix), POINTER(c_uint64),
                                                  c_uint32, POINTER(c_uint32)]),
     "clsparseAmdComputeRowBlocksHost": (c_uint64, [POINTER(c_uint32), c_uint32, POINTER(c_uint64)]),
@@ -265,6 +276,23 @@ def gen_aniso2d(nx: int, ny: int, corr_len: float = 16.0, sigma: float = 1.0,
           load().clsparseAmdGenAnisoDiffusion2D(nx, ny, corr_len, sigma, ratio, theta, seed,
                                                 ctypes.byref(m)))
     return _take_host_csr(m)
+
+
+def read_mtx_host(path: str, read_explicit_zeroes: bool = False):
+    """(row_pointer, col_indices, values f64, num_cols) of a Matrix Market file."""
+    m = clsparseAmdHostCsr()
+    check("clsparseAmdCsrMatrixfromFileHost",
+          load().clsparseAmdCsrMatrixfromFileHost(path.encode(), 1 if read_explicit_zeroes else 0,
+                                                  ctypes.byref(m)))
+    return _take_host_csr(m)
+
+
+def mtx_header(path: str):
+    """(nnz upper bound, rows, cols) from the file header."""
+    nnz, r, c = c_uint32(), c_uint32(), c_uint32()
+    check("clsparseHeaderfromFile", load().clsparseHeaderfromFile(
+        ctypes.byref(nnz), ctypes.byref(r), ctypes.byref(c), path.encode()))
+    return nnz.value, r.value, c.value
 
 
 def gen_uniform(n: int, lo: float = -1.0, hi: float = 1.0, seed: int = 1) -> np.ndarray:

@@ -34,7 +34,8 @@ constexpr int kStreamPerLane = kBlkSize / 64;  // 16 nnz per lane per stream blo
 template <typename T, bool EXT>
 struct StreamLds
 {
-    T p[kWavesPerWg][kBlkSize];
+    T p[kWavesPerWg][kBlkSize];         // products, or alpha * values (row-wise blocks)
+    uint32_t c[kWavesPerWg][kBlkSize];  // columns (row-wise blocks)
 };
 template <typename T>
 struct StreamLds<T, true>
@@ -116,6 +117,14 @@ __device__ __forceinline__ Acc<T, EXT> strided_dot(const T* __restrict__ vals,
     return acc;
 }
 
+// a * b rounded on its own: never contracted into a following add.
+template <typename T>
+__device__ __forceinline__ T mul_rounded(T a, T b)
+{
+#pragma clang fp contract(off)
+    return a * b;
+}
+
 // Row offsets (relative to the block's first nnz) of rows row .. row+255 held
 // in registers: lane l keeps row l + 64k in slot k. Loaded together with the
 // block's values so the reduction needs no dependent global loads.
@@ -165,26 +174,52 @@ __global__ __launch_bounds__(64 * kWavesPerWg) void csrmv_adaptive_kernel(
             }
         }
         T* buf = lds.p[wave];
+        // Blocks of more than 64 short rows (stencils): each lane walks whole
+        // rows, so one gather instruction reads x at the same offset from 64
+        // consecutive rows -- a few cache lines instead of one per row run.
+        // The block's alpha*values and columns are staged in LDS for that.
+        const bool rowwise = !EXT && in_regs && nrows > 64;
         {
+            // 16-byte loads: lane owns elements 4*lane + 256*k .. +3.
             T v[kStreamPerLane];
             uint32_t c[kStreamPerLane];
 #pragma unroll
-            for (int k = 0; k < kStreamPerLane; ++k) {
-                const uint32_t i = k * 64 + lane;
-                if (i < cnt) {
-                    v[k] = __builtin_nontemporal_load(vals + base + i);
-                    c[k] = __builtin_nontemporal_load(cols + base + i);
+            for (int k = 0; k < kStreamPerLane / 4; ++k) {
+                const uint32_t i0 = 256 * k + 4 * lane;
+                if (i0 + 3 < cnt) {
+                    __builtin_memcpy(&v[4 * k], vals + base + i0, 4 * sizeof(T));
+                    __builtin_memcpy(&c[4 * k], cols + base + i0, 4 * sizeof(uint32_t));
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (i0 + u < cnt) {
+                            v[4 * k + u] = vals[base + i0 + u];
+                            c[4 * k + u] = cols[base + i0 + u];
+                        }
                 }
             }
+            if (rowwise) {
+                if constexpr (!EXT) {
 #pragma unroll
-            for (int k = 0; k < kStreamPerLane; ++k) {
-                const uint32_t i = k * 64 + lane;
-                if (i < cnt) {
-                    const T a = alpha * v[k];
-                    const T xv = xg(c[k]);
-                    const T p = a * xv;
-                    buf[i] = p;
-                    if constexpr (EXT) lds.e[wave][i] = fma(a, xv, -p);
+                    for (int k = 0; k < kStreamPerLane; ++k) {
+                        const uint32_t i = 256 * (k / 4) + 4 * lane + (k % 4);
+                        if (i < cnt) {
+                            buf[i] = alpha * v[k];
+                            lds.c[wave][i] = c[k];
+                        }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < kStreamPerLane; ++k) {
+                    const uint32_t i = 256 * (k / 4) + 4 * lane + (k % 4);
+                    if (i < cnt) {
+                        const T a = alpha * v[k];
+                        const T xv = xg(c[k]);
+                        const T p = a * xv;
+                        buf[i] = p;
+                        if constexpr (EXT) lds.e[wave][i] = fma(a, xv, -p);
+                    }
                 }
             }
         }
@@ -209,6 +244,39 @@ __global__ __launch_bounds__(64 * kWavesPerWg) void csrmv_adaptive_kernel(
             }
             acc = group_reduce(acc, tpr);
             if (t == 0 && lr < nrows) write_row<T, EXT>(y, row + lr, acc, beta);
+        } else if (rowwise) {
+            if constexpr (!EXT) {
+                // Each lane walks whole rows: 8 gathers in flight, each
+                // product rounded on its own and added in row order (the
+                // same sums as the product-staging path).
+#pragma unroll
+                for (int k = 0; k < kRowSlots; ++k) {
+                    const uint32_t nxt = __shfl(ro[k], static_cast<int>((lane + 1) & 63), 64);
+                    const uint32_t wrap = (k + 1 < kRowSlots) ? __shfl(ro[k + 1 < kRowSlots ? k + 1 : k], 0, 64) : 0u;
+                    const uint32_t lr = lane + 64 * k;
+                    if (lr < nrows) {
+                        const uint32_t s = ro[k];
+                        const uint32_t e = (lane < 63) ? nxt : wrap;
+                        Acc<T, EXT> acc;
+                        for (uint32_t j = s; j < e; j += 8) {
+                            T pr[8];
+#pragma unroll
+                            for (int u = 0; u < 8; ++u) {
+                                const uint32_t idx = min(j + u, uint32_t(kBlkSize - 1));
+                                const T a = buf[idx];
+                                const uint32_t col = lds.c[wave][idx];
+                                T xv = T(0);
+                                if (j + u < e) xv = xg(col);
+                                pr[u] = mul_rounded(a, xv);
+                            }
+#pragma unroll
+                            for (int u = 0; u < 8; ++u)
+                                if (j + u < e) acc.add(pr[u]);
+                        }
+                        write_row<T, EXT>(y, row + lr, acc, beta);
+                    }
+                }
+            }
         } else if (in_regs) {
             // More rows than lanes: each lane reduces whole rows from LDS.
 #pragma unroll

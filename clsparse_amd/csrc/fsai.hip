@@ -87,9 +87,37 @@ struct MlpDev
     const float* p;  // FP32 master parameters (biases, w3, b3 are read from here)
 };
 
+// Per-lane register copy of the network, loaded once per wave: the MFMA A
+// fragments of W1, W2 (and W2^T for the backward pass) and the lane's slice
+// of b1, b2, w3 (units 16m + 4(l/16) + e).
+struct NetRegs
+{
+    bf16x4 w1[4];
+    bf16x4 w2[4][4];  // [m][kb]
+    float b1[16], b2[16], w3[16];
+    float b3;
+};
+
+__device__ __forceinline__ void load_net(const MlpDev& net, uint32_t lane, NetRegs& w)
+{
+    const uint32_t q = lane >> 4, c16 = lane & 15;
+    for (int m = 0; m < 4; ++m) {
+        w.w1[m] = load_bf16x4(net.W1 + (16 * m + c16) * kFeat + 4 * q);
+        for (int kb = 0; kb < 4; ++kb)
+            w.w2[m][kb] = load_bf16x4(net.W2 + (16 * m + c16) * kHid + 16 * kb + 4 * q);
+        for (int e = 0; e < 4; ++e) {
+            const int u = 16 * m + 4 * q + e;
+            w.b1[4 * m + e] = net.p[kOffB1 + u];
+            w.b2[4 * m + e] = net.p[kOffB2 + u];
+            w.w3[4 * m + e] = net.p[kOffW3 + u];
+        }
+    }
+    w.b3 = net.p[kOffB3];
+}
+
 // Forward for 16 nonzeros [r0, r0+16) held by one wave. Lane l works on
 // nonzero l % 16 and hidden units 16m + 4(l/16) + e of block m (e = 0..3).
-// Returns pre-activations a1, a2 and the BF16 activations h1 (for backward).
+// Keeps pre-activations a1, a2 and the BF16 activations h1 (for backward).
 struct TileFwd
 {
     f32x4 a1[4];
@@ -98,7 +126,7 @@ struct TileFwd
     bf16x4 x;  // B operand X^T (features 4(l/16)..+3 of nonzero l%16)
 };
 
-__device__ __forceinline__ void tile_forward(const MlpDev& net, const uint16_t* __restrict__ X,
+__device__ __forceinline__ void tile_forward(const NetRegs& w, const uint16_t* __restrict__ X,
                                              uint32_t r0, uint32_t nnz, uint32_t lane, TileFwd& t)
 {
     const uint32_t nz = r0 + (lane & 15);
@@ -109,10 +137,9 @@ __device__ __forceinline__ void tile_forward(const MlpDev& net, const uint16_t* 
         t.x = bf16x4{0, 0, 0, 0};
     // Layer 1: A1^T (64 x 16nnz) = W1 (64 x 16) . X^T (16 x 16nnz)
     for (int m = 0; m < 4; ++m) {
-        const bf16x4 w = load_bf16x4(net.W1 + (16 * m + (lane & 15)) * kFeat + 4 * q);
         f32x4 c = {0.f, 0.f, 0.f, 0.f};
-        c = mfma16(w, t.x, c);
-        for (int e = 0; e < 4; ++e) c[e] += net.p[kOffB1 + 16 * m + 4 * q + e];
+        c = mfma16(w.w1[m], t.x, c);
+        for (int e = 0; e < 4; ++e) c[e] += w.b1[4 * m + e];
         t.a1[m] = c;
         f32x4 h;
         for (int e = 0; e < 4; ++e) h[e] = c[e] > 0.f ? c[e] : 0.f;
@@ -121,28 +148,24 @@ __device__ __forceinline__ void tile_forward(const MlpDev& net, const uint16_t* 
     // Layer 2: A2^T (64 x 16) = W2 (64 x 64) . H1^T (64 x 16)
     for (int m = 0; m < 4; ++m) {
         f32x4 c = {0.f, 0.f, 0.f, 0.f};
-        for (int kb = 0; kb < 4; ++kb) {
-            const bf16x4 w = load_bf16x4(net.W2 + (16 * m + (lane & 15)) * kHid + 16 * kb + 4 * q);
-            c = mfma16(w, t.h1[kb], c);
-        }
-        for (int e = 0; e < 4; ++e) c[e] += net.p[kOffB2 + 16 * m + 4 * q + e];
+        for (int kb = 0; kb < 4; ++kb) c = mfma16(w.w2[m][kb], t.h1[kb], c);
+        for (int e = 0; e < 4; ++e) c[e] += w.b2[4 * m + e];
         t.a2[m] = c;
     }
 }
 
 // o = w3 . relu(a2) + b3 for the lane's nonzero (all 4 lanes of a nonzero agree).
-__device__ __forceinline__ float tile_output(const MlpDev& net, const TileFwd& t, uint32_t lane)
+__device__ __forceinline__ float tile_output(const NetRegs& w, const TileFwd& t)
 {
-    const uint32_t q = lane >> 4;
     float s = 0.f;
     for (int m = 0; m < 4; ++m)
         for (int e = 0; e < 4; ++e) {
             const float h = t.a2[m][e] > 0.f ? bf2f(f2bf(t.a2[m][e])) : 0.f;
-            s += net.p[kOffW3 + 16 * m + 4 * q + e] * h;
+            s += w.w3[4 * m + e] * h;
         }
     s += __shfl_xor(s, 16, 64);
     s += __shfl_xor(s, 32, 64);
-    return s + net.p[kOffB3];
+    return s + w.b3;
 }
 
 __global__ __launch_bounds__(64 * kMlpWaves) void mlp_forward_kernel(MlpDev net,
@@ -153,18 +176,21 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_forward_kernel(MlpDev net,
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t tiles = (nnz + 15) / 16;
+    NetRegs w;
+    load_net(net, lane, w);
     for (uint32_t tile = blockIdx.x * kMlpWaves + wave; tile < tiles; tile += gridDim.x * kMlpWaves) {
         TileFwd t;
-        tile_forward(net, X, tile * 16, nnz, lane, t);
-        const float o = tile_output(net, t, lane);
+        tile_forward(w, X, tile * 16, nnz, lane, t);
+        const float o = tile_output(w, t);
         const uint32_t nz = tile * 16 + (lane & 15);
         if (lane < 16 && nz < nnz) out[nz] = o;
     }
 }
 
-// Backward: recompute the tile's forward pass, then accumulate this wave's
-// parameter gradients; one partial gradient vector per wave, reduced later
-// in a fixed order.
+// Backward: recompute the tile's forward pass, then accumulate the wave's
+// parameter gradients; the 4 waves of a workgroup add theirs into one LDS
+// vector in wave order, and each workgroup writes one partial gradient vector
+// (reduced later in a fixed order).
 __global__ __launch_bounds__(64 * kMlpWaves) void mlp_backward_kernel(MlpDev net,
                                                                        const uint16_t* __restrict__ X,
                                                                        const float* __restrict__ gout,
@@ -173,9 +199,17 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_backward_kernel(MlpDev net
 {
     __shared__ uint16_t sh_t[kMlpWaves][kHid][16];   // a 64 x 16 tile, nnz contiguous
     __shared__ uint16_t sh_x[kMlpWaves][kFeat][16];  // X^T tile, nnz contiguous
+    __shared__ float sh_g[kNumParams];               // the workgroup's gradient
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t q = lane >> 4, c16 = lane & 15;
+
+    NetRegs w;
+    load_net(net, lane, w);
+    bf16x4 w2t[4][4];  // A fragments of W2^T: rows = input unit, K = output unit
+    for (int m = 0; m < 4; ++m)
+        for (int kb = 0; kb < 4; ++kb)
+            w2t[m][kb] = load_bf16x4(net.W2T + (16 * m + c16) * kHid + 16 * kb + 4 * q);
 
     f32x4 dW2[4][4];  // block (m = output unit, n = input unit)
     f32x4 dW1[4];     // block m (rows = hidden), 16 features
@@ -190,7 +224,7 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_backward_kernel(MlpDev net
     const uint32_t tiles = (nnz + 15) / 16;
     for (uint32_t tile = blockIdx.x * kMlpWaves + wave; tile < tiles; tile += gridDim.x * kMlpWaves) {
         TileFwd t;
-        tile_forward(net, X, tile * 16, nnz, lane, t);
+        tile_forward(w, X, tile * 16, nnz, lane, t);
         const uint32_t nz = tile * 16 + c16;
         const float g = (nz < nnz) ? gout[nz] : 0.f;
         if (q == 0) db3 += g;
@@ -199,25 +233,21 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_backward_kernel(MlpDev net
         f32x4 d2[4];
         for (int m = 0; m < 4; ++m)
             for (int e = 0; e < 4; ++e) {
-                const int u = 16 * m + 4 * q + e;
                 const bool on = t.a2[m][e] > 0.f;
                 const float h2 = on ? bf2f(f2bf(t.a2[m][e])) : 0.f;
                 dw3[4 * m + e] += g * h2;
-                const float d = on ? g * net.p[kOffW3 + u] : 0.f;
+                const float d = on ? g * w.w3[4 * m + e] : 0.f;
                 d2[m][e] = d;
                 db2[4 * m + e] += d;
             }
         bf16x4 d2b[4];
         for (int m = 0; m < 4; ++m) d2b[m] = pack_bf16x4(d2[m]);
 
-        // D1^T = (W2^T D2^T) * relu'(a1): A = W2T rows (input unit), K = output unit.
+        // D1^T = (W2^T D2^T) * relu'(a1)
         f32x4 d1[4];
         for (int m = 0; m < 4; ++m) {
             f32x4 c = {0.f, 0.f, 0.f, 0.f};
-            for (int kb = 0; kb < 4; ++kb) {
-                const bf16x4 w = load_bf16x4(net.W2T + (16 * m + c16) * kHid + 16 * kb + 4 * q);
-                c = mfma16(w, d2b[kb], c);
-            }
+            for (int kb = 0; kb < 4; ++kb) c = mfma16(w2t[m][kb], d2b[kb], c);
             for (int e = 0; e < 4; ++e) {
                 const float d = t.a1[m][e] > 0.f ? c[e] : 0.f;
                 d1[m][e] = d;
@@ -225,7 +255,7 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_backward_kernel(MlpDev net
             }
         }
 
-        // dW2 += D2^T H1: both operands with the nonzeros in the K slot,
+        // dW2 += D2^T H1 and dW1 += D1^T X: the nonzeros go in the K slot,
         // through LDS ([unit][nnz] BF16 tiles).
         for (int m = 0; m < 4; ++m)
             for (int e = 0; e < 4; ++e) sh_t[wave][16 * m + 4 * q + e][c16] = static_cast<uint16_t>(d2b[m][e]);
@@ -236,13 +266,9 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_backward_kernel(MlpDev net
         for (int m = 0; m < 4; ++m)
             for (int e = 0; e < 4; ++e) sh_t[wave][16 * m + 4 * q + e][c16] = static_cast<uint16_t>(t.h1[m][e]);
         wave_sync_lds();
-        for (int n = 0; n < 4; ++n) {
-            const bf16x4 bH1 = load_bf16x4(&sh_t[wave][16 * n + c16][4 * q]);
-            for (int m = 0; m < 4; ++m) dW2[m][n] = mfma16(aD2[m], bH1, dW2[m][n]);
-        }
+        bf16x4 bH1[4];
+        for (int n = 0; n < 4; ++n) bH1[n] = load_bf16x4(&sh_t[wave][16 * n + c16][4 * q]);
         wave_sync_lds();
-
-        // dW1 += D1^T X
         for (int m = 0; m < 4; ++m) {
             const bf16x4 d1b = pack_bf16x4(d1[m]);
             for (int e = 0; e < 4; ++e) sh_t[wave][16 * m + 4 * q + e][c16] = static_cast<uint16_t>(d1b[e]);
@@ -250,11 +276,12 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_backward_kernel(MlpDev net
         for (int e = 0; e < 4; ++e) sh_x[wave][4 * q + e][c16] = static_cast<uint16_t>(t.x[e]);
         wave_sync_lds();
         const bf16x4 bX = load_bf16x4(&sh_x[wave][c16][4 * q]);
-        for (int m = 0; m < 4; ++m) {
-            const bf16x4 aD1 = load_bf16x4(&sh_t[wave][16 * m + c16][4 * q]);
-            dW1[m] = mfma16(aD1, bX, dW1[m]);
-        }
+        bf16x4 aD1[4];
+        for (int m = 0; m < 4; ++m) aD1[m] = load_bf16x4(&sh_t[wave][16 * m + c16][4 * q]);
         wave_sync_lds();
+        for (int n = 0; n < 4; ++n)
+            for (int m = 0; m < 4; ++m) dW2[m][n] = mfma16(aD2[m], bH1[n], dW2[m][n]);
+        for (int m = 0; m < 4; ++m) dW1[m] = mfma16(aD1[m], bX, dW1[m]);
     }
 
     // Bias-like sums: lanes with the same q hold the same units for 16
@@ -266,32 +293,59 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_backward_kernel(MlpDev net
     }
     db3 = wave_sum(db3);
 
-    float* out = partials + static_cast<size_t>(blockIdx.x * kMlpWaves + wave) * kNumParams;
-    // dW2 block (m, n): lane l holds rows 16m + 4q + e, column 16n + c16.
-    for (int m = 0; m < 4; ++m)
-        for (int n = 0; n < 4; ++n)
-            for (int e = 0; e < 4; ++e)
-                out[kOffW2 + (16 * m + 4 * q + e) * kHid + 16 * n + c16] = dW2[m][n][e];
-    for (int m = 0; m < 4; ++m)
-        for (int e = 0; e < 4; ++e) out[kOffW1 + (16 * m + 4 * q + e) * kFeat + c16] = dW1[m][e];
-    if (c16 == 0)
-        for (int m = 0; m < 4; ++m)
-            for (int e = 0; e < 4; ++e) {
-                out[kOffB1 + 16 * m + 4 * q + e] = db1[4 * m + e];
-                out[kOffB2 + 16 * m + 4 * q + e] = db2[4 * m + e];
-                out[kOffW3 + 16 * m + 4 * q + e] = dw3[4 * m + e];
-            }
-    if (lane == 0) out[kOffB3] = db3;
+    // Waves 0..3 in order: the first stores, the others add.
+    for (uint32_t wv = 0; wv < static_cast<uint32_t>(kMlpWaves); ++wv) {
+        if (wave == wv) {
+            const bool first = (wv == 0);
+            auto put = [&](int idx, float v) { sh_g[idx] = first ? v : sh_g[idx] + v; };
+            // dW2 block (m, n): lane l holds rows 16m + 4q + e, column 16n + c16.
+            for (int m = 0; m < 4; ++m)
+                for (int n = 0; n < 4; ++n)
+                    for (int e = 0; e < 4; ++e)
+                        put(kOffW2 + (16 * m + 4 * q + e) * kHid + 16 * n + c16, dW2[m][n][e]);
+            for (int m = 0; m < 4; ++m)
+                for (int e = 0; e < 4; ++e) put(kOffW1 + (16 * m + 4 * q + e) * kFeat + c16, dW1[m][e]);
+            if (c16 == 0)
+                for (int m = 0; m < 4; ++m)
+                    for (int e = 0; e < 4; ++e) {
+                        put(kOffB1 + 16 * m + 4 * q + e, db1[4 * m + e]);
+                        put(kOffB2 + 16 * m + 4 * q + e, db2[4 * m + e]);
+                        put(kOffW3 + 16 * m + 4 * q + e, dw3[4 * m + e]);
+                    }
+            if (lane == 0) put(kOffB3, db3);
+        }
+        __syncthreads();
+    }
+    float* out = partials + static_cast<size_t>(blockIdx.x) * kNumParams;
+    for (uint32_t i = threadIdx.x; i < static_cast<uint32_t>(kNumParams); i += 64 * kMlpWaves)
+        out[i] = sh_g[i];
 }
 
-// grad[p] = sum over partial vectors (fixed order).
-__global__ __launch_bounds__(256) void grad_reduce_kernel(const float* __restrict__ partials,
-                                                          uint32_t count, float* __restrict__ grad)
+// Two-stage fixed-order sum of `count` partial gradient vectors:
+// stage 1: block (x, c) sums vectors [c*per, (c+1)*per) for 256 parameters;
+// stage 2: the chunk sums in chunk order.
+constexpr uint32_t kGradChunks = 32;
+
+__global__ __launch_bounds__(256) void grad_reduce_stage1(const float* __restrict__ partials,
+                                                          uint32_t count, double* __restrict__ chunk)
+{
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= static_cast<uint32_t>(kNumParams)) return;
+    const uint32_t per = (count + kGradChunks - 1) / kGradChunks;
+    const uint32_t k0 = blockIdx.y * per;
+    const uint32_t k1 = min(count, k0 + per);
+    double s = 0.0;
+    for (uint32_t k = k0; k < k1; ++k) s += partials[static_cast<size_t>(k) * kNumParams + p];
+    chunk[static_cast<size_t>(blockIdx.y) * kNumParams + p] = s;
+}
+
+__global__ __launch_bounds__(256) void grad_reduce_stage2(const double* __restrict__ chunk,
+                                                          float* __restrict__ grad)
 {
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
     if (p >= static_cast<uint32_t>(kNumParams)) return;
     double s = 0.0;
-    for (uint32_t k = 0; k < count; ++k) s += partials[static_cast<size_t>(k) * kNumParams + p];
+    for (uint32_t c = 0; c < kGradChunks; ++c) s += chunk[static_cast<size_t>(c) * kNumParams + p];
     grad[p] = static_cast<float>(s);
 }
 
@@ -484,7 +538,8 @@ struct _clsparseAmdMlp
     float* m1 = nullptr;  // Adam moments
     float* m2 = nullptr;
     float* grad = nullptr;
-    float* partials = nullptr;  // kMlpGrid * kMlpWaves partial gradients
+    float* partials = nullptr;  // kMlpGrid partial gradients (one per workgroup)
+    double* chunk = nullptr;    // kGradChunks stage-1 sums
     uint16_t* W1 = nullptr;
     uint16_t* W2 = nullptr;
     uint16_t* W2T = nullptr;
@@ -589,7 +644,8 @@ extern "C" clsparseStatus clsparseAmdMlpCreate(clsparseAmdMlp* out, cl_ulong see
     if (e == hipSuccess) e = dmalloc(&m->m1, kNumParams);
     if (e == hipSuccess) e = dmalloc(&m->m2, kNumParams);
     if (e == hipSuccess) e = dmalloc(&m->grad, kNumParams);
-    if (e == hipSuccess) e = dmalloc(&m->partials, static_cast<size_t>(kMlpGrid) * kMlpWaves * kNumParams);
+    if (e == hipSuccess) e = dmalloc(&m->partials, static_cast<size_t>(kMlpGrid) * kNumParams);
+    if (e == hipSuccess) e = dmalloc(&m->chunk, static_cast<size_t>(kGradChunks) * kNumParams);
     if (e == hipSuccess) e = dmalloc(&m->W1, kHid * kFeat);
     if (e == hipSuccess) e = dmalloc(&m->W2, kHid * kHid);
     if (e == hipSuccess) e = dmalloc(&m->W2T, kHid * kHid);
@@ -611,7 +667,7 @@ extern "C" clsparseStatus clsparseAmdMlpCreate(clsparseAmdMlp* out, cl_ulong see
 extern "C" clsparseStatus clsparseAmdMlpDestroy(clsparseAmdMlp m)
 {
     if (m == nullptr) return clsparseInvalidValue;
-    void* ptrs[] = {m->p, m->m1, m->m2, m->grad, m->partials, m->W1, m->W2, m->W2T};
+    void* ptrs[] = {m->p, m->m1, m->m2, m->grad, m->partials, m->chunk, m->W1, m->W2, m->W2T};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete m;
@@ -793,8 +849,12 @@ clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_
         return clsparseSuccess;
     }
     if (net) {
-        hipLaunchKernelGGL(mlp_forward_kernel, dim3(kMlpGrid), dim3(64 * kMlpWaves), 0, st, net->dev(),
-                           f->X, f->nnzG, f->o);
+        // Enough workgroups for several waves per SIMD (the forward kernel
+        // keeps no per-workgroup state).
+        const uint32_t tiles = (f->nnzG + 15) / 16;
+        const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((tiles + kMlpWaves - 1) / kMlpWaves, 4096));
+        hipLaunchKernelGGL(mlp_forward_kernel, dim3(grid), dim3(64 * kMlpWaves), 0, st, net->dev(), f->X,
+                           f->nnzG, f->o);
     }
     CLS_HIP_TRY(hipMemsetAsync(f->bad, 0, sizeof(int), st));
     const float* o = net ? f->o : nullptr;
@@ -915,8 +975,10 @@ extern "C" clsparseStatus clsparseAmdFsaiTrainStep(clsparseAmdFsai f, clsparseAm
                        f->dhat, f->gout);
     hipLaunchKernelGGL(mlp_backward_kernel, dim3(kMlpGrid), dim3(64 * kMlpWaves), 0, st, net->dev(), f->X,
                        f->gout, nnzG, net->partials);
-    hipLaunchKernelGGL(grad_reduce_kernel, dim3(blocks_for(kNumParams)), dim3(256), 0, st, net->partials,
-                       static_cast<uint32_t>(kMlpGrid * kMlpWaves), net->grad);
+    hipLaunchKernelGGL(grad_reduce_stage1, dim3(blocks_for(kNumParams), kGradChunks), dim3(256), 0, st,
+                       net->partials, static_cast<uint32_t>(kMlpGrid), net->chunk);
+    hipLaunchKernelGGL(grad_reduce_stage2, dim3(blocks_for(kNumParams)), dim3(256), 0, st, net->chunk,
+                       net->grad);
     if (lr > 0.f) {
         net->step += 1;
         const float b1t = std::pow(0.9f, static_cast<float>(net->step));

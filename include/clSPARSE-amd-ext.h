@@ -195,6 +195,13 @@ CLSPARSE_EXPORT clsparseStatus
                                    cl_double ratio, cl_double theta, cl_ulong seed,
                                    clsparseAmdHostCsr* out);
 
+/* The FP64 CSR Matrix Market reader into host memory (no device): the same
+ * parsing, symmetric expansion, zero filtering and ordering as
+ * clsparseDCsrMatrixfromFile. Free with clsparseAmdHostCsrFree. */
+CLSPARSE_EXPORT clsparseStatus
+    clsparseAmdCsrMatrixfromFileHost(const char* filePath, cl_bool read_explicit_zeroes,
+                                     clsparseAmdHostCsr* out);
+
 /* Fills out[0..n) with U[lo, hi) from a SplitMix64 stream of `seed`. */
 CLSPARSE_EXPORT clsparseStatus
     clsparseAmdGenUniform(cl_double* out, size_t n, cl_double lo, cl_double hi, cl_ulong seed);

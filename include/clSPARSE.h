@@ -231,6 +231,27 @@ CLSPARSE_EXPORT clsparseStatus
     clsparseDcsrcg(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
                    clSParseSolverControl solverControl, clsparseControl control);
 
+/* ---- Matrix Market file input: src/include/clSPARSE.h L490-611 ------------
+ * Coordinate .mtx files (real, integer or pattern; general or symmetric).
+ * The caller allocates the device arrays (sizes from clsparseHeaderfromFile);
+ * entries are sorted by (row, column), symmetric files are expanded, and
+ * explicit zeroes are dropped unless read_explicit_zeroes is set. */
+CLSPARSE_EXPORT clsparseStatus                                            /* L508-509 */
+    clsparseHeaderfromFile(clsparseIdx_t* nnz, clsparseIdx_t* row, clsparseIdx_t* col,
+                           const char* filePath);
+CLSPARSE_EXPORT clsparseStatus                                            /* L533-534 */
+    clsparseSCooMatrixfromFile(clsparseCooMatrix* cooMatx, const char* filePath,
+                               clsparseControl control, cl_bool read_explicit_zeroes);
+CLSPARSE_EXPORT clsparseStatus                                            /* L559-560 */
+    clsparseDCooMatrixfromFile(clsparseCooMatrix* cooMatx, const char* filePath,
+                               clsparseControl control, cl_bool read_explicit_zeroes);
+CLSPARSE_EXPORT clsparseStatus                                            /* L584-585 */
+    clsparseSCsrMatrixfromFile(clsparseCsrMatrix* csrMatx, const char* filePath,
+                               clsparseControl control, cl_bool read_explicit_zeroes);
+CLSPARSE_EXPORT clsparseStatus                                            /* L610-611 */
+    clsparseDCsrMatrixfromFile(clsparseCsrMatrix* csrMatx, const char* filePath,
+                               clsparseControl control, cl_bool read_explicit_zeroes);
+
 /* ---- CSR-Adaptive metadata: src/include/clSPARSE.h L612-659 --------------- */
 typedef struct _clsparseMetaSizeResult
 {
