@@ -236,21 +236,21 @@ CLSPARSE_EXPORT clsparseStatus
  * The caller allocates the device arrays (sizes from clsparseHeaderfromFile);
  * entries are sorted by (row, column), symmetric files are expanded, and
  * explicit zeroes are dropped unless read_explicit_zeroes is set. */
-CLSPARSE_EXPORT clsparseStatus                                            /* L508-509 */
+CLSPARSE_EXPORT clsparseStatus
     clsparseHeaderfromFile(clsparseIdx_t* nnz, clsparseIdx_t* row, clsparseIdx_t* col,
-                           const char* filePath);
-CLSPARSE_EXPORT clsparseStatus                                            /* L533-534 */
+                           const char* filePath);  /* L508-509 */
+CLSPARSE_EXPORT clsparseStatus
     clsparseSCooMatrixfromFile(clsparseCooMatrix* cooMatx, const char* filePath,
-                               clsparseControl control, cl_bool read_explicit_zeroes);
-CLSPARSE_EXPORT clsparseStatus                                            /* L559-560 */
+                               clsparseControl control, cl_bool read_explicit_zeroes);  /* L533-534 */
+CLSPARSE_EXPORT clsparseStatus
     clsparseDCooMatrixfromFile(clsparseCooMatrix* cooMatx, const char* filePath,
-                               clsparseControl control, cl_bool read_explicit_zeroes);
-CLSPARSE_EXPORT clsparseStatus                                            /* L584-585 */
+                               clsparseControl control, cl_bool read_explicit_zeroes);  /* L559-560 */
+CLSPARSE_EXPORT clsparseStatus
     clsparseSCsrMatrixfromFile(clsparseCsrMatrix* csrMatx, const char* filePath,
-                               clsparseControl control, cl_bool read_explicit_zeroes);
-CLSPARSE_EXPORT clsparseStatus                                            /* L610-611 */
+                               clsparseControl control, cl_bool read_explicit_zeroes);  /* L584-585 */
+CLSPARSE_EXPORT clsparseStatus
     clsparseDCsrMatrixfromFile(clsparseCsrMatrix* csrMatx, const char* filePath,
-                               clsparseControl control, cl_bool read_explicit_zeroes);
+                               clsparseControl control, cl_bool read_explicit_zeroes);  /* L610-611 */
 
 /* ---- CSR-Adaptive metadata: src/include/clSPARSE.h L612-659 --------------- */
 typedef struct _clsparseMetaSizeResult
