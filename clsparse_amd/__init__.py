@@ -125,6 +125,10 @@ _SIGS = {
                                  POINTER(cldenseVector), _V, _V]),
     "clsparseDcsrcg": (c_int32, [POINTER(cldenseVector), POINTER(clsparseCsrMatrix),
                                  POINTER(cldenseVector), _V, _V]),
+    "clsparseScsrbicgStab": (c_int32, [POINTER(cldenseVector), POINTER(clsparseCsrMatrix),
+                                       POINTER(cldenseVector), _V, _V]),
+    "clsparseDcsrbicgStab": (c_int32, [POINTER(cldenseVector), POINTER(clsparseCsrMatrix),
+                                       POINTER(cldenseVector), _V, _V]),
     "clsparseCsrMetaSize": (clsparseMetaSizeResult, [POINTER(clsparseCsrMatrix), _V]),
     "clsparseCsrMetaCreate": (c_int32, [POINTER(clsparseCsrMatrix), _V]),
     "clsparseCsrMetaDelete": (c_int32, [POINTER(clsparseCsrMatrix)]),

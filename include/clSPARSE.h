@@ -230,6 +230,14 @@ CLSPARSE_EXPORT clsparseStatus
 CLSPARSE_EXPORT clsparseStatus
     clsparseDcsrcg(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
                    clSParseSolverControl solverControl, clsparseControl control);
+/* BiCGStab with right Jacobi / no preconditioning, same solver control and
+ * ||r||_1 / ||b||_1 stopping rule (src/include/clSPARSE.h L450-482). */
+CLSPARSE_EXPORT clsparseStatus
+    clsparseScsrbicgStab(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
+                         clSParseSolverControl solverControl, clsparseControl control);  /* L463-465 */
+CLSPARSE_EXPORT clsparseStatus
+    clsparseDcsrbicgStab(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
+                         clSParseSolverControl solverControl, clsparseControl control);  /* L480-482 */
 
 /* ---- Matrix Market file input: src/include/clSPARSE.h L490-611 ------------
  * Coordinate .mtx files (real, integer or pattern; general or symmetric).

@@ -55,6 +55,10 @@ def load():
         lib.orc_cg_s.restype = CgResult
         lib.orc_cg_s.argtypes = [c_uint32, u32p, u32p, f32p, f32p, f32p, c_int, c_int, c_double,
                                  c_double, c_int]
+        lib.orc_bicgstab_d.restype = CgResult
+        lib.orc_bicgstab_d.argtypes = lib.orc_cg_d.argtypes
+        lib.orc_bicgstab_s.restype = CgResult
+        lib.orc_bicgstab_s.argtypes = lib.orc_cg_s.argtypes
         _lib = lib
     return _lib
 
@@ -123,6 +127,25 @@ def nrm1(x):
 def nrm2(x):
     x = np.ascontiguousarray(x, dtype=np.float64)
     return load().orc_nrm2_d(x.size, _p(x, c_double))
+
+
+def bicgstab(rp, ci, v, b, x0, precond=1, max_iters=100, rel_tol=1e-2, abs_tol=1e-5,
+             n_iters_prior=0):
+    """CPU restatement of clsparse[S/D]csrbicgStab -> (x, CgResult)."""
+    rp = np.ascontiguousarray(rp, dtype=np.uint32)
+    ci = np.ascontiguousarray(ci, dtype=np.uint32)
+    n = rp.size - 1
+    lib = load()
+    if np.asarray(v).dtype == np.float32:
+        t, fn, dt = ctypes.c_float, lib.orc_bicgstab_s, np.float32
+    else:
+        t, fn, dt = c_double, lib.orc_bicgstab_d, np.float64
+    v = np.ascontiguousarray(v, dtype=dt)
+    b = np.ascontiguousarray(b, dtype=dt)
+    x = np.array(x0, dtype=dt, copy=True)
+    r = fn(n, _p(rp, c_uint32), _p(ci, c_uint32), _p(v, t), _p(b, t), _p(x, t), precond,
+           max_iters, rel_tol, abs_tol, n_iters_prior)
+    return x, r
 
 
 def cg(rp, ci, v, b, x0, precond=1, max_iters=100, rel_tol=1e-2, abs_tol=1e-5, n_iters_prior=0):

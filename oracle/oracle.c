@@ -22,6 +22,7 @@
  * Parity pins (which checks fix each function) are listed in the header of
  * tests/test_oracle.py.
  */
+#include <float.h>
 #include <math.h>
 #include <stddef.h>
 #include <stdint.h>
@@ -386,4 +387,162 @@ orc_cg_result orc_cg_s(uint32_t n, const uint32_t* rp, const uint32_t* ci, const
                        double absTol, int nIters_prior)
 {
     ORC_CG_BODY(float)
+}
+
+/* ---- BiCGStab ------------------------------------------------------------------
+ * Restates bicgStab<T> of src/library/solvers/biconjugate-gradients-stabilized.hpp
+ * L41-268 (right preconditioning with M = Jacobi or identity, L1 residual
+ * relative to ||b||_1, two finished() checks per iteration; finished uses the
+ * solver control's nIters, which holds the value from before the call until
+ * the end of the first iteration). Dots and norms accumulate in double. */
+
+#define ORC_BICG_BODY(T)                                                                    \
+    orc_cg_result res;                                                                      \
+    memset(&res, 0, sizeof(res));                                                           \
+    res.rel_tol_out = relTol;                                                               \
+    res.abs_tol_out = absTol;                                                               \
+    res.n_iters = nIters_prior;                                                             \
+    double norm_b = 0;                                                                      \
+    for (uint32_t i = 0; i < n; ++i) norm_b += fabs((double)b[i]);                          \
+    if (norm_b <= (double)ORC_MIN(T)) { /* L75-83 */                                        \
+        res.n_iters = 0;                                                                    \
+        res.rel_tol_out = 0;                                                                \
+        res.abs_tol_out = 0;                                                                \
+        for (uint32_t i = 0; i < n; ++i) x[i] = b[i];                                       \
+        res.early_exit = 1;                                                                 \
+        return res;                                                                         \
+    }                                                                                       \
+    T* invD = (T*)malloc(n * sizeof(T));                                                    \
+    T* r = (T*)calloc(n, sizeof(T));                                                        \
+    T* rs = (T*)calloc(n, sizeof(T));                                                       \
+    T* p = (T*)calloc(n, sizeof(T));                                                        \
+    T* s = (T*)calloc(n, sizeof(T));                                                        \
+    T* Mp = (T*)calloc(n, sizeof(T));                                                       \
+    T* AMp = (T*)calloc(n, sizeof(T));                                                      \
+    T* Ms = (T*)calloc(n, sizeof(T));                                                       \
+    T* AMs = (T*)calloc(n, sizeof(T));                                                      \
+    for (uint32_t i = 0; i < n; ++i) {                                                      \
+        T d = (T)1;                                                                         \
+        for (uint32_t j = rp[i]; j < rp[i + 1]; ++j)                                        \
+            if (ci[j] == i) {                                                               \
+                d = (T)1 / v[j];                                                            \
+                break;                                                                      \
+            }                                                                               \
+        invD[i] = precond ? d : (T)1;                                                       \
+    }                                                                                       \
+    for (uint32_t i = 0; i < n; ++i) { /* L103-109: r = b - A x */                          \
+        T acc = 0;                                                                          \
+        for (uint32_t j = rp[i]; j < rp[i + 1]; ++j) acc = ORC_FMA(v[j], x[ci[j]], acc);     \
+        r[i] = b[i] - acc;                                                                  \
+    }                                                                                       \
+    double nr = 0;                                                                          \
+    for (uint32_t i = 0; i < n; ++i) nr += fabs((double)r[i]);                              \
+    double residuum = nr / norm_b;                                                          \
+    res.initial_residual = residuum;                                                        \
+    res.current_residual = residuum;                                                        \
+    if (residuum <= relTol || residuum <= absTol * residuum || nIters_prior >= maxIters) {  \
+        res.n_iters = 0;                                                                    \
+        res.early_exit = 1;                                                                 \
+        goto done;                                                                          \
+    }                                                                                       \
+    for (uint32_t i = 0; i < n; ++i) {                                                      \
+        p[i] = r[i];                                                                        \
+        rs[i] = r[i];                                                                       \
+    }                                                                                       \
+    double rr_old = 0;                                                                      \
+    for (uint32_t i = 0; i < n; ++i) rr_old += (double)rs[i] * (double)r[i];                \
+    int iteration = 0;                                                                      \
+    int cur_n = nIters_prior;                                                               \
+    for (;;) {                                                                              \
+        for (uint32_t i = 0; i < n; ++i) Mp[i] = p[i] * invD[i];                            \
+        for (uint32_t i = 0; i < n; ++i) {                                                  \
+            T acc = 0;                                                                      \
+            for (uint32_t j = rp[i]; j < rp[i + 1]; ++j) acc = ORC_FMA(v[j], Mp[ci[j]], acc);\
+            AMp[i] = acc;                                                                   \
+        }                                                                                   \
+        double ra = 0;                                                                      \
+        for (uint32_t i = 0; i < n; ++i) ra += (double)rs[i] * (double)AMp[i];              \
+        const T alpha = (T)(rr_old / ra);                                                   \
+        double ns = 0;                                                                      \
+        for (uint32_t i = 0; i < n; ++i) {                                                  \
+            s[i] = r[i] - alpha * AMp[i];                                                   \
+            ns += fabs((double)s[i]);                                                       \
+        }                                                                                   \
+        residuum = ns / norm_b;                                                             \
+        res.current_residual = residuum;                                                    \
+        if (residuum <= relTol || residuum <= absTol * res.initial_residual ||              \
+            cur_n >= maxIters) { /* L186-195 */                                             \
+            res.n_iters = iteration;                                                        \
+            for (uint32_t i = 0; i < n; ++i) x[i] = x[i] + alpha * Mp[i];                   \
+            res.executed = iteration + 1;                                                   \
+            break;                                                                          \
+        }                                                                                   \
+        for (uint32_t i = 0; i < n; ++i) Ms[i] = s[i] * invD[i];                            \
+        for (uint32_t i = 0; i < n; ++i) {                                                  \
+            T acc = 0;                                                                      \
+            for (uint32_t j = rp[i]; j < rp[i + 1]; ++j) acc = ORC_FMA(v[j], Ms[ci[j]], acc);\
+            AMs[i] = acc;                                                                   \
+        }                                                                                   \
+        double us = 0, ww = 0;                                                              \
+        for (uint32_t i = 0; i < n; ++i) {                                                  \
+            us += (double)AMs[i] * (double)s[i];                                            \
+            ww += (double)AMs[i] * (double)AMs[i];                                          \
+        }                                                                                   \
+        const T omega = (T)(us / ww);                                                       \
+        nr = 0;                                                                             \
+        double rr_new = 0;                                                                  \
+        for (uint32_t i = 0; i < n; ++i) {                                                  \
+            x[i] = x[i] + alpha * Mp[i];                                                    \
+            x[i] = x[i] + omega * Ms[i];                                                    \
+            r[i] = s[i] - omega * AMs[i];                                                   \
+            nr += fabs((double)r[i]);                                                       \
+            rr_new += (double)rs[i] * (double)r[i];                                         \
+        }                                                                                   \
+        residuum = nr / norm_b;                                                             \
+        res.current_residual = residuum;                                                    \
+        if (residuum <= relTol || residuum <= absTol * res.initial_residual ||              \
+            cur_n >= maxIters) { /* L233-238 */                                             \
+            res.n_iters = iteration;                                                        \
+            res.executed = iteration + 1;                                                   \
+            break;                                                                          \
+        }                                                                                   \
+        T beta = (T)(rr_new / rr_old);                                                      \
+        beta = beta * alpha;                                                                \
+        beta = beta / omega;                                                                \
+        rr_old = rr_new;                                                                    \
+        const T bo = beta * omega;                                                          \
+        for (uint32_t i = 0; i < n; ++i) {                                                  \
+            p[i] = beta * p[i] + r[i];                                                      \
+            p[i] = p[i] - bo * AMp[i];                                                      \
+        }                                                                                   \
+        iteration++;                                                                        \
+        cur_n = iteration;                                                                  \
+        res.n_iters = iteration;                                                            \
+    }                                                                                       \
+done:                                                                                       \
+    free(invD);                                                                             \
+    free(r);                                                                                \
+    free(rs);                                                                               \
+    free(p);                                                                                \
+    free(s);                                                                                \
+    free(Mp);                                                                               \
+    free(AMp);                                                                              \
+    free(Ms);                                                                               \
+    free(AMs);                                                                              \
+    return res;
+
+#define ORC_MIN(T) _Generic((T)0, float: FLT_MIN, default: DBL_MIN)
+
+orc_cg_result orc_bicgstab_d(uint32_t n, const uint32_t* rp, const uint32_t* ci, const double* v,
+                             const double* b, double* x, int precond, int maxIters, double relTol,
+                             double absTol, int nIters_prior)
+{
+    ORC_BICG_BODY(double)
+}
+
+orc_cg_result orc_bicgstab_s(uint32_t n, const uint32_t* rp, const uint32_t* ci, const float* v,
+                             const float* b, float* x, int precond, int maxIters, double relTol,
+                             double absTol, int nIters_prior)
+{
+    ORC_BICG_BODY(float)
 }

@@ -1,0 +1,119 @@
+"""BiCGStab (clsparse[S/D]csrbicgStab) against the CPU restatement of
+src/library/solvers/biconjugate-gradients-stabilized.hpp L41-268."""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+import clsparse_amd as cls
+import oracle
+
+from gpu_helpers import say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctl():
+    c = cls.Control()
+    yield c
+    c.release()
+
+
+def _convection_diffusion(nx, ny, c=0.4):
+    """5-point Laplacian plus an upwind convection term in x: nonsymmetric."""
+    n = nx * ny
+    lap = cls.gen_poisson2d(nx, ny)
+    a = sp.csr_matrix((lap[2], lap[1], lap[0]), shape=(n, n)).tolil()
+    for y in range(ny):
+        for x in range(nx):
+            i = y * nx + x
+            a[i, i] += c
+            if x > 0:
+                a[i, i - 1] -= c
+    a = sp.csr_matrix(a)
+    a.sort_indices()
+    return a.indptr.astype(np.uint32), a.indices.astype(np.uint32), a.data, n
+
+
+def _solve(ctl, rp, ci, v, n, b, dtype, precond, max_iters, rel, absol, n_prior=None):
+    lib = cls.load()
+    tdt = torch.float32 if dtype == np.float32 else torch.float64
+    A = cls.CsrMatrix(rp, ci, np.asarray(v, dtype=dtype), n, dtype=dtype)
+    A.meta_create(ctl)
+    x = torch.zeros(n, dtype=tdt, device="cuda")
+    xd, bd = cls.DenseVector(x), cls.DenseVector(torch.tensor(np.asarray(b, dtype=dtype), device="cuda"))
+    sc = cls.SolverControl(precond, max_iters, rel, absol)
+    try:
+        fn = lib.clsparseScsrbicgStab if dtype == np.float32 else lib.clsparseDcsrbicgStab
+        st = fn(ctypes.byref(xd.s), ctypes.byref(A.s), ctypes.byref(bd.s), sc.handle, ctl.handle)
+        return st, sc.stats(), x.cpu().numpy()
+    finally:
+        sc.release()
+
+
+@pytest.mark.parametrize("precond", [cls.DIAGONAL, cls.NOPRECOND])
+def test_nonsymmetric_fp64_matches_oracle(ctl, precond):
+    rp, ci, v, n = _convection_diffusion(48, 40)
+    b = cls.gen_uniform(n, -1, 1, seed=21)
+    say(f"bicgstab fp64 precond={precond}")
+    st, s, x = _solve(ctl, rp, ci, v, n, b, np.float64, precond, 2000, 1e-10, 0.0)
+    assert st == 0
+    xo, ro = oracle.bicgstab(rp, ci, v, b, np.zeros(n), precond, 2000, 1e-10, 0.0)
+    assert abs(s.n_iters - ro.n_iters) <= 1, (s.n_iters, ro.n_iters)
+    r = b - oracle.csrmv(rp, ci, v, x)[0]
+    assert np.abs(r).sum() / np.abs(b).sum() <= 1e-10 * 1.5
+    assert np.abs(x - xo).max() <= 1e-7 * np.abs(xo).max()
+
+
+def test_fp32_converges(ctl):
+    rp, ci, v, n = _convection_diffusion(32, 32)
+    b = cls.gen_uniform(n, -1, 1, seed=5)
+    st, s, x = _solve(ctl, rp, ci, v, n, b, np.float32, cls.DIAGONAL, 1000, 1e-5, 0.0)
+    assert st == 0 and s.status_detail == 0
+    _, ro = oracle.bicgstab(rp, ci, v.astype(np.float32), b.astype(np.float32), np.zeros(n, np.float32),
+                            1, 1000, 1e-5, 0.0)
+    assert abs(s.n_iters - ro.n_iters) <= 3
+    r = b - oracle.csrmv(rp, ci, v, x.astype(np.float64))[0]
+    assert np.abs(r).sum() / np.abs(b).sum() <= 2e-5
+
+
+def test_diagonal_matrix_stops_at_first_check_with_exact_solution(ctl):
+    d = np.array([2.0, 5.0, 0.25, 9.0, 3.0])
+    a = sp.diags(d).tocsr()
+    b = np.array([1.0, -2.0, 3.0, 0.5, 7.0])
+    st, s, x = _solve(ctl, a.indptr.astype(np.uint32), a.indices.astype(np.uint32), a.data, 5, b,
+                      np.float64, cls.DIAGONAL, 100, 1e-12, 0.0)
+    assert st == 0 and s.n_iters == 0
+    np.testing.assert_allclose(x, b / d, rtol=1e-15)
+
+
+def test_zero_rhs_and_max_iters(ctl):
+    rp, ci, v, n = _convection_diffusion(20, 20)
+    st, s, x = _solve(ctl, rp, ci, v, n, np.zeros(n), np.float64, cls.DIAGONAL, 100, 1e-10, 0.0)
+    assert st == 0 and s.n_iters == 0 and np.all(x == 0)
+    assert s.relative_tolerance == 0.0 and s.absolute_tolerance == 0.0
+    b = cls.gen_uniform(n, -1, 1, seed=8)
+    st, s, x = _solve(ctl, rp, ci, v, n, b, np.float64, cls.DIAGONAL, 5, 1e-14, 0.0)
+    xo, ro = oracle.bicgstab(rp, ci, v, b, np.zeros(n), 1, 5, 1e-14, 0.0)
+    assert st == 0 and s.n_iters == ro.n_iters == 5 and s.status_detail == 1
+    np.testing.assert_allclose(x, xo, rtol=1e-9, atol=1e-12)
+
+
+def test_error_codes(ctl):
+    lib = cls.load()
+    x = cls.DenseVector(torch.zeros(4, dtype=torch.float64, device="cuda"))
+    a = sp.identity(3, format="csr")
+    A = cls.CsrMatrix(a.indptr, a.indices, a.data, 3)
+    sc = cls.SolverControl(cls.DIAGONAL, 10, 1e-6, 0.0)
+    try:
+        assert lib.clsparseDcsrbicgStab(ctypes.byref(x.s), ctypes.byref(A.s), ctypes.byref(x.s),
+                                        sc.handle, None) == cls.clsparseInvalidControlObject
+        assert lib.clsparseDcsrbicgStab(ctypes.byref(x.s), ctypes.byref(A.s), ctypes.byref(x.s),
+                                        None, ctl.handle) == cls.clsparseInvalidSolverControlObject
+        assert lib.clsparseDcsrbicgStab(ctypes.byref(x.s), ctypes.byref(A.s), ctypes.byref(x.s),
+                                        sc.handle, ctl.handle) == cls.clsparseInvalidSystemSize
+    finally:
+        sc.release()

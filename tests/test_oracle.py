@@ -233,3 +233,45 @@ def test_oracle_cg_manufactured_solution():
     x, r = oracle.cg(rp, ci, v, b, np.zeros(m), 1, 1000, 1e-12, 0.0)
     assert r.current_residual <= 1e-12
     np.testing.assert_allclose(x, xs, atol=1e-9)
+
+
+# ---- BiCGStab restatement (biconjugate-gradients-stabilized.hpp L41-268) ------
+
+def _nonsym(n=300, seed=0):
+    import scipy.sparse as sp
+    main = 4 * np.ones(n)
+    a = sp.diags([-1.3 * np.ones(n - 1), main, -0.7 * np.ones(n - 1)], [-1, 0, 1]).tocsr()
+    return a.indptr.astype(np.uint32), a.indices.astype(np.uint32), a.data, a
+
+
+def test_bicgstab_diagonal_jacobi_first_check_exact():
+    import scipy.sparse as sp
+    d = np.array([2.0, 5.0, 0.25, 9.0])
+    a = sp.diags(d).tocsr()
+    b = np.array([1.0, -2.0, 3.0, 0.5])
+    x, r = oracle.bicgstab(a.indptr, a.indices, a.data, b, np.zeros(4), 1, 100, 1e-12, 0.0)
+    assert r.n_iters == 0 and r.executed == 1
+    np.testing.assert_array_equal(x, b / d)
+
+
+def test_bicgstab_manufactured_nonsymmetric_solution():
+    rp, ci, v, a = _nonsym()
+    xs = np.random.default_rng(1).uniform(-1, 1, a.shape[0])
+    b = a @ xs
+    for precond in (0, 1):
+        x, r = oracle.bicgstab(rp, ci, v, b, np.zeros(a.shape[0]), precond, 1000, 1e-13, 0.0)
+        assert np.abs(x - xs).max() < 1e-11
+        assert r.current_residual <= 1e-13
+
+
+def test_bicgstab_zero_rhs_and_max_iters_quirks():
+    rp, ci, v, a = _nonsym(50)
+    x, r = oracle.bicgstab(rp, ci, v, np.zeros(50), np.ones(50), 1, 100, 1e-6, 1e-3)
+    assert r.early_exit == 1 and r.n_iters == 0 and r.rel_tol_out == 0 and r.abs_tol_out == 0
+    assert np.all(x == 0)
+    b = np.random.default_rng(2).uniform(-1, 1, 50)
+    x, r = oracle.bicgstab(rp, ci, v, b, np.zeros(50), 1, 3, 1e-15, 0.0)
+    assert r.n_iters == 3 and r.executed == 4
+    # a solver control that already holds nIters >= maxIters stops before the loop
+    x, r = oracle.bicgstab(rp, ci, v, b, np.zeros(50), 1, 3, 1e-15, 0.0, n_iters_prior=5)
+    assert r.early_exit == 1 and r.n_iters == 0
