@@ -147,6 +147,12 @@ _SIGS = {
     "clsparseDCsrMatrixfromFile": (c_int32, [POINTER(clsparseCsrMatrix), ctypes.c_char_p, _V, c_uint32]),
     "clsparseAmdCsrMatrixfromFileHost": (c_int32, [ctypes.c_char_p, c_uint32,
                                                    POINTER(clsparseAmdHostCsr)]),
+    "clsparseScsrmm": (c_int32, [POINTER(clsparseScalar), POINTER(clsparseCsrMatrix),
+                                 POINTER(cldenseMatrix), POINTER(clsparseScalar),
+                                 POINTER(cldenseMatrix), _V]),
+    "clsparseDcsrmm": (c_int32, [POINTER(clsparseScalar), POINTER(clsparseCsrMatrix),
+                                 POINTER(cldenseMatrix), POINTER(clsparseScalar),
+                                 POINTER(cldenseMatrix), _V]),
     "clsparseAmdCsrMetaGetRowBlocks": (c_int32, [POINTER(clsparseCsrMatrix), POINTER(c_uint64),
                                                  c_uint32, POINTER(c_uint32)]),
     "clsparseAmdComputeRowBlocksHost": (c_uint64, [POINTER(c_uint32), c_uint32, POINTER(c_uint64)]),

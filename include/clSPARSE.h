@@ -355,6 +355,19 @@ CLSPARSE_EXPORT clsparseStatus
                    const cldenseVector* x, const clsparseScalar* beta,
                    cldenseVector* y, const clsparseControl control);
 
+/* ---- sparse BLAS-3: src/include/clSPARSE.h L1145-1187 ----------------------
+ * C = alpha * A * B + beta * C with B and C row-major (lead_dim = row stride;
+ * the major field is not read, as in the reference). Size mismatches between
+ * A, B and C give clsparseInvalidSize. */
+CLSPARSE_EXPORT clsparseStatus
+    clsparseScsrmm(const clsparseScalar* alpha, const clsparseCsrMatrix* sparseMatA,
+                   const cldenseMatrix* denseMatB, const clsparseScalar* beta,
+                   cldenseMatrix* denseMatC, const clsparseControl control);  /* L1159-1165 */
+CLSPARSE_EXPORT clsparseStatus
+    clsparseDcsrmm(const clsparseScalar* alpha, const clsparseCsrMatrix* sparseMatA,
+                   const cldenseMatrix* denseMatB, const clsparseScalar* beta,
+                   cldenseMatrix* denseMatC, const clsparseControl control);  /* L1181-1187 */
+
 #ifdef __cplusplus
 }
 #endif

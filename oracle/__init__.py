@@ -55,6 +55,9 @@ def load():
         lib.orc_cg_s.restype = CgResult
         lib.orc_cg_s.argtypes = [c_uint32, u32p, u32p, f32p, f32p, f32p, c_int, c_int, c_double,
                                  c_double, c_int]
+        lib.orc_csrmm_d.restype = None
+        lib.orc_csrmm_d.argtypes = [c_uint32, u32p, u32p, f64p, f64p, c_uint32, f64p, c_uint32,
+                                    c_uint32, c_double, c_double, f64p]
         lib.orc_bicgstab_d.restype = CgResult
         lib.orc_bicgstab_d.argtypes = lib.orc_cg_d.argtypes
         lib.orc_bicgstab_s.restype = CgResult
@@ -127,6 +130,21 @@ def nrm1(x):
 def nrm2(x):
     x = np.ascontiguousarray(x, dtype=np.float64)
     return load().orc_nrm2_d(x.size, _p(x, c_double))
+
+
+def csrmm(rp, ci, v, B, C, alpha=1.0, beta=0.0):
+    """C = alpha A B + beta C for row-major dense B (k x m) and C (n x m)
+    -> (C_out, scale) in FP64 (long-double sums)."""
+    rp = np.ascontiguousarray(rp, dtype=np.uint32)
+    ci = np.ascontiguousarray(ci, dtype=np.uint32)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    out = np.array(C, dtype=np.float64, copy=True, order="C")
+    n, m = out.shape
+    scale = np.zeros((n, m))
+    load().orc_csrmm_d(n, _p(rp, c_uint32), _p(ci, c_uint32), _p(v, c_double), _p(B, c_double),
+                       B.shape[1], _p(out, c_double), m, m, alpha, beta, _p(scale, c_double))
+    return out, scale
 
 
 def bicgstab(rp, ci, v, b, x0, precond=1, max_iters=100, rel_tol=1e-2, abs_tol=1e-5,

@@ -546,3 +546,34 @@ orc_cg_result orc_bicgstab_s(uint32_t n, const uint32_t* rp, const uint32_t* ci,
 {
     ORC_BICG_BODY(float)
 }
+
+/* ---- csrmm -----------------------------------------------------------------------
+ * C = alpha * A * B + beta * C with row-major B and C (element (k, c) at
+ * k * ld + c), as csrmv_batched of src/library/kernels/csrmm_general.cl
+ * L59-167 computes it column by column: alpha == 0 contributes nothing,
+ * beta == 0 never reads C. Long-double sums; scale[r*ncols+c] receives
+ * |alpha| sum |a b| + |beta c| for error bounds. */
+void orc_csrmm_d(uint32_t n, const uint32_t* rp, const uint32_t* ci, const double* v, const double* B,
+                 uint32_t ldb, double* C, uint32_t ldc, uint32_t ncols, double alpha, double beta,
+                 double* scale)
+{
+#pragma omp parallel for schedule(dynamic, 64)
+    for (uint32_t r = 0; r < n; ++r) {
+        for (uint32_t c = 0; c < ncols; ++c) {
+            long double s = 0, a = 0;
+            if (alpha != 0)
+                for (uint32_t j = rp[r]; j < rp[r + 1]; ++j) {
+                    const long double t = (long double)alpha * v[j] * B[(size_t)ci[j] * ldb + c];
+                    s += t;
+                    a += fabsl(t);
+                }
+            double* out = &C[(size_t)r * ldc + c];
+            if (beta != 0) {
+                s += (long double)beta * *out;
+                a += fabsl((long double)beta * *out);
+            }
+            *out = (double)s;
+            scale[(size_t)r * ncols + c] = (double)a;
+        }
+    }
+}

@@ -275,3 +275,22 @@ def test_bicgstab_zero_rhs_and_max_iters_quirks():
     # a solver control that already holds nIters >= maxIters stops before the loop
     x, r = oracle.bicgstab(rp, ci, v, b, np.zeros(50), 1, 3, 1e-15, 0.0, n_iters_prior=5)
     assert r.early_exit == 1 and r.n_iters == 0
+
+
+def test_csrmm_oracle_matches_scipy_and_special_cases():
+    import scipy.sparse as sp
+    m = 350
+    a = sp.random(400, m, density=0.03, random_state=7, format="csr")
+    a.sort_indices()
+    rp, ci, v = a.indptr.astype(np.uint32), a.indices.astype(np.uint32), a.data
+    rng = np.random.default_rng(3)
+    B = rng.uniform(-1, 1, (m, 5))
+    C = rng.uniform(-1, 1, (400, 5))
+    out, scale = oracle.csrmm(rp, ci, v, B, C, alpha=1.5, beta=-0.5)
+    np.testing.assert_allclose(out, 1.5 * (a @ B) - 0.5 * C, rtol=0, atol=1e-13)
+    assert np.all(scale > 0)
+    # alpha = 0: beta * C only; beta = 0: C is never read (NaN stays out)
+    out, _ = oracle.csrmm(rp, ci, v, B, C, alpha=0.0, beta=2.0)
+    np.testing.assert_array_equal(out, 2.0 * C)
+    out, _ = oracle.csrmm(rp, ci, v, B, np.full((400, 5), np.nan), alpha=1.0, beta=0.0)
+    assert np.all(np.isfinite(out))
