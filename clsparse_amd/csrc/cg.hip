@@ -551,7 +551,7 @@ extern "C" clsparseStatus clsparseDcsrcg(cldenseVector* x, const clsparseCsrMatr
 namespace {
 
 constexpr int kStagnationChunks = 16;  // learned FSAI must halve ||r|| within this many chunks
-constexpr int kPilotChunks = 4;        // chunks per preconditioner in the pilot race
+constexpr int kPilotChunks = 3;        // chunks per preconditioner in the pilot race
 
 // CLSPARSE_AMD_FSAI_POLICY=always keeps the learned factor unless it breaks
 // down or stagnates; the default ("race") also runs the pilot race.

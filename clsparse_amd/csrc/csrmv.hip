@@ -128,7 +128,7 @@ __device__ __forceinline__ T mul_rounded(T a, T b)
 // Row offsets (relative to the block's first nnz) of rows row .. row+255 held
 // in registers: lane l keeps row l + 64k in slot k. Loaded together with the
 // block's values so the reduction needs no dependent global loads.
-constexpr int kRowSlots = 4;
+constexpr int kRowSlots = 8;
 
 template <typename T, bool EXT>
 __global__ __launch_bounds__(64 * kWavesPerWg) void csrmv_adaptive_kernel(
