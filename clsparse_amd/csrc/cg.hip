@@ -248,12 +248,13 @@ template <typename T, typename TX = T>
 class CgEngine
 {
 public:
-    // G, GT: the factor of M = G G^T and its transpose (kPrecFsai only).
+    // G, GT: the factor of M = G G^T and its transpose (kPrecFsai only);
+    // g_bf16: their values are BF16 bit patterns (FP32 engines only).
     CgEngine(const clsparseCsrMatrix* A, TX* x, const T* b, size_t n, int prec, int crit,
              CgBuffers<T>& buf, hipStream_t s, bool ext, const clsparseCsrMatrix* G = nullptr,
-             const clsparseCsrMatrix* GT = nullptr)
+             const clsparseCsrMatrix* GT = nullptr, bool g_bf16 = false)
         : A_(A), x_(x), b_(b), n_(n), prec_(prec), crit_(crit), buf_(buf), s_(s), ext_(ext),
-          g_(grid_for(n)), G_(G), GT_(GT)
+          g_(grid_for(n)), G_(G), GT_(GT), g_bf16_(g_bf16)
     {
     }
     ~CgEngine()
@@ -336,8 +337,8 @@ public:
     clsparseStatus precondition(T* out, T* tmp)
     {
         if (prec_ == kPrecFsai) {
-            CLS_TRY(launch_csrmv<T>(buf_.one, GT_, buf_.r, buf_.zero, tmp, s_, false, nullptr));
-            CLS_TRY(launch_csrmv<T>(buf_.one, G_, tmp, buf_.zero, out, s_, false, nullptr));
+            CLS_TRY(spmv_g(GT_, buf_.r, tmp, nullptr));
+            CLS_TRY(spmv_g(G_, tmp, out, nullptr));
         } else {
             hipLaunchKernelGGL((precond_kernel<T>), dim3(ew_grid(n_)), dim3(256), 0, s_, n_, buf_.r,
                                buf_.invD, prec_, out);
@@ -367,15 +368,24 @@ private:
     }
 
     // z = G (G^T r) through y (free between the r update and the next A p);
-    // partials of <r, z>.
+    // partials of <r, z> = ||G^T r||^2, read from y alone.
     clsparseStatus apply_fsai(double* part_rz, const int* done)
     {
-        CLS_TRY(launch_csrmv<T>(buf_.one, GT_, buf_.r, buf_.zero, buf_.y, s_, false, done));
-        CLS_TRY(launch_csrmv<T>(buf_.one, G_, buf_.y, buf_.zero, buf_.z, s_, false, done));
-        hipLaunchKernelGGL((cg_dot_kernel<T>), dim3(g_), dim3(256), 0, s_, n_, buf_.r, buf_.z,
+        CLS_TRY(spmv_g(GT_, buf_.r, buf_.y, done));
+        CLS_TRY(spmv_g(G_, buf_.y, buf_.z, done));
+        hipLaunchKernelGGL((cg_dot_kernel<T>), dim3(g_), dim3(256), 0, s_, n_, buf_.y, buf_.y,
                            part_rz, buf_.st);
         CLS_HIP_TRY(hipGetLastError());
         return clsparseSuccess;
+    }
+
+    // out = M_ in for one factor of M, in its stored precision.
+    clsparseStatus spmv_g(const clsparseCsrMatrix* M_, const T* in, T* out, const int* done)
+    {
+        if constexpr (std::is_same_v<T, float>) {
+            if (g_bf16_) return launch_csrmv_bf16(buf_.one, M_, in, buf_.zero, out, s_, done);
+        }
+        return launch_csrmv<T>(buf_.one, M_, in, buf_.zero, out, s_, false, done);
     }
 
     const clsparseCsrMatrix* A_;
@@ -390,6 +400,7 @@ private:
     uint32_t g_;
     const clsparseCsrMatrix* G_;
     const clsparseCsrMatrix* GT_;
+    bool g_bf16_;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
 };
@@ -683,10 +694,16 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     int used = prec;  // statistics: 0 none, 1 Jacobi, 2 FSAI, 3 FSAI -> Jacobi
     const clsparseCsrMatrix* G = nullptr;
     const clsparseCsrMatrix* GT = nullptr;
+    bool g_bf16 = false;
     if (fsai) {
         if constexpr (kMixed) {
-            G = fsai_G32(fsai);
-            GT = fsai_GT32(fsai);
+            // The factor's values in BF16 (CLSPARSE_AMD_FSAI_BF16=0: FP32):
+            // M = G G^T stays symmetric positive definite for any rounding of
+            // a nonsingular G, and the SpMVs with G move fewer bytes.
+            const char* e = std::getenv("CLSPARSE_AMD_FSAI_BF16");
+            g_bf16 = !(e && e[0] == '0');
+            G = g_bf16 ? fsai_G16(fsai) : fsai_G32(fsai);
+            GT = g_bf16 ? fsai_GT16(fsai) : fsai_GT32(fsai);
         } else {
             G = fsai_G64(fsai);
             GT = fsai_GT64(fsai);
@@ -729,7 +746,7 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     *hs = init;
     CLS_HIP_TRY(hipMemcpyAsync(buf.st, hs, sizeof(CgDevState), hipMemcpyHostToDevice, s));
 
-    CgEngine<TI, double> eng_main(&AI, xp, rI, n, prec, kCritL2, buf, s, false, G, GT);
+    CgEngine<TI, double> eng_main(&AI, xp, rI, n, prec, kCritL2, buf, s, false, G, GT, g_bf16);
     CgEngine<TI, double> eng_jac(&AI, xp, rI, n, kPrecJacobi, kCritL2, buf, s, false);
     CgEngine<TI, double>* eng = &eng_main;
 

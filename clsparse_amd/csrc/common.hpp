@@ -184,8 +184,16 @@ const clsparseCsrMatrix* fsai_G32(clsparseAmdFsai f);
 const clsparseCsrMatrix* fsai_GT32(clsparseAmdFsai f);
 const clsparseCsrMatrix* fsai_G64(clsparseAmdFsai f);
 const clsparseCsrMatrix* fsai_GT64(clsparseAmdFsai f);
+const clsparseCsrMatrix* fsai_G16(clsparseAmdFsai f);   // BF16 values, FP32 compute
+const clsparseCsrMatrix* fsai_GT16(clsparseAmdFsai f);
 bool fsai_valid(clsparseAmdFsai f);
 uint32_t fsai_n(clsparseAmdFsai f);
+
+// FP32 CSR-Adaptive SpMV whose values are stored as BF16 bit patterns
+// (metadata required).
+clsparseStatus launch_csrmv_bf16(const float* alpha, const clsparseCsrMatrix* A, const float* x,
+                                 const float* beta, float* y, hipStream_t stream,
+                                 const int* done_flag);
 
 // Deterministic reductions: result written to a device scalar of type R.
 enum class RedOp { Sum, Abs, Sqr, Dot };

@@ -85,10 +85,21 @@ struct XGather<float>
     }
 };
 
+// Stored value -> compute type. Values are stored as T, or (FP32 compute
+// only) as BF16 bit patterns: the preconditioner factors of the learned FSAI.
+template <typename T, typename TV>
+__device__ __forceinline__ T val_to(TV v)
+{
+    if constexpr (sizeof(TV) == 2)
+        return __uint_as_float(static_cast<uint32_t>(v) << 16);
+    else
+        return v;
+}
+
 // Sum over [beg, end) of (alpha * val) * x[col], lanes strided by 64, four
 // loads in flight per lane per step.
-template <typename T, bool EXT>
-__device__ __forceinline__ Acc<T, EXT> strided_dot(const T* __restrict__ vals,
+template <typename T, typename TV, bool EXT>
+__device__ __forceinline__ Acc<T, EXT> strided_dot(const TV* __restrict__ vals,
                                                    const uint32_t* __restrict__ cols,
                                                    const XGather<T>& xg, T alpha,
                                                    uint32_t beg, uint32_t end, uint32_t lane)
@@ -96,7 +107,7 @@ __device__ __forceinline__ Acc<T, EXT> strided_dot(const T* __restrict__ vals,
     Acc<T, EXT> acc;
     uint32_t j = beg + lane;
     for (; j + 192 < end; j += 256) {
-        T v[4];
+        TV v[4];
         uint32_t c[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -107,12 +118,12 @@ __device__ __forceinline__ Acc<T, EXT> strided_dot(const T* __restrict__ vals,
 #pragma unroll
         for (int u = 0; u < 4; ++u) xv[u] = xg(c[u]);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc.mul_add(alpha * v[u], xv[u]);
+        for (int u = 0; u < 4; ++u) acc.mul_add(alpha * val_to<T>(v[u]), xv[u]);
     }
     for (; j < end; j += 64) {
-        const T v = __builtin_nontemporal_load(vals + j);
+        const TV v = __builtin_nontemporal_load(vals + j);
         const uint32_t c = __builtin_nontemporal_load(cols + j);
-        acc.mul_add(alpha * v, xg(c));
+        acc.mul_add(alpha * val_to<T>(v), xg(c));
     }
     return acc;
 }
@@ -130,9 +141,9 @@ __device__ __forceinline__ T mul_rounded(T a, T b)
 // block's values so the reduction needs no dependent global loads.
 constexpr int kRowSlots = 8;
 
-template <typename T, bool EXT>
+template <typename T, typename TV, bool EXT>
 __global__ __launch_bounds__(64 * kWavesPerWg) void csrmv_adaptive_kernel(
-    const T* __restrict__ vals, const uint32_t* __restrict__ cols,
+    const TV* __restrict__ vals, const uint32_t* __restrict__ cols,
     const uint32_t* __restrict__ rowPtr, const T* __restrict__ x, T* __restrict__ y,
     const BlockDesc* __restrict__ desc, uint32_t numBlocks, uint32_t bpw,
     const T* __restrict__ pAlpha, const T* __restrict__ pBeta, double* __restrict__ partials,
@@ -181,13 +192,13 @@ __global__ __launch_bounds__(64 * kWavesPerWg) void csrmv_adaptive_kernel(
         const bool rowwise = !EXT && in_regs && nrows > 64;
         {
             // 16-byte loads: lane owns elements 4*lane + 256*k .. +3.
-            T v[kStreamPerLane];
+            TV v[kStreamPerLane];
             uint32_t c[kStreamPerLane];
 #pragma unroll
             for (int k = 0; k < kStreamPerLane / 4; ++k) {
                 const uint32_t i0 = 256 * k + 4 * lane;
                 if (i0 + 3 < cnt) {
-                    __builtin_memcpy(&v[4 * k], vals + base + i0, 4 * sizeof(T));
+                    __builtin_memcpy(&v[4 * k], vals + base + i0, 4 * sizeof(TV));
                     __builtin_memcpy(&c[4 * k], cols + base + i0, 4 * sizeof(uint32_t));
                 } else {
 #pragma unroll
@@ -204,7 +215,7 @@ __global__ __launch_bounds__(64 * kWavesPerWg) void csrmv_adaptive_kernel(
                     for (int k = 0; k < kStreamPerLane; ++k) {
                         const uint32_t i = 256 * (k / 4) + 4 * lane + (k % 4);
                         if (i < cnt) {
-                            buf[i] = alpha * v[k];
+                            buf[i] = alpha * val_to<T>(v[k]);
                             lds.c[wave][i] = c[k];
                         }
                     }
@@ -214,7 +225,7 @@ __global__ __launch_bounds__(64 * kWavesPerWg) void csrmv_adaptive_kernel(
                 for (int k = 0; k < kStreamPerLane; ++k) {
                     const uint32_t i = 256 * (k / 4) + 4 * lane + (k % 4);
                     if (i < cnt) {
-                        const T a = alpha * v[k];
+                        const T a = alpha * val_to<T>(v[k]);
                         const T xv = xg(c[k]);
                         const T p = a * xv;
                         buf[i] = p;
@@ -322,12 +333,12 @@ __global__ __launch_bounds__(64 * kWavesPerWg) void csrmv_adaptive_kernel(
         }
     } else if (nrows == 1) {
         // ---- CSR-Vector: one row, <= 3072 nnz --------------------------
-        Acc<T, EXT> acc = strided_dot<T, EXT>(vals, cols, xg, alpha, base, base + cnt, lane);
+        Acc<T, EXT> acc = strided_dot<T, TV, EXT>(vals, cols, xg, alpha, base, base + cnt, lane);
         acc = wave_reduce(acc);
         if (lane == 0) write_row<T, EXT>(y, row, acc, beta);
     } else {
         // ---- CSR-LongRows segment --------------------------------------
-        Acc<T, EXT> acc = strided_dot<T, EXT>(vals, cols, xg, alpha, base, base + cnt, lane);
+        Acc<T, EXT> acc = strided_dot<T, TV, EXT>(vals, cols, xg, alpha, base, base + cnt, lane);
         acc = wave_reduce(acc);
         if (lane == 0) {
             // Segment partial (sum, and its running error in extended mode).
@@ -462,11 +473,11 @@ clsparseStatus launch_csrmv(const T* alpha, const clsparseCsrMatrix* A, const T*
         const uint32_t per_wg = kWavesPerWg * bpw;
         const uint32_t wgs = (numBlocks + per_wg - 1) / per_wg;
         if (ext)
-            hipLaunchKernelGGL((csrmv_adaptive_kernel<T, true>), dim3(wgs), dim3(64 * kWavesPerWg), 0, s, vals,
+            hipLaunchKernelGGL((csrmv_adaptive_kernel<T, T, true>), dim3(wgs), dim3(64 * kWavesPerWg), 0, s, vals,
                                cols, rp, x, y, m->d_desc, numBlocks, bpw, alpha, beta,
                                m->d_partials, done, A->num_cols);
         else
-            hipLaunchKernelGGL((csrmv_adaptive_kernel<T, false>), dim3(wgs), dim3(64 * kWavesPerWg), 0, s,
+            hipLaunchKernelGGL((csrmv_adaptive_kernel<T, T, false>), dim3(wgs), dim3(64 * kWavesPerWg), 0, s,
                                vals, cols, rp, x, y, m->d_desc, numBlocks, bpw, alpha, beta,
                                m->d_partials, done, A->num_cols);
         CLS_HIP_TRY(hipGetLastError());
@@ -479,6 +490,35 @@ clsparseStatus launch_csrmv(const T* alpha, const clsparseCsrMatrix* A, const T*
         else
             hipLaunchKernelGGL((csrmv_longrow_combine<T, false>), dim3(g), dim3(64), 0, s,
                                m->d_long, m->num_long_rows, m->d_partials, y, beta, done);
+        CLS_HIP_TRY(hipGetLastError());
+    }
+    return clsparseSuccess;
+}
+
+// FP32 SpMV whose matrix values are BF16 bit patterns (CSR-Adaptive only).
+clsparseStatus launch_csrmv_bf16(const float* alpha, const clsparseCsrMatrix* A, const float* x,
+                                 const float* beta, float* y, hipStream_t s, const int* done)
+{
+    if (A->num_rows == 0) return clsparseSuccess;
+    const matrix_meta* m = get_meta(A);
+    if (m == nullptr || m->magic != 0x4d455441u || m->rowBlockSize == 0 || m->num_rows != A->num_rows ||
+        m->num_nonzeros != A->num_nonzeros)
+        return clsparseStructInvalid;
+    const uint16_t* vals = dev_ptr<uint16_t>(A->values, A->off_values);
+    const uint32_t* cols = dev_ptr<uint32_t>(A->col_indices, A->off_col_indices);
+    const uint32_t* rp = dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer);
+    const uint32_t numBlocks = m->entries - 1;
+    if (numBlocks > 0) {
+        const uint32_t wgs = (numBlocks + kWavesPerWg - 1) / kWavesPerWg;
+        hipLaunchKernelGGL((csrmv_adaptive_kernel<float, uint16_t, false>), dim3(wgs), dim3(64 * kWavesPerWg), 0,
+                           s, vals, cols, rp, x, y, m->d_desc, numBlocks, 1u, alpha, beta, m->d_partials,
+                           done, A->num_cols);
+        CLS_HIP_TRY(hipGetLastError());
+    }
+    if (m->num_long_rows > 0) {
+        const uint32_t g = (m->num_long_rows + 63) / 64;
+        hipLaunchKernelGGL((csrmv_longrow_combine<float, false>), dim3(g), dim3(64), 0, s, m->d_long,
+                           m->num_long_rows, m->d_partials, y, beta, done);
         CLS_HIP_TRY(hipGetLastError());
     }
     return clsparseSuccess;

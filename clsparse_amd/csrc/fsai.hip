@@ -455,7 +455,8 @@ __global__ __launch_bounds__(256) void feature_kernel(uint32_t nnzG, const uint3
     dst[1] = uint4{w[4], w[5], w[6], w[7]};
 }
 
-// G_k = (delta + o_k) dhat_i, scattered to G (CSR) and G^T (CSR) in both precisions.
+// G_k = (delta + o_k) dhat_i, scattered to G (CSR) and G^T (CSR) in both
+// precisions; the FP32 assembly also writes BF16 copies (g16, gt16).
 template <typename T>
 __global__ __launch_bounds__(256) void assemble_kernel(uint32_t nnzG, const uint32_t* __restrict__ grow,
                                                        const uint32_t* __restrict__ gcol,
@@ -463,6 +464,8 @@ __global__ __launch_bounds__(256) void assemble_kernel(uint32_t nnzG, const uint
                                                        const float* __restrict__ o,
                                                        const double* __restrict__ dhat,
                                                        T* __restrict__ gval, T* __restrict__ gtval,
+                                                       uint16_t* __restrict__ g16,
+                                                       uint16_t* __restrict__ gt16,
                                                        int* __restrict__ bad)
 {
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
@@ -472,6 +475,11 @@ __global__ __launch_bounds__(256) void assemble_kernel(uint32_t nnzG, const uint
     if (!isfinite(g)) *bad = 1;
     gval[k] = static_cast<T>(g);
     gtval[tpos[k]] = static_cast<T>(g);
+    if (g16) {
+        const uint16_t h = f2bf(static_cast<float>(g));
+        g16[k] = h;
+        gt16[tpos[k]] = h;
+    }
 }
 
 // Hutchinson gradient for one probe: gG[k] += c (u_i v_j + w_i z_j).
@@ -562,8 +570,10 @@ struct _clsparseAmdFsai
     float* o = nullptr;     // network output per G nonzero
     double *g64 = nullptr, *gt64 = nullptr;
     float *g32 = nullptr, *gt32 = nullptr;
+    uint16_t *g16 = nullptr, *gt16 = nullptr;  // BF16 copies of the FP32 values
     int* bad = nullptr;
     clsparseCsrMatrix G64{}, GT64{}, G32{}, GT32{};
+    clsparseCsrMatrix G16{}, GT16{};  // share G32's / GT32's metadata (not owned)
     bool valid = true;
     // Training scratch (FP64 vectors of length n, and the G gradient).
     double *t = nullptr, *u = nullptr, *s = nullptr, *v = nullptr, *t2 = nullptr, *w = nullptr,
@@ -608,7 +618,7 @@ void free_fsai(_clsparseAmdFsai* f)
     clsparseCsrMetaDelete(&f->G32);
     clsparseCsrMetaDelete(&f->GT32);
     void* ptrs[] = {f->g_rp, f->g_ci, f->gt_rp, f->gt_ci, f->g_row, f->g_src, f->g_tpos,
-                    f->dhat, f->stats, f->X, f->o, f->g64, f->gt64, f->g32, f->gt32, f->bad,
+                    f->dhat, f->stats, f->X, f->o, f->g64, f->gt64, f->g32, f->gt32, f->g16, f->gt16, f->bad,
                     f->t, f->u, f->s, f->v, f->t2, f->w, f->z, f->gG, f->sc, f->gout};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -778,6 +788,8 @@ extern "C" clsparseStatus clsparseAmdFsaiCreate(clsparseAmdFsai* out, const clsp
     if (e == hipSuccess) e = dmalloc(&f->gt64, nnzG);
     if (e == hipSuccess) e = dmalloc(&f->g32, nnzG);
     if (e == hipSuccess) e = dmalloc(&f->gt32, nnzG);
+    if (e == hipSuccess) e = dmalloc(&f->g16, nnzG);
+    if (e == hipSuccess) e = dmalloc(&f->gt16, nnzG);
     if (e == hipSuccess) e = dmalloc(&f->bad, 1);
     if (e != hipSuccess) {
         free_fsai(f);
@@ -799,6 +811,10 @@ extern "C" clsparseStatus clsparseAmdFsaiCreate(clsparseAmdFsai* out, const clsp
     if (s == clsparseSuccess) s = clsparseCsrMetaCreate(&f->GT64, control);
     if (s == clsparseSuccess) s = clsparseCsrMetaCreate(&f->G32, control);
     if (s == clsparseSuccess) s = clsparseCsrMetaCreate(&f->GT32, control);
+    f->G16 = f->G32;
+    f->G16.values = f->g16;
+    f->GT16 = f->GT32;
+    f->GT16.values = f->gt16;
     if (s != clsparseSuccess) {
         free_fsai(f);
         return s;
@@ -859,9 +875,11 @@ clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_
     CLS_HIP_TRY(hipMemsetAsync(f->bad, 0, sizeof(int), st));
     const float* o = net ? f->o : nullptr;
     hipLaunchKernelGGL((assemble_kernel<double>), dim3(blocks_for(f->nnzG)), dim3(256), 0, st, f->nnzG,
-                       f->g_row, f->g_ci, f->g_tpos, o, f->dhat, f->g64, f->gt64, f->bad);
+                       f->g_row, f->g_ci, f->g_tpos, o, f->dhat, f->g64, f->gt64,
+                       static_cast<uint16_t*>(nullptr), static_cast<uint16_t*>(nullptr), f->bad);
     hipLaunchKernelGGL((assemble_kernel<float>), dim3(blocks_for(f->nnzG)), dim3(256), 0, st, f->nnzG,
-                       f->g_row, f->g_ci, f->g_tpos, o, f->dhat, f->g32, f->gt32, f->bad);
+                       f->g_row, f->g_ci, f->g_tpos, o, f->dhat, f->g32, f->gt32, f->g16, f->gt16,
+                       f->bad);
     CLS_HIP_TRY(hipGetLastError());
     int bad = 0;
     CLS_HIP_TRY(hipMemcpyAsync(&bad, f->bad, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -995,6 +1013,8 @@ extern "C" clsparseStatus clsparseAmdFsaiTrainStep(clsparseAmdFsai f, clsparseAm
 const clsparseCsrMatrix* fsai_G32(clsparseAmdFsai f) { return &f->G32; }
 const clsparseCsrMatrix* fsai_GT32(clsparseAmdFsai f) { return &f->GT32; }
 const clsparseCsrMatrix* fsai_G64(clsparseAmdFsai f) { return &f->G64; }
+const clsparseCsrMatrix* fsai_G16(clsparseAmdFsai f) { return &f->G16; }
+const clsparseCsrMatrix* fsai_GT16(clsparseAmdFsai f) { return &f->GT16; }
 const clsparseCsrMatrix* fsai_GT64(clsparseAmdFsai f) { return &f->GT64; }
 bool fsai_valid(clsparseAmdFsai f) { return f->valid; }
 uint32_t fsai_n(clsparseAmdFsai f) { return f->n; }

@@ -133,13 +133,18 @@ CLSPARSE_EXPORT clsparseStatus
                              cl_uint probes, cl_ulong seed, cl_float lr, clsparseControl control,
                              cl_double* loss, cl_double* grad_g);
 
-/* The mixed-precision solver of clsparseAmdDcsrcgMixed with M = G G^T
- * (two CSR-Adaptive SpMVs per iteration) in place of Jacobi. Falls back to
- * Jacobi for the rest of the solve when G holds a non-finite value or a
- * missing diagonal, when the recurrence residual turns non-finite (x is
- * restored to the last replacement point), or when it fails to halve within
- * 16 chunks of 32 iterations. The FP64 true residual decides convergence;
- * stats.preconditioner reports 2 (FSAI throughout) or 3 (fell back). */
+/* The mixed-precision solver of clsparseAmdDcsrcgMixed with M = G G^T in
+ * place of Jacobi: two CSR-Adaptive SpMVs per iteration, with G's values
+ * stored in BF16 (M stays symmetric positive definite; CLSPARSE_AMD_FSAI_BF16=0
+ * keeps FP32). Jacobi takes over for the rest of the solve when G holds a
+ * non-finite value or lacks a diagonal entry; when the recurrence residual
+ * turns non-finite (x goes back to the last replacement point); when the
+ * residual fails to halve within 16 chunks of 32 iterations; or when a pilot
+ * race -- 3 chunks with G, then 3 with Jacobi, each timed -- finds Jacobi
+ * reducing the residual faster per second (CLSPARSE_AMD_FSAI_POLICY=always
+ * skips the race). The FP64 true residual decides convergence;
+ * stats.preconditioner reports 2 (G to the end) or 3 (Jacobi took over) and
+ * stats.pilot_rate_* the two measured rates. */
 CLSPARSE_EXPORT clsparseStatus
     clsparseAmdDcsrcgFsai(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
                           clsparseAmdFsai fsai, clSParseSolverControl solverControl,

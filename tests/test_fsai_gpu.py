@@ -182,6 +182,7 @@ def _solve_jacobi_mixed(ctl, A, b, tol=1e-8, max_iters=20000):
 
 def test_fsai_solver_with_jacobi_factor_matches_jacobi(ctl, monkeypatch):
     monkeypatch.setenv("CLSPARSE_AMD_FSAI_POLICY", "always")
+    monkeypatch.setenv("CLSPARSE_AMD_FSAI_BF16", "0")  # FP32 G = D^-1/2 exactly
     rp, ci, v, m, A = _sample_matrix(ctl, 40, 40, seed=12)
     b = cls.gen_uniform(m, -1, 1, seed=4)
     f = cls.Fsai(A, ctl)
@@ -278,3 +279,31 @@ def test_pilot_race_picks_a_preconditioner_and_converges(ctl, monkeypatch):
         assert np.linalg.norm(b - Acpu @ x) / np.linalg.norm(b) <= 1e-8
     finally:
         f.release()
+
+
+def test_bf16_factor_matches_fp32_factor(ctl, monkeypatch):
+    """The BF16 copy of a trained G preconditions as well as the FP32 one."""
+    monkeypatch.setenv("CLSPARSE_AMD_FSAI_POLICY", "always")
+    from clsparse_amd import fsai as fs
+    params, _ = fs.load_weights()
+    rp, ci, v, m = _aniso(64, 64, seed=77, ratio=200.0, theta=0.3)
+    A = cls.CsrMatrix(rp, ci, v, m)
+    A.meta_create(ctl)
+    net = cls.Mlp(seed=1)
+    net.set_params(params)
+    f = cls.Fsai(A, ctl)
+    try:
+        f.apply_net(net, ctl)
+        b = cls.gen_uniform(m, -1, 1, seed=9)
+        say("fsai: BF16 vs FP32 factor")
+        monkeypatch.setenv("CLSPARSE_AMD_FSAI_BF16", "0")
+        st32, _ = _solve_fsai(ctl, A, f, b)
+        monkeypatch.setenv("CLSPARSE_AMD_FSAI_BF16", "1")
+        st16, x16 = _solve_fsai(ctl, A, f, b)
+        assert st32.status_detail == 0 and st16.status_detail == 0
+        assert st16.iterations <= 1.05 * st32.iterations + 8
+        Acpu = sp.csr_matrix((v, ci, rp), shape=(m, m))
+        assert np.linalg.norm(b - Acpu @ x16) / np.linalg.norm(b) <= 1e-8
+    finally:
+        f.release()
+        net.release()
