@@ -116,7 +116,24 @@ class SpmvRun:
         self.A.meta_delete()
 
 
-def solver_workload(cls, torch, ctl, n1d=128, tol=1e-10):
+def cpu_cg_baseline(rp, ci, v, b, iters_needed, sample_iters=24):
+    """The oracle's FP64 Jacobi-CG (single thread, the reference's algorithm) on
+    `sample_iters` iterations of the same system; seconds per iteration and the
+    projection to the GPU solve's iteration count."""
+    import numpy as np
+
+    import oracle
+    t0 = time.perf_counter()
+    _, r = oracle.cg(rp, ci, v, b, np.zeros(rp.size - 1), 1, sample_iters, 1e-300, 0.0)
+    dt = time.perf_counter() - t0
+    per = dt / max(r.executed, 1)
+    return {"kind": "port", "cores": 1, "seconds_per_iteration": round(per, 5),
+            "projected_seconds": round(per * iters_needed, 3),
+            "sample": f"{r.executed} iterations of the oracle's FP64 Jacobi-CG on the same matrix, "
+                      "single thread, projected to the FP64 GPU solve's iteration count"}
+
+
+def solver_workload(cls, torch, ctl, n1d=128, tol=1e-10, cpu=True):
     """Mixed-precision refinement vs FP64 Jacobi-PCG to an FP64 true relative residual."""
     import numpy as np
     rp, ci, v, m = cls.gen_diffusion3d(n1d, n1d, n1d, sigma=1.0, seed=3)
@@ -148,11 +165,13 @@ def solver_workload(cls, torch, ctl, n1d=128, tol=1e-10):
         out[name] = best
     out["speedup_mixed_vs_fp64"] = round(out["fp64_jacobi_pcg"]["seconds"] /
                                          out["mixed_fp32_inner"]["seconds"], 3)
+    if cpu:
+        out["cpu_baseline"] = cpu_cg_baseline(rp, ci, v, b, out["fp64_jacobi_pcg"]["iterations"])
     A.meta_delete()
     return out
 
 
-def fsai_workload(cls, torch, ctl, n1d=160, tol=1e-8, weights=None):
+def fsai_workload(cls, torch, ctl, n1d=160, tol=1e-8, weights=None, cpu=True):
     """Learned-FSAI mixed PCG vs Jacobi (mixed, FP64) on the held-out 3-D problem."""
     import numpy as np
 
@@ -220,6 +239,9 @@ def fsai_workload(cls, torch, ctl, n1d=160, tol=1e-8, weights=None):
     out["speedup_jacobi_mixed_vs_fp64_jacobi"] = round(base / out["jacobi_mixed"]["seconds"], 3)
     out["iteration_ratio_learned_vs_untrained"] = round(
         out["learned_fsai_mixed"]["iterations"] / out["untrained_fsai_mixed"]["iterations"], 3)
+    if cpu:
+        out["cpu_baseline"] = cpu_cg_baseline(rp, ci, v, b, out["fp64_jacobi_pcg"]["iterations"],
+                                              sample_iters=12)
     f.release()
     net.release()
     A.meta_delete()
@@ -380,9 +402,9 @@ def main():
 
     extra = {}
     if not args.no_solver:
-        extra["solver"] = solver_workload(cls, torch, ctl)
+        extra["solver"] = solver_workload(cls, torch, ctl, cpu=not args.no_cpu)
     if not args.no_fsai:
-        extra["fsai"] = fsai_workload(cls, torch, ctl, weights=args.weights)
+        extra["fsai"] = fsai_workload(cls, torch, ctl, weights=args.weights, cpu=not args.no_cpu)
 
     out = {
         "metric": METRIC,
