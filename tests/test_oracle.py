@@ -294,3 +294,66 @@ def test_csrmm_oracle_matches_scipy_and_special_cases():
     np.testing.assert_array_equal(out, 2.0 * C)
     out, _ = oracle.csrmm(rp, ci, v, B, np.full((400, 5), np.nan), alpha=1.0, beta=0.0)
     assert np.all(np.isfinite(out))
+
+
+def test_bicgstab_iteration_count_depends_on_summation_order():
+    """Why the GPU parity test compares BiCGStab iteration counts within 10 %:
+    a numpy restatement with sequential dots matches the oracle exactly, and
+    only reordering the dot-product sums moves the count."""
+    import scipy.sparse as sp
+    nx, ny, c = 48, 40, 0.4
+    n = nx * ny
+    lap = sp.diags([-np.ones(n - nx), -np.ones(n - 1), 4 * np.ones(n), -np.ones(n - 1),
+                    -np.ones(n - nx)], [-nx, -1, 0, 1, nx], format="lil")
+    for i in range(1, n):
+        if i % nx == 0:
+            lap[i, i - 1] = 0.0
+            lap[i - 1, i] = 0.0
+    a = sp.csr_matrix(lap)
+    conv = sp.diags([c * np.ones(n), -c * np.ones(n - 1)], [0, -1], format="lil")
+    for i in range(1, n):
+        if i % nx == 0:
+            conv[i, i - 1] = 0.0
+    a = sp.csr_matrix(a + conv)
+    a.eliminate_zeros()
+    a.sort_indices()
+    b = np.random.default_rng(21).uniform(-1, 1, n)
+
+    def run(dot):
+        inv = 1 / a.diagonal()
+        x = np.zeros(n)
+        r = b.copy()
+        nb = np.abs(b).sum()
+        p, rs = r.copy(), r.copy()
+        rr = dot(rs, r)
+        for it in range(2000):
+            mp = p * inv
+            amp = a @ mp
+            alpha = rr / dot(rs, amp)
+            s = r - alpha * amp
+            if np.abs(s).sum() / nb <= 1e-10:
+                return it
+            ms = s * inv
+            ams = a @ ms
+            omega = dot(ams, s) / dot(ams, ams)
+            x = x + alpha * mp + omega * ms
+            r = s - omega * ams
+            if np.abs(r).sum() / nb <= 1e-10:
+                return it
+            rrn = dot(rs, r)
+            beta = rrn / rr * alpha / omega
+            rr = rrn
+            p = beta * p + r - beta * omega * amp
+        return 2000
+
+    def seq(u, w):
+        acc = 0.0
+        for t in (u * w).tolist():
+            acc += t
+        return acc
+
+    _, ro = oracle.bicgstab(a.indptr.astype(np.uint32), a.indices.astype(np.uint32), a.data, b,
+                            np.zeros(n), 1, 2000, 1e-10, 0.0)
+    counts = {run(seq), run(lambda u, w: float(np.sum(u * w))), run(lambda u, w: seq(u[::-1], w[::-1]))}
+    assert ro.n_iters in counts or min(abs(ro.n_iters - k) for k in counts) <= 2
+    assert max(counts) - min(counts) <= 0.15 * max(counts)
