@@ -110,6 +110,9 @@ struct matrix_meta
     uint64_t* d_rowBlocks = nullptr;  // rowBlockSize words; entries filled, rest zero
     std::vector<uint64_t> h_rowBlocks;  // host copy of the filled entries
     BlockDesc* d_desc = nullptr;      // entries - 1 kernel descriptors
+    // Stream blocks of 65..511 rows (the row-wise path) as a fraction of all
+    // blocks: near 1 for stencil and finite-element matrices.
+    double rowwise_fraction = 0.0;
 
     // Long-row combine tables (deterministic replacement of the reference's
     // atomic + spin-flag protocol, csrmv_adaptive.cl L564-686).

@@ -269,7 +269,9 @@ extern "C" clsparseStatus clsparseCsrMetaCreate(clsparseCsrMatrix* csrMatx,
             d.cnt = static_cast<uint32_t>(ve - vs);
         }
         desc[e] = d;
+        if (d.nrows > 64 && d.nrows < 512) m->rowwise_fraction += 1.0;
     }
+    if (!desc.empty()) m->rowwise_fraction /= static_cast<double>(desc.size());
 
     hipError_t err = hipMalloc(&m->d_rowBlocks, size_words * sizeof(uint64_t));
     if (err == hipSuccess)

@@ -31,14 +31,21 @@ namespace {
 constexpr int kWavesPerWg = 1;  // one wave per workgroup: each wave frees its LDS on exit
 constexpr int kStreamPerLane = kBlkSize / 64;  // 16 nnz per lane per stream block
 
-template <typename T, bool EXT>
+// LDS of one stream block: products (or alpha * values on the row-wise path),
+// the columns for the row-wise path, the product errors in extended mode.
+template <typename T, bool EXT, bool ROWWISE>
 struct StreamLds
 {
-    T p[kWavesPerWg][kBlkSize];         // products, or alpha * values (row-wise blocks)
-    uint32_t c[kWavesPerWg][kBlkSize];  // columns (row-wise blocks)
+    T p[kWavesPerWg][kBlkSize];
 };
 template <typename T>
-struct StreamLds<T, true>
+struct StreamLds<T, false, true>
+{
+    T p[kWavesPerWg][kBlkSize];
+    uint32_t c[kWavesPerWg][kBlkSize];
+};
+template <typename T, bool ROWWISE>
+struct StreamLds<T, true, ROWWISE>
 {
     T p[kWavesPerWg][kBlkSize];
     T e[kWavesPerWg][kBlkSize];
@@ -141,7 +148,9 @@ __device__ __forceinline__ T mul_rounded(T a, T b)
 // block's values so the reduction needs no dependent global loads.
 constexpr int kRowSlots = 8;
 
-template <typename T, typename TV, bool EXT>
+// ROWWISE: compile the row-wise path (its column staging costs LDS and with it
+// occupancy, which the gather-bound matrices with irregular rows need more).
+template <typename T, typename TV, bool EXT, bool ROWWISE>
 __global__ __launch_bounds__(64 * kWavesPerWg) void csrmv_adaptive_kernel(
     const TV* __restrict__ vals, const uint32_t* __restrict__ cols,
     const uint32_t* __restrict__ rowPtr, const T* __restrict__ x, T* __restrict__ y,
@@ -149,7 +158,7 @@ __global__ __launch_bounds__(64 * kWavesPerWg) void csrmv_adaptive_kernel(
     const T* __restrict__ pAlpha, const T* __restrict__ pBeta, double* __restrict__ partials,
     const int* __restrict__ done, uint32_t ncols)
 {
-    __shared__ StreamLds<T, EXT> lds;
+    __shared__ StreamLds<T, EXT, ROWWISE> lds;
 
     if (done != nullptr && *done != 0) return;
 
@@ -189,7 +198,7 @@ __global__ __launch_bounds__(64 * kWavesPerWg) void csrmv_adaptive_kernel(
         // rows, so one gather instruction reads x at the same offset from 64
         // consecutive rows -- a few cache lines instead of one per row run.
         // The block's alpha*values and columns are staged in LDS for that.
-        const bool rowwise = !EXT && in_regs && nrows > 64;
+        const bool rowwise = ROWWISE && !EXT && in_regs && nrows > 64;
         {
             // 16-byte loads: lane owns elements 4*lane + 256*k .. +3.
             TV v[kStreamPerLane];
@@ -210,7 +219,7 @@ __global__ __launch_bounds__(64 * kWavesPerWg) void csrmv_adaptive_kernel(
                 }
             }
             if (rowwise) {
-                if constexpr (!EXT) {
+                if constexpr (!EXT && ROWWISE) {
 #pragma unroll
                     for (int k = 0; k < kStreamPerLane; ++k) {
                         const uint32_t i = 256 * (k / 4) + 4 * lane + (k % 4);
@@ -256,7 +265,7 @@ __global__ __launch_bounds__(64 * kWavesPerWg) void csrmv_adaptive_kernel(
             acc = group_reduce(acc, tpr);
             if (t == 0 && lr < nrows) write_row<T, EXT>(y, row + lr, acc, beta);
         } else if (rowwise) {
-            if constexpr (!EXT) {
+            if constexpr (!EXT && ROWWISE) {
                 // Each lane walks whole rows: 8 gathers in flight, each
                 // product rounded on its own and added in row order (the
                 // same sums as the product-staging path).
@@ -472,13 +481,20 @@ clsparseStatus launch_csrmv(const T* alpha, const clsparseCsrMatrix* A, const T*
         if (const char* e = std::getenv("CLSPARSE_AMD_BPW")) bpw = std::max(1, std::atoi(e));
         const uint32_t per_wg = kWavesPerWg * bpw;
         const uint32_t wgs = (numBlocks + per_wg - 1) / per_wg;
+        // FP32 always compiles the row-wise path (its LDS still fits five waves
+        // per SIMD); FP64 only where nearly every block takes it.
+        const bool rowwise = sizeof(T) == 4 || m->rowwise_fraction >= 0.9;
         if (ext)
-            hipLaunchKernelGGL((csrmv_adaptive_kernel<T, T, true>), dim3(wgs), dim3(64 * kWavesPerWg), 0, s, vals,
-                               cols, rp, x, y, m->d_desc, numBlocks, bpw, alpha, beta,
+            hipLaunchKernelGGL((csrmv_adaptive_kernel<T, T, true, false>), dim3(wgs), dim3(64 * kWavesPerWg), 0,
+                               s, vals, cols, rp, x, y, m->d_desc, numBlocks, bpw, alpha, beta,
+                               m->d_partials, done, A->num_cols);
+        else if (rowwise)
+            hipLaunchKernelGGL((csrmv_adaptive_kernel<T, T, false, true>), dim3(wgs), dim3(64 * kWavesPerWg), 0,
+                               s, vals, cols, rp, x, y, m->d_desc, numBlocks, bpw, alpha, beta,
                                m->d_partials, done, A->num_cols);
         else
-            hipLaunchKernelGGL((csrmv_adaptive_kernel<T, T, false>), dim3(wgs), dim3(64 * kWavesPerWg), 0, s,
-                               vals, cols, rp, x, y, m->d_desc, numBlocks, bpw, alpha, beta,
+            hipLaunchKernelGGL((csrmv_adaptive_kernel<T, T, false, false>), dim3(wgs), dim3(64 * kWavesPerWg), 0,
+                               s, vals, cols, rp, x, y, m->d_desc, numBlocks, bpw, alpha, beta,
                                m->d_partials, done, A->num_cols);
         CLS_HIP_TRY(hipGetLastError());
     }
@@ -510,7 +526,7 @@ clsparseStatus launch_csrmv_bf16(const float* alpha, const clsparseCsrMatrix* A,
     const uint32_t numBlocks = m->entries - 1;
     if (numBlocks > 0) {
         const uint32_t wgs = (numBlocks + kWavesPerWg - 1) / kWavesPerWg;
-        hipLaunchKernelGGL((csrmv_adaptive_kernel<float, uint16_t, false>), dim3(wgs), dim3(64 * kWavesPerWg), 0,
+        hipLaunchKernelGGL((csrmv_adaptive_kernel<float, uint16_t, false, true>), dim3(wgs), dim3(64 * kWavesPerWg), 0,
                            s, vals, cols, rp, x, y, m->d_desc, numBlocks, 1u, alpha, beta, m->d_partials,
                            done, A->num_cols);
         CLS_HIP_TRY(hipGetLastError());
