@@ -203,7 +203,9 @@ def fsai_workload(cls, torch, ctl, n1d=160, tol=1e-8, weights=None, cpu=True):
 
     def solve(fn, with_fsai):
         best = None
-        for _ in range(2):  # the second solve is timed (the first captures graphs)
+        first = None
+        for _ in range(2):  # the second solve is the reported one (the first captures graphs
+            # and, for the learned factor, runs the pilot race whose outcome the factor keeps)
             x = torch.zeros(m, dtype=torch.float64, device="cuda")
             sc = cls.SolverControl(cls.DIAGONAL, 50000, tol, 0.0)
             xd, bd = cls.DenseVector(x), cls.DenseVector(bt)
@@ -223,6 +225,9 @@ def fsai_workload(cls, torch, ctl, n1d=160, tol=1e-8, weights=None, cpu=True):
             best = {"seconds": round(dt, 5), "iterations": s.iterations,
                     "replacements": s.outer_iterations, "true_rel_residual": s.true_residual,
                     "converged": s.status_detail == 0, "preconditioner": s.preconditioner}
+            if first is None:
+                first = {"seconds": best["seconds"], "iterations": best["iterations"]}
+        best["first_solve"] = first
         return best
 
     out["learned_fsai_mixed"] = solve(lib.clsparseAmdDcsrcgFsai, True)

@@ -566,7 +566,7 @@ constexpr int kPilotChunks = 3;        // chunks per preconditioner in the pilot
 
 // CLSPARSE_AMD_FSAI_POLICY=always keeps the learned factor unless it breaks
 // down or stagnates; the default ("race") also runs the pilot race.
-bool fsai_race()
+bool race_enabled()
 {
     const char* e = std::getenv("CLSPARSE_AMD_FSAI_POLICY");
     return !(e && std::strcmp(e, "always") == 0);
@@ -749,6 +749,17 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     CgEngine<TI, double> eng_main(&AI, xp, rI, n, prec, kCritL2, buf, s, false, G, GT, g_bf16);
     CgEngine<TI, double> eng_jac(&AI, xp, rI, n, kPrecJacobi, kCritL2, buf, s, false);
     CgEngine<TI, double>* eng = &eng_main;
+    // A pilot race already run for this factor decides without a new one
+    // (the factor object forgets it when G changes).
+    int race_known = 0;
+    double known_rf = 0.0, known_rj = 0.0;
+    if (fsai && prec == kPrecFsai && race_enabled()) {
+        race_known = fsai_race(fsai, &known_rf, &known_rj);
+        if (race_known == 2) {
+            eng = &eng_jac;
+            used = 3;
+        }
+    }
 
     // (Re)starts the TI loop from the current x: r = (TI)(b - A x), z = M r, p = z.
     auto restart = [&]() -> clsparseStatus {
@@ -781,7 +792,7 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
         // -ln(res_end / res_start) / seconds is measured over all chunks but
         // the first of each (which captures the graph). The faster one goes on.
         enum Phase { kPilotFsai, kPilotJacobi, kMain };
-        Phase phase = (eng->prec() == kPrecFsai && fsai_race()) ? kPilotFsai : kMain;
+        Phase phase = (eng->prec() == kPrecFsai && race_enabled() && race_known == 0) ? kPilotFsai : kMain;
         int phase_chunks = 0;
         double phase_res0 = 0.0, rate_fsai = 0.0, rate_jac = 0.0;
         auto phase_t0 = std::chrono::steady_clock::now();
@@ -829,6 +840,7 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
                     } else {
                         rate_jac = rate;
                         phase = kMain;
+                        fsai_set_race(fsai, rate_fsai > rate_jac ? 1 : 2, rate_fsai, rate_jac);
                         if (rate_fsai > rate_jac) {
                             used = 2;
                             CLS_TRY(switch_to(&eng_main, &over));
@@ -893,8 +905,8 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
             CLS_HIP_TRY(hipGetLastError());
             res_at_replace = rel;
         }
-        rate_fsai_out = rate_fsai;
-        rate_jac_out = rate_jac;
+        rate_fsai_out = race_known ? known_rf : rate_fsai;
+        rate_jac_out = race_known ? known_rj : rate_jac;
     }
     CLS_TRY(leave_solver_stream(control));
     const auto t1 = std::chrono::steady_clock::now();

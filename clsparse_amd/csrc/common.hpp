@@ -190,6 +190,9 @@ const clsparseCsrMatrix* fsai_GT64(clsparseAmdFsai f);
 const clsparseCsrMatrix* fsai_G16(clsparseAmdFsai f);   // BF16 values, FP32 compute
 const clsparseCsrMatrix* fsai_GT16(clsparseAmdFsai f);
 bool fsai_valid(clsparseAmdFsai f);
+// Pilot-race outcome stored with the factor (0 not run, 1 G, 2 Jacobi).
+int fsai_race(clsparseAmdFsai f, double* rate_fsai, double* rate_jacobi);
+void fsai_set_race(clsparseAmdFsai f, int outcome, double rate_fsai, double rate_jacobi);
 uint32_t fsai_n(clsparseAmdFsai f);
 
 // FP32 CSR-Adaptive SpMV whose values are stored as BF16 bit patterns

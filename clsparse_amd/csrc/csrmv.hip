@@ -143,7 +143,34 @@ __device__ __forceinline__ T mul_rounded(T a, T b)
     return a * b;
 }
 
-// Row offsets (relative to the block's first nnz) of rows row .. row+255 held
+// One row's sum from the block's staged alpha*values and columns: W gathers
+// in flight per step, products added in row order. Indices past the row stay
+// inside the tile; their gathers are masked off.
+template <int W, typename T, bool EXT, typename XG>
+__device__ __forceinline__ Acc<T, EXT> row_from_lds(const T* __restrict__ buf,
+                                                    const uint32_t* __restrict__ cols, uint32_t s,
+                                                    uint32_t e, const XG& xg)
+{
+    Acc<T, EXT> acc;
+    for (uint32_t j = s; j < e; j += W) {
+        T pr[W];
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+            const uint32_t idx = min(j + u, uint32_t(kBlkSize - 1));
+            const T a = buf[idx];
+            const uint32_t col = cols[idx];
+            T xv = T(0);
+            if (j + u < e) xv = xg(col);
+            pr[u] = mul_rounded(a, xv);
+        }
+#pragma unroll
+        for (int u = 0; u < W; ++u)
+            if (j + u < e) acc.add(pr[u]);
+    }
+    return acc;
+}
+
+// Row offsets (relative to the block's first nnz) of rows row .. row+511 held
 // in registers: lane l keeps row l + 64k in slot k. Loaded together with the
 // block's values so the reduction needs no dependent global loads.
 constexpr int kRowSlots = 8;
@@ -266,9 +293,11 @@ __global__ __launch_bounds__(64 * kWavesPerWg) void csrmv_adaptive_kernel(
             if (t == 0 && lr < nrows) write_row<T, EXT>(y, row + lr, acc, beta);
         } else if (rowwise) {
             if constexpr (!EXT && ROWWISE) {
-                // Each lane walks whole rows: 8 gathers in flight, each
-                // product rounded on its own and added in row order (the
-                // same sums as the product-staging path).
+                // Each lane walks whole rows, 4 or 8 gathers in flight (by
+                // the block's mean row length), each product rounded on its
+                // own and added in row order (the same sums as the
+                // product-staging path).
+                const bool narrow = cnt <= 4u * nrows;
 #pragma unroll
                 for (int k = 0; k < kRowSlots; ++k) {
                     const uint32_t nxt = __shfl(ro[k], static_cast<int>((lane + 1) & 63), 64);
@@ -277,22 +306,9 @@ __global__ __launch_bounds__(64 * kWavesPerWg) void csrmv_adaptive_kernel(
                     if (lr < nrows) {
                         const uint32_t s = ro[k];
                         const uint32_t e = (lane < 63) ? nxt : wrap;
-                        Acc<T, EXT> acc;
-                        for (uint32_t j = s; j < e; j += 8) {
-                            T pr[8];
-#pragma unroll
-                            for (int u = 0; u < 8; ++u) {
-                                const uint32_t idx = min(j + u, uint32_t(kBlkSize - 1));
-                                const T a = buf[idx];
-                                const uint32_t col = lds.c[wave][idx];
-                                T xv = T(0);
-                                if (j + u < e) xv = xg(col);
-                                pr[u] = mul_rounded(a, xv);
-                            }
-#pragma unroll
-                            for (int u = 0; u < 8; ++u)
-                                if (j + u < e) acc.add(pr[u]);
-                        }
+                        const Acc<T, EXT> acc = narrow
+                                                    ? row_from_lds<4, T, EXT>(buf, lds.c[wave], s, e, xg)
+                                                    : row_from_lds<8, T, EXT>(buf, lds.c[wave], s, e, xg);
                         write_row<T, EXT>(y, row + lr, acc, beta);
                     }
                 }

@@ -560,6 +560,10 @@ struct _clsparseAmdFsai
     int device = 0;
     uint32_t n = 0, nnzG = 0, nnzA = 0;
     bool full_diag = true;  // every row of A stores its diagonal (else G is singular)
+    // Outcome of the solver's pilot race for the current G (0 not run, 1 G,
+    // 2 Jacobi) and the two measured rates; reset whenever G changes.
+    int race = 0;
+    double race_rate_fsai = 0.0, race_rate_jacobi = 0.0;
     // Pattern of G (CSR, lower) and of G^T (CSR, upper); per-G-nonzero row,
     // column, source index in A and position in G^T.
     uint32_t *g_rp = nullptr, *g_ci = nullptr, *gt_rp = nullptr, *gt_ci = nullptr;
@@ -885,6 +889,7 @@ clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_
     CLS_HIP_TRY(hipMemcpyAsync(&bad, f->bad, sizeof(int), hipMemcpyDeviceToHost, st));
     CLS_HIP_TRY(hipStreamSynchronize(st));
     f->valid = (bad == 0) && f->full_diag;
+    f->race = 0;
     return clsparseSuccess;
 }
 
@@ -1017,4 +1022,16 @@ const clsparseCsrMatrix* fsai_G16(clsparseAmdFsai f) { return &f->G16; }
 const clsparseCsrMatrix* fsai_GT16(clsparseAmdFsai f) { return &f->GT16; }
 const clsparseCsrMatrix* fsai_GT64(clsparseAmdFsai f) { return &f->GT64; }
 bool fsai_valid(clsparseAmdFsai f) { return f->valid; }
+int fsai_race(clsparseAmdFsai f, double* rate_fsai, double* rate_jacobi)
+{
+    *rate_fsai = f->race_rate_fsai;
+    *rate_jacobi = f->race_rate_jacobi;
+    return f->race;
+}
+void fsai_set_race(clsparseAmdFsai f, int outcome, double rate_fsai, double rate_jacobi)
+{
+    f->race = outcome;
+    f->race_rate_fsai = rate_fsai;
+    f->race_rate_jacobi = rate_jacobi;
+}
 uint32_t fsai_n(clsparseAmdFsai f) { return f->n; }

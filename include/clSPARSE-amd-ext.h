@@ -142,7 +142,9 @@ CLSPARSE_EXPORT clsparseStatus
  * residual fails to halve within 16 chunks of 32 iterations; or when a pilot
  * race -- 3 chunks with G, then 3 with Jacobi, each timed -- finds Jacobi
  * reducing the residual faster per second (CLSPARSE_AMD_FSAI_POLICY=always
- * skips the race). The FP64 true residual decides convergence;
+ * skips the race). The race's outcome is kept with the factor and reused by
+ * later solves until the factor changes (clsparseAmdFsaiApplyNet,
+ * clsparseAmdFsaiUpdateValues). The FP64 true residual decides convergence;
  * stats.preconditioner reports 2 (G to the end) or 3 (Jacobi took over) and
  * stats.pilot_rate_* the two measured rates. */
 CLSPARSE_EXPORT clsparseStatus
