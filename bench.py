@@ -16,7 +16,8 @@ and by FP64 Jacobi-PCG.
 Other modes (diagnostics, not the bench line):
   --mode copy      plain streaming kernel of known size, to calibrate rocprof counters
   --mode locality  the same power-law rows with columns folded into smaller windows
-                   (x-gather locality sweep), and the 128^3 stencil SpMV in FP64/FP32
+                   (x-gather locality sweep), and stencil SpMVs: s64 / s32 (128^3),
+                   s32:160 (160^3, z faces x100), t32:160 (its lower triangle)
   --mode cg        only the solver workload
   --mode fsai      only the learned-FSAI workload: held-out 3-D anisotropic diffusion
                    on 160^3 (z faces x100, kappa ~ LogNormal(0, 1)) solved to an FP64 true
@@ -352,7 +353,7 @@ def main():
         res = []
         items = args.set.split(",")
         for it in items:
-            if it.startswith("s"):
+            if it[0] in "st":
                 continue
             bits = int(it)
             cif = (ci & np.uint32((1 << bits) - 1)) if bits < 21 else ci
@@ -363,16 +364,29 @@ def main():
                         "GBps": round(run.nbytes / (launch * 1e-3) / 1e9, 1)})
             run.close()
             log(str(res[-1]))
-        stencils = [it for it in items if it.startswith("s")]
-        if stencils:
-            rp3, ci3, v3, m3 = cls.gen_diffusion3d(128, 128, 128, sigma=1.0, seed=3)
-            x3 = cls.gen_uniform(m3, -1.0, 1.0, seed=5)
+        # Stencils: "s64" / "s32" (FP64 / FP32, 128^3), "s32:160" (size), and
+        # "t32:160" for the lower triangle (the pattern of the FSAI factor).
+        import scipy.sparse as sp
+        stencils = [it for it in items if it[0] in "st"]
+        cache = {}
         for it in stencils:
-            dt = np.float64 if it == "s64" else np.float32
+            kind, _, size = it.partition(":")
+            n1d = int(size) if size else 128
+            dt = np.float64 if kind[1:] == "64" else np.float32
+            if n1d not in cache:
+                rp3, ci3, v3, m3 = cls.gen_diffusion3d(n1d, n1d, n1d, sigma=1.0, ax=1.0, ay=1.0,
+                                                        az=100.0 if size else 1.0, seed=3)
+                cache[n1d] = (rp3, ci3, v3, m3, cls.gen_uniform(m3, -1.0, 1.0, seed=5))
+            rp3, ci3, v3, m3, x3 = cache[n1d]
+            if kind[0] == "t":
+                lo = sp.tril(sp.csr_matrix((v3, ci3, rp3), shape=(m3, m3))).tocsr()
+                lo.sort_indices()
+                rp3, ci3, v3 = lo.indptr.astype(np.uint32), lo.indices.astype(np.uint32), lo.data
             run = SpmvRun(cls, torch, rp3, ci3, v3, m3, x3, dt, ctl)
             wall, launch = run.time(args.steps, args.warmup, stream)
-            res.append({"matrix": f"diffusion3d 128^3 {np.dtype(dt).name}",
-                        "launch_us": round(launch * 1e3, 2),
+            what = "tril of " if kind[0] == "t" else ""
+            res.append({"matrix": f"{what}diffusion3d {n1d}^3 {np.dtype(dt).name}",
+                        "nnz": int(rp3[-1]), "launch_us": round(launch * 1e3, 2),
                         "GBps": round(run.nbytes / (launch * 1e-3) / 1e9, 1)})
             run.close()
             log(str(res[-1]))
