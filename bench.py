@@ -117,6 +117,26 @@ class SpmvRun:
         self.A.meta_delete()
 
 
+def poisson_workload(cls, torch, ctl, stream, cpu=True):
+    """FP32 SpMV on the 2-D 5-point Poisson matrix, 256 x 256 (the reference's
+    CPU-runnable case): microseconds per call beside the CPU oracle."""
+    import numpy as np
+    rp, ci, v, m = cls.gen_poisson2d(256, 256)
+    x = cls.gen_uniform(m, -1.0, 1.0, seed=2)
+    run = SpmvRun(cls, torch, rp, ci, v, m, x, np.float32, ctl)
+    wall, launch = run.time(200, 20, stream)
+    out = {"workload": "poisson2d_256x256_fp32_spmv", "n": m, "nnz": int(rp[-1]),
+           "launch_us": round(launch * 1e3, 2), "wall_us_per_call": round(wall * 1e3, 2),
+           "GBps": round(run.nbytes / (launch * 1e-3) / 1e9, 1),
+           "note": "3.4 MB: L2/Infinity-Cache resident, so not an HBM figure"}
+    run.close()
+    if cpu:
+        med, threads, calls = cpu_baseline(rp, ci, v, x, budget_s=1.0, max_calls=50)
+        out["cpu_baseline"] = {"us_per_call": round(med * 1e6, 1), "cores": threads, "kind": "port",
+                               "sample": f"{calls} calls, oracle OpenMP CSR SpMV (FP64 values)"}
+    return out
+
+
 def cpu_cg_baseline(rp, ci, v, b, iters_needed, sample_iters=24):
     """The oracle's FP64 Jacobi-CG (single thread, the reference's algorithm) on
     `sample_iters` iterations of the same system; seconds per iteration and the
@@ -419,7 +439,7 @@ def main():
                "sample": f"{calls} calls of the same FP64 power-law SpMV (full matrix), median; "
                          f"oracle OpenMP CSR SpMV, {threads} threads"}
 
-    extra = {}
+    extra = {"poisson_fp32": poisson_workload(cls, torch, ctl, stream, cpu=not args.no_cpu)}
     if not args.no_solver:
         extra["solver"] = solver_workload(cls, torch, ctl, cpu=not args.no_cpu)
     if not args.no_fsai:

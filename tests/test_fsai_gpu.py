@@ -321,3 +321,28 @@ def test_bf16_factor_matches_fp32_factor(ctl, monkeypatch):
     finally:
         f.release()
         net.release()
+
+
+def test_missing_diagonal_factor_falls_back_to_jacobi(ctl):
+    """A row of A without a stored diagonal makes G singular: the factor is
+    marked invalid when it is assembled and the solver runs Jacobi from the
+    start (that matrix is not positive definite, so only the choice of
+    preconditioner and a clean return are checked)."""
+    rp, ci, v, m = _aniso(20, 20, seed=31)
+    a = sp.csr_matrix((v, ci, rp), shape=(m, m)).tolil()
+    a[5, 5] = 0.0
+    a = sp.csr_matrix(a)
+    a.eliminate_zeros()
+    a.sort_indices()
+    assert a[5, 5] == 0 and a.indptr[6] - a.indptr[5] > 0
+    A = cls.CsrMatrix(a.indptr.astype(np.uint32), a.indices.astype(np.uint32), a.data, m)
+    A.meta_create(ctl)
+    f = cls.Fsai(A, ctl)
+    try:
+        b = cls.gen_uniform(m, -1, 1, seed=3)
+        say("fsai: missing diagonal -> Jacobi")
+        st, x = _solve_fsai(ctl, A, f, b, max_iters=50)
+        assert st.preconditioner == 3
+        assert st.iterations <= 50
+    finally:
+        f.release()
