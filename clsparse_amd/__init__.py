@@ -92,7 +92,8 @@ class clsparseAmdSolverStats(Structure):
                 ("n_iters", c_int32), ("max_iters", c_int32),
                 ("relative_tolerance", c_double), ("absolute_tolerance", c_double),
                 ("initial_residual", c_double), ("current_residual", c_double),
-                ("pilot_rate_fsai", c_double), ("pilot_rate_jacobi", c_double)]
+                ("pilot_rate_fsai", c_double), ("pilot_rate_jacobi", c_double),
+                ("a_diagonals", c_int32), ("g_diagonals", c_int32)]
 
 
 class clsparseAmdHostCsr(Structure):
@@ -186,6 +187,7 @@ _SIGS = {
                                            POINTER(c_double), POINTER(c_uint32)]),
     "clsparseAmdFsaiGetFeatures": (c_int32, [_V, POINTER(ctypes.c_uint16)]),
     "clsparseAmdFsaiGetOutput": (c_int32, [_V, POINTER(c_float)]),
+    "clsparseAmdFsaiGetDiagonals": (c_int32, [_V, POINTER(c_int32), POINTER(c_int32)]),
     "clsparseAmdFsaiTrainStep": (c_int32, [_V, _V, POINTER(clsparseCsrMatrix), c_uint32, c_uint64,
                                            c_float, _V, POINTER(c_double), POINTER(c_double)]),
     "clsparseAmdDcsrcgFsai": (c_int32, [POINTER(cldenseVector), POINTER(clsparseCsrMatrix),

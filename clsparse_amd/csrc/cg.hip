@@ -25,6 +25,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <type_traits>
+#include <vector>
 
 #include "common.hpp"
 #include "solver_common.hpp"
@@ -33,6 +34,11 @@ namespace {
 
 constexpr int kHist = 64;   // residual history ring (VERBOSE printing)
 constexpr int kChunk = 32;  // iterations per captured graph (even: both parities)
+// Rows per thread per step of the grid-stride loops: the loads of kRows rows
+// are issued together (the grid is capped at kReducePartialsMax workgroups,
+// so one row at a time leaves the kernels waiting on memory latency). Rows
+// are still visited, and partial sums formed, in grid-stride order.
+constexpr int kRows = 4;
 
 enum Crit : int { kCritL1Ref = 0, kCritL2 = 1 };
 enum Prec : int { kPrecNone = 0, kPrecJacobi = 1, kPrecFsai = 2 };
@@ -113,8 +119,20 @@ __global__ __launch_bounds__(256) void cg_dot_kernel(size_t n, const T* __restri
     if (st->done) return;
     double a = 0.0;
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
-    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
-        a += static_cast<double>(y[i]) * static_cast<double>(p[i]);
+    for (size_t i0 = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i0 < n; i0 += kRows * stride) {
+        T yv[kRows], pv[kRows];
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            const size_t i = i0 + u * stride;
+            if (i < n) {
+                yv[u] = y[i];
+                pv[u] = p[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kRows; ++u)
+            if (i0 + u * stride < n) a += static_cast<double>(yv[u]) * static_cast<double>(pv[u]);
+    }
     a = block_sum256(a, sh);
     if (threadIdx.x == 0) part_yp[blockIdx.x] = a;
 }
@@ -138,16 +156,33 @@ __global__ __launch_bounds__(256) void cg_update_kernel(
     const int crit = st->crit;
     double nr = 0.0, rzn = 0.0;
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
-    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
-        const T pi = p[i];
-        x[i] = x[i] + alpha_x * static_cast<TX>(pi);
-        const T ri = r[i] - alpha * y[i];
-        r[i] = ri;
-        nr += norm_term(crit, static_cast<double>(ri));
-        if (prec == kPrecFsai) continue;  // z and <r, z> follow the G G^T apply
-        const T zi = (prec == kPrecJacobi) ? ri * invD[i] : ri;
-        z[i] = zi;
-        rzn += static_cast<double>(ri) * static_cast<double>(zi);
+    for (size_t i0 = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i0 < n; i0 += kRows * stride) {
+        T pv[kRows], rv[kRows], yv[kRows], dv[kRows];
+        TX xv[kRows];
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            const size_t i = i0 + u * stride;
+            if (i < n) {
+                pv[u] = p[i];
+                xv[u] = x[i];
+                rv[u] = r[i];
+                yv[u] = y[i];
+                if (prec == kPrecJacobi) dv[u] = invD[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            const size_t i = i0 + u * stride;
+            if (i >= n) continue;
+            x[i] = xv[u] + alpha_x * static_cast<TX>(pv[u]);
+            const T ri = rv[u] - alpha * yv[u];
+            r[i] = ri;
+            nr += norm_term(crit, static_cast<double>(ri));
+            if (prec == kPrecFsai) continue;  // z and <r, z> follow the G G^T apply
+            const T zi = (prec == kPrecJacobi) ? ri * dv[u] : ri;
+            z[i] = zi;
+            rzn += static_cast<double>(ri) * static_cast<double>(zi);
+        }
     }
     nr = block_sum256(nr, sh);
     rzn = block_sum256(rzn, sh);
@@ -172,8 +207,22 @@ __global__ __launch_bounds__(256) void cg_pupdate_kernel(size_t n, T* __restrict
     const double rz_old = sum_partials(part_rz_prev, g, sh);
     const T beta = static_cast<T>(rz_new / rz_old);
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
-    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
-        p[i] = z[i] + beta * p[i];
+    for (size_t i0 = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i0 < n; i0 += kRows * stride) {
+        T zv[kRows], pv[kRows];
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            const size_t i = i0 + u * stride;
+            if (i < n) {
+                zv[u] = z[i];
+                pv[u] = p[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            const size_t i = i0 + u * stride;
+            if (i < n) p[i] = zv[u] + beta * pv[u];
+        }
+    }
 
     if (blockIdx.x == 0) {
         const double s = sum_partials(part_nr, g, sh);
@@ -203,6 +252,231 @@ __global__ __launch_bounds__(256) void precond_kernel(size_t n, const T* __restr
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
     for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
         out[i] = (prec == kPrecJacobi) ? r[i] * invD[i] : r[i];
+}
+
+// ---- learned-FSAI factor in diagonal storage (BF16 values, FP32 compute) ----
+// Row i of G holds G(i, i + off[d]) = vals[d n + i]; entries outside the
+// pattern are zero and skipped, so they never read the input vector. Each
+// product is rounded on its own and the row is summed in ascending column
+// order: the same sums as the CSR SpMV of the factor.
+
+// Diagonal storage of the FP32 copy of A in the mixed solvers (stencils of
+// structured grids): at most kDiaMax diagonals.
+constexpr int kDiaMax = 9;
+static_assert(kDiaMax >= kFsaiMaxDiags, "offset table holds the factor's diagonals");
+
+struct DiaOffsets
+{
+    int32_t o[kDiaMax];
+};
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+__device__ __forceinline__ float mul_rounded_f(float a, float b)
+{
+#pragma clang fp contract(off)
+    return a * b;
+}
+
+// y = G^T r and partials of ||y||^2 (one per workgroup, g workgroups).
+// Row i of G^T holds G(i - off[d], i) at column i - off[d]: ascending column
+// order walks the diagonals from off = 0 down.
+template <int K>
+__global__ __launch_bounds__(256) void fsai_gt_dia_kernel(size_t n, DiaOffsets off,
+                                                          const uint16_t* __restrict__ vals,
+                                                          const float* __restrict__ r,
+                                                          float* __restrict__ y,
+                                                          double* __restrict__ part,
+                                                          const int* __restrict__ done)
+{
+    __shared__ double sh[4];
+    if (done != nullptr && *done != 0) return;
+    double a = 0.0;
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i0 = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i0 < n; i0 += kRows * stride) {
+        uint16_t h[kRows][K];
+        float rv[kRows][K];
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            const size_t i = i0 + u * stride;
+#pragma unroll
+            for (int d = 0; d < K; ++d) {
+                const size_t j = i - static_cast<int64_t>(off.o[d]);
+                const bool in = i < n && j < n;
+                h[u][d] = in ? vals[static_cast<size_t>(d) * n + j] : uint16_t(0);
+                rv[u][d] = in ? r[j] : 0.f;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            const size_t i = i0 + u * stride;
+            if (i >= n) continue;
+            float acc = 0.f;
+#pragma unroll
+            for (int d = K - 1; d >= 0; --d)
+                if (h[u][d] != 0) acc += mul_rounded_f(bf16_to_f32(h[u][d]), rv[u][d]);
+            y[i] = acc;
+            a += static_cast<double>(acc) * static_cast<double>(acc);
+        }
+    }
+    a = block_sum256(a, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
+// z = G y.
+template <int K>
+__global__ __launch_bounds__(256) void fsai_g_dia_kernel(size_t n, DiaOffsets off,
+                                                         const uint16_t* __restrict__ vals,
+                                                         const float* __restrict__ y,
+                                                         float* __restrict__ z,
+                                                         const int* __restrict__ done)
+{
+    if (done != nullptr && *done != 0) return;
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i0 = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i0 < n; i0 += kRows * stride) {
+        uint16_t h[kRows][K];
+        float yv[kRows][K];
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            const size_t i = i0 + u * stride;
+#pragma unroll
+            for (int d = 0; d < K; ++d) {
+                const int64_t j = static_cast<int64_t>(i) + off.o[d];
+                const bool in = i < n && j >= 0;
+                h[u][d] = in ? vals[static_cast<size_t>(d) * n + i] : uint16_t(0);
+                yv[u][d] = in ? y[j] : 0.f;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            const size_t i = i0 + u * stride;
+            if (i >= n) continue;
+            float acc = 0.f;
+#pragma unroll
+            for (int d = 0; d < K; ++d)
+                if (h[u][d] != 0) acc += mul_rounded_f(bf16_to_f32(h[u][d]), yv[u][d]);
+            z[i] = acc;
+        }
+    }
+}
+
+// y = A p from diagonal storage (vals[d n + i] = A(i, i + off[d]), off
+// ascending, zero outside the pattern) with partials of <y, p> in the
+// grid-stride order of cg_dot_kernel (g workgroups).
+template <int K>
+__global__ __launch_bounds__(256) void dia_spmv_dot_kernel(size_t n, DiaOffsets off,
+                                                           const float* __restrict__ vals,
+                                                           const float* __restrict__ p,
+                                                           float* __restrict__ y,
+                                                           double* __restrict__ part,
+                                                           const int* __restrict__ done)
+{
+    __shared__ double sh[4];
+    if (done != nullptr && *done != 0) return;
+    double a = 0.0;
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i0 = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i0 < n; i0 += kRows * stride) {
+        float v[kRows][K], xv[kRows][K], pi[kRows];
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            const size_t i = i0 + u * stride;
+#pragma unroll
+            for (int d = 0; d < K; ++d) {
+                const int64_t j = static_cast<int64_t>(i) + off.o[d];
+                const bool in = i < n && j >= 0 && j < static_cast<int64_t>(n);
+                v[u][d] = in ? vals[static_cast<size_t>(d) * n + i] : 0.f;
+                xv[u][d] = in ? p[j] : 0.f;
+            }
+            pi[u] = i < n ? p[i] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            const size_t i = i0 + u * stride;
+            if (i >= n) continue;
+            float acc = 0.f;
+#pragma unroll
+            for (int d = 0; d < K; ++d)
+                if (v[u][d] != 0.f) acc += mul_rounded_f(v[u][d], xv[u][d]);
+            y[i] = acc;
+            a += static_cast<double>(acc) * static_cast<double>(pi[u]);
+        }
+    }
+    a = block_sum256(a, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
+// dia[d n + i] = (float) A(i, i + off[d]) for every stored entry; *bad = 1 if
+// a row has an entry off the K diagonals or columns not strictly increasing
+// (then the CSR SpMV stays). dia must be zero beforehand.
+__global__ __launch_bounds__(256) void dia_fill_kernel(uint32_t n, const uint32_t* __restrict__ rp,
+                                                       const uint32_t* __restrict__ ci,
+                                                       const double* __restrict__ v, int K,
+                                                       DiaOffsets off, float* __restrict__ dia,
+                                                       int* __restrict__ bad)
+{
+    const uint32_t row = blockIdx.x * 256 + threadIdx.x;
+    if (row >= n) return;
+    int64_t prev = INT64_MIN;
+    for (uint32_t j = rp[row]; j < rp[row + 1]; ++j) {
+        const int64_t o = static_cast<int64_t>(ci[j]) - static_cast<int64_t>(row);
+        int d = -1;
+        for (int k = 0; k < K; ++k)
+            if (off.o[k] == o) d = k;
+        if (o <= prev || d < 0) {
+            *bad = 1;
+            return;
+        }
+        prev = o;
+        dia[static_cast<size_t>(d) * n + row] = static_cast<float>(v[j]);
+    }
+}
+
+// Column - row offsets of kDiaSamples rows spread over the matrix: out[16 k]
+// = entry count of sample row k (16: more than 15 entries or an offset
+// outside int32), then its offsets.
+constexpr int kDiaSamples = 64;
+__global__ __launch_bounds__(64) void dia_sample_kernel(uint32_t n, const uint32_t* __restrict__ rp,
+                                                        const uint32_t* __restrict__ ci,
+                                                        int32_t* __restrict__ out)
+{
+    const uint32_t k = threadIdx.x;
+    if (k >= static_cast<uint32_t>(kDiaSamples)) return;
+    // Golden-ratio sequence: no aliasing with the row length of a grid (a
+    // regular stride such as n / 64 lands every sample on the same column of
+    // a 160 x 160 x 160 grid, a boundary column).
+    const double t = (k + 0.5) * 0.6180339887498949;
+    const uint32_t row = min(n - 1, static_cast<uint32_t>((t - floor(t)) * n));
+    int32_t* o = out + 16 * k;
+    const uint32_t b = rp[row], e = rp[row + 1];
+    if (e < b || e - b > 15) {
+        o[0] = 16;
+        return;
+    }
+    o[0] = static_cast<int32_t>(e - b);
+    for (uint32_t j = b; j < e; ++j) {
+        const int64_t d = static_cast<int64_t>(ci[j]) - static_cast<int64_t>(row);
+        if (d < INT32_MIN || d > INT32_MAX) {
+            o[0] = 16;
+            return;
+        }
+        o[1 + j - b] = static_cast<int32_t>(d);
+    }
+}
+
+template <int K>
+void launch_dia_spmv_dot(size_t n, uint32_t g, const DiaOffsets& off, const float* vals, const float* p,
+                         float* y, double* part, const int* done, hipStream_t s)
+{
+    hipLaunchKernelGGL((dia_spmv_dot_kernel<K>), dim3(g), dim3(256), 0, s, n, off, vals, p, y, part, done);
+}
+
+template <int K>
+void launch_dia_apply(size_t n, uint32_t g, const DiaOffsets& off, const uint16_t* vals, const float* r,
+                      float* tmp, float* out, double* part, const int* done, hipStream_t s)
+{
+    hipLaunchKernelGGL((fsai_gt_dia_kernel<K>), dim3(g), dim3(256), 0, s, n, off, vals, r, tmp, part,
+                       done);
+    hipLaunchKernelGGL((fsai_g_dia_kernel<K>), dim3(g), dim3(256), 0, s, n, off, vals, tmp, out, done);
 }
 
 template <typename T>
@@ -249,13 +523,26 @@ class CgEngine
 {
 public:
     // G, GT: the factor of M = G G^T and its transpose (kPrecFsai only);
-    // g_bf16: their values are BF16 bit patterns (FP32 engines only).
+    // g_bf16: their values are BF16 bit patterns (FP32 engines only);
+    // dia: the BF16 factor in diagonal storage, used instead of G and GT.
     CgEngine(const clsparseCsrMatrix* A, TX* x, const T* b, size_t n, int prec, int crit,
              CgBuffers<T>& buf, hipStream_t s, bool ext, const clsparseCsrMatrix* G = nullptr,
-             const clsparseCsrMatrix* GT = nullptr, bool g_bf16 = false)
+             const clsparseCsrMatrix* GT = nullptr, bool g_bf16 = false,
+             const FsaiDia* dia = nullptr, const float* a_dia = nullptr, int a_ndiag = 0,
+             const DiaOffsets* a_off = nullptr)
         : A_(A), x_(x), b_(b), n_(n), prec_(prec), crit_(crit), buf_(buf), s_(s), ext_(ext),
           g_(grid_for(n)), G_(G), GT_(GT), g_bf16_(g_bf16)
     {
+        if (a_dia != nullptr && std::is_same_v<T, float>) {
+            a_dia_ = a_dia;
+            a_ndiag_ = a_ndiag;
+            a_off_ = *a_off;
+        }
+        if (dia != nullptr && std::is_same_v<T, float> && g_bf16) {
+            use_dia_ = true;
+            dia_ = *dia;
+            for (int d = 0; d < kFsaiMaxDiags; ++d) dia_off_.o[d] = dia->off[d];
+        }
     }
     ~CgEngine()
     {
@@ -336,7 +623,9 @@ public:
     // through `tmp`), for the residual replacement of the reliable solver.
     clsparseStatus precondition(T* out, T* tmp)
     {
-        if (prec_ == kPrecFsai) {
+        if (prec_ == kPrecFsai && use_dia_) {
+            CLS_TRY(dia_apply(buf_.r, tmp, out, buf_.part_yp, nullptr));
+        } else if (prec_ == kPrecFsai) {
             CLS_TRY(spmv_g(GT_, buf_.r, tmp, nullptr));
             CLS_TRY(spmv_g(G_, tmp, out, nullptr));
         } else {
@@ -352,9 +641,13 @@ private:
     clsparseStatus enqueue_iteration(int par)
     {
         const int* done = &buf_.st->done;
-        CLS_TRY(launch_csrmv<T>(buf_.one, A_, buf_.p, buf_.zero, buf_.y, s_, ext_, done));
-        hipLaunchKernelGGL((cg_dot_kernel<T>), dim3(g_), dim3(256), 0, s_, n_, buf_.y, buf_.p,
-                           buf_.part_yp, buf_.st);
+        if (a_dia_ != nullptr) {
+            CLS_TRY(spmv_dot_dia(done));
+        } else {
+            CLS_TRY(launch_csrmv<T>(buf_.one, A_, buf_.p, buf_.zero, buf_.y, s_, ext_, done));
+            hipLaunchKernelGGL((cg_dot_kernel<T>), dim3(g_), dim3(256), 0, s_, n_, buf_.y, buf_.p,
+                               buf_.part_yp, buf_.st);
+        }
         hipLaunchKernelGGL((cg_update_kernel<T, TX>), dim3(g_), dim3(256), 0, s_, n_, x_, buf_.r,
                            buf_.z, buf_.p, buf_.y, buf_.invD, prec_, buf_.part_yp,
                            buf_.part_rz[par ^ 1], buf_.part_rz[par], buf_.part_nr[par], g_,
@@ -371,12 +664,57 @@ private:
     // partials of <r, z> = ||G^T r||^2, read from y alone.
     clsparseStatus apply_fsai(double* part_rz, const int* done)
     {
+        if (use_dia_) return dia_apply(buf_.r, buf_.y, buf_.z, part_rz, done);
         CLS_TRY(spmv_g(GT_, buf_.r, buf_.y, done));
         CLS_TRY(spmv_g(G_, buf_.y, buf_.z, done));
         hipLaunchKernelGGL((cg_dot_kernel<T>), dim3(g_), dim3(256), 0, s_, n_, buf_.y, buf_.y,
                            part_rz, buf_.st);
         CLS_HIP_TRY(hipGetLastError());
         return clsparseSuccess;
+    }
+
+    // y = A p and <y, p> partials from A's diagonal storage.
+    clsparseStatus spmv_dot_dia(const int* done)
+    {
+        if constexpr (std::is_same_v<T, float>) {
+            const float* v = a_dia_;
+            switch (a_ndiag_) {
+            case 1: launch_dia_spmv_dot<1>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_); break;
+            case 2: launch_dia_spmv_dot<2>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_); break;
+            case 3: launch_dia_spmv_dot<3>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_); break;
+            case 4: launch_dia_spmv_dot<4>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_); break;
+            case 5: launch_dia_spmv_dot<5>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_); break;
+            case 6: launch_dia_spmv_dot<6>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_); break;
+            case 7: launch_dia_spmv_dot<7>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_); break;
+            case 8: launch_dia_spmv_dot<8>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_); break;
+            case 9: launch_dia_spmv_dot<9>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_); break;
+            default: return clsparseInvalidValue;
+            }
+            CLS_HIP_TRY(hipGetLastError());
+            return clsparseSuccess;
+        }
+        return clsparseInvalidValue;
+    }
+
+    // tmp = G^T r (with ||tmp||^2 partials), out = G tmp, diagonal storage.
+    clsparseStatus dia_apply(const T* r, T* tmp, T* out, double* part, const int* done)
+    {
+        if constexpr (std::is_same_v<T, float>) {
+            switch (dia_.ndiag) {
+            case 1: launch_dia_apply<1>(n_, g_, dia_off_, dia_.vals, r, tmp, out, part, done, s_); break;
+            case 2: launch_dia_apply<2>(n_, g_, dia_off_, dia_.vals, r, tmp, out, part, done, s_); break;
+            case 3: launch_dia_apply<3>(n_, g_, dia_off_, dia_.vals, r, tmp, out, part, done, s_); break;
+            case 4: launch_dia_apply<4>(n_, g_, dia_off_, dia_.vals, r, tmp, out, part, done, s_); break;
+            case 5: launch_dia_apply<5>(n_, g_, dia_off_, dia_.vals, r, tmp, out, part, done, s_); break;
+            case 6: launch_dia_apply<6>(n_, g_, dia_off_, dia_.vals, r, tmp, out, part, done, s_); break;
+            case 7: launch_dia_apply<7>(n_, g_, dia_off_, dia_.vals, r, tmp, out, part, done, s_); break;
+            case 8: launch_dia_apply<8>(n_, g_, dia_off_, dia_.vals, r, tmp, out, part, done, s_); break;
+            default: return clsparseInvalidValue;
+            }
+            CLS_HIP_TRY(hipGetLastError());
+            return clsparseSuccess;
+        }
+        return clsparseInvalidValue;
     }
 
     // out = M_ in for one factor of M, in its stored precision.
@@ -401,6 +739,12 @@ private:
     const clsparseCsrMatrix* G_;
     const clsparseCsrMatrix* GT_;
     bool g_bf16_;
+    bool use_dia_ = false;
+    FsaiDia dia_{};
+    const float* a_dia_ = nullptr;
+    int a_ndiag_ = 0;
+    DiaOffsets a_off_{};
+    DiaOffsets dia_off_{};
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
 };
@@ -618,6 +962,41 @@ __global__ void state_resume(CgDevState* st, const double* __restrict__ rnorm)
     }
 }
 
+// Union of the column - row offsets of kDiaSamples rows spread over A,
+// ascending, when it has at most kDiaMax entries and that many diagonals of
+// length n would store at most 1.5 nnz values; *ndiag = 0 otherwise. The
+// fill kernel checks every row against the result.
+clsparseStatus dia_candidates(const clsparseCsrMatrix* A, int32_t* scratch, hipStream_t s,
+                              DiaOffsets* off, int* ndiag)
+{
+    *ndiag = 0;
+    const uint32_t n = A->num_rows;
+    hipLaunchKernelGGL(dia_sample_kernel, dim3(1), dim3(64), 0, s, n,
+                       dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer),
+                       dev_ptr<uint32_t>(A->col_indices, A->off_col_indices), scratch);
+    CLS_HIP_TRY(hipGetLastError());
+    std::vector<int32_t> h(16 * kDiaSamples);
+    CLS_HIP_TRY(hipMemcpyAsync(h.data(), scratch, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    CLS_HIP_TRY(hipStreamSynchronize(s));
+    std::vector<int32_t> offs;
+    for (int k = 0; k < kDiaSamples; ++k) {
+        const int32_t cnt = h[16 * k];
+        if (cnt > 15) return clsparseSuccess;
+        for (int32_t j = 0; j < cnt; ++j) {
+            const int32_t o = h[16 * k + 1 + j];
+            if (std::find(offs.begin(), offs.end(), o) == offs.end()) offs.push_back(o);
+        }
+        if (offs.size() > static_cast<size_t>(kDiaMax)) return clsparseSuccess;
+    }
+    if (offs.empty()) return clsparseSuccess;
+    if (static_cast<double>(offs.size()) * n > 1.5 * static_cast<double>(A->num_nonzeros))
+        return clsparseSuccess;
+    std::sort(offs.begin(), offs.end());
+    for (size_t d = 0; d < offs.size(); ++d) off->o[d] = offs[d];
+    *ndiag = static_cast<int>(offs.size());
+    return clsparseSuccess;
+}
+
 template <typename TI>
 clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
                               clSParseSolverControl sc, clsparseControl control,
@@ -643,12 +1022,26 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     double delta = 0.1;
     if (const char* e = std::getenv("CLSPARSE_AMD_RELIABLE_DELTA")) delta = std::atof(e);
 
+    // Mixed solver: candidate diagonals of A from rows spread over it (a
+    // stencil's interior rows hold all of them); the fill kernel below checks
+    // every row against them. CLSPARSE_AMD_DIA=0: CSR always.
+    DiaOffsets a_off{};
+    int a_ndiag = 0;
+    if constexpr (kMixed) {
+        const char* ed = std::getenv("CLSPARSE_AMD_DIA");
+        if (!(ed && ed[0] == '0') && n >= 8 && nnz > 0)
+            CLS_TRY(dia_candidates(A, reinterpret_cast<int32_t*>(control->red_partials), s, &a_off,
+                                   &a_ndiag));
+    }
+
     // Workspace: TI values of A (mixed only), FP64 residual, FP64 copy of x
     // (restored if a preconditioner breaks down), scalars, TI start residual,
-    // then the CG buffers.
+    // then the CG buffers; with a factor also the pilot race's start point
+    // and the learned factor's pilot end point.
     const size_t bytes = (kMixed ? align_up(nnz * sizeof(TI)) : 0) + 2 * align_up(n * sizeof(double)) +
                          align_up(8 * sizeof(double)) + align_up(n * sizeof(TI)) +
-                         CgBuffers<TI>::bytes(n);
+                         CgBuffers<TI>::bytes(n) + (fsai ? 2 * align_up(n * sizeof(double)) : 0) +
+                         (a_ndiag ? align_up(a_ndiag * n * sizeof(float)) + align_up(sizeof(int)) : 0);
     void* work = nullptr;
     CLS_TRY(control_workspace(control, bytes, &work));
     void* pinned = nullptr;
@@ -662,6 +1055,10 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     TI* rI = c.take<TI>(n);            // TI start residual; FSAI temporary afterwards
     CgBuffers<TI> buf;
     buf.carve(c, n);
+    double* xstart = fsai ? c.take<double>(n) : nullptr;
+    double* xpilot = fsai ? c.take<double>(n) : nullptr;
+    float* a_dia = a_ndiag ? c.take<float>(a_ndiag * n) : nullptr;
+    int* a_bad = a_ndiag ? c.take<int>(1) : nullptr;
 
     CLS_TRY(enter_solver_stream(control));
     const auto t0 = std::chrono::steady_clock::now();
@@ -689,12 +1086,31 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
                            dev_ptr<double>(A->values, A->off_values), valsI);
         AI.values = valsI;
         AI.off_values = 0;
+        if (a_ndiag) {
+            hipLaunchKernelGGL((fill_kernel<float>), dim3(ew_grid(a_ndiag * n)), dim3(256), 0, s,
+                               a_ndiag * n, a_dia, 0.f);
+            hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(256), 0, s, size_t(1), a_bad, 0);
+            hipLaunchKernelGGL(dia_fill_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
+                               static_cast<uint32_t>(n), dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer),
+                               dev_ptr<uint32_t>(A->col_indices, A->off_col_indices),
+                               dev_ptr<double>(A->values, A->off_values), a_ndiag, a_off, a_dia, a_bad);
+            CLS_HIP_TRY(hipGetLastError());
+            int bad = 0;
+            CLS_HIP_TRY(hipMemcpyAsync(&bad, a_bad, sizeof(int), hipMemcpyDeviceToHost, s));
+            CLS_HIP_TRY(hipStreamSynchronize(s));
+            if (bad) {
+                a_ndiag = 0;
+                a_dia = nullptr;
+            }
+        }
     }
     int prec = (sc->preconditioner == DIAGONAL) ? kPrecJacobi : kPrecNone;
     int used = prec;  // statistics: 0 none, 1 Jacobi, 2 FSAI, 3 FSAI -> Jacobi
     const clsparseCsrMatrix* G = nullptr;
     const clsparseCsrMatrix* GT = nullptr;
     bool g_bf16 = false;
+    FsaiDia dia{};
+    bool have_dia = false;
     if (fsai) {
         if constexpr (kMixed) {
             // The factor's values in BF16 (CLSPARSE_AMD_FSAI_BF16=0: FP32):
@@ -704,6 +1120,10 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
             g_bf16 = !(e && e[0] == '0');
             G = g_bf16 ? fsai_G16(fsai) : fsai_G32(fsai);
             GT = g_bf16 ? fsai_GT16(fsai) : fsai_GT32(fsai);
+            // Diagonal storage of the BF16 factor where its pattern allows
+            // (CLSPARSE_AMD_FSAI_DIA=0: always the CSR SpMVs).
+            const char* ed = std::getenv("CLSPARSE_AMD_FSAI_DIA");
+            have_dia = g_bf16 && !(ed && ed[0] == '0') && fsai_dia(fsai, &dia);
         } else {
             G = fsai_G64(fsai);
             GT = fsai_GT64(fsai);
@@ -746,8 +1166,10 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     *hs = init;
     CLS_HIP_TRY(hipMemcpyAsync(buf.st, hs, sizeof(CgDevState), hipMemcpyHostToDevice, s));
 
-    CgEngine<TI, double> eng_main(&AI, xp, rI, n, prec, kCritL2, buf, s, false, G, GT, g_bf16);
-    CgEngine<TI, double> eng_jac(&AI, xp, rI, n, kPrecJacobi, kCritL2, buf, s, false);
+    CgEngine<TI, double> eng_main(&AI, xp, rI, n, prec, kCritL2, buf, s, false, G, GT, g_bf16,
+                                  have_dia ? &dia : nullptr, a_dia, a_ndiag, &a_off);
+    CgEngine<TI, double> eng_jac(&AI, xp, rI, n, kPrecJacobi, kCritL2, buf, s, false, nullptr,
+                                 nullptr, false, nullptr, a_dia, a_ndiag, &a_off);
     CgEngine<TI, double>* eng = &eng_main;
     // A pilot race already run for this factor decides without a new one
     // (the factor object forgets it when G changes).
@@ -788,11 +1210,16 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
         int chunks_since_best = 0;
         int iter_at_chunk = 0;
         // Pilot race of the learned factor against Jacobi: kPilotChunks chunks
-        // of each (FSAI first, then Jacobi from where FSAI left x); the rate
+        // of each from the same starting point (FSAI first; its end point is
+        // kept, x goes back to the start for Jacobi); the rate
         // -ln(res_end / res_start) / seconds is measured over all chunks but
-        // the first of each (which captures the graph). The faster one goes on.
+        // the first of each (which captures the graph). The faster one goes on
+        // from its own end point. (Jacobi started where FSAI stopped would
+        // meet a residual whose easily damped components are already gone.)
         enum Phase { kPilotFsai, kPilotJacobi, kMain };
         Phase phase = (eng->prec() == kPrecFsai && race_enabled() && race_known == 0) ? kPilotFsai : kMain;
+        if (phase == kPilotFsai)
+            CLS_HIP_TRY(hipMemcpyAsync(xstart, xp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         int phase_chunks = 0;
         double phase_res0 = 0.0, rate_fsai = 0.0, rate_jac = 0.0;
         auto phase_t0 = std::chrono::steady_clock::now();
@@ -836,6 +1263,8 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
                         rate_fsai = rate;
                         phase = kPilotJacobi;
                         used = 3;
+                        CLS_HIP_TRY(hipMemcpyAsync(xpilot, xp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+                        CLS_HIP_TRY(hipMemcpyAsync(xp, xstart, n * sizeof(double), hipMemcpyDeviceToDevice, s));
                         CLS_TRY(switch_to(&eng_jac, &over));
                     } else {
                         rate_jac = rate;
@@ -843,6 +1272,7 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
                         fsai_set_race(fsai, rate_fsai > rate_jac ? 1 : 2, rate_fsai, rate_jac);
                         if (rate_fsai > rate_jac) {
                             used = 2;
+                            CLS_HIP_TRY(hipMemcpyAsync(xp, xpilot, n * sizeof(double), hipMemcpyDeviceToDevice, s));
                             CLS_TRY(switch_to(&eng_main, &over));
                         }
                     }
@@ -921,6 +1351,8 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     sc->stats.preconditioner = used;
     sc->stats.pilot_rate_fsai = rate_fsai_out;
     sc->stats.pilot_rate_jacobi = rate_jac_out;
+    sc->stats.a_diagonals = a_ndiag;
+    sc->stats.g_diagonals = have_dia ? dia.ndiag : 0;
     return clsparseSuccess;
 }
 

@@ -195,6 +195,19 @@ int fsai_race(clsparseAmdFsai f, double* rate_fsai, double* rate_jacobi);
 void fsai_set_race(clsparseAmdFsai f, int outcome, double rate_fsai, double rate_jacobi);
 uint32_t fsai_n(clsparseAmdFsai f);
 
+// Diagonal storage of the BF16 factor, present when G's pattern lies on at
+// most kFsaiMaxDiags diagonals (structured-grid stencils): vals[k * n + i] =
+// G(i, i + off[k]), zero where the pattern has no entry; off ascending, all
+// <= 0 (G is lower triangular), the last one 0.
+constexpr int kFsaiMaxDiags = 8;
+struct FsaiDia
+{
+    int ndiag;
+    int32_t off[kFsaiMaxDiags];
+    const uint16_t* vals;
+};
+bool fsai_dia(clsparseAmdFsai f, FsaiDia* out);
+
 // FP32 CSR-Adaptive SpMV whose values are stored as BF16 bit patterns
 // (metadata required).
 clsparseStatus launch_csrmv_bf16(const float* alpha, const clsparseCsrMatrix* A, const float* x,

@@ -466,6 +466,8 @@ __global__ __launch_bounds__(256) void assemble_kernel(uint32_t nnzG, const uint
                                                        T* __restrict__ gval, T* __restrict__ gtval,
                                                        uint16_t* __restrict__ g16,
                                                        uint16_t* __restrict__ gt16,
+                                                       const size_t* __restrict__ dia_pos,
+                                                       uint16_t* __restrict__ dia16,
                                                        int* __restrict__ bad)
 {
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
@@ -479,6 +481,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(uint32_t nnzG, const uint
         const uint16_t h = f2bf(static_cast<float>(g));
         g16[k] = h;
         gt16[tpos[k]] = h;
+        if (dia16) dia16[dia_pos[k]] = h;
     }
 }
 
@@ -575,6 +578,12 @@ struct _clsparseAmdFsai
     double *g64 = nullptr, *gt64 = nullptr;
     float *g32 = nullptr, *gt32 = nullptr;
     uint16_t *g16 = nullptr, *gt16 = nullptr;  // BF16 copies of the FP32 values
+    // Diagonal storage of the BF16 values (see FsaiDia): ndiag = 0 when G's
+    // pattern needs more than kFsaiMaxDiags diagonals or too much padding.
+    int ndiag = 0;
+    int32_t dia_off[kFsaiMaxDiags] = {};
+    uint16_t* dia16 = nullptr;   // [ndiag][n]
+    size_t* dia_pos = nullptr;   // per G nonzero: its index in dia16
     int* bad = nullptr;
     clsparseCsrMatrix G64{}, GT64{}, G32{}, GT32{};
     clsparseCsrMatrix G16{}, GT16{};  // share G32's / GT32's metadata (not owned)
@@ -623,7 +632,7 @@ void free_fsai(_clsparseAmdFsai* f)
     clsparseCsrMetaDelete(&f->GT32);
     void* ptrs[] = {f->g_rp, f->g_ci, f->gt_rp, f->gt_ci, f->g_row, f->g_src, f->g_tpos,
                     f->dhat, f->stats, f->X, f->o, f->g64, f->gt64, f->g32, f->gt32, f->g16, f->gt16, f->bad,
-                    f->t, f->u, f->s, f->v, f->t2, f->w, f->z, f->gG, f->sc, f->gout};
+                    f->dia16, f->dia_pos, f->t, f->u, f->s, f->v, f->t2, f->w, f->z, f->gG, f->sc, f->gout};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete f;
@@ -765,6 +774,26 @@ extern "C" clsparseStatus clsparseAmdFsaiCreate(clsparseAmdFsai* out, const clsp
         }
     }
 
+    // Diagonal storage when the pattern lies on few diagonals: distinct
+    // offsets col - row (all <= 0), ascending; at most 1.5 nnzG slots.
+    std::vector<int64_t> offs;
+    for (uint32_t k = 0; k < nnzG && offs.size() <= static_cast<size_t>(kFsaiMaxDiags); ++k) {
+        const int64_t o = static_cast<int64_t>(gci[k]) - static_cast<int64_t>(grow[k]);
+        if (std::find(offs.begin(), offs.end(), o) == offs.end()) offs.push_back(o);
+    }
+    const bool use_dia = nnzG > 0 && offs.size() <= static_cast<size_t>(kFsaiMaxDiags) &&
+                         static_cast<double>(offs.size()) * n <= 1.5 * nnzG;
+    std::vector<size_t> dpos;
+    if (use_dia) {
+        std::sort(offs.begin(), offs.end());
+        dpos.resize(nnzG);
+        for (uint32_t k = 0; k < nnzG; ++k) {
+            const int64_t o = static_cast<int64_t>(gci[k]) - static_cast<int64_t>(grow[k]);
+            const size_t d = static_cast<size_t>(std::lower_bound(offs.begin(), offs.end(), o) - offs.begin());
+            dpos[k] = d * n + grow[k];
+        }
+    }
+
     _clsparseAmdFsai* f = new (std::nothrow) _clsparseAmdFsai;
     if (!f) return clsparseOutOfHostMemory;
     f->device = control->device;
@@ -795,6 +824,16 @@ extern "C" clsparseStatus clsparseAmdFsaiCreate(clsparseAmdFsai* out, const clsp
     if (e == hipSuccess) e = dmalloc(&f->g16, nnzG);
     if (e == hipSuccess) e = dmalloc(&f->gt16, nnzG);
     if (e == hipSuccess) e = dmalloc(&f->bad, 1);
+    if (use_dia) {
+        f->ndiag = static_cast<int>(offs.size());
+        for (int d = 0; d < f->ndiag; ++d) f->dia_off[d] = static_cast<int32_t>(offs[d]);
+        if (e == hipSuccess) e = dmalloc(&f->dia16, offs.size() * n);
+        if (e == hipSuccess)
+            e = hipMemsetAsync(f->dia16, 0, offs.size() * n * sizeof(uint16_t), st);  // padding stays 0
+        if (e == hipSuccess) e = dmalloc(&f->dia_pos, nnzG);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(f->dia_pos, dpos.data(), nnzG * sizeof(size_t), hipMemcpyHostToDevice, st);
+    }
     if (e != hipSuccess) {
         free_fsai(f);
         return clsparseOutOfResources;
@@ -880,10 +919,11 @@ clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_
     const float* o = net ? f->o : nullptr;
     hipLaunchKernelGGL((assemble_kernel<double>), dim3(blocks_for(f->nnzG)), dim3(256), 0, st, f->nnzG,
                        f->g_row, f->g_ci, f->g_tpos, o, f->dhat, f->g64, f->gt64,
-                       static_cast<uint16_t*>(nullptr), static_cast<uint16_t*>(nullptr), f->bad);
+                       static_cast<uint16_t*>(nullptr), static_cast<uint16_t*>(nullptr),
+                       static_cast<const size_t*>(nullptr), static_cast<uint16_t*>(nullptr), f->bad);
     hipLaunchKernelGGL((assemble_kernel<float>), dim3(blocks_for(f->nnzG)), dim3(256), 0, st, f->nnzG,
                        f->g_row, f->g_ci, f->g_tpos, o, f->dhat, f->g32, f->gt32, f->g16, f->gt16,
-                       f->bad);
+                       f->dia_pos, f->dia16, f->bad);
     CLS_HIP_TRY(hipGetLastError());
     int bad = 0;
     CLS_HIP_TRY(hipMemcpyAsync(&bad, f->bad, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -947,6 +987,15 @@ extern "C" clsparseStatus clsparseAmdFsaiGetOutput(clsparseAmdFsai f, cl_float* 
 {
     if (f == nullptr || host == nullptr) return clsparseInvalidValue;
     CLS_HIP_TRY(hipMemcpy(host, f->o, f->nnzG * sizeof(float), hipMemcpyDeviceToHost));
+    return clsparseSuccess;
+}
+
+extern "C" clsparseStatus clsparseAmdFsaiGetDiagonals(clsparseAmdFsai f, cl_int* ndiag, cl_int* offsets)
+{
+    if (f == nullptr || ndiag == nullptr) return clsparseInvalidValue;
+    *ndiag = f->ndiag;
+    if (offsets)
+        for (int d = 0; d < kFsaiMaxDiags; ++d) offsets[d] = d < f->ndiag ? f->dia_off[d] : 0;
     return clsparseSuccess;
 }
 
@@ -1035,3 +1084,11 @@ void fsai_set_race(clsparseAmdFsai f, int outcome, double rate_fsai, double rate
     f->race_rate_jacobi = rate_jacobi;
 }
 uint32_t fsai_n(clsparseAmdFsai f) { return f->n; }
+bool fsai_dia(clsparseAmdFsai f, FsaiDia* out)
+{
+    if (f->ndiag == 0) return false;
+    out->ndiag = f->ndiag;
+    for (int d = 0; d < kFsaiMaxDiags; ++d) out->off[d] = f->dia_off[d];
+    out->vals = f->dia16;
+    return true;
+}

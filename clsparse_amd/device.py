@@ -243,6 +243,14 @@ class Fsai:
             self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
         return out[: self.nnz()]
 
+    def diagonals(self) -> list:
+        """Column - row offsets of the factor's diagonal storage ([] = CSR)."""
+        nd = ctypes.c_int32(0)
+        offs = (ctypes.c_int32 * 8)()
+        check("clsparseAmdFsaiGetDiagonals", load().clsparseAmdFsaiGetDiagonals(
+            self.handle, ctypes.byref(nd), offs))
+        return [int(offs[d]) for d in range(nd.value)]
+
     def train_step(self, net: Mlp, A: CsrMatrix, control: Control, probes: int = 8,
                    seed: int = 1, lr: float = 0.0, want_grad: bool = False):
         """Returns (loss, dL/dG per nonzero of G or None)."""

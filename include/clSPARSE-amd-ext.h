@@ -55,6 +55,11 @@ typedef struct clsparseAmdSolverStats_
      * second of the pilot race, for the factor and for Jacobi (0 if not run). */
     cl_double pilot_rate_fsai;
     cl_double pilot_rate_jacobi;
+    /* FP32 inner solver: diagonals of the diagonal storage A's values were
+     * moved to for the solve (0: CSR-Adaptive SpMV), and of the learned
+     * factor's storage (0: CSR SpMVs or no factor). */
+    cl_int a_diagonals;
+    cl_int g_diagonals;
 } clsparseAmdSolverStats;
 
 CLSPARSE_EXPORT clsparseStatus
@@ -122,6 +127,12 @@ CLSPARSE_EXPORT clsparseStatus
 CLSPARSE_EXPORT clsparseStatus clsparseAmdFsaiGetFeatures(clsparseAmdFsai fsai, uint16_t* bf16);
 /* Network output o per nonzero of G from the last evaluation. */
 CLSPARSE_EXPORT clsparseStatus clsparseAmdFsaiGetOutput(clsparseAmdFsai fsai, cl_float* out);
+/* Storage of the BF16 factor used by the FP32 solver: *ndiag = number of
+ * diagonals when G's pattern lies on at most 8 of them (structured grids) and
+ * the factor is applied from diagonal storage, else 0 (CSR SpMVs). offsets
+ * (8 entries, may be NULL) receives the diagonals' column - row offsets. */
+CLSPARSE_EXPORT clsparseStatus clsparseAmdFsaiGetDiagonals(clsparseAmdFsai fsai, cl_int* ndiag,
+                                                           cl_int* offsets);
 
 /* One training step on A: assembles G from the network, estimates
  * L = ||(G^T A G - I) Z||_F^2 / (probes * n) with `probes` Rademacher columns

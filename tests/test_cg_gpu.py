@@ -242,3 +242,70 @@ def test_mixed_reliable_updates_track_fp64_iterations(ctl):
         sc.release()
     say(f"iterations mixed {res['mixed']} fp64 {res['fp64']}")
     assert res["mixed"] <= 1.15 * res["fp64"] + 32
+
+
+def _mixed_solve(ctl, rp, ci, v, m, b, tol=1e-9):
+    lib = cls.load()
+    A = cls.CsrMatrix(rp, ci, v, m)
+    A.meta_create(ctl)
+    x = torch.zeros(m, dtype=torch.float64, device="cuda")
+    sc = cls.SolverControl(cls.DIAGONAL, 20000, tol, 0.0)
+    xd, bd = cls.DenseVector(x), cls.DenseVector(torch.tensor(b, device="cuda"))
+    try:
+        assert lib.clsparseAmdDcsrcgMixed(ctypes.byref(xd.s), ctypes.byref(A.s), ctypes.byref(bd.s),
+                                          sc.handle, ctl.handle) == 0
+        return sc.stats(), x.cpu().numpy()
+    finally:
+        sc.release()
+        A.meta_delete()
+
+
+@pytest.mark.parametrize("case", ["diffusion3d", "diffusion3d_pow2", "poisson2d", "q1_2d"])
+def test_mixed_diagonal_storage_matches_csr(ctl, monkeypatch, case):
+    """Stencil matrices go to diagonal storage in the FP32 inner solver (7, 5
+    and 9 diagonals). The row sums are formed in the same order as the CSR
+    SpMV, so the solve takes the same iterations as with CSR. (32 x 32 x 16:
+    rows sampled at a regular stride n / 64 would all sit on the x = 0
+    boundary and miss the -1 diagonal.)"""
+    if case == "diffusion3d":
+        rp, ci, v, m = cls.gen_diffusion3d(28, 26, 24, sigma=1.0, ax=1.0, ay=1.0, az=20.0, seed=17)
+        nd = 7
+    elif case == "diffusion3d_pow2":
+        rp, ci, v, m = cls.gen_diffusion3d(32, 32, 16, sigma=1.0, ax=1.0, ay=1.0, az=20.0, seed=18)
+        nd = 7
+    elif case == "poisson2d":
+        rp, ci, v, m = cls.gen_poisson2d(120, 110)
+        nd = 5
+    else:
+        rp, ci, v, m = cls.gen_aniso2d(90, 80, corr_len=6.0, sigma=1.0, ratio=30.0, theta=0.4, seed=2)
+        nd = 9
+    b = cls.gen_uniform(m, -1, 1, seed=23)
+    say(f"mixed CG diagonal storage vs CSR: {case}")
+    st_d, x_d = _mixed_solve(ctl, rp, ci, v, m, b)
+    monkeypatch.setenv("CLSPARSE_AMD_DIA", "0")
+    st_c, x_c = _mixed_solve(ctl, rp, ci, v, m, b)
+    assert st_d.a_diagonals == nd and st_c.a_diagonals == 0
+    assert st_d.status_detail == 0 and st_c.status_detail == 0
+    assert abs(st_d.iterations - st_c.iterations) <= max(2, st_c.iterations // 50)
+    for x in (x_d, x_c):
+        r = b - oracle.csrmv(rp, ci, v, x)[0]
+        assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-9
+
+
+def test_mixed_keeps_csr_for_entries_off_the_diagonals(ctl):
+    """One symmetric pair of entries far from the stencil's diagonals: the
+    fill kernel finds it and the solve runs on the CSR SpMV."""
+    import scipy.sparse as sp
+    rp, ci, v, m = cls.gen_poisson2d(60, 50)
+    a = sp.csr_matrix((v, ci, rp), shape=(m, m)).tolil()
+    a[3, m - 7] = -0.05
+    a[m - 7, 3] = -0.05
+    a = sp.csr_matrix(a)
+    a.sort_indices()
+    rp2, ci2 = a.indptr.astype(np.uint32), a.indices.astype(np.uint32)
+    b = cls.gen_uniform(m, -1, 1, seed=24)
+    say("mixed CG: stencil with a far entry -> CSR")
+    st, x = _mixed_solve(ctl, rp2, ci2, a.data, m, b)
+    assert st.a_diagonals == 0 and st.status_detail == 0
+    r = b - oracle.csrmv(rp2, ci2, a.data, x)[0]
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-9

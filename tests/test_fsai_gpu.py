@@ -346,3 +346,101 @@ def test_missing_diagonal_factor_falls_back_to_jacobi(ctl):
         assert st.iterations <= 50
     finally:
         f.release()
+
+
+def _tril_offsets(rp, ci):
+    """Distinct column - row offsets of the lower triangle, ascending."""
+    rows = np.repeat(np.arange(rp.size - 1, dtype=np.int64), np.diff(rp.astype(np.int64)))
+    d = ci.astype(np.int64) - rows
+    return np.unique(d[d <= 0]).tolist(), int((d <= 0).sum())
+
+
+def test_factor_storage_follows_the_pattern(ctl):
+    """Stencil factors are applied from diagonal storage; a pattern on many
+    diagonals keeps the CSR SpMVs."""
+    rp3, ci3, v3, m3 = cls.gen_diffusion3d(12, 10, 8, sigma=1.0, ax=1.0, ay=1.0, az=100.0, seed=3)
+    rp2, ci2, v2, m2 = _aniso(24, 20, seed=3)
+    a = sp.random(600, 600, density=0.02, random_state=np.random.default_rng(5), format="csr")
+    a = a + a.T + sp.identity(600) * 50.0
+    a = sp.csr_matrix(a)
+    a.sort_indices()
+    cases = [(rp3, ci3, v3, m3), (rp2, ci2, v2, m2),
+             (a.indptr.astype(np.uint32), a.indices.astype(np.uint32), a.data, 600)]
+    say("fsai: factor storage by pattern")
+    for k, (rp, ci, v, m) in enumerate(cases):
+        A = cls.CsrMatrix(rp, ci, v, m)
+        A.meta_create(ctl)
+        f = cls.Fsai(A, ctl)
+        try:
+            offs, nnz_tril = _tril_offsets(rp, ci)
+            expect = offs if (len(offs) <= 8 and len(offs) * m <= 1.5 * nnz_tril) else []
+            assert f.diagonals() == expect, (k, f.diagonals(), offs)
+        finally:
+            f.release()
+    assert _tril_offsets(rp3, ci3)[0] == [-120, -12, -1, 0]
+    assert len(_tril_offsets(rp2, ci2)[0]) == 5
+
+
+@pytest.mark.parametrize("dia", ["1", "0"])
+def test_first_iteration_applies_the_bf16_factor(ctl, monkeypatch, dia):
+    """One PCG iteration from x = 0 gives x = alpha G (G^T b): its direction
+    checks the factor application (diagonal storage, or the CSR SpMVs of G
+    and G^T) against the product with the BF16-rounded factor."""
+    monkeypatch.setenv("CLSPARSE_AMD_FSAI_POLICY", "always")
+    monkeypatch.setenv("CLSPARSE_AMD_FSAI_DIA", dia)
+    from clsparse_amd import fsai as fs
+    params, _ = fs.load_weights()
+    rp, ci, v, m = cls.gen_diffusion3d(16, 14, 12, sigma=1.0, ax=1.0, ay=1.0, az=100.0, seed=4)
+    A = cls.CsrMatrix(rp, ci, v, m)
+    A.meta_create(ctl)
+    net = cls.Mlp(seed=1)
+    net.set_params(params)
+    f = cls.Fsai(A, ctl)
+    try:
+        f.apply_net(net, ctl)
+        assert len(f.diagonals()) == 4
+        b = cls.gen_uniform(m, -1, 1, seed=12)
+        say(f"fsai: first iteration, diagonal storage={dia}")
+        st, x = _solve_fsai(ctl, A, f, b, max_iters=1)
+        grp, gci, gva = f.values()
+        G = sp.csr_matrix((of.bf16_round(gva.astype(np.float32)).astype(np.float64), gci, grp),
+                          shape=(m, m))
+        z0 = G @ (G.T @ b.astype(np.float32).astype(np.float64))
+        err = np.linalg.norm(x / np.linalg.norm(x) - z0 / np.linalg.norm(z0))
+        assert err <= 1e-5, err
+        assert st.preconditioner == 2 and st.iterations == 1
+    finally:
+        f.release()
+        net.release()
+
+
+def test_diagonal_storage_solves_like_the_csr_factor(ctl, monkeypatch):
+    """The learned factor on a 3-D anisotropic stencil: the solve from
+    diagonal storage and the one through the CSR SpMVs of G and G^T take the
+    same iterations (the row sums are formed in the same order; only a
+    matrix's last row block may sum in another order) and both reach 1e-8."""
+    monkeypatch.setenv("CLSPARSE_AMD_FSAI_POLICY", "always")
+    from clsparse_amd import fsai as fs
+    params, _ = fs.load_weights()
+    rp, ci, v, m = cls.gen_diffusion3d(24, 24, 24, sigma=1.0, ax=1.0, ay=1.0, az=100.0, seed=5)
+    A = cls.CsrMatrix(rp, ci, v, m)
+    A.meta_create(ctl)
+    net = cls.Mlp(seed=1)
+    net.set_params(params)
+    f = cls.Fsai(A, ctl)
+    Acpu = sp.csr_matrix((v, ci, rp), shape=(m, m))
+    try:
+        f.apply_net(net, ctl)
+        b = cls.gen_uniform(m, -1, 1, seed=13)
+        say("fsai: diagonal storage vs CSR factor, 24^3")
+        monkeypatch.setenv("CLSPARSE_AMD_FSAI_DIA", "0")
+        st_c, x_c = _solve_fsai(ctl, A, f, b)
+        monkeypatch.setenv("CLSPARSE_AMD_FSAI_DIA", "1")
+        st_d, x_d = _solve_fsai(ctl, A, f, b)
+        assert st_c.status_detail == 0 and st_d.status_detail == 0
+        assert abs(st_c.iterations - st_d.iterations) <= max(2, st_c.iterations // 50)
+        for x in (x_c, x_d):
+            assert np.linalg.norm(b - Acpu @ x) / np.linalg.norm(b) <= 1e-8
+    finally:
+        f.release()
+        net.release()
