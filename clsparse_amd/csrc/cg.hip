@@ -56,6 +56,11 @@ struct CgDevState
     int iter;
     int converged;
     int crit;
+    // Copy of done taken by the x / r update of each iteration: the p update
+    // skips on it, since its own workgroup 0 may set done while other
+    // workgroups of the same launch have yet to start (they would skip their
+    // rows of p, which a residual replacement still uses).
+    int stop;
     double hist[kHist];
 };
 
@@ -145,10 +150,12 @@ __global__ __launch_bounds__(256) void cg_update_kernel(
     const T* __restrict__ y, const T* __restrict__ invD, int prec,
     const double* __restrict__ part_yp, const double* __restrict__ part_rz_prev,
     double* __restrict__ part_rz, double* __restrict__ part_nr, uint32_t g,
-    const CgDevState* __restrict__ st)
+    CgDevState* __restrict__ st)
 {
     __shared__ double sh[4];
-    if (st->done) return;
+    const int done = st->done;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->stop = done;
+    if (done) return;
     const double yp = sum_partials(part_yp, g, sh);
     const double rz = sum_partials(part_rz_prev, g, sh);
     const T alpha = static_cast<T>(rz / yp);
@@ -192,7 +199,30 @@ __global__ __launch_bounds__(256) void cg_update_kernel(
     }
 }
 
-// beta = rz_new / rz_old; p = z + beta p; block 0 evaluates finished().
+// End of an iteration: workgroup 0 of the p update evaluates finished()
+// from the ||r|| partials.
+__device__ void cg_finish(CgDevState* __restrict__ st, const double* __restrict__ part_nr, uint32_t g,
+                          double* sh)
+{
+    if (blockIdx.x != 0) return;
+    const double s = sum_partials(part_nr, g, sh);
+    if (threadIdx.x == 0) {
+        const double norm = st->crit == kCritL2 ? sqrt(s) : s;
+        const double res = norm / st->norm_ref;
+        const int it = st->iter;
+        st->cur_res = res;
+        st->hist[it % kHist] = res;
+        const bool conv = (res <= st->rel_tol) || (res <= st->abs_tol * st->init_res);
+        if (conv || it >= st->max_iters) {
+            st->done = 1;
+            st->n_iters = it;
+            st->converged = conv ? 1 : 0;
+        }
+        st->iter = it + 1;
+    }
+}
+
+// beta = rz_new / rz_old; p = z + beta p; workgroup 0 evaluates finished().
 template <typename T>
 __global__ __launch_bounds__(256) void cg_pupdate_kernel(size_t n, T* __restrict__ p,
                                                          const T* __restrict__ z,
@@ -202,7 +232,7 @@ __global__ __launch_bounds__(256) void cg_pupdate_kernel(size_t n, T* __restrict
                                                          uint32_t g, CgDevState* __restrict__ st)
 {
     __shared__ double sh[4];
-    if (st->done) return;
+    if (st->stop) return;
     const double rz_new = sum_partials(part_rz, g, sh);
     const double rz_old = sum_partials(part_rz_prev, g, sh);
     const T beta = static_cast<T>(rz_new / rz_old);
@@ -224,23 +254,7 @@ __global__ __launch_bounds__(256) void cg_pupdate_kernel(size_t n, T* __restrict
         }
     }
 
-    if (blockIdx.x == 0) {
-        const double s = sum_partials(part_nr, g, sh);
-        if (threadIdx.x == 0) {
-            const double norm = st->crit == kCritL2 ? sqrt(s) : s;
-            const double res = norm / st->norm_ref;
-            const int it = st->iter;
-            st->cur_res = res;
-            st->hist[it % kHist] = res;
-            const bool conv = (res <= st->rel_tol) || (res <= st->abs_tol * st->init_res);
-            if (conv || it >= st->max_iters) {
-                st->done = 1;
-                st->n_iters = it;
-                st->converged = conv ? 1 : 0;
-            }
-            st->iter = it + 1;
-        }
-    }
+    cg_finish(st, part_nr, g, sh);
 }
 
 // out = M r for the elementwise preconditioners.
@@ -470,6 +484,63 @@ void launch_dia_spmv_dot(size_t n, uint32_t g, const DiaOffsets& off, const floa
     hipLaunchKernelGGL((dia_spmv_dot_kernel<K>), dim3(g), dim3(256), 0, s, n, off, vals, p, y, part, done);
 }
 
+// p = G y + beta p with beta = ||y||^2_new / <r, z>_old (the p update of
+// cg_pupdate_kernel with z = G y formed in registers, never stored), then
+// the convergence test.
+template <int K>
+__global__ __launch_bounds__(256) void fsai_g_pupdate_dia_kernel(
+    size_t n, DiaOffsets off, const uint16_t* __restrict__ vals, const float* __restrict__ y,
+    float* __restrict__ p, const double* __restrict__ part_rz, const double* __restrict__ part_rz_prev,
+    const double* __restrict__ part_nr, uint32_t g, CgDevState* __restrict__ st)
+{
+    __shared__ double sh[4];
+    if (st->stop) return;
+    const double rz_new = sum_partials(part_rz, g, sh);
+    const double rz_old = sum_partials(part_rz_prev, g, sh);
+    const float beta = static_cast<float>(rz_new / rz_old);
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i0 = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i0 < n; i0 += kRows * stride) {
+        uint16_t h[kRows][K];
+        float yv[kRows][K], pv[kRows];
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            const size_t i = i0 + u * stride;
+#pragma unroll
+            for (int d = 0; d < K; ++d) {
+                const int64_t j = static_cast<int64_t>(i) + off.o[d];
+                const bool in = i < n && j >= 0;
+                h[u][d] = in ? vals[static_cast<size_t>(d) * n + i] : uint16_t(0);
+                yv[u][d] = in ? y[j] : 0.f;
+            }
+            pv[u] = i < n ? p[i] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            const size_t i = i0 + u * stride;
+            if (i >= n) continue;
+            float zi = 0.f;
+#pragma unroll
+            for (int d = 0; d < K; ++d)
+                if (h[u][d] != 0) zi += mul_rounded_f(bf16_to_f32(h[u][d]), yv[u][d]);
+            p[i] = zi + beta * pv[u];
+        }
+    }
+    cg_finish(st, part_nr, g, sh);
+}
+
+// One iteration's preconditioner and p update: y = G^T r with <r, z>
+// partials, then p = G y + beta p and the convergence test.
+template <int K>
+void launch_dia_iteration(size_t n, uint32_t g, const DiaOffsets& off, const uint16_t* vals, const float* r,
+                          float* y, float* p, double* part_rz, const double* part_rz_prev,
+                          const double* part_nr, CgDevState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL((fsai_gt_dia_kernel<K>), dim3(g), dim3(256), 0, s, n, off, vals, r, y, part_rz,
+                       &st->done);
+    hipLaunchKernelGGL((fsai_g_pupdate_dia_kernel<K>), dim3(g), dim3(256), 0, s, n, off, vals, y, p,
+                       part_rz, part_rz_prev, part_nr, g, st);
+}
+
 template <int K>
 void launch_dia_apply(size_t n, uint32_t g, const DiaOffsets& off, const uint16_t* vals, const float* r,
                       float* tmp, float* out, double* part, const int* done, hipStream_t s)
@@ -637,6 +708,15 @@ public:
     }
     int prec() const { return prec_; }
 
+    // z = M r of the current recurrence residual into the z buffer, where the
+    // iterations keep z in registers only (the residual replacement re-bases
+    // p on it). tmp: n values of scratch.
+    clsparseStatus materialize_z(T* tmp)
+    {
+        if (prec_ == kPrecFsai && use_dia_) return dia_apply(buf_.r, tmp, buf_.z, buf_.part_yp, nullptr);
+        return clsparseSuccess;
+    }
+
 private:
     clsparseStatus enqueue_iteration(int par)
     {
@@ -652,12 +732,42 @@ private:
                            buf_.z, buf_.p, buf_.y, buf_.invD, prec_, buf_.part_yp,
                            buf_.part_rz[par ^ 1], buf_.part_rz[par], buf_.part_nr[par], g_,
                            buf_.st);
+        if (prec_ == kPrecFsai && use_dia_) return dia_iteration(par);
         if (prec_ == kPrecFsai) CLS_TRY(apply_fsai(buf_.part_rz[par], done));
         hipLaunchKernelGGL((cg_pupdate_kernel<T>), dim3(g_), dim3(256), 0, s_, n_, buf_.p, buf_.z,
                            buf_.part_rz[par], buf_.part_rz[par ^ 1], buf_.part_nr[par], g_,
                            buf_.st);
         CLS_HIP_TRY(hipGetLastError());
         return clsparseSuccess;
+    }
+
+    // The FSAI preconditioner and p update of one iteration from diagonal
+    // storage (z stays in registers: materialize_z forms it when needed).
+    clsparseStatus dia_iteration(int par)
+    {
+        if constexpr (std::is_same_v<T, float>) {
+            const uint16_t* v = dia_.vals;
+            double* rz = buf_.part_rz[par];
+            const double* rz0 = buf_.part_rz[par ^ 1];
+            const double* nr = buf_.part_nr[par];
+#define CLS_DIA_ITER(K) \
+    launch_dia_iteration<K>(n_, g_, dia_off_, v, buf_.r, buf_.y, buf_.p, rz, rz0, nr, buf_.st, s_)
+            switch (dia_.ndiag) {
+            case 1: CLS_DIA_ITER(1); break;
+            case 2: CLS_DIA_ITER(2); break;
+            case 3: CLS_DIA_ITER(3); break;
+            case 4: CLS_DIA_ITER(4); break;
+            case 5: CLS_DIA_ITER(5); break;
+            case 6: CLS_DIA_ITER(6); break;
+            case 7: CLS_DIA_ITER(7); break;
+            case 8: CLS_DIA_ITER(8); break;
+            default: return clsparseInvalidValue;
+            }
+#undef CLS_DIA_ITER
+            CLS_HIP_TRY(hipGetLastError());
+            return clsparseSuccess;
+        }
+        return clsparseInvalidValue;
     }
 
     // z = G (G^T r) through y (free between the r update and the next A p);
@@ -1318,6 +1428,7 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
             CLS_HIP_TRY(hipMemcpyAsync(xsave, xp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
             // r = (TI) r64; z' = M r into y; <r, z'> partials -> part_yp;
             // re-base p and z; the new <r, z> becomes the previous one.
+            CLS_TRY(eng->materialize_z(rI));
             hipLaunchKernelGGL((rr_cast<TI>), dim3(ew_grid(n)), dim3(256), 0, s, n, r64, buf.r);
             CLS_HIP_TRY(hipGetLastError());
             CLS_TRY(eng->precondition(buf.y, rI));
