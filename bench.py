@@ -113,6 +113,34 @@ class SpmvRun:
         launch_ms = sorted(a.elapsed_time(b) for a, b in per)[len(per) // 2]
         return wall_ms, launch_ms
 
+    def sum_ax(self, calls, stream):
+        """Median time (ms) of clsparseAmd[S|D]csrSumAx on the same A and x (one
+        pass over values, columns and x gathers, no rows, no y) and its value."""
+        torch = self.torch
+        lib = self.cls.load()
+        tdt = self.alpha.t.dtype if hasattr(self.alpha, "t") else torch.float64
+        out = self.cls.Scalar(0.0, tdt)
+        fn = lib.clsparseAmdDcsrSumAx if tdt == torch.float64 else lib.clsparseAmdScsrSumAx
+        import ctypes
+
+        def call():
+            st = fn(ctypes.byref(self.A.s), ctypes.byref(self.x.s), ctypes.byref(out.s),
+                    self.ctl.handle)
+            if st != 0:
+                raise self.cls.ClsparseError("csrSumAx", st)
+        for _ in range(3):
+            call()
+        per = []
+        for _ in range(calls):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            call()
+            b.record(stream)
+            per.append((a, b))
+        torch.cuda.synchronize()
+        ms = sorted(a.elapsed_time(b) for a, b in per)[len(per) // 2]
+        return ms, float(out.t.double().item())
+
     def close(self):
         self.A.meta_delete()
 
@@ -429,6 +457,16 @@ def main():
     sub_v = np.concatenate([v[rp[r]:rp[r + 1]] for r in rows])
     yref, scale = oracle.csrmv(sub_rp, sub_ci, sub_v, x)
     max_err = float(np.max(np.abs(yg[rows] - yref) / np.maximum(scale, 1e-300)))
+    # Floor of any CSR SpMV that gathers x directly: the same value, column and
+    # x loads in storage order, with no rows and no y (1^T A x, which also
+    # checks the sum of y).
+    floor_ms, sum_ax = run.sum_ax(20, stream)
+    ysum = float(np.sum(yg))
+    gather_floor = {"sum_ax_us": round(floor_ms * 1e3, 2), "spmv_launch_us": round(launch_ms * 1e3, 2),
+                    "spmv_over_floor": round(launch_ms / floor_ms, 3),
+                    "checksum_rel_diff": abs(sum_ax - ysum) / max(float(np.sum(np.abs(yg))), 1e-300),
+                    "what": "clsparseAmdDcsrSumAx on the same matrix and x: values, columns and "
+                            "x gathers in storage order, no row structure, no y"}
     run.close()
 
     cpu = None
@@ -439,7 +477,8 @@ def main():
                "sample": f"{calls} calls of the same FP64 power-law SpMV (full matrix), median; "
                          f"oracle OpenMP CSR SpMV, {threads} threads"}
 
-    extra = {"poisson_fp32": poisson_workload(cls, torch, ctl, stream, cpu=not args.no_cpu)}
+    extra = {"gather_floor": gather_floor,
+             "poisson_fp32": poisson_workload(cls, torch, ctl, stream, cpu=not args.no_cpu)}
     if not args.no_solver:
         extra["solver"] = solver_workload(cls, torch, ctl, cpu=not args.no_cpu)
     if not args.no_fsai:

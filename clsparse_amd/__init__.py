@@ -210,6 +210,8 @@ for _s in ("S", "D"):
     for _op in ("add", "sub", "mul", "div"):
         _SIGS[f"cldense{_s}{_op}"] = (c_int32, [POINTER(cldenseVector), POINTER(cldenseVector),
                                                 POINTER(cldenseVector), _V])
+    _SIGS[f"clsparseAmd{_s}csrSumAx"] = (c_int32, [POINTER(clsparseCsrMatrix), POINTER(cldenseVector),
+                                                   POINTER(clsparseScalar), _V])
 _SIGS["cldenseIreduce"] = (c_int32, [POINTER(clsparseScalar), POINTER(cldenseVector), _V])
 
 EXPORTED = tuple(sorted(_SIGS))

@@ -606,3 +606,96 @@ extern "C" clsparseStatus clsparseDcsrmv(const clsparseScalar* alpha,
 {
     return csrmv_entry<double>(alpha, matx, x, beta, y, control);
 }
+
+// ---- sum of A x in storage order ---------------------------------------------
+// sum = sum_j vals[j] * x[cols[j]] over all stored entries, ignoring the rows:
+// 1^T (A x), the checksum of an SpMV, from exactly the loads an SpMV of A
+// cannot avoid (values, columns, x gathers) -- its time is the floor for any
+// CSR SpMV that gathers x directly. FP64 accumulation, fixed order.
+
+namespace {
+
+constexpr int kSumAxUnroll = 8;
+constexpr uint32_t kSumAxMaxGrid = 4096;
+
+template <typename T>
+__global__ __launch_bounds__(256) void sum_ax_kernel(uint64_t nnz, const T* __restrict__ vals,
+                                                     const uint32_t* __restrict__ cols,
+                                                     const T* __restrict__ x, double* __restrict__ part)
+{
+    __shared__ double sh[4];
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256;
+    double acc = 0.0;
+    for (uint64_t j0 = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x; j0 < nnz;
+         j0 += kSumAxUnroll * stride) {
+        T v[kSumAxUnroll];
+        uint32_t c[kSumAxUnroll];
+#pragma unroll
+        for (int u = 0; u < kSumAxUnroll; ++u) {
+            const uint64_t j = j0 + u * stride;
+            const bool in = j < nnz;
+            v[u] = in ? __builtin_nontemporal_load(vals + j) : T(0);
+            c[u] = in ? __builtin_nontemporal_load(cols + j) : 0u;
+        }
+        T xv[kSumAxUnroll];
+#pragma unroll
+        for (int u = 0; u < kSumAxUnroll; ++u) xv[u] = (j0 + u * stride < nnz) ? x[c[u]] : T(0);
+#pragma unroll
+        for (int u = 0; u < kSumAxUnroll; ++u)
+            acc += static_cast<double>(v[u]) * static_cast<double>(xv[u]);
+    }
+    acc = block_sum256(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void sum_ax_finish(const double* __restrict__ part, uint32_t g,
+                                                     T* __restrict__ out)
+{
+    __shared__ double sh[4];
+    double a = 0.0;
+    for (uint32_t i = threadIdx.x; i < g; i += 256) a += part[i];
+    a = block_sum256(a, sh);
+    if (threadIdx.x == 0) *out = static_cast<T>(a);
+}
+
+template <typename T>
+clsparseStatus sum_ax_entry(const clsparseCsrMatrix* A, const cldenseVector* x, clsparseScalar* sum,
+                            clsparseControl control)
+{
+    CLS_TRY(check_init_control(control));
+    if (A == nullptr || x == nullptr || sum == nullptr) return clsparseInvalidValue;
+    if (x->num_values < A->num_cols) return clsparseInvalidSize;
+    if (sum->value == nullptr || (A->num_nonzeros > 0 &&
+                                  (A->values == nullptr || A->col_indices == nullptr || x->values == nullptr)))
+        return clsparseInvalidMemObject;
+    const uint64_t nnz = A->num_nonzeros;
+    const uint32_t g = static_cast<uint32_t>(std::max<uint64_t>(
+        1, std::min<uint64_t>(kSumAxMaxGrid, (nnz + 256 * kSumAxUnroll - 1) / (256 * kSumAxUnroll))));
+    void* work = nullptr;
+    CLS_TRY(control_workspace(control, g * sizeof(double), &work));
+    double* part = static_cast<double*>(work);
+    hipStream_t s = control->stream;
+    hipLaunchKernelGGL((sum_ax_kernel<T>), dim3(g), dim3(256), 0, s, nnz,
+                       dev_ptr<T>(A->values, A->off_values),
+                       dev_ptr<uint32_t>(A->col_indices, A->off_col_indices),
+                       dev_ptr<T>(x->values, x->off_values), part);
+    hipLaunchKernelGGL((sum_ax_finish<T>), dim3(1), dim3(256), 0, s, part, g,
+                       dev_ptr<T>(sum->value, sum->off_value));
+    CLS_HIP_TRY(hipGetLastError());
+    return finish_op(control);
+}
+
+}  // namespace
+
+extern "C" clsparseStatus clsparseAmdScsrSumAx(const clsparseCsrMatrix* A, const cldenseVector* x,
+                                               clsparseScalar* sum, const clsparseControl control)
+{
+    return sum_ax_entry<float>(A, x, sum, control);
+}
+
+extern "C" clsparseStatus clsparseAmdDcsrSumAx(const clsparseCsrMatrix* A, const cldenseVector* x,
+                                               clsparseScalar* sum, const clsparseControl control)
+{
+    return sum_ax_entry<double>(A, x, sum, control);
+}

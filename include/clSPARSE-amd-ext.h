@@ -185,6 +185,16 @@ CLSPARSE_EXPORT clsparseStatus clsparseAmdHostCsrFree(clsparseAmdHostCsr* m);
 CLSPARSE_EXPORT clsparseStatus
     clsparseAmdGenPoisson2D(cl_uint nx, cl_uint ny, clsparseAmdHostCsr* out);
 
+/* sum = sum over the stored entries of values[j] * x[col_indices[j]] =
+ * 1^T (A x), accumulated in FP64 in a fixed order and written to the device
+ * scalar in the matrix's precision. One pass in storage order that ignores the
+ * rows: the checksum of an SpMV, and the time of the loads an SpMV of A cannot
+ * avoid (values, columns and x gathers, no row structure, no y). */
+CLSPARSE_EXPORT clsparseStatus clsparseAmdScsrSumAx(const clsparseCsrMatrix* A, const cldenseVector* x,
+                                                    clsparseScalar* sum, const clsparseControl control);
+CLSPARSE_EXPORT clsparseStatus clsparseAmdDcsrSumAx(const clsparseCsrMatrix* A, const cldenseVector* x,
+                                                    clsparseScalar* sum, const clsparseControl control);
+
 /* Power-law rows: length = clip(round(LogNormal(ln(mean_len) - sigma^2/2, sigma)),
  * 1, max_len); then n_dense rows at seeded positions get dense_len entries.
  * Columns uniform without replacement over [0, n), sorted; values U[-1, 1]. */

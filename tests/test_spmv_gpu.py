@@ -194,3 +194,37 @@ def test_meta_shape_mismatch_is_rejected(ctl):
     assert st == cls.clsparseStructInvalid
     assert cls.load().clsparseDcsrmv(None, ctypes.byref(A.s), None, None, None, ctl.handle) == \
         cls.clsparseInvalidValue
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["f32", "f64"])
+def test_sum_ax_is_the_spmv_checksum(ctl, dtype):
+    """clsparseAmd[S|D]csrSumAx = 1^T A x = sum of the SpMV's y, against the
+    oracle's FP64 SpMV summed in long double; a matrix without entries sums to
+    0; a short x is refused."""
+    import ctypes
+    lib = cls.load()
+    tdt = torch.float32 if dtype == np.float32 else torch.float64
+    fn = lib.clsparseAmdScsrSumAx if dtype == np.float32 else lib.clsparseAmdDcsrSumAx
+    say(f"sum_ax {dtype.__name__}")
+    for rp, ci, v, m in (cls.gen_powerlaw(50000, 16.0, 1.5, 200000, 2, 20000, seed=21),
+                         cls.gen_poisson2d(40, 30)):
+        vt = v.astype(dtype)
+        x = cls.gen_uniform(m, -1, 1, seed=8).astype(dtype)
+        A = cls.CsrMatrix(rp, ci, vt, m, dtype=dtype)
+        xd = cls.DenseVector(torch.tensor(x, device="cuda"))
+        out = cls.Scalar(0.0, tdt)
+        assert fn(ctypes.byref(A.s), ctypes.byref(xd.s), ctypes.byref(out.s), ctl.handle) == 0
+        y, _ = oracle.csrmv(rp, ci, vt.astype(np.float64), x.astype(np.float64))
+        ref = float(np.sum(y.astype(np.longdouble)))
+        _, scale = oracle.csrmv(rp, ci, np.abs(vt.astype(np.float64)), np.abs(x.astype(np.float64)))
+        bound = 1e-13 * float(np.sum(scale)) + (2.0 ** -24 if dtype == np.float32 else 0.0) * abs(ref)
+        assert abs(out.item() - ref) <= bound, (out.item(), ref)
+    empty = cls.CsrMatrix(np.zeros(4, np.uint32), np.zeros(0, np.uint32), np.zeros(0, dtype), 3,
+                          dtype=dtype)
+    xd = cls.DenseVector(torch.ones(3, dtype=tdt, device="cuda"))
+    out = cls.Scalar(5.0, tdt)
+    assert fn(ctypes.byref(empty.s), ctypes.byref(xd.s), ctypes.byref(out.s), ctl.handle) == 0
+    assert out.item() == 0.0
+    short = cls.DenseVector(torch.ones(2, dtype=tdt, device="cuda"))
+    assert fn(ctypes.byref(empty.s), ctypes.byref(short.s), ctypes.byref(out.s),
+              ctl.handle) == cls.clsparseInvalidSize
