@@ -193,8 +193,7 @@ def solver_workload(cls, torch, ctl, n1d=128, tol=1e-10, cpu=True):
     lib = cls.load()
     out = {"workload": f"diffusion3d_{n1d}^3_lognormal1_jacobi", "n": m, "nnz": int(rp[-1]),
            "tolerance": tol}
-    for name, fn in (("mixed_fp32_inner", lib.clsparseAmdDcsrcgMixed),
-                     ("fp64_jacobi_pcg", lib.clsparseAmdDcsrcgRefined)):
+    def run(name, fn):
         best = None
         for _ in range(2):  # second solve is the timed one (first builds graphs, warms caches)
             x = torch.zeros(m, dtype=torch.float64, device="cuda")
@@ -210,14 +209,37 @@ def solver_workload(cls, torch, ctl, n1d=128, tol=1e-10, cpu=True):
             sc.release()
             cls.check(name, st)
             best = {"seconds": round(dt, 5), "iterations": s.iterations,
-                    "refinements": s.outer_iterations, "true_rel_residual": s.true_residual}
+                    "refinements": s.outer_iterations, "true_rel_residual": s.true_residual,
+                    "a_diagonals": s.a_diagonals}
         out[name] = best
+
+    run("mixed_fp32_inner", lib.clsparseAmdDcsrcgMixed)
+    run("fp64_jacobi_pcg", lib.clsparseAmdDcsrcgRefined)
+    with csr_only():
+        run("fp64_jacobi_pcg_csr", lib.clsparseAmdDcsrcgRefined)
     out["speedup_mixed_vs_fp64"] = round(out["fp64_jacobi_pcg"]["seconds"] /
                                          out["mixed_fp32_inner"]["seconds"], 3)
+    out["speedup_mixed_vs_fp64_csr"] = round(out["fp64_jacobi_pcg_csr"]["seconds"] /
+                                             out["mixed_fp32_inner"]["seconds"], 3)
     if cpu:
         out["cpu_baseline"] = cpu_cg_baseline(rp, ci, v, b, out["fp64_jacobi_pcg"]["iterations"])
     A.meta_delete()
     return out
+
+
+class csr_only:
+    """Within the block the solvers keep A in CSR (CLSPARSE_AMD_DIA=0): the
+    FP64 baseline with the reference's SpMV."""
+
+    def __enter__(self):
+        self.old = os.environ.get("CLSPARSE_AMD_DIA")
+        os.environ["CLSPARSE_AMD_DIA"] = "0"
+
+    def __exit__(self, *exc):
+        if self.old is None:
+            del os.environ["CLSPARSE_AMD_DIA"]
+        else:
+            os.environ["CLSPARSE_AMD_DIA"] = self.old
 
 
 def fsai_workload(cls, torch, ctl, n1d=160, tol=1e-8, weights=None, cpu=True):
@@ -273,7 +295,8 @@ def fsai_workload(cls, torch, ctl, n1d=160, tol=1e-8, weights=None, cpu=True):
             cls.check("solve", st)
             best = {"seconds": round(dt, 5), "iterations": s.iterations,
                     "replacements": s.outer_iterations, "true_rel_residual": s.true_residual,
-                    "converged": s.status_detail == 0, "preconditioner": s.preconditioner}
+                    "converged": s.status_detail == 0, "preconditioner": s.preconditioner,
+                    "a_diagonals": s.a_diagonals, "g_diagonals": s.g_diagonals}
             if first is None:
                 first = {"seconds": best["seconds"], "iterations": best["iterations"]}
         best["first_solve"] = first
@@ -285,11 +308,18 @@ def fsai_workload(cls, torch, ctl, n1d=160, tol=1e-8, weights=None, cpu=True):
     log(f"Jacobi mixed: {out['jacobi_mixed']}")
     out["fp64_jacobi_pcg"] = solve(lib.clsparseAmdDcsrcgRefined, False)
     log(f"FP64 Jacobi: {out['fp64_jacobi_pcg']}")
+    with csr_only():
+        out["fp64_jacobi_pcg_csr"] = solve(lib.clsparseAmdDcsrcgRefined, False)
+    log(f"FP64 Jacobi, CSR-Adaptive SpMV: {out['fp64_jacobi_pcg_csr']}")
     f.apply_net(None, ctl)  # zero network: the same solver with G = D^-1/2
     out["untrained_fsai_mixed"] = solve(lib.clsparseAmdDcsrcgFsai, True)
     log(f"untrained FSAI mixed: {out['untrained_fsai_mixed']}")
     base = out["fp64_jacobi_pcg"]["seconds"]
     out["speedup_learned_fsai_vs_fp64_jacobi"] = round(base / out["learned_fsai_mixed"]["seconds"], 3)
+    out["speedup_learned_fsai_vs_fp64_jacobi_csr"] = round(
+        out["fp64_jacobi_pcg_csr"]["seconds"] / out["learned_fsai_mixed"]["seconds"], 3)
+    out["speedup_learned_fsai_first_solve_vs_fp64_jacobi"] = round(
+        base / out["learned_fsai_mixed"]["first_solve"]["seconds"], 3)
     out["speedup_jacobi_mixed_vs_fp64_jacobi"] = round(base / out["jacobi_mixed"]["seconds"], 3)
     out["iteration_ratio_learned_vs_untrained"] = round(
         out["learned_fsai_mixed"]["iterations"] / out["untrained_fsai_mixed"]["iterations"], 3)

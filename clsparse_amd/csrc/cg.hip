@@ -292,6 +292,13 @@ __device__ __forceinline__ float mul_rounded_f(float a, float b)
     return a * b;
 }
 
+template <typename T>
+__device__ __forceinline__ T mul_rounded_t(T a, T b)
+{
+#pragma clang fp contract(off)
+    return a * b;
+}
+
 // y = G^T r and partials of ||y||^2 (one per workgroup, g workgroups).
 // Row i of G^T holds G(i - off[d], i) at column i - off[d]: ascending column
 // order walks the diagonals from off = 0 down.
@@ -377,11 +384,11 @@ __global__ __launch_bounds__(256) void fsai_g_dia_kernel(size_t n, DiaOffsets of
 // y = A p from diagonal storage (vals[d n + i] = A(i, i + off[d]), off
 // ascending, zero outside the pattern) with partials of <y, p> in the
 // grid-stride order of cg_dot_kernel (g workgroups).
-template <int K>
+template <int K, typename T>
 __global__ __launch_bounds__(256) void dia_spmv_dot_kernel(size_t n, DiaOffsets off,
-                                                           const float* __restrict__ vals,
-                                                           const float* __restrict__ p,
-                                                           float* __restrict__ y,
+                                                           const T* __restrict__ vals,
+                                                           const T* __restrict__ p,
+                                                           T* __restrict__ y,
                                                            double* __restrict__ part,
                                                            const int* __restrict__ done)
 {
@@ -390,7 +397,7 @@ __global__ __launch_bounds__(256) void dia_spmv_dot_kernel(size_t n, DiaOffsets 
     double a = 0.0;
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
     for (size_t i0 = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i0 < n; i0 += kRows * stride) {
-        float v[kRows][K], xv[kRows][K], pi[kRows];
+        T v[kRows][K], xv[kRows][K], pi[kRows];
 #pragma unroll
         for (int u = 0; u < kRows; ++u) {
             const size_t i = i0 + u * stride;
@@ -398,19 +405,19 @@ __global__ __launch_bounds__(256) void dia_spmv_dot_kernel(size_t n, DiaOffsets 
             for (int d = 0; d < K; ++d) {
                 const int64_t j = static_cast<int64_t>(i) + off.o[d];
                 const bool in = i < n && j >= 0 && j < static_cast<int64_t>(n);
-                v[u][d] = in ? vals[static_cast<size_t>(d) * n + i] : 0.f;
-                xv[u][d] = in ? p[j] : 0.f;
+                v[u][d] = in ? vals[static_cast<size_t>(d) * n + i] : T(0);
+                xv[u][d] = in ? p[j] : T(0);
             }
-            pi[u] = i < n ? p[i] : 0.f;
+            pi[u] = i < n ? p[i] : T(0);
         }
 #pragma unroll
         for (int u = 0; u < kRows; ++u) {
             const size_t i = i0 + u * stride;
             if (i >= n) continue;
-            float acc = 0.f;
+            T acc = T(0);
 #pragma unroll
             for (int d = 0; d < K; ++d)
-                if (v[u][d] != 0.f) acc += mul_rounded_f(v[u][d], xv[u][d]);
+                if (v[u][d] != T(0)) acc += mul_rounded_t(v[u][d], xv[u][d]);
             y[i] = acc;
             a += static_cast<double>(acc) * static_cast<double>(pi[u]);
         }
@@ -419,13 +426,14 @@ __global__ __launch_bounds__(256) void dia_spmv_dot_kernel(size_t n, DiaOffsets 
     if (threadIdx.x == 0) part[blockIdx.x] = a;
 }
 
-// dia[d n + i] = (float) A(i, i + off[d]) for every stored entry; *bad = 1 if
+// dia[d n + i] = (TO) A(i, i + off[d]) for every stored entry; *bad = 1 if
 // a row has an entry off the K diagonals or columns not strictly increasing
 // (then the CSR SpMV stays). dia must be zero beforehand.
+template <typename TA, typename TO>
 __global__ __launch_bounds__(256) void dia_fill_kernel(uint32_t n, const uint32_t* __restrict__ rp,
                                                        const uint32_t* __restrict__ ci,
-                                                       const double* __restrict__ v, int K,
-                                                       DiaOffsets off, float* __restrict__ dia,
+                                                       const TA* __restrict__ v, int K,
+                                                       DiaOffsets off, TO* __restrict__ dia,
                                                        int* __restrict__ bad)
 {
     const uint32_t row = blockIdx.x * 256 + threadIdx.x;
@@ -441,7 +449,7 @@ __global__ __launch_bounds__(256) void dia_fill_kernel(uint32_t n, const uint32_
             return;
         }
         prev = o;
-        dia[static_cast<size_t>(d) * n + row] = static_cast<float>(v[j]);
+        dia[static_cast<size_t>(d) * n + row] = static_cast<TO>(v[j]);
     }
 }
 
@@ -477,11 +485,12 @@ __global__ __launch_bounds__(64) void dia_sample_kernel(uint32_t n, const uint32
     }
 }
 
-template <int K>
-void launch_dia_spmv_dot(size_t n, uint32_t g, const DiaOffsets& off, const float* vals, const float* p,
-                         float* y, double* part, const int* done, hipStream_t s)
+template <int K, typename T>
+void launch_dia_spmv_dot(size_t n, uint32_t g, const DiaOffsets& off, const T* vals, const T* p, T* y,
+                         double* part, const int* done, hipStream_t s)
 {
-    hipLaunchKernelGGL((dia_spmv_dot_kernel<K>), dim3(g), dim3(256), 0, s, n, off, vals, p, y, part, done);
+    hipLaunchKernelGGL((dia_spmv_dot_kernel<K, T>), dim3(g), dim3(256), 0, s, n, off, vals, p, y, part,
+                       done);
 }
 
 // p = G y + beta p with beta = ||y||^2_new / <r, z>_old (the p update of
@@ -599,12 +608,12 @@ public:
     CgEngine(const clsparseCsrMatrix* A, TX* x, const T* b, size_t n, int prec, int crit,
              CgBuffers<T>& buf, hipStream_t s, bool ext, const clsparseCsrMatrix* G = nullptr,
              const clsparseCsrMatrix* GT = nullptr, bool g_bf16 = false,
-             const FsaiDia* dia = nullptr, const float* a_dia = nullptr, int a_ndiag = 0,
+             const FsaiDia* dia = nullptr, const T* a_dia = nullptr, int a_ndiag = 0,
              const DiaOffsets* a_off = nullptr)
         : A_(A), x_(x), b_(b), n_(n), prec_(prec), crit_(crit), buf_(buf), s_(s), ext_(ext),
           g_(grid_for(n)), G_(G), GT_(GT), g_bf16_(g_bf16)
     {
-        if (a_dia != nullptr && std::is_same_v<T, float>) {
+        if (a_dia != nullptr) {
             a_dia_ = a_dia;
             a_ndiag_ = a_ndiag;
             a_off_ = *a_off;
@@ -786,24 +795,23 @@ private:
     // y = A p and <y, p> partials from A's diagonal storage.
     clsparseStatus spmv_dot_dia(const int* done)
     {
-        if constexpr (std::is_same_v<T, float>) {
-            const float* v = a_dia_;
-            switch (a_ndiag_) {
-            case 1: launch_dia_spmv_dot<1>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_); break;
-            case 2: launch_dia_spmv_dot<2>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_); break;
-            case 3: launch_dia_spmv_dot<3>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_); break;
-            case 4: launch_dia_spmv_dot<4>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_); break;
-            case 5: launch_dia_spmv_dot<5>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_); break;
-            case 6: launch_dia_spmv_dot<6>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_); break;
-            case 7: launch_dia_spmv_dot<7>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_); break;
-            case 8: launch_dia_spmv_dot<8>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_); break;
-            case 9: launch_dia_spmv_dot<9>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_); break;
-            default: return clsparseInvalidValue;
-            }
-            CLS_HIP_TRY(hipGetLastError());
-            return clsparseSuccess;
+        const T* v = a_dia_;
+#define CLS_DIA_SPMV(K) launch_dia_spmv_dot<K, T>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_)
+        switch (a_ndiag_) {
+        case 1: CLS_DIA_SPMV(1); break;
+        case 2: CLS_DIA_SPMV(2); break;
+        case 3: CLS_DIA_SPMV(3); break;
+        case 4: CLS_DIA_SPMV(4); break;
+        case 5: CLS_DIA_SPMV(5); break;
+        case 6: CLS_DIA_SPMV(6); break;
+        case 7: CLS_DIA_SPMV(7); break;
+        case 8: CLS_DIA_SPMV(8); break;
+        case 9: CLS_DIA_SPMV(9); break;
+        default: return clsparseInvalidValue;
         }
-        return clsparseInvalidValue;
+#undef CLS_DIA_SPMV
+        CLS_HIP_TRY(hipGetLastError());
+        return clsparseSuccess;
     }
 
     // tmp = G^T r (with ||tmp||^2 partials), out = G tmp, diagonal storage.
@@ -851,13 +859,77 @@ private:
     bool g_bf16_;
     bool use_dia_ = false;
     FsaiDia dia_{};
-    const float* a_dia_ = nullptr;
+    const T* a_dia_ = nullptr;
     int a_ndiag_ = 0;
     DiaOffsets a_off_{};
     DiaOffsets dia_off_{};
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
 };
+
+bool dia_enabled()
+{
+    const char* e = std::getenv("CLSPARSE_AMD_DIA");
+    return !(e && e[0] == '0');
+}
+
+// Union of the column - row offsets of kDiaSamples rows spread over A,
+// ascending, when it has at most kDiaMax entries and that many diagonals of
+// length n would store at most 1.5 nnz values; *ndiag = 0 otherwise. The
+// fill kernel checks every row against the result.
+clsparseStatus dia_candidates(const clsparseCsrMatrix* A, int32_t* scratch, hipStream_t s,
+                              DiaOffsets* off, int* ndiag)
+{
+    *ndiag = 0;
+    const uint32_t n = A->num_rows;
+    if (n < 8 || A->num_nonzeros == 0 || !dia_enabled()) return clsparseSuccess;
+    hipLaunchKernelGGL(dia_sample_kernel, dim3(1), dim3(64), 0, s, n,
+                       dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer),
+                       dev_ptr<uint32_t>(A->col_indices, A->off_col_indices), scratch);
+    CLS_HIP_TRY(hipGetLastError());
+    std::vector<int32_t> h(16 * kDiaSamples);
+    CLS_HIP_TRY(hipMemcpyAsync(h.data(), scratch, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    CLS_HIP_TRY(hipStreamSynchronize(s));
+    std::vector<int32_t> offs;
+    for (int k = 0; k < kDiaSamples; ++k) {
+        const int32_t cnt = h[16 * k];
+        if (cnt > 15) return clsparseSuccess;
+        for (int32_t j = 0; j < cnt; ++j) {
+            const int32_t o = h[16 * k + 1 + j];
+            if (std::find(offs.begin(), offs.end(), o) == offs.end()) offs.push_back(o);
+        }
+        if (offs.size() > static_cast<size_t>(kDiaMax)) return clsparseSuccess;
+    }
+    if (offs.empty()) return clsparseSuccess;
+    if (static_cast<double>(offs.size()) * n > 1.5 * static_cast<double>(A->num_nonzeros))
+        return clsparseSuccess;
+    std::sort(offs.begin(), offs.end());
+    for (size_t d = 0; d < offs.size(); ++d) off->o[d] = offs[d];
+    *ndiag = static_cast<int>(offs.size());
+    return clsparseSuccess;
+}
+
+// Fills `vals` (ndiag * n values, from the workspace) with A's values in
+// diagonal storage; *ndiag = 0 when a row does not fit the candidates (the
+// CSR SpMV stays).
+template <typename TA, typename TO>
+clsparseStatus dia_fill(const clsparseCsrMatrix* A, size_t n, int* ndiag, const DiaOffsets& off, TO* vals,
+                        int* bad, hipStream_t s)
+{
+    hipLaunchKernelGGL((fill_kernel<TO>), dim3(ew_grid(*ndiag * n)), dim3(256), 0, s, *ndiag * n, vals,
+                       TO(0));
+    hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(256), 0, s, size_t(1), bad, 0);
+    hipLaunchKernelGGL((dia_fill_kernel<TA, TO>), dim3((n + 255) / 256), dim3(256), 0, s,
+                       static_cast<uint32_t>(n), dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer),
+                       dev_ptr<uint32_t>(A->col_indices, A->off_col_indices),
+                       dev_ptr<TA>(A->values, A->off_values), *ndiag, off, vals, bad);
+    CLS_HIP_TRY(hipGetLastError());
+    int h = 0;
+    CLS_HIP_TRY(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    CLS_HIP_TRY(hipStreamSynchronize(s));
+    if (h) *ndiag = 0;
+    return clsparseSuccess;
+}
 
 template <typename T>
 clsparseStatus cg_solve(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
@@ -874,14 +946,25 @@ clsparseStatus cg_solve(cldenseVector* x, const clsparseCsrMatrix* A, const clde
     const T* bp = dev_ptr<T>(b->values, b->off_values);
     if (n > 0 && (xp == nullptr || bp == nullptr)) return clsparseInvalidMemObject;
 
+    // Stencil matrices: A's values in diagonal storage for the iterations
+    // (same row sums as the CSR-Adaptive SpMV; not with extended precision).
+    DiaOffsets a_off{};
+    int a_ndiag = 0;
+    if (!control->extended_precision)
+        CLS_TRY(dia_candidates(A, reinterpret_cast<int32_t*>(control->red_partials), s, &a_off, &a_ndiag));
+
     void* work = nullptr;
-    CLS_TRY(control_workspace(control, CgBuffers<T>::bytes(n), &work));
+    CLS_TRY(control_workspace(control, CgBuffers<T>::bytes(n) +
+                                           (a_ndiag ? align_up(a_ndiag * n * sizeof(T)) + align_up(sizeof(int)) : 0),
+                              &work));
     void* pinned = nullptr;
     CLS_TRY(control_pinned(control, sizeof(CgDevState) + 64, &pinned));
     CgDevState* hs = static_cast<CgDevState*>(pinned);
     Carve c{static_cast<char*>(work)};
     CgBuffers<T> buf;
     buf.carve(c, n);
+    T* a_dia = a_ndiag ? c.take<T>(a_ndiag * n) : nullptr;
+    int* a_bad = a_ndiag ? c.take<int>(1) : nullptr;
 
     CLS_TRY(enter_solver_stream(control));
 
@@ -926,7 +1009,12 @@ clsparseStatus cg_solve(cldenseVector* x, const clsparseCsrMatrix* A, const clde
     *hs = init;
     CLS_HIP_TRY(hipMemcpyAsync(buf.st, hs, sizeof(CgDevState), hipMemcpyHostToDevice, s));
 
-    CgEngine<T> eng(A, xp, bp, n, prec, kCritL1Ref, buf, s, control->extended_precision);
+    if (a_ndiag) {
+        CLS_TRY((dia_fill<T, T>(A, n, &a_ndiag, a_off, a_dia, a_bad, s)));
+        if (a_ndiag == 0) a_dia = nullptr;
+    }
+    CgEngine<T> eng(A, xp, bp, n, prec, kCritL1Ref, buf, s, control->extended_precision, nullptr, nullptr,
+                    false, nullptr, a_dia, a_ndiag, &a_off);
     CLS_TRY(eng.setup(false));
     CLS_HIP_TRY(hipMemcpyAsync(hs, buf.st, sizeof(CgDevState), hipMemcpyDeviceToHost, s));
     CLS_HIP_TRY(hipStreamSynchronize(s));
@@ -961,6 +1049,7 @@ clsparseStatus cg_solve(cldenseVector* x, const clsparseCsrMatrix* A, const clde
     sc->stats.l1_residual = hs->cur_res;
     sc->stats.seconds = std::chrono::duration<double>(t1 - t0).count();
     sc->stats.status_detail = hs->converged ? 0 : 1;
+    sc->stats.a_diagonals = a_ndiag;
     return clsparseSuccess;
 }
 
@@ -1072,41 +1161,6 @@ __global__ void state_resume(CgDevState* st, const double* __restrict__ rnorm)
     }
 }
 
-// Union of the column - row offsets of kDiaSamples rows spread over A,
-// ascending, when it has at most kDiaMax entries and that many diagonals of
-// length n would store at most 1.5 nnz values; *ndiag = 0 otherwise. The
-// fill kernel checks every row against the result.
-clsparseStatus dia_candidates(const clsparseCsrMatrix* A, int32_t* scratch, hipStream_t s,
-                              DiaOffsets* off, int* ndiag)
-{
-    *ndiag = 0;
-    const uint32_t n = A->num_rows;
-    hipLaunchKernelGGL(dia_sample_kernel, dim3(1), dim3(64), 0, s, n,
-                       dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer),
-                       dev_ptr<uint32_t>(A->col_indices, A->off_col_indices), scratch);
-    CLS_HIP_TRY(hipGetLastError());
-    std::vector<int32_t> h(16 * kDiaSamples);
-    CLS_HIP_TRY(hipMemcpyAsync(h.data(), scratch, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    CLS_HIP_TRY(hipStreamSynchronize(s));
-    std::vector<int32_t> offs;
-    for (int k = 0; k < kDiaSamples; ++k) {
-        const int32_t cnt = h[16 * k];
-        if (cnt > 15) return clsparseSuccess;
-        for (int32_t j = 0; j < cnt; ++j) {
-            const int32_t o = h[16 * k + 1 + j];
-            if (std::find(offs.begin(), offs.end(), o) == offs.end()) offs.push_back(o);
-        }
-        if (offs.size() > static_cast<size_t>(kDiaMax)) return clsparseSuccess;
-    }
-    if (offs.empty()) return clsparseSuccess;
-    if (static_cast<double>(offs.size()) * n > 1.5 * static_cast<double>(A->num_nonzeros))
-        return clsparseSuccess;
-    std::sort(offs.begin(), offs.end());
-    for (size_t d = 0; d < offs.size(); ++d) off->o[d] = offs[d];
-    *ndiag = static_cast<int>(offs.size());
-    return clsparseSuccess;
-}
-
 template <typename TI>
 clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
                               clSParseSolverControl sc, clsparseControl control,
@@ -1132,17 +1186,12 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     double delta = 0.1;
     if (const char* e = std::getenv("CLSPARSE_AMD_RELIABLE_DELTA")) delta = std::atof(e);
 
-    // Mixed solver: candidate diagonals of A from rows spread over it (a
-    // stencil's interior rows hold all of them); the fill kernel below checks
-    // every row against them. CLSPARSE_AMD_DIA=0: CSR always.
+    // Candidate diagonals of A from rows spread over it (a stencil's interior
+    // rows hold all of them); the fill kernel below checks every row against
+    // them. CLSPARSE_AMD_DIA=0: CSR always.
     DiaOffsets a_off{};
     int a_ndiag = 0;
-    if constexpr (kMixed) {
-        const char* ed = std::getenv("CLSPARSE_AMD_DIA");
-        if (!(ed && ed[0] == '0') && n >= 8 && nnz > 0)
-            CLS_TRY(dia_candidates(A, reinterpret_cast<int32_t*>(control->red_partials), s, &a_off,
-                                   &a_ndiag));
-    }
+    CLS_TRY(dia_candidates(A, reinterpret_cast<int32_t*>(control->red_partials), s, &a_off, &a_ndiag));
 
     // Workspace: TI values of A (mixed only), FP64 residual, FP64 copy of x
     // (restored if a preconditioner breaks down), scalars, TI start residual,
@@ -1151,7 +1200,7 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     const size_t bytes = (kMixed ? align_up(nnz * sizeof(TI)) : 0) + 2 * align_up(n * sizeof(double)) +
                          align_up(8 * sizeof(double)) + align_up(n * sizeof(TI)) +
                          CgBuffers<TI>::bytes(n) + (fsai ? 2 * align_up(n * sizeof(double)) : 0) +
-                         (a_ndiag ? align_up(a_ndiag * n * sizeof(float)) + align_up(sizeof(int)) : 0);
+                         (a_ndiag ? align_up(a_ndiag * n * sizeof(TI)) + align_up(sizeof(int)) : 0);
     void* work = nullptr;
     CLS_TRY(control_workspace(control, bytes, &work));
     void* pinned = nullptr;
@@ -1167,7 +1216,7 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     buf.carve(c, n);
     double* xstart = fsai ? c.take<double>(n) : nullptr;
     double* xpilot = fsai ? c.take<double>(n) : nullptr;
-    float* a_dia = a_ndiag ? c.take<float>(a_ndiag * n) : nullptr;
+    TI* a_dia = a_ndiag ? c.take<TI>(a_ndiag * n) : nullptr;
     int* a_bad = a_ndiag ? c.take<int>(1) : nullptr;
 
     CLS_TRY(enter_solver_stream(control));
@@ -1196,23 +1245,10 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
                            dev_ptr<double>(A->values, A->off_values), valsI);
         AI.values = valsI;
         AI.off_values = 0;
-        if (a_ndiag) {
-            hipLaunchKernelGGL((fill_kernel<float>), dim3(ew_grid(a_ndiag * n)), dim3(256), 0, s,
-                               a_ndiag * n, a_dia, 0.f);
-            hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(256), 0, s, size_t(1), a_bad, 0);
-            hipLaunchKernelGGL(dia_fill_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
-                               static_cast<uint32_t>(n), dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer),
-                               dev_ptr<uint32_t>(A->col_indices, A->off_col_indices),
-                               dev_ptr<double>(A->values, A->off_values), a_ndiag, a_off, a_dia, a_bad);
-            CLS_HIP_TRY(hipGetLastError());
-            int bad = 0;
-            CLS_HIP_TRY(hipMemcpyAsync(&bad, a_bad, sizeof(int), hipMemcpyDeviceToHost, s));
-            CLS_HIP_TRY(hipStreamSynchronize(s));
-            if (bad) {
-                a_ndiag = 0;
-                a_dia = nullptr;
-            }
-        }
+    }
+    if (a_ndiag) {
+        CLS_TRY((dia_fill<double, TI>(A, n, &a_ndiag, a_off, a_dia, a_bad, s)));
+        if (a_ndiag == 0) a_dia = nullptr;
     }
     int prec = (sc->preconditioner == DIAGONAL) ? kPrecJacobi : kPrecNone;
     int used = prec;  // statistics: 0 none, 1 Jacobi, 2 FSAI, 3 FSAI -> Jacobi

@@ -55,8 +55,9 @@ typedef struct clsparseAmdSolverStats_
      * second of the pilot race, for the factor and for Jacobi (0 if not run). */
     cl_double pilot_rate_fsai;
     cl_double pilot_rate_jacobi;
-    /* FP32 inner solver: diagonals of the diagonal storage A's values were
-     * moved to for the solve (0: CSR-Adaptive SpMV), and of the learned
+    /* CG solvers (clsparse[S/D]csrcg and the refinement solvers): diagonals
+     * of the diagonal storage A's values were moved to for the iterations of
+     * the solve (0: CSR-Adaptive SpMV); learned-FSAI solver: diagonals of the
      * factor's storage (0: CSR SpMVs or no factor). */
     cl_int a_diagonals;
     cl_int g_diagonals;

@@ -38,15 +38,21 @@ def _solve(ctl, rp, ci, v, m, b, x0, dtype, precond, max_iters, rel, abst, meta=
     return st, x.cpu().numpy(), stats
 
 
+@pytest.mark.parametrize("dia", ["1", "0"], ids=["diagonal-storage", "csr"])
 @pytest.mark.parametrize("meta", [True, False], ids=["adaptive", "vector"])
 @pytest.mark.parametrize("precond", [cls.DIAGONAL, cls.NOPRECOND], ids=["jacobi", "none"])
-def test_poisson_fp64_matches_oracle(ctl, precond, meta):
-    say(f"cg poisson fp64 precond={precond} meta={meta}")
+def test_poisson_fp64_matches_oracle(ctl, monkeypatch, precond, meta, dia):
+    """The 5-point Poisson matrix runs the iterations from diagonal storage by
+    default; CLSPARSE_AMD_DIA=0 keeps the CSR SpMV (CSR-Adaptive with
+    metadata, CSR-Vector without)."""
+    monkeypatch.setenv("CLSPARSE_AMD_DIA", dia)
+    say(f"cg poisson fp64 precond={precond} meta={meta} dia={dia}")
     rp, ci, v, m = cls.gen_poisson2d(64, 64)
     b = cls.gen_uniform(m, -1, 1, seed=3)
     st, x, s = _solve(ctl, rp, ci, v, m, b, np.zeros(m), np.float64, precond, 2000, 1e-10, 0.0,
                       meta)
     assert st == 0
+    assert s.a_diagonals == (5 if dia == "1" else 0)
     xr, r = oracle.cg(rp, ci, v, b, np.zeros(m), precond, 2000, 1e-10, 0.0)
     assert abs(s.n_iters - r.n_iters) <= 1, (s.n_iters, r.n_iters)
     assert s.current_residual <= 1e-10 and s.iterations == s.n_iters + 1
@@ -309,3 +315,36 @@ def test_mixed_keeps_csr_for_entries_off_the_diagonals(ctl):
     assert st.a_diagonals == 0 and st.status_detail == 0
     r = b - oracle.csrmv(rp2, ci2, a.data, x)[0]
     assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-9
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["f32", "f64"])
+def test_reference_cg_diagonal_storage_matches_csr(ctl, monkeypatch, dtype):
+    """clsparse[S/D]csrcg on a 3-D stencil: the iterations from diagonal storage
+    and from the CSR-Adaptive SpMV (same row sums) stop at the same iteration
+    count with the same solution to rounding."""
+    rp, ci, v, m = cls.gen_diffusion3d(30, 28, 26, sigma=1.0, ax=1.0, ay=1.0, az=10.0, seed=19)
+    b = cls.gen_uniform(m, -1, 1, seed=29)
+    tol = 1e-5 if dtype == np.float32 else 1e-10
+    say(f"reference CG diagonal storage vs CSR {dtype.__name__}")
+    st_d, x_d, s_d = _solve(ctl, rp, ci, v, m, b, np.zeros(m), dtype, cls.DIAGONAL, 5000, tol, 0.0)
+    monkeypatch.setenv("CLSPARSE_AMD_DIA", "0")
+    st_c, x_c, s_c = _solve(ctl, rp, ci, v, m, b, np.zeros(m), dtype, cls.DIAGONAL, 5000, tol, 0.0)
+    assert st_d == 0 and st_c == 0
+    assert s_d.a_diagonals == 7 and s_c.a_diagonals == 0
+    assert abs(s_d.n_iters - s_c.n_iters) <= (2 if dtype == np.float32 else 1)
+    scale = np.abs(x_c).max()
+    assert np.abs(x_d - x_c).max() <= (1e-3 if dtype == np.float32 else 1e-8) * scale
+
+
+def test_reference_cg_extended_precision_keeps_csr(ctl):
+    """Extended precision needs the compensated CSR sums: no diagonal storage."""
+    rp, ci, v, m = cls.gen_poisson2d(40, 40)
+    b = cls.gen_uniform(m, -1, 1, seed=31)
+    ext = cls.Control(extended_precision=True)
+    try:
+        st, x, s = _solve(ext, rp, ci, v, m, b, np.zeros(m), np.float64, cls.DIAGONAL, 2000, 1e-10, 0.0)
+        assert st == 0 and s.a_diagonals == 0
+        _, r = oracle.cg(rp, ci, v, b, np.zeros(m), 1, 2000, 1e-10, 0.0)
+        assert abs(s.n_iters - r.n_iters) <= 1
+    finally:
+        ext.release()
