@@ -312,9 +312,10 @@ __global__ __launch_bounds__(256) void fsai_gt_dia_kernel(size_t n, DiaOffsets o
 {
     __shared__ double sh[4];
     if (done != nullptr && *done != 0) return;
+    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);  // contiguous rows per XCD
     double a = 0.0;
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
-    for (size_t i0 = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i0 < n; i0 += kRows * stride) {
+    for (size_t i0 = static_cast<size_t>(blk) * 256 + threadIdx.x; i0 < n; i0 += kRows * stride) {
         uint16_t h[kRows][K];
         float rv[kRows][K];
 #pragma unroll
@@ -341,7 +342,7 @@ __global__ __launch_bounds__(256) void fsai_gt_dia_kernel(size_t n, DiaOffsets o
         }
     }
     a = block_sum256(a, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = a;
+    if (threadIdx.x == 0) part[blk] = a;
 }
 
 // z = G y.
@@ -383,8 +384,13 @@ __global__ __launch_bounds__(256) void fsai_g_dia_kernel(size_t n, DiaOffsets of
 
 // y = A p from diagonal storage (vals[d n + i] = A(i, i + off[d]), off
 // ascending, zero outside the pattern) with partials of <y, p> in the
-// grid-stride order of cg_dot_kernel (g workgroups).
-template <int K, typename T>
+// grid-stride order of cg_dot_kernel (g workgroups). SYM: A is symmetric with
+// offsets -o..o, and the upper diagonals are read from the lower ones,
+// A(i, i + o) = A(i + o, i) = vals[d' n + i + o] with off[d'] = -o, so only
+// (K + 1) / 2 diagonals are read. Workgroups are remapped so that each XCD
+// walks contiguous row ranges (the shifted reads of p -- and of the mirrored
+// diagonals -- then meet lines its own L2 holds).
+template <int K, bool SYM, typename T>
 __global__ __launch_bounds__(256) void dia_spmv_dot_kernel(size_t n, DiaOffsets off,
                                                            const T* __restrict__ vals,
                                                            const T* __restrict__ p,
@@ -392,11 +398,13 @@ __global__ __launch_bounds__(256) void dia_spmv_dot_kernel(size_t n, DiaOffsets 
                                                            double* __restrict__ part,
                                                            const int* __restrict__ done)
 {
+    constexpr int KL = SYM ? (K + 1) / 2 : K;
     __shared__ double sh[4];
     if (done != nullptr && *done != 0) return;
+    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
     double a = 0.0;
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
-    for (size_t i0 = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i0 < n; i0 += kRows * stride) {
+    for (size_t i0 = static_cast<size_t>(blk) * 256 + threadIdx.x; i0 < n; i0 += kRows * stride) {
         T v[kRows][K], xv[kRows][K], pi[kRows];
 #pragma unroll
         for (int u = 0; u < kRows; ++u) {
@@ -405,7 +413,9 @@ __global__ __launch_bounds__(256) void dia_spmv_dot_kernel(size_t n, DiaOffsets 
             for (int d = 0; d < K; ++d) {
                 const int64_t j = static_cast<int64_t>(i) + off.o[d];
                 const bool in = i < n && j >= 0 && j < static_cast<int64_t>(n);
-                v[u][d] = in ? vals[static_cast<size_t>(d) * n + i] : T(0);
+                const size_t at = d < KL ? static_cast<size_t>(d) * n + i
+                                         : static_cast<size_t>(K - 1 - d) * n + static_cast<size_t>(j);
+                v[u][d] = in ? vals[at] : T(0);
                 xv[u][d] = in ? p[j] : T(0);
             }
             pi[u] = i < n ? p[i] : T(0);
@@ -423,7 +433,100 @@ __global__ __launch_bounds__(256) void dia_spmv_dot_kernel(size_t n, DiaOffsets 
         }
     }
     a = block_sum256(a, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = a;
+    if (threadIdx.x == 0) part[blk] = a;
+}
+
+// The same product through buffer loads: the value of diagonal d at row i is
+// the buffer offset d*n*sizeof(T) (one scalar register per diagonal) plus
+// i*sizeof(T) (one vector register per row), where the 64-bit flat addresses
+// took two vector registers per load and left two waves per SIMD; reads of p
+// outside [0, n) return 0 from the buffer's range check. Needs
+// K * n * sizeof(T) < 2^32.
+template <typename T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff)
+{
+    if constexpr (sizeof(T) == 4)
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
+    else
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
+}
+
+// The grid (at most kReducePartialsMax workgroups of 4 waves) puts 4 waves on
+// a SIMD: tell the scheduler so, or it trades the loads in flight for an
+// occupancy the launch never reaches.
+template <int K, bool SYM, typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void dia_spmv_dot_buf_kernel(uint32_t n, DiaOffsets off,
+                                                               const T* __restrict__ vals,
+                                                               const T* __restrict__ p,
+                                                               T* __restrict__ y,
+                                                               double* __restrict__ part,
+                                                               const int* __restrict__ done)
+{
+    constexpr int KL = SYM ? (K + 1) / 2 : K;
+    constexpr uint32_t S = sizeof(T);
+    constexpr int R = S == 8 ? 2 : kRows;  // FP64: fewer rows per step keep it under 128 VGPRs
+    __shared__ double sh[4];
+    if (done != nullptr && *done != 0) return;
+    const __amdgpu_buffer_rsrc_t rv =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(vals), 0, K * n * S, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(p), 0, n * S, 0x00020000);
+    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+    double a = 0.0;
+    const uint32_t stride = gridDim.x * 256;
+    for (uint32_t i0 = blk * 256 + threadIdx.x; i0 < n; i0 += R * stride) {
+        T v[R][K], xv[R][K], pi[R];
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const uint32_t i = i0 + u * stride;
+#pragma unroll
+            for (int d = 0; d < K; ++d) {
+                const uint32_t j = i + static_cast<uint32_t>(off.o[d]);  // wraps below 0: out of range
+                xv[u][d] = buf_load<T>(rx, j * S, 0);
+                if (d < KL) {
+                    v[u][d] = buf_load<T>(rv, i * S, static_cast<uint32_t>(d) * n * S);
+                } else {
+                    // Mirrored value at row j of the lower diagonal; an offset past
+                    // the buffer (no branch) reads 0 where j >= n.
+                    const uint32_t at = j < n ? (static_cast<uint32_t>(K - 1 - d) * n + j) * S : 0xFFFFFFFFu;
+                    v[u][d] = buf_load<T>(rv, at, 0);
+                }
+            }
+            pi[u] = buf_load<T>(rx, i * S, 0);
+        }
+        // Rows past n are computed too (from in-range or zero loads) and only
+        // their store and dot term are dropped: a branch around the whole row
+        // lets the compiler sink that row's loads into it, one row at a time.
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const uint32_t i = i0 + u * stride;
+            T acc = T(0);
+#pragma unroll
+            for (int d = 0; d < K; ++d)
+                if (v[u][d] != T(0)) acc += mul_rounded_t(v[u][d], xv[u][d]);
+            if (i < n) y[i] = acc;
+            a += i < n ? static_cast<double>(acc) * static_cast<double>(pi[u]) : 0.0;
+        }
+    }
+    a = block_sum256(a, sh);
+    if (threadIdx.x == 0) part[blk] = a;
+}
+
+// *bad = 1 unless A(i, i + o) == A(i + o, i) for every upper diagonal o of
+// the full diagonal storage (offsets already checked to be -o..o).
+template <typename T>
+__global__ __launch_bounds__(256) void dia_sym_check_kernel(uint32_t n, int K, DiaOffsets off,
+                                                            const T* __restrict__ vals, int* __restrict__ bad)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    for (int d = (K + 1) / 2; d < K; ++d) {
+        const uint64_t j = static_cast<uint64_t>(i) + static_cast<uint32_t>(off.o[d]);
+        if (j >= n) continue;
+        if (!(vals[static_cast<size_t>(d) * n + i] == vals[static_cast<size_t>(K - 1 - d) * n + j])) {
+            *bad = 1;
+            return;
+        }
+    }
 }
 
 // dia[d n + i] = (TO) A(i, i + off[d]) for every stored entry; *bad = 1 if
@@ -485,12 +588,16 @@ __global__ __launch_bounds__(64) void dia_sample_kernel(uint32_t n, const uint32
     }
 }
 
-template <int K, typename T>
+template <int K, bool SYM, typename T>
 void launch_dia_spmv_dot(size_t n, uint32_t g, const DiaOffsets& off, const T* vals, const T* p, T* y,
                          double* part, const int* done, hipStream_t s)
 {
-    hipLaunchKernelGGL((dia_spmv_dot_kernel<K, T>), dim3(g), dim3(256), 0, s, n, off, vals, p, y, part,
-                       done);
+    if (static_cast<uint64_t>(K) * n * sizeof(T) < (uint64_t(1) << 32) - 16)
+        hipLaunchKernelGGL((dia_spmv_dot_buf_kernel<K, SYM, T>), dim3(g), dim3(256), 0, s,
+                           static_cast<uint32_t>(n), off, vals, p, y, part, done);
+    else
+        hipLaunchKernelGGL((dia_spmv_dot_kernel<K, SYM, T>), dim3(g), dim3(256), 0, s, n, off, vals, p, y,
+                           part, done);
 }
 
 // p = G y + beta p with beta = ||y||^2_new / <r, z>_old (the p update of
@@ -507,8 +614,9 @@ __global__ __launch_bounds__(256) void fsai_g_pupdate_dia_kernel(
     const double rz_new = sum_partials(part_rz, g, sh);
     const double rz_old = sum_partials(part_rz_prev, g, sh);
     const float beta = static_cast<float>(rz_new / rz_old);
+    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);  // contiguous rows per XCD
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
-    for (size_t i0 = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i0 < n; i0 += kRows * stride) {
+    for (size_t i0 = static_cast<size_t>(blk) * 256 + threadIdx.x; i0 < n; i0 += kRows * stride) {
         uint16_t h[kRows][K];
         float yv[kRows][K], pv[kRows];
 #pragma unroll
@@ -609,13 +717,14 @@ public:
              CgBuffers<T>& buf, hipStream_t s, bool ext, const clsparseCsrMatrix* G = nullptr,
              const clsparseCsrMatrix* GT = nullptr, bool g_bf16 = false,
              const FsaiDia* dia = nullptr, const T* a_dia = nullptr, int a_ndiag = 0,
-             const DiaOffsets* a_off = nullptr)
+             const DiaOffsets* a_off = nullptr, bool a_sym = false)
         : A_(A), x_(x), b_(b), n_(n), prec_(prec), crit_(crit), buf_(buf), s_(s), ext_(ext),
           g_(grid_for(n)), G_(G), GT_(GT), g_bf16_(g_bf16)
     {
         if (a_dia != nullptr) {
             a_dia_ = a_dia;
             a_ndiag_ = a_ndiag;
+            a_sym_ = a_sym;
             a_off_ = *a_off;
         }
         if (dia != nullptr && std::is_same_v<T, float> && g_bf16) {
@@ -796,18 +905,29 @@ private:
     clsparseStatus spmv_dot_dia(const int* done)
     {
         const T* v = a_dia_;
-#define CLS_DIA_SPMV(K) launch_dia_spmv_dot<K, T>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_)
-        switch (a_ndiag_) {
-        case 1: CLS_DIA_SPMV(1); break;
-        case 2: CLS_DIA_SPMV(2); break;
-        case 3: CLS_DIA_SPMV(3); break;
-        case 4: CLS_DIA_SPMV(4); break;
-        case 5: CLS_DIA_SPMV(5); break;
-        case 6: CLS_DIA_SPMV(6); break;
-        case 7: CLS_DIA_SPMV(7); break;
-        case 8: CLS_DIA_SPMV(8); break;
-        case 9: CLS_DIA_SPMV(9); break;
-        default: return clsparseInvalidValue;
+#define CLS_DIA_SPMV(K, S) \
+    launch_dia_spmv_dot<K, S, T>(n_, g_, a_off_, v, buf_.p, buf_.y, buf_.part_yp, done, s_)
+        if (a_sym_) {
+            switch (a_ndiag_) {
+            case 3: CLS_DIA_SPMV(3, true); break;
+            case 5: CLS_DIA_SPMV(5, true); break;
+            case 7: CLS_DIA_SPMV(7, true); break;
+            case 9: CLS_DIA_SPMV(9, true); break;
+            default: return clsparseInvalidValue;
+            }
+        } else {
+            switch (a_ndiag_) {
+            case 1: CLS_DIA_SPMV(1, false); break;
+            case 2: CLS_DIA_SPMV(2, false); break;
+            case 3: CLS_DIA_SPMV(3, false); break;
+            case 4: CLS_DIA_SPMV(4, false); break;
+            case 5: CLS_DIA_SPMV(5, false); break;
+            case 6: CLS_DIA_SPMV(6, false); break;
+            case 7: CLS_DIA_SPMV(7, false); break;
+            case 8: CLS_DIA_SPMV(8, false); break;
+            case 9: CLS_DIA_SPMV(9, false); break;
+            default: return clsparseInvalidValue;
+            }
         }
 #undef CLS_DIA_SPMV
         CLS_HIP_TRY(hipGetLastError());
@@ -861,6 +981,7 @@ private:
     FsaiDia dia_{};
     const T* a_dia_ = nullptr;
     int a_ndiag_ = 0;
+    bool a_sym_ = false;
     DiaOffsets a_off_{};
     DiaOffsets dia_off_{};
     hipGraph_t graph_ = nullptr;
@@ -914,8 +1035,9 @@ clsparseStatus dia_candidates(const clsparseCsrMatrix* A, int32_t* scratch, hipS
 // CSR SpMV stays).
 template <typename TA, typename TO>
 clsparseStatus dia_fill(const clsparseCsrMatrix* A, size_t n, int* ndiag, const DiaOffsets& off, TO* vals,
-                        int* bad, hipStream_t s)
+                        int* bad, hipStream_t s, bool* sym)
 {
+    *sym = false;
     hipLaunchKernelGGL((fill_kernel<TO>), dim3(ew_grid(*ndiag * n)), dim3(256), 0, s, *ndiag * n, vals,
                        TO(0));
     hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(256), 0, s, size_t(1), bad, 0);
@@ -927,7 +1049,23 @@ clsparseStatus dia_fill(const clsparseCsrMatrix* A, size_t n, int* ndiag, const 
     int h = 0;
     CLS_HIP_TRY(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, s));
     CLS_HIP_TRY(hipStreamSynchronize(s));
-    if (h) *ndiag = 0;
+    if (h) {
+        *ndiag = 0;
+        return clsparseSuccess;
+    }
+    // Symmetric storage: offsets -o..o (odd count, 0 in the middle) and equal
+    // mirrored values; then only the lower diagonals are read.
+    const int K = *ndiag;
+    bool offs_sym = (K % 2 == 1) && K >= 3 && off.o[K / 2] == 0;
+    for (int d = 0; offs_sym && d < K; ++d) offs_sym = off.o[d] == -off.o[K - 1 - d];
+    const char* e = std::getenv("CLSPARSE_AMD_DIA_SYM");
+    if (!offs_sym || (e && e[0] == '0')) return clsparseSuccess;
+    hipLaunchKernelGGL((dia_sym_check_kernel<TO>), dim3((n + 255) / 256), dim3(256), 0, s,
+                       static_cast<uint32_t>(n), K, off, vals, bad);
+    CLS_HIP_TRY(hipGetLastError());
+    CLS_HIP_TRY(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    CLS_HIP_TRY(hipStreamSynchronize(s));
+    *sym = (h == 0);
     return clsparseSuccess;
 }
 
@@ -1009,12 +1147,13 @@ clsparseStatus cg_solve(cldenseVector* x, const clsparseCsrMatrix* A, const clde
     *hs = init;
     CLS_HIP_TRY(hipMemcpyAsync(buf.st, hs, sizeof(CgDevState), hipMemcpyHostToDevice, s));
 
+    bool a_sym = false;
     if (a_ndiag) {
-        CLS_TRY((dia_fill<T, T>(A, n, &a_ndiag, a_off, a_dia, a_bad, s)));
+        CLS_TRY((dia_fill<T, T>(A, n, &a_ndiag, a_off, a_dia, a_bad, s, &a_sym)));
         if (a_ndiag == 0) a_dia = nullptr;
     }
     CgEngine<T> eng(A, xp, bp, n, prec, kCritL1Ref, buf, s, control->extended_precision, nullptr, nullptr,
-                    false, nullptr, a_dia, a_ndiag, &a_off);
+                    false, nullptr, a_dia, a_ndiag, &a_off, a_sym);
     CLS_TRY(eng.setup(false));
     CLS_HIP_TRY(hipMemcpyAsync(hs, buf.st, sizeof(CgDevState), hipMemcpyDeviceToHost, s));
     CLS_HIP_TRY(hipStreamSynchronize(s));
@@ -1246,8 +1385,9 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
         AI.values = valsI;
         AI.off_values = 0;
     }
+    bool a_sym = false;
     if (a_ndiag) {
-        CLS_TRY((dia_fill<double, TI>(A, n, &a_ndiag, a_off, a_dia, a_bad, s)));
+        CLS_TRY((dia_fill<double, TI>(A, n, &a_ndiag, a_off, a_dia, a_bad, s, &a_sym)));
         if (a_ndiag == 0) a_dia = nullptr;
     }
     int prec = (sc->preconditioner == DIAGONAL) ? kPrecJacobi : kPrecNone;
@@ -1313,9 +1453,9 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     CLS_HIP_TRY(hipMemcpyAsync(buf.st, hs, sizeof(CgDevState), hipMemcpyHostToDevice, s));
 
     CgEngine<TI, double> eng_main(&AI, xp, rI, n, prec, kCritL2, buf, s, false, G, GT, g_bf16,
-                                  have_dia ? &dia : nullptr, a_dia, a_ndiag, &a_off);
+                                  have_dia ? &dia : nullptr, a_dia, a_ndiag, &a_off, a_sym);
     CgEngine<TI, double> eng_jac(&AI, xp, rI, n, kPrecJacobi, kCritL2, buf, s, false, nullptr,
-                                 nullptr, false, nullptr, a_dia, a_ndiag, &a_off);
+                                 nullptr, false, nullptr, a_dia, a_ndiag, &a_off, a_sym);
     CgEngine<TI, double>* eng = &eng_main;
     // A pilot race already run for this factor decides without a new one
     // (the factor object forgets it when G changes).

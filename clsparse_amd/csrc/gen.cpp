@@ -353,7 +353,8 @@ extern "C" clsparseStatus clsparseAmdGenDiffusion3D(cl_uint nx, cl_uint ny, cl_u
         const uint32_t iy = static_cast<uint32_t>((i / nx) % ny);
         const uint32_t iz = static_cast<uint32_t>(i / sxy);
         const double ki = kappa[i];
-        auto face = [&](uint64_t j, double a) { return a * 2.0 * ki * kappa[j] / (ki + kappa[j]); };
+        // Harmonic mean of the two cells, symmetric in (i, j) bit for bit.
+        auto face = [&](uint64_t j, double a) { return a * 2.0 * (ki * kappa[j]) / (ki + kappa[j]); };
         double diag = 0.0;
         double off[6];
         uint64_t col[6];

@@ -207,7 +207,8 @@ CLSPARSE_EXPORT clsparseStatus
 /* 3-D 7-point variable-coefficient diffusion on nx*ny*nz cells with Dirichlet
  * boundaries: cell coefficient kappa ~ LogNormal(0, sigma), face coefficient
  * the harmonic mean of the two cells (the boundary face uses the cell's own
- * kappa), scaled per axis by (ax, ay, az). SPD, diagonally dominant. */
+ * kappa), scaled per axis by (ax, ay, az). SPD, diagonally dominant, and
+ * exactly symmetric in floating point. */
 CLSPARSE_EXPORT clsparseStatus
     clsparseAmdGenDiffusion3D(cl_uint nx, cl_uint ny, cl_uint nz, cl_double sigma,
                               cl_double ax, cl_double ay, cl_double az, cl_ulong seed,

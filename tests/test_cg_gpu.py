@@ -348,3 +348,43 @@ def test_reference_cg_extended_precision_keeps_csr(ctl):
         assert abs(s.n_iters - r.n_iters) <= 1
     finally:
         ext.release()
+
+
+def test_symmetric_half_storage_is_bitwise_the_full_one(ctl, monkeypatch):
+    """A symmetric stencil reads only its lower diagonals (the upper values come
+    from the mirrored positions, checked equal): the solve is bit for bit the
+    one from full diagonal storage."""
+    rp, ci, v, m = cls.gen_diffusion3d(26, 24, 22, sigma=1.0, ax=1.0, ay=1.0, az=5.0, seed=37)
+    b = cls.gen_uniform(m, -1, 1, seed=41)
+    say("reference CG: symmetric half storage vs full")
+    st_h, x_h, s_h = _solve(ctl, rp, ci, v, m, b, np.zeros(m), np.float64, cls.DIAGONAL, 5000, 1e-10, 0.0)
+    monkeypatch.setenv("CLSPARSE_AMD_DIA_SYM", "0")
+    st_f, x_f, s_f = _solve(ctl, rp, ci, v, m, b, np.zeros(m), np.float64, cls.DIAGONAL, 5000, 1e-10, 0.0)
+    assert st_h == 0 and st_f == 0 and s_h.a_diagonals == 7 and s_f.a_diagonals == 7
+    assert s_h.n_iters == s_f.n_iters
+    assert np.array_equal(x_h, x_f)
+
+
+def test_nonsymmetric_stencil_uses_full_storage(ctl, monkeypatch):
+    """Upwind convection makes the 5-point matrix nonsymmetric: the mirrored
+    values differ, the full diagonal storage is used, and 20 CG iterations
+    give the CSR result to rounding."""
+    import scipy.sparse as sp
+    nx = ny = 40
+    lap = cls.gen_poisson2d(nx, ny)
+    n = nx * ny
+    a = sp.csr_matrix((lap[2], lap[1], lap[0]), shape=(n, n)).tolil()
+    for i in range(n):
+        a[i, i] += 0.3
+        if i % nx > 0:
+            a[i, i - 1] -= 0.3
+    a = sp.csr_matrix(a)
+    a.sort_indices()
+    rp, ci, v = a.indptr.astype(np.uint32), a.indices.astype(np.uint32), a.data
+    b = cls.gen_uniform(n, -1, 1, seed=43)
+    say("reference CG: nonsymmetric stencil, 20 iterations, diagonal vs CSR")
+    _, x_d, s_d = _solve(ctl, rp, ci, v, n, b, np.zeros(n), np.float64, cls.DIAGONAL, 20, 1e-30, 0.0)
+    monkeypatch.setenv("CLSPARSE_AMD_DIA", "0")
+    _, x_c, s_c = _solve(ctl, rp, ci, v, n, b, np.zeros(n), np.float64, cls.DIAGONAL, 20, 1e-30, 0.0)
+    assert s_d.a_diagonals == 5 and s_c.a_diagonals == 0
+    assert np.abs(x_d - x_c).max() <= 1e-10 * np.abs(x_c).max()
