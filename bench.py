@@ -25,6 +25,8 @@ Other modes (diagnostics, not the bench line):
                    FP64 Jacobi-PCG, with the committed trained network
   --mode fsai-train  trains the network on the seeded 2-D anisotropic family (512^2,
                    2000 matrices, the last 200 held out) and writes the weights
+  --mode next      the rows after the SpMV/CG path: extended-precision SpMV, BiCGStab,
+                   csrmm and the Matrix Market reader, each with its bytes and a CPU figure
 """
 import argparse
 import ctypes
@@ -143,6 +145,207 @@ class SpmvRun:
 
     def close(self):
         self.A.meta_delete()
+
+
+def median_call_ms(torch, stream, call, calls=20, warmup=3):
+    """Median of HIP-event times of single calls on `stream`."""
+    for _ in range(warmup):
+        call()
+    per = []
+    for _ in range(calls):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        call()
+        b.record(stream)
+        per.append((a, b))
+    torch.cuda.synchronize()
+    return sorted(a.elapsed_time(b) for a, b in per)[len(per) // 2]
+
+
+def copy_peak(cls, torch, ctl, stream, n=1 << 26):
+    """cldenseDscale on 512 MiB (read 8 B + write 8 B per element): the
+    streaming rate one library kernel reaches, beside the 8 TB/s figure."""
+    lib = cls.load()
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    r, y = cls.DenseVector(dst), cls.DenseVector(src)
+    one = cls.Scalar(1.0, torch.float64)
+
+    def call():
+        cls.check("cldenseDscale", lib.cldenseDscale(ctypes.byref(r.s), ctypes.byref(one.s),
+                                                     ctypes.byref(y.s), ctl.handle))
+    ms = median_call_ms(torch, stream, call)
+    return {"kernel": "cldenseDscale (y = 1 * x), FP64", "bytes": 16 * n, "us": round(ms * 1e3, 2),
+            "GBps": round(16 * n / (ms * 1e-3) / 1e9, 1)}
+
+
+def next_rows(cls, torch, ctl, stream, cpu=True):
+    """Extended-precision SpMV, BiCGStab, csrmm and the Matrix Market reader."""
+    import numpy as np
+    import scipy.sparse as sp
+
+    import oracle
+    lib = cls.load()
+    out = {}
+
+    # Extended precision: the power-law FP64 SpMV with compensated sums.
+    rp, ci, v, m = cls.gen_powerlaw(1 << 21, 16.0, 1.5, 200000, 8, 100000, seed=1)
+    x = cls.gen_uniform(m, -1.0, 1.0, seed=2)
+    res = {}
+    for name, ext in (("plain", False), ("extended", True)):
+        c = cls.Control(stream=stream, async_mode=True, extended_precision=ext)
+        run = SpmvRun(cls, torch, rp, ci, v, m, x, np.float64, c)
+        _, launch = run.time(50, 5, stream)
+        res[name] = {"launch_us": round(launch * 1e3, 2),
+                     "GBps": round(run.nbytes / (launch * 1e-3) / 1e9, 1)}
+        run.close()
+        c.release()
+    res["bytes_per_call"] = spmv_bytes(rp.size - 1, m, int(rp[-1]))
+    out["extended_precision_spmv"] = res
+    log(f"extended precision: {res}")
+    del rp, ci, v, x
+
+    # BiCGStab: 3-D convection-diffusion (upwind in x), nonsymmetric, Jacobi, 1e-8.
+    n1 = 96
+    rp, ci, v, m = cls.gen_diffusion3d(n1, n1, n1, sigma=1.0, ax=1.0, ay=1.0, az=1.0, seed=7)
+    A0 = sp.csr_matrix((v, ci, rp), shape=(m, m))
+    conv = 0.5
+    rows = np.arange(m)
+    left = rows[rows % n1 > 0]
+    U = sp.csr_matrix((np.concatenate([np.full(m, conv), np.full(left.size, -conv)]),
+                       (np.concatenate([rows, left]), np.concatenate([rows, left - 1]))), shape=(m, m))
+    Ab = (A0 + U).tocsr()
+    Ab.sort_indices()
+    rpb, cib, vb = Ab.indptr.astype(np.uint32), Ab.indices.astype(np.uint32), Ab.data
+    b = cls.gen_uniform(m, -1.0, 1.0, seed=8)
+    A = cls.CsrMatrix(rpb, cib, vb, m)
+    A.meta_create(ctl)
+    bt = torch.tensor(b, device="cuda")
+    tol = 1e-8
+    best = None
+    for _ in range(2):
+        xt = torch.zeros(m, dtype=torch.float64, device="cuda")
+        sc = cls.SolverControl(cls.DIAGONAL, 20000, tol, 0.0)
+        xd, bd = cls.DenseVector(xt), cls.DenseVector(bt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = lib.clsparseDcsrbicgStab(ctypes.byref(xd.s), ctypes.byref(A.s), ctypes.byref(bd.s),
+                                      sc.handle, ctl.handle)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        stats = sc.stats()
+        sc.release()
+        cls.check("clsparseDcsrbicgStab", st)
+        best = (dt, stats.n_iters + 1, stats.current_residual)
+    nnzb = int(rpb[-1])
+    it_bytes = 2 * spmv_bytes(m, m, nnzb) + 184 * m
+    r = b - oracle.csrmv(rpb, cib, vb, xt.cpu().numpy())[0]
+    res = {"workload": f"convection_diffusion3d_{n1}^3_upwind{conv}", "n": m, "nnz": nnzb,
+           "tolerance_l1_relative": tol, "seconds": round(best[0], 5), "iterations": best[1],
+           "l1_residual": best[2], "true_rel_residual_l2": float(np.linalg.norm(r) / np.linalg.norm(b)),
+           "bytes_per_iteration": it_bytes,
+           "GBps": round(it_bytes * best[1] / best[0] / 1e9, 1)}
+    if cpu:
+        k = 4
+        t0 = time.perf_counter()
+        oracle.bicgstab(rpb, cib, vb, b, np.zeros(m), 1, k, 1e-30, 0.0)
+        per = (time.perf_counter() - t0) / k
+        res["cpu_baseline"] = {"kind": "port", "cores": 1, "seconds_per_iteration": round(per, 5),
+                               "projected_seconds": round(per * best[1], 3),
+                               "sample": f"{k} iterations of the oracle's FP64 BiCGStab on the same "
+                                         "matrix, single thread, projected"}
+    out["bicgstab"] = res
+    log(f"bicgstab: {res}")
+    A.meta_delete()
+
+    # csrmm: 2-D Poisson 1024^2 (FP64) times a dense 32-column B.
+    rp, ci, v, m = cls.gen_poisson2d(1024, 1024)
+    k = 32
+    nnz = int(rp[-1])
+    A = cls.CsrMatrix(rp, ci, v, m)
+    B = torch.tensor(cls.gen_uniform(m * k, -1.0, 1.0, seed=9), device="cuda")
+    C = torch.zeros(m * k, dtype=torch.float64, device="cuda")
+
+    def dense(t, rows, cols):
+        d = cls.cldenseMatrix()
+        lib.cldenseInitMatrix(ctypes.byref(d))
+        d.num_rows, d.num_cols, d.lead_dim, d.major = rows, cols, cols, 1
+        d.values, d.off_values = t.data_ptr(), 0
+        return d
+    Bd, Cd = dense(B, m, k), dense(C, m, k)
+    one, zero = cls.Scalar(1.0, torch.float64), cls.Scalar(0.0, torch.float64)
+
+    def call():
+        cls.check("clsparseDcsrmm", lib.clsparseDcsrmm(ctypes.byref(one.s), ctypes.byref(A.s),
+                                                       ctypes.byref(Bd), ctypes.byref(zero.s),
+                                                       ctypes.byref(Cd), ctl.handle))
+    ms = median_call_ms(torch, stream, call)
+    nbytes = 12 * nnz + 4 * (m + 1) + 2 * 8 * m * k
+    res = {"workload": f"poisson2d_1024^2_fp64_times_{k}_columns", "n": m, "nnz": nnz, "ncols": k,
+           "us": round(ms * 1e3, 2), "bytes": nbytes, "GBps": round(nbytes / (ms * 1e-3) / 1e9, 1),
+           "GFLOPs": round(2 * nnz * k / (ms * 1e-3) / 1e9, 1),
+           "bytes_rule": "A once (values, columns, row pointer) + B once + C written once"}
+    rows = np.arange(0, m, 997)
+    Bh = B.cpu().numpy().reshape(m, k)
+    sub_rp = np.concatenate([[0], np.cumsum(np.diff(rp.astype(np.int64))[rows])]).astype(np.uint32)
+    sub_ci = np.concatenate([ci[rp[r]:rp[r + 1]] for r in rows])
+    sub_v = np.concatenate([v[rp[r]:rp[r + 1]] for r in rows])
+    ref, scale = oracle.csrmm(sub_rp, sub_ci, sub_v, Bh, np.zeros((rows.size, k)))
+    got = C.cpu().numpy().reshape(m, k)[rows]
+    res["max_rel_err_sample"] = float(np.max(np.abs(got - ref) / np.maximum(scale, 1e-300)))
+    if cpu:
+        t0 = time.perf_counter()
+        oracle.csrmm(rp, ci, v, Bh, np.zeros((m, k)))
+        res["cpu_baseline"] = {"kind": "port", "cores": 1, "us": round((time.perf_counter() - t0) * 1e6, 1),
+                               "sample": "one full call of the oracle's csrmm (long-double sums), "
+                                         "single thread"}
+    out["csrmm"] = res
+    log(f"csrmm: {res}")
+    del B, C
+
+    # Matrix Market: power-law 2^16 x 2^16 written with 17 significant digits.
+    import tempfile
+    rp, ci, v, m = cls.gen_powerlaw(1 << 16, 16.0, 1.5, 20000, 2, 5000, seed=3)
+    nnz = int(rp[-1])
+    rows = np.repeat(np.arange(1, m + 1), np.diff(rp.astype(np.int64)))
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "powerlaw.mtx")
+        with open(path, "w") as f:
+            f.write("%%MatrixMarket matrix coordinate real general\n")
+            f.write(f"{m} {m} {nnz}\n")
+            np.savetxt(f, np.column_stack([rows, ci.astype(np.int64) + 1, v]), fmt="%d %d %.17g")
+        size = os.path.getsize(path)
+        nnz_h, nr, nc = cls.mtx_header(path)
+        vals = torch.zeros(nnz_h, dtype=torch.float64, device="cuda")
+        colt = torch.zeros(nnz_h, dtype=torch.int32, device="cuda")
+        rpt = torch.zeros(nr + 1, dtype=torch.int32, device="cuda")
+        Am = cls.clsparseCsrMatrix()
+        lib.clsparseInitCsrMatrix(ctypes.byref(Am))
+        Am.values, Am.col_indices, Am.row_pointer = vals.data_ptr(), colt.data_ptr(), rpt.data_ptr()
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            cls.check("clsparseDCsrMatrixfromFile",
+                      lib.clsparseDCsrMatrixfromFile(ctypes.byref(Am), path.encode(), ctl.handle, 0))
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        t = sorted(times)[1]
+        ok = bool(np.array_equal(rpt.cpu().numpy().view(np.uint32), rp) and
+                  np.array_equal(colt.cpu().numpy().view(np.uint32), ci) and
+                  np.array_equal(vals.cpu().numpy(), v))
+        res = {"file_bytes": size, "nnz": nnz, "seconds": round(t, 4),
+               "MBps": round(size / t / 1e6, 1), "Mnnz_per_s": round(nnz / t / 1e6, 2),
+               "round_trip_exact": ok,
+               "what": "clsparseDCsrMatrixfromFile: parse, sort, CSR build and upload"}
+        if cpu:
+            import scipy.io
+            t0 = time.perf_counter()
+            scipy.io.mmread(path)
+            res["cpu_comparison"] = {"kind": "scipy.io.mmread (coordinate parse only)",
+                                     "seconds": round(time.perf_counter() - t0, 4)}
+    out["matrix_market"] = res
+    log(f"matrix market: {res}")
+    return out
 
 
 def poisson_workload(cls, torch, ctl, stream, cpu=True):
@@ -363,7 +566,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-solver", action="store_true", help="skip the solver workload")
     ap.add_argument("--mode", default="spmv",
-                    choices=["spmv", "copy", "locality", "cg", "fsai", "fsai-train"])
+                    choices=["spmv", "copy", "locality", "cg", "fsai", "fsai-train", "next"])
     ap.add_argument("--no-fsai", action="store_true", help="skip the learned-FSAI workload")
     ap.add_argument("--weights", default=None, help="learned-FSAI weights (.npy)")
     ap.add_argument("--train-matrices", type=int, default=1800)
@@ -407,6 +610,11 @@ def main():
 
     ctl = cls.Control(stream=stream, async_mode=True)
 
+    if args.mode == "next":
+        print(json.dumps({"mode": "next", "copy_peak": copy_peak(cls, torch, ctl, stream),
+                          "next": next_rows(cls, torch, ctl, stream, cpu=not args.no_cpu)}))
+        ctl.release()
+        return
     if args.mode == "cg":
         print(json.dumps({"mode": "cg", "solver": solver_workload(cls, torch, ctl)}))
         ctl.release()
@@ -507,7 +715,7 @@ def main():
                "sample": f"{calls} calls of the same FP64 power-law SpMV (full matrix), median; "
                          f"oracle OpenMP CSR SpMV, {threads} threads"}
 
-    extra = {"gather_floor": gather_floor,
+    extra = {"gather_floor": gather_floor, "copy_peak": copy_peak(cls, torch, ctl, stream),
              "poisson_fp32": poisson_workload(cls, torch, ctl, stream, cpu=not args.no_cpu)}
     if not args.no_solver:
         extra["solver"] = solver_workload(cls, torch, ctl, cpu=not args.no_cpu)
