@@ -6,18 +6,23 @@
 // element (k, c) of B at off_values + k * lead_dim + c; alpha == 0 gives
 // beta * C without reading B, beta == 0 never reads C, beta == 1 adds C.
 // The reference runs one CSR-Vector SpMV per column of C (A re-read for every
-// column). Here a wave owns a row of C: it walks the row's nonzeros once and
-// its lanes cover up to 64 columns of B at a time, so every B row segment is
-// one coalesced read; with fewer columns the spare lanes take every
-// (64/width)-th nonzero and the partial sums meet through lane permutes.
-// Per element of C the sum is fixed-order, so results are deterministic.
+// column). Here W lanes (the column tile, a power of two >= the columns of C up
+// to 64) form a row slot, and a wave runs 64 / W rows of C at once: each slot
+// loads W nonzeros of its row with one coalesced read, hands each nonzero's
+// value and column to its lanes through lane permutes, and keeps 8 B row
+// segments in flight per step, every one a coalesced read of W columns. Waves
+// walk consecutive rows and workgroups are remapped so each XCD covers one
+// contiguous range of rows (neighbouring rows share B rows in its L2). Each
+// element of C is summed over the row's nonzeros in storage order, so results
+// are deterministic.
 #include <algorithm>
 
 #include "common.hpp"
 
 namespace {
 
-// W = lanes per column group (power of two, 1..64); 64 / W nonzero slices.
+constexpr int kBInFlight = 8;
+
 template <typename T, int W>
 __global__ __launch_bounds__(256) void csrmm_kernel(uint32_t num_rows, uint32_t ncols,
                                                     const uint32_t* __restrict__ rp,
@@ -27,27 +32,51 @@ __global__ __launch_bounds__(256) void csrmm_kernel(uint32_t num_rows, uint32_t 
                                                     const T* __restrict__ pAlpha,
                                                     const T* __restrict__ pBeta)
 {
-    constexpr int kSlices = 64 / W;
+    constexpr uint32_t kSlots = 64 / W;
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t c_in = lane % W;  // column inside the tile
-    const uint32_t slice = lane / W;
+    const uint32_t slot = lane / W;
+    const uint32_t t = lane % W;
+    const int base = static_cast<int>(slot * W);  // first lane of this slot
     const T alpha = *pAlpha;
     const T beta = *pBeta;
+    const uint32_t wave = xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     const uint32_t waves = gridDim.x * 4;
-    for (uint32_t row = blockIdx.x * 4 + (threadIdx.x >> 6); row < num_rows; row += waves) {
-        const uint32_t beg = rp[row], end = rp[row + 1];
+    for (uint32_t row0 = wave * kSlots; row0 < num_rows; row0 += waves * kSlots) {
+        const uint32_t row = row0 + slot;
+        const bool live = row < num_rows;
+        const uint32_t beg = live ? rp[row] : 0u;
+        const uint32_t end = live ? rp[row + 1] : 0u;
         for (uint32_t c0 = 0; c0 < ncols; c0 += W) {
-            const uint32_t c = c0 + c_in;
+            const uint32_t c = c0 + t;
+            const bool colok = c < ncols;
             T sum = T(0);
-            if (alpha != T(0) && c < ncols) {
-                for (uint32_t j = beg + slice; j < end; j += kSlices)
-                    sum = fma(alpha * v[j], B[static_cast<size_t>(ci[j]) * ldb + c], sum);
+            if (alpha != T(0)) {
+                for (uint32_t j0 = beg; j0 < end; j0 += W) {
+                    // Lane t of the slot holds nonzero j0 + t of the row.
+                    const uint32_t jj = j0 + t;
+                    T av = T(0);
+                    uint32_t cc = 0;
+                    if (jj < end) {
+                        av = alpha * v[jj];
+                        cc = ci[jj];
+                    }
+                    const uint32_t cnt = min(static_cast<uint32_t>(W), end - j0);
+                    for (uint32_t q = 0; q < cnt; q += kBInFlight) {
+                        T a8[kBInFlight], b8[kBInFlight];
+#pragma unroll
+                        for (int u = 0; u < kBInFlight; ++u) {
+                            const int src = base + static_cast<int>(min(q + u, static_cast<uint32_t>(W - 1)));
+                            a8[u] = __shfl(av, src, 64);
+                            const uint32_t col = __shfl(cc, src, 64);
+                            b8[u] = (colok && q + u < cnt) ? B[static_cast<size_t>(col) * ldb + c] : T(0);
+                        }
+#pragma unroll
+                        for (int u = 0; u < kBInFlight; ++u)
+                            if (q + u < cnt) sum = fma(a8[u], b8[u], sum);
+                    }
+                }
             }
-            // lanes with the same column and different slices: lane, lane+W, ...
-            if constexpr (kSlices > 1) {
-                for (int off = W; off < 64; off <<= 1) sum += __shfl_xor(sum, off, 64);
-            }
-            if (slice == 0 && c < ncols) {
+            if (live && colok) {
                 T* out = C + static_cast<size_t>(row) * ldc + c;
                 if (beta == T(0))
                     *out = sum;
@@ -84,7 +113,10 @@ clsparseStatus csrmm_entry(const clsparseScalar* alpha, const clsparseCsrMatrix*
     T* Cp = dev_ptr<T>(C->values, C->off_values);
     if (rp == nullptr || Bp == nullptr || Cp == nullptr || pa == nullptr || pb == nullptr)
         return clsparseInvalidMemObject;
-    const uint32_t grid = std::min<uint32_t>((n + 3) / 4, 65536);
+    // Enough waves for several per SIMD; each walks rows in steps of the grid.
+    const uint32_t slots = ncols > 32 ? 1u : ncols > 16 ? 2u : ncols > 8 ? 4u : ncols > 4 ? 8u
+                                       : ncols > 2 ? 16u : ncols > 1 ? 32u : 64u;
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((n + 4 * slots - 1) / (4 * slots), 16384));
     hipStream_t s = control->stream;
 #define CLS_CSRMM(W)                                                                                 \
     hipLaunchKernelGGL((csrmm_kernel<T, W>), dim3(grid), dim3(256), 0, s, n, ncols, rp, ci, v, Bp,   \
