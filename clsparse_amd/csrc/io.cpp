@@ -16,15 +16,26 @@
 // Differences (supersets): duplicates keep their file order (stable sort);
 // an index outside the declared size, or fewer entries than the header
 // announces, is an error (clsparseInvalidFile) instead of undefined
-// behaviour; the file is parsed from one buffered read, not per-token fscanf.
+// behaviour; the file is parsed from one read, not per-token fscanf: in
+// parallel over newline-aligned chunks (one thread per chunk) when every entry
+// sits inside its chunk, else sequentially; the (row, column) order comes from
+// a stable counting sort by row and a stable sort of each row by column.
 #include <algorithm>
 #include <cctype>
 #include <cerrno>
+#include <charconv>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <string_view>
+#include <thread>
 #include <vector>
+
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include "common.hpp"
 
@@ -50,17 +61,32 @@ bool has_mtx_extension(const char* path)
     return dot != std::string::npos && s.substr(dot + 1) == "mtx";
 }
 
-bool read_file(const char* path, std::string& out)
+// The file mapped read-only (no copy of the text).
+struct MappedFile
 {
-    FILE* f = std::fopen(path, "rb");
-    if (!f) return false;
-    char buf[1 << 16];
-    size_t got;
-    while ((got = std::fread(buf, 1, sizeof(buf), f)) > 0) out.append(buf, got);
-    const bool ok = !std::ferror(f);
-    std::fclose(f);
-    return ok;
-}
+    const char* data = nullptr;
+    size_t size = 0;
+    explicit MappedFile(const char* path)
+    {
+        const int fd = ::open(path, O_RDONLY);
+        if (fd < 0) return;
+        struct stat st;
+        if (::fstat(fd, &st) == 0 && st.st_size > 0) {
+            void* m = ::mmap(nullptr, static_cast<size_t>(st.st_size), PROT_READ, MAP_PRIVATE, fd, 0);
+            if (m != MAP_FAILED) {
+                data = static_cast<const char*>(m);
+                size = static_cast<size_t>(st.st_size);
+            }
+        }
+        ::close(fd);
+    }
+    ~MappedFile()
+    {
+        if (data) ::munmap(const_cast<char*>(data), size);
+    }
+    MappedFile(const MappedFile&) = delete;
+    MappedFile& operator=(const MappedFile&) = delete;
+};
 
 std::string lower(std::string s)
 {
@@ -70,10 +96,10 @@ std::string lower(std::string s)
 
 // Banner "%%MatrixMarket matrix coordinate <data> <storage>", comment lines,
 // then the size line "rows cols nnz" (blank lines before it are skipped).
-bool parse_header(const std::string& text, MmHeader& h)
+bool parse_header(std::string_view text, MmHeader& h)
 {
     size_t pos = text.find('\n');
-    const std::string first = text.substr(0, pos);
+    const std::string first(text.substr(0, pos));
     char banner[64] = {0}, mtx[64] = {0}, crd[64] = {0}, data[64] = {0}, storage[64] = {0};
     if (std::sscanf(first.c_str(), "%63s %63s %63s %63s %63s", banner, mtx, crd, data, storage) != 5)
         return false;
@@ -96,7 +122,7 @@ bool parse_header(const std::string& text, MmHeader& h)
     while (pos != std::string::npos) {
         const size_t start = pos + 1;
         pos = text.find('\n', start);
-        const std::string line = text.substr(start, pos == std::string::npos ? std::string::npos : pos - start);
+        const std::string line(text.substr(start, pos == std::string::npos ? std::string::npos : pos - start));
         if (!line.empty() && line[0] == '%') continue;
         unsigned long long r = 0, c = 0, z = 0;
         if (std::sscanf(line.c_str(), "%llu %llu %llu", &r, &c, &z) == 3) {
@@ -118,57 +144,209 @@ struct Entry
     T v;
 };
 
-template <typename T>
-T parse_value(const char* p, char** end);
-template <>
-float parse_value<float>(const char* p, char** end)
+bool is_space(char ch)
 {
-    return std::strtof(p, end);
+    return ch == ' ' || ch == '\t' || ch == '\n' || ch == '\v' || ch == '\f' || ch == '\r';
 }
-template <>
-double parse_value<double>(const char* p, char** end)
+
+// One unsigned integer token as strtoull reads it (leading white space, an
+// optional '+'); a '-' sign makes the token invalid here (strtoull would
+// wrap it past any matrix size, which the range check rejects).
+bool parse_index(const char*& p, const char* end, uint64_t& out)
 {
-    return std::strtod(p, end);
+    while (p < end && is_space(*p)) ++p;
+    if (p < end && *p == '+') ++p;
+    const auto r = std::from_chars(p, end, out);
+    if (r.ec != std::errc() || r.ptr == p) return false;
+    p = r.ptr;
+    return true;
+}
+
+// One value token: from_chars on the common decimal forms, strtod / strtof
+// (which the reference's fscanf follows) for anything else.
+template <typename T>
+bool parse_number(const char*& p, const char* end, T& out)
+{
+    while (p < end && is_space(*p)) ++p;
+    const char* q = p;
+    if (q < end && *q == '+' && q + 1 < end && q[1] != '-' && q[1] != '+') ++q;
+    const auto r = std::from_chars(q, end, out);
+    if (r.ec == std::errc() && r.ptr != q && (r.ptr == end || is_space(*r.ptr))) {
+        p = r.ptr;
+        return true;
+    }
+    // Slow path on a NUL-terminated copy of the token.
+    const char* t = p;
+    while (t < end && !is_space(*t)) ++t;
+    const std::string tok(p, t);
+    char* e = nullptr;
+    if constexpr (sizeof(T) == 4)
+        out = std::strtof(tok.c_str(), &e);
+    else
+        out = std::strtod(tok.c_str(), &e);
+    if (e == tok.c_str()) return false;
+    p += (e - tok.c_str());
+    return true;
+}
+
+template <typename T>
+struct Raw
+{
+    uint64_t r, c;
+    T v;
+};
+
+// Entries of [p, end) until `want` are read or the text ends; *crossed is set
+// when an entry runs past `end` (it continues beyond this chunk). Returns
+// false on a malformed token.
+template <typename T>
+bool parse_range(const char* p, const char* end, const char* hard_end, bool pattern, uint64_t want,
+                 std::vector<Raw<T>>& out, bool* crossed)
+{
+    *crossed = false;
+    while (out.size() < want) {
+        while (p < end && is_space(*p)) ++p;
+        if (p >= end) return true;
+        Raw<T> e{0, 0, T(0)};
+        if (!parse_index(p, hard_end, e.r) || !parse_index(p, hard_end, e.c)) return false;
+        if (!pattern && !parse_number<T>(p, hard_end, e.v)) return false;
+        if (p > end) {
+            *crossed = true;
+            return true;
+        }
+        out.push_back(e);
+    }
+    return true;
+}
+
+unsigned reader_threads()
+{
+    const unsigned t = std::thread::hardware_concurrency();
+    return t == 0 ? 1u : std::min(t, 16u);
+}
+
+// Raw entries [0, take) -> 0-based entries: pattern values are drawn by the
+// caller beforehand; range check, zero filtering, symmetric expansion.
+template <typename T>
+bool filter_entries(const Raw<T>* raw, size_t take, const MmHeader& h, bool explicit_zeroes,
+                    std::vector<Entry<T>>& e)
+{
+    e.reserve(h.symmetric ? 2 * take : take);
+    for (size_t k = 0; k < take; ++k) {
+        const Raw<T>& x = raw[k];
+        if (x.r < 1 || x.c < 1 || x.r > h.rows || x.c > h.cols) return false;
+        if (!explicit_zeroes && x.v == T(0)) continue;
+        const uint32_t r = static_cast<uint32_t>(x.r - 1), c = static_cast<uint32_t>(x.c - 1);
+        e.push_back({r, c, x.v});
+        if (h.symmetric && r != c) e.push_back({c, r, x.v});
+    }
+    return true;
 }
 
 // Entries (0-based, symmetric expansion, zero filtering), sorted by (row, col).
 template <typename T>
 clsparseStatus read_entries(const char* path, bool explicit_zeroes, MmHeader& h, std::vector<Entry<T>>& out)
 {
-    std::string text;
-    if (!read_file(path, text)) return clsparseInvalidFile;
-    if (!parse_header(text, h)) return clsparseInvalidFile;
+    const MappedFile file(path);
+    if (!file.data) return clsparseInvalidFile;
+    if (!parse_header(std::string_view(file.data, file.size), h)) return clsparseInvalidFile;
     if (h.rows > 0xFFFFFFFFull || h.cols > 0xFFFFFFFFull || h.nnz > 0xFFFFFFFFull)
         return clsparseInvalidFile;
-    out.clear();
-    out.reserve(h.symmetric ? 2 * h.nnz : h.nnz);
-    const char* p = text.c_str() + h.body;
-    for (uint64_t i = 0; i < h.nnz; ++i) {
-        char* e = nullptr;
-        errno = 0;
-        const unsigned long long ir = std::strtoull(p, &e, 10);
-        if (e == p || errno) return clsparseInvalidFile;
-        p = e;
-        const unsigned long long ic = std::strtoull(p, &e, 10);
-        if (e == p || errno) return clsparseInvalidFile;
-        p = e;
-        T v;
-        if (h.data == MmData::Pattern) {
-            v = static_cast<T>(kMaxRandVal * (std::rand() / (RAND_MAX + 1.0)));
-        } else {
-            v = parse_value<T>(p, &e);
-            if (e == p) return clsparseInvalidFile;
-            p = e;
+    const bool pattern = h.data == MmData::Pattern;
+    const char* body = file.data + h.body;
+    const char* text_end = file.data + file.size;
+
+    // Raw entries per chunk; take[k] of chunk k are among the first nnz.
+    const size_t len = static_cast<size_t>(text_end - body);
+    unsigned nt = len >= (1u << 20) ? reader_threads() : 1u;
+    std::vector<std::vector<Raw<T>>> part;
+    std::vector<size_t> take;
+    bool done = false;
+    if (nt > 1) {
+        std::vector<const char*> cut(nt + 1, text_end);
+        cut[0] = body;
+        for (unsigned k = 1; k < nt; ++k) {
+            const char* q = std::max(body + len * k / nt, cut[k - 1]);
+            const void* nl = std::memchr(q, '\n', static_cast<size_t>(text_end - q));
+            cut[k] = nl ? static_cast<const char*>(nl) + 1 : text_end;
         }
-        if (ir < 1 || ic < 1 || ir > h.rows || ic > h.cols) return clsparseInvalidFile;
-        if (!explicit_zeroes && v == T(0)) continue;
-        const uint32_t r = static_cast<uint32_t>(ir - 1), c = static_cast<uint32_t>(ic - 1);
-        out.push_back({r, c, v});
-        if (h.symmetric && r != c) out.push_back({c, r, v});
+        part.assign(nt, {});
+        std::vector<char> ok(nt, 1), crossed(nt, 0);
+        std::vector<std::thread> th;
+        for (unsigned k = 0; k < nt; ++k)
+            th.emplace_back([&, k]() {
+                bool cr = false;
+                part[k].reserve(static_cast<size_t>(h.nnz / nt + 16));
+                ok[k] = parse_range<T>(cut[k], cut[k + 1], text_end, pattern, h.nnz, part[k], &cr);
+                crossed[k] = cr;
+            });
+        for (auto& t : th) t.join();
+        // Usable when every chunk up to the nnz-th entry parsed cleanly and no
+        // entry ran across a chunk boundary; otherwise parse sequentially.
+        bool regular = true;
+        uint64_t have = 0;
+        take.assign(nt, 0);
+        for (unsigned k = 0; k < nt && have < h.nnz; ++k) {
+            if (!ok[k] || crossed[k]) {
+                regular = false;
+                break;
+            }
+            take[k] = static_cast<size_t>(std::min<uint64_t>(part[k].size(), h.nnz - have));
+            have += take[k];
+        }
+        done = regular && have >= h.nnz;
     }
-    std::stable_sort(out.begin(), out.end(), [](const Entry<T>& a, const Entry<T>& b) {
-        return a.r != b.r ? a.r < b.r : a.c < b.c;
-    });
+    if (!done) {
+        nt = 1;
+        part.assign(1, {});
+        part[0].reserve(static_cast<size_t>(h.nnz));
+        bool cr = false;
+        if (!parse_range<T>(body, text_end, text_end, pattern, h.nnz, part[0], &cr)) return clsparseInvalidFile;
+        if (part[0].size() < h.nnz) return clsparseInvalidFile;
+        take.assign(1, part[0].size());
+    }
+
+    // Pattern values are drawn in entry order; then each chunk is filtered on
+    // its own thread.
+    if (pattern)
+        for (unsigned k = 0; k < nt; ++k)
+            for (size_t i = 0; i < take[k]; ++i)
+                part[k][i].v = static_cast<T>(kMaxRandVal * (std::rand() / (RAND_MAX + 1.0)));
+    std::vector<std::vector<Entry<T>>> ent(nt);
+    {
+        std::vector<char> ok(nt, 1);
+        std::vector<std::thread> th;
+        for (unsigned k = 1; k < nt; ++k)
+            th.emplace_back([&, k]() { ok[k] = filter_entries<T>(part[k].data(), take[k], h, explicit_zeroes, ent[k]); });
+        ok[0] = filter_entries<T>(part[0].data(), take[0], h, explicit_zeroes, ent[0]);
+        for (auto& t : th) t.join();
+        for (unsigned k = 0; k < nt; ++k)
+            if (!ok[k]) return clsparseInvalidFile;
+    }
+    std::vector<std::vector<Raw<T>>>().swap(part);
+
+    // Stable (row, column) order: counting sort by row over the chunks in
+    // file order, then each row by column.
+    std::vector<uint64_t> start(static_cast<size_t>(h.rows) + 1, 0);
+    size_t total = 0;
+    for (const auto& e : ent) {
+        total += e.size();
+        for (const auto& x : e) ++start[x.r + 1];
+    }
+    for (uint64_t r = 0; r < h.rows; ++r) start[r + 1] += start[r];
+    out.resize(total);
+    {
+        std::vector<uint64_t> fill(start.begin(), start.end() - 1);
+        for (const auto& e : ent)
+            for (const auto& x : e) out[fill[x.r]++] = x;
+    }
+    std::vector<std::vector<Entry<T>>>().swap(ent);
+    for (uint64_t r = 0; r < h.rows; ++r) {
+        auto b = out.begin() + static_cast<std::ptrdiff_t>(start[r]);
+        auto en = out.begin() + static_cast<std::ptrdiff_t>(start[r + 1]);
+        if (!std::is_sorted(b, en, [](const Entry<T>& x, const Entry<T>& y) { return x.c < y.c; }))
+            std::stable_sort(b, en, [](const Entry<T>& x, const Entry<T>& y) { return x.c < y.c; });
+    }
     return clsparseSuccess;
 }
 

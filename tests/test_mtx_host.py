@@ -127,3 +127,95 @@ def test_generated_matrix_roundtrip(tmp_path):
     np.testing.assert_array_equal(rp2, rp)
     np.testing.assert_array_equal(ci2, ci)
     np.testing.assert_array_equal(v2, v)
+
+
+def _big_text(rng, n=40000, per_row=6, header_nnz=None, symmetric=False, shuffle=True):
+    """A file over 1 MB (the reader splits it over threads): random entries,
+    optionally shuffled so the (row, column) sort has work to do."""
+    rows = np.repeat(np.arange(1, n + 1), per_row)
+    cols = rng.integers(1, n + 1, rows.size)
+    if symmetric:
+        keep = cols <= rows
+        rows, cols = rows[keep], cols[keep]
+    vals = rng.uniform(-1, 1, rows.size)
+    vals[rng.random(rows.size) < 0.01] = 0.0  # explicit zeroes to drop
+    order = rng.permutation(rows.size) if shuffle else np.arange(rows.size)
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    kind = "symmetric" if symmetric else "general"
+    body = "\n".join(f"{r} {c} {x:.17g}" for r, c, x in zip(rows, cols, vals))
+    nnz = rows.size if header_nnz is None else header_nnz
+    return f"%%MatrixMarket matrix coordinate real {kind}\n{n} {n} {nnz}\n{body}\n", rows.size
+
+
+def _expected_csr(text):
+    """The reader's rules restated with numpy: zeroes dropped, a symmetric
+    file's off-diagonal entry followed by its mirror, then a stable sort by
+    (row, column) -- duplicates stay, in file order."""
+    lines = text.splitlines()
+    kind = lines[0].split()[-1]
+    n, _, nnz = (int(t) for t in lines[1].split())
+    arr = np.array([ln.split() for ln in lines[2:2 + nnz]], dtype=np.float64)
+    r, c, v = arr[:, 0].astype(np.int64) - 1, arr[:, 1].astype(np.int64) - 1, arr[:, 2]
+    keep = v != 0
+    r, c, v = r[keep], c[keep], v[keep]
+    if kind == "symmetric":
+        off = r != c
+        idx = np.repeat(np.arange(r.size), 1 + off)
+        first = np.ones(idx.size, bool)
+        first[1:] = idx[1:] != idx[:-1]
+        rr = np.where(first, r[idx], c[idx])
+        cc = np.where(first, c[idx], r[idx])
+        r, c, v = rr, cc, v[idx]
+    order = np.lexsort((c, r))
+    rp = np.concatenate([[0], np.cumsum(np.bincount(r, minlength=n))])
+    return rp, c[order], v[order]
+
+
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_large_file_parallel_parse_matches_scipy(tmp_path, symmetric):
+    """Over 1 MB the body is parsed in newline-aligned chunks on several
+    threads; duplicates, zeroes and symmetric expansion as in the small case
+    (scipy sums duplicates, so the expected CSR is formed with numpy)."""
+    text, _ = _big_text(np.random.default_rng(7 + symmetric), symmetric=symmetric)
+    assert len(text) > (1 << 20)
+    path = _write(tmp_path, "big.mtx", text)
+    rp, ci, v, ncols = cls.read_mtx_host(path)
+    erp, eci, ev = _expected_csr(text)
+    np.testing.assert_array_equal(rp, erp)
+    np.testing.assert_array_equal(ci, eci)
+    np.testing.assert_array_equal(v, ev)
+
+
+def test_large_file_entries_across_lines_and_trailing_text(tmp_path):
+    """Entries whose tokens span lines (accepted by the reference's fscanf)
+    make the chunked parse fall back to the sequential one; text after the
+    announced entries is ignored; a bad token among them is an error."""
+    rng = np.random.default_rng(3)
+    text, count = _big_text(rng, shuffle=False)
+    head, body = text.split("\n", 2)[:2], text.split("\n", 2)[2]
+    lines = body.splitlines()
+    mid = len(lines) // 2
+    r, c, x = lines[mid].split()
+    lines[mid] = f"{r}\n{c}\n{x}"  # one entry over three lines, in the middle of the file
+    spread = "\n".join(head) + "\n" + "\n".join(lines) + "\n"
+    ref_path = _write(tmp_path, "ref.mtx", text)
+    p1 = _write(tmp_path, "spread.mtx", spread)
+    ref = cls.read_mtx_host(ref_path)
+    got = cls.read_mtx_host(p1)
+    for a, b in zip(got[:3], ref[:3]):
+        np.testing.assert_array_equal(a, b)
+    trailing = text + "this is not an entry\n" * 50000
+    got = cls.read_mtx_host(_write(tmp_path, "trail.mtx", trailing))
+    for a, b in zip(got[:3], ref[:3]):
+        np.testing.assert_array_equal(a, b)
+    lines = body.splitlines()
+    lines[len(lines) // 3] = "12 x 1.0"
+    bad = "\n".join(head) + "\n" + "\n".join(lines) + "\n"
+    lib = cls.load()
+    import ctypes
+    m = cls.clsparseAmdHostCsr()
+    assert lib.clsparseAmdCsrMatrixfromFileHost(_write(tmp_path, "bad.mtx", bad).encode(), 0,
+                                                ctypes.byref(m)) == cls.clsparseInvalidFile
+    short, _ = _big_text(rng, header_nnz=count + 5)
+    assert lib.clsparseAmdCsrMatrixfromFileHost(_write(tmp_path, "short.mtx", short).encode(), 0,
+                                                ctypes.byref(m)) == cls.clsparseInvalidFile
