@@ -236,14 +236,20 @@ def next_rows(cls, torch, ctl, stream, cpu=True):
         stats = sc.stats()
         sc.release()
         cls.check("clsparseDcsrbicgStab", st)
-        best = (dt, stats.n_iters + 1, stats.current_residual)
+        best = (dt, stats.n_iters + 1, stats.current_residual, stats.a_diagonals)
     nnzb = int(rpb[-1])
-    it_bytes = 2 * spmv_bytes(m, m, nnzb) + 184 * m
+    # Bytes of one iteration as executed: two SpMVs (diagonal storage: K values
+    # + x + y per row; CSR: the reference formula), the vector kernels
+    # (s update 40, <AMs,s>/<AMs,AMs> 16, x/r update 64, p update 48 B per row)
+    # and <r*, A Mp> (fused into the first diagonal SpMV: 8, else its own 16).
+    K = best[3]
+    spmv = (K * 8 + 16) * m if K else spmv_bytes(m, m, nnzb)
+    it_bytes = 2 * spmv + (8 if K else 16) * m + 168 * m
     r = b - oracle.csrmv(rpb, cib, vb, xt.cpu().numpy())[0]
     res = {"workload": f"convection_diffusion3d_{n1}^3_upwind{conv}", "n": m, "nnz": nnzb,
            "tolerance_l1_relative": tol, "seconds": round(best[0], 5), "iterations": best[1],
            "l1_residual": best[2], "true_rel_residual_l2": float(np.linalg.norm(r) / np.linalg.norm(b)),
-           "bytes_per_iteration": it_bytes,
+           "a_diagonals": K, "bytes_per_iteration": it_bytes,
            "GBps": round(it_bytes * best[1] / best[0] / 1e9, 1)}
     if cpu:
         k = 4

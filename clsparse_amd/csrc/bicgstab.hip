@@ -28,10 +28,14 @@
 
 #include "common.hpp"
 #include "solver_common.hpp"
+#include "dia.hpp"
 
 namespace {
 
 constexpr int kBiChunk = 32;  // iterations per captured graph (even: both parities)
+// Rows per thread per step of the grid-stride loops (their loads issued
+// together; rows and partial sums still in grid-stride order).
+constexpr int kBiRows = 4;
 
 struct BiDevState
 {
@@ -113,9 +117,26 @@ __global__ __launch_bounds__(256) void bi_dot(size_t n, const T* __restrict__ a,
     if (st->done) return;
     double s1 = 0.0, s2 = 0.0;
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
-    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
-        s1 += static_cast<double>(a[i]) * static_cast<double>(b[i]);
-        if (c) s2 += static_cast<double>(c[i]) * static_cast<double>(d[i]);
+    for (size_t i0 = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i0 < n; i0 += kBiRows * stride) {
+        T av[kBiRows], bv[kBiRows], cv[kBiRows], dv[kBiRows];
+#pragma unroll
+        for (int u = 0; u < kBiRows; ++u) {
+            const size_t i = i0 + u * stride;
+            if (i < n) {
+                av[u] = a[i];
+                bv[u] = b[i];
+                if (c) {
+                    cv[u] = c[i];
+                    dv[u] = d[i];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kBiRows; ++u) {
+            if (i0 + u * stride >= n) continue;
+            s1 += static_cast<double>(av[u]) * static_cast<double>(bv[u]);
+            if (c) s2 += static_cast<double>(cv[u]) * static_cast<double>(dv[u]);
+        }
     }
     s1 = block_sum256(s1, sh);
     if (c) s2 = block_sum256(s2, sh);
@@ -139,12 +160,28 @@ __global__ __launch_bounds__(256) void bi_s_update(size_t n, const T* __restrict
     if (st->done) return;
     const T alpha = static_cast<T>(sum_partials(part_rr_old, g, sh) / sum_partials(part_ra, g, sh));
     double ns = 0.0;
+    const bool pre = st->precond != 0;
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
-    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
-        const T si = r[i] - alpha * AMp[i];
-        s[i] = si;
-        Ms[i] = precond_apply(st, invD, si, i);
-        ns += fabs(static_cast<double>(si));
+    for (size_t i0 = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i0 < n; i0 += kBiRows * stride) {
+        T rv[kBiRows], av[kBiRows], dv[kBiRows];
+#pragma unroll
+        for (int u = 0; u < kBiRows; ++u) {
+            const size_t i = i0 + u * stride;
+            if (i < n) {
+                rv[u] = r[i];
+                av[u] = AMp[i];
+                if (pre) dv[u] = invD[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kBiRows; ++u) {
+            const size_t i = i0 + u * stride;
+            if (i >= n) continue;
+            const T si = rv[u] - alpha * av[u];
+            s[i] = si;
+            Ms[i] = pre ? si * dv[u] : si;
+            ns += fabs(static_cast<double>(si));
+        }
     }
     ns = block_sum256(ns, sh);
     if (threadIdx.x == 0) part_s[blockIdx.x] = ns;
@@ -189,14 +226,32 @@ __global__ __launch_bounds__(256) void bi_xr_update(
     const T omega = static_cast<T>(sum_partials(part_us, g, sh) / sum_partials(part_ww, g, sh));
     double nr = 0.0, rr = 0.0;
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
-    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
-        T xi = x[i] + alpha * Mp[i];
-        xi = xi + omega * Ms[i];
-        x[i] = xi;
-        const T ri = s[i] - omega * AMs[i];
-        r[i] = ri;
-        nr += fabs(static_cast<double>(ri));
-        rr += static_cast<double>(rs[i]) * static_cast<double>(ri);
+    for (size_t i0 = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i0 < n; i0 += kBiRows * stride) {
+        T xv[kBiRows], mp[kBiRows], ms[kBiRows], sv[kBiRows], ams[kBiRows], rsv[kBiRows];
+#pragma unroll
+        for (int u = 0; u < kBiRows; ++u) {
+            const size_t i = i0 + u * stride;
+            if (i < n) {
+                xv[u] = x[i];
+                mp[u] = Mp[i];
+                ms[u] = Ms[i];
+                sv[u] = s[i];
+                ams[u] = AMs[i];
+                rsv[u] = rs[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kBiRows; ++u) {
+            const size_t i = i0 + u * stride;
+            if (i >= n) continue;
+            T xi = xv[u] + alpha * mp[u];
+            xi = xi + omega * ms[u];
+            x[i] = xi;
+            const T ri = sv[u] - omega * ams[u];
+            r[i] = ri;
+            nr += fabs(static_cast<double>(ri));
+            rr += static_cast<double>(rsv[u]) * static_cast<double>(ri);
+        }
     }
     nr = block_sum256(nr, sh);
     rr = block_sum256(rr, sh);
@@ -225,12 +280,29 @@ __global__ __launch_bounds__(256) void bi_p_update(
     beta = beta * alpha;
     beta = beta / omega;
     const T bo = beta * omega;
+    const bool pre = st->precond != 0;
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
-    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
-        T pi = beta * p[i] + r[i];
-        pi = pi - bo * AMp[i];
-        p[i] = pi;
-        Mp[i] = precond_apply(st, invD, pi, i);
+    for (size_t i0 = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i0 < n; i0 += kBiRows * stride) {
+        T pv[kBiRows], rv[kBiRows], av[kBiRows], dv[kBiRows];
+#pragma unroll
+        for (int u = 0; u < kBiRows; ++u) {
+            const size_t i = i0 + u * stride;
+            if (i < n) {
+                pv[u] = p[i];
+                rv[u] = r[i];
+                av[u] = AMp[i];
+                if (pre) dv[u] = invD[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kBiRows; ++u) {
+            const size_t i = i0 + u * stride;
+            if (i >= n) continue;
+            T pi = beta * pv[u] + rv[u];
+            pi = pi - bo * av[u];
+            p[i] = pi;
+            Mp[i] = pre ? pi * dv[u] : pi;
+        }
     }
     if (blockIdx.x == 0) {
         const double res = sum_partials(part_n, g, sh) / st->norm_b;
@@ -296,20 +368,29 @@ struct BiBuffers
     }
 };
 
+// adia: A in diagonal storage (stencils), else the CSR SpMV; its first SpMV
+// forms the <r*, A Mp> partials itself (same grid-stride order as bi_dot).
 template <typename T>
-clsparseStatus enqueue_bi_iteration(const clsparseCsrMatrix* A, T* x, BiBuffers<T>& B, size_t n,
-                                    uint32_t g, int par, hipStream_t s, bool ext)
+clsparseStatus enqueue_bi_iteration(const clsparseCsrMatrix* A, const DiaMatrix<T>& adia, T* x,
+                                    BiBuffers<T>& B, size_t n, uint32_t g, int par, hipStream_t s, bool ext)
 {
     const int* done = &B.st->done;
     double* rr_old = B.part_rr[par ^ 1];
     double* rr_new = B.part_rr[par];
-    CLS_TRY(launch_csrmv<T>(B.one, A, B.Mp, B.zero, B.AMp, s, ext, done));
-    hipLaunchKernelGGL((bi_dot<T>), dim3(g), dim3(256), 0, s, n, B.rs, B.AMp, B.part_ra,
-                       static_cast<const T*>(nullptr), static_cast<const T*>(nullptr), nullptr, B.st);
+    if (adia.vals != nullptr) {
+        CLS_TRY(launch_dia_spmv<T>(adia, n, g, B.Mp, B.AMp, B.rs, B.part_ra, done, s));
+    } else {
+        CLS_TRY(launch_csrmv<T>(B.one, A, B.Mp, B.zero, B.AMp, s, ext, done));
+        hipLaunchKernelGGL((bi_dot<T>), dim3(g), dim3(256), 0, s, n, B.rs, B.AMp, B.part_ra,
+                           static_cast<const T*>(nullptr), static_cast<const T*>(nullptr), nullptr, B.st);
+    }
     hipLaunchKernelGGL((bi_s_update<T>), dim3(g), dim3(256), 0, s, n, B.r, B.AMp, B.s, B.Ms, B.invD, rr_old,
                        B.part_ra, B.part_s, g, B.st);
     hipLaunchKernelGGL(bi_check_s, dim3(1), dim3(256), 0, s, B.st, B.part_s, rr_old, B.part_ra, g);
-    CLS_TRY(launch_csrmv<T>(B.one, A, B.Ms, B.zero, B.AMs, s, ext, done));
+    if (adia.vals != nullptr)
+        CLS_TRY(launch_dia_spmv<T>(adia, n, g, B.Ms, B.AMs, static_cast<const T*>(nullptr), nullptr, done, s));
+    else
+        CLS_TRY(launch_csrmv<T>(B.one, A, B.Ms, B.zero, B.AMs, s, ext, done));
     hipLaunchKernelGGL((bi_dot<T>), dim3(g), dim3(256), 0, s, n, B.AMs, B.s, B.part_us,
                        static_cast<const T*>(B.AMs), static_cast<const T*>(B.AMs), B.part_ww, B.st);
     hipLaunchKernelGGL((bi_xr_update<T>), dim3(g), dim3(256), 0, s, n, x, B.r, B.rs, B.s, B.Mp, B.Ms, B.AMs,
@@ -334,14 +415,26 @@ clsparseStatus bicg_solve(cldenseVector* x, const clsparseCsrMatrix* A, const cl
     const T* bp = dev_ptr<T>(b->values, b->off_values);
     if (n > 0 && (xp == nullptr || bp == nullptr)) return clsparseInvalidMemObject;
 
+    // Stencil matrices: A's values in diagonal storage for the iterations
+    // (not with extended precision, which needs the compensated CSR sums).
+    DiaOffsets a_off{};
+    int a_ndiag = 0;
+    const bool ext = control->extended_precision;
+    // (after the caller's queued work: A's arrays and the partials buffer)
+    CLS_TRY(enter_solver_stream(control));
+    if (!ext) CLS_TRY(dia_candidates(A, reinterpret_cast<int32_t*>(control->red_partials), s, &a_off, &a_ndiag));
     void* work = nullptr;
-    CLS_TRY(control_workspace(control, BiBuffers<T>::bytes(n), &work));
+    CLS_TRY(control_workspace(control, BiBuffers<T>::bytes(n) +
+                                           (a_ndiag ? align_up(a_ndiag * n * sizeof(T)) + align_up(sizeof(int)) : 0),
+                              &work));
     void* pinned = nullptr;
     CLS_TRY(control_pinned(control, sizeof(BiDevState) + 64, &pinned));
     BiDevState* hs = static_cast<BiDevState*>(pinned);
     Carve c{static_cast<char*>(work)};
     BiBuffers<T> B;
     B.carve(c, n);
+    T* a_vals = a_ndiag ? c.take<T>(a_ndiag * n) : nullptr;
+    int* a_bad = a_ndiag ? c.take<int>(1) : nullptr;
     const uint32_t g = grid_for(n);
 
     CLS_TRY(enter_solver_stream(control));
@@ -382,8 +475,13 @@ clsparseStatus bicg_solve(cldenseVector* x, const clsparseCsrMatrix* A, const cl
     *hs = init;
     CLS_HIP_TRY(hipMemcpyAsync(B.st, hs, sizeof(BiDevState), hipMemcpyHostToDevice, s));
 
+    DiaMatrix<T> adia{};
+    if (a_ndiag) {
+        CLS_TRY((dia_fill<T, T>(A, n, &a_ndiag, a_off, a_vals, a_bad, s, &adia.sym)));
+        if (a_ndiag) adia = DiaMatrix<T>{a_vals, a_ndiag, adia.sym, a_off};
+    }
+
     // y = A x (into AMp), r = b - y, r* = r, p = r, Mp = M p (L103-134)
-    const bool ext = control->extended_precision;
     CLS_TRY(launch_csrmv<T>(B.one, A, xp, B.zero, B.AMp, s, ext, nullptr));
     hipLaunchKernelGGL((bi_init<T>), dim3(g), dim3(256), 0, s, n, bp, B.AMp, B.r, B.rs, B.p, B.Mp, B.invD,
                        B.st, B.part_n, B.part_rr[1]);
@@ -406,7 +504,7 @@ clsparseStatus bicg_solve(cldenseVector* x, const clsparseCsrMatrix* A, const cl
     if (use_graph) {
         CLS_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         for (int k = 0; k < kBiChunk && status == clsparseSuccess; ++k)
-            status = enqueue_bi_iteration<T>(A, xp, B, n, g, k & 1, s, ext);
+            status = enqueue_bi_iteration<T>(A, adia, xp, B, n, g, k & 1, s, ext);
         const hipError_t e = hipStreamEndCapture(s, &graph);
         if (status == clsparseSuccess && e != hipSuccess) status = clsparseInvalidKernelExecution;
         if (status == clsparseSuccess && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
@@ -418,7 +516,7 @@ clsparseStatus bicg_solve(cldenseVector* x, const clsparseCsrMatrix* A, const cl
             if (hipGraphLaunch(exec, s) != hipSuccess) status = clsparseInvalidKernelExecution;
         } else {
             for (int k = 0; k < kBiChunk && status == clsparseSuccess; ++k)
-                status = enqueue_bi_iteration<T>(A, xp, B, n, g, k & 1, s, ext);
+                status = enqueue_bi_iteration<T>(A, adia, xp, B, n, g, k & 1, s, ext);
         }
         if (status != clsparseSuccess) break;
         CLS_HIP_TRY(hipMemcpyAsync(hs, B.st, sizeof(BiDevState), hipMemcpyDeviceToHost, s));
@@ -451,6 +549,7 @@ clsparseStatus bicg_solve(cldenseVector* x, const clsparseCsrMatrix* A, const cl
     sc->stats.seconds = std::chrono::duration<double>(t1 - t0).count();
     sc->stats.status_detail = hs->converged ? 0 : 1;
     sc->stats.preconditioner = prec;
+    sc->stats.a_diagonals = a_ndiag;
     return clsparseSuccess;
 }
 

@@ -120,3 +120,23 @@ def test_error_codes(ctl):
                                         sc.handle, ctl.handle) == cls.clsparseInvalidSystemSize
     finally:
         sc.release()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_diagonal_storage_matches_csr(ctl, monkeypatch, dtype):
+    """The nonsymmetric 5-point matrix runs from full diagonal storage (its
+    mirrored values differ); the solve with CLSPARSE_AMD_DIA=0 (CSR-Adaptive)
+    stops within a few iterations of it and both meet the tolerance."""
+    rp, ci, v, n = _convection_diffusion(72, 64)
+    b = cls.gen_uniform(n, -1, 1, seed=33)
+    tol = 1e-10 if dtype == np.float64 else 1e-5
+    say(f"bicgstab diagonal storage vs CSR {np.dtype(dtype).name}")
+    st_d, s_d, x_d = _solve(ctl, rp, ci, v, n, b, dtype, cls.DIAGONAL, 5000, tol, 0.0)
+    monkeypatch.setenv("CLSPARSE_AMD_DIA", "0")
+    st_c, s_c, x_c = _solve(ctl, rp, ci, v, n, b, dtype, cls.DIAGONAL, 5000, tol, 0.0)
+    assert st_d == 0 and st_c == 0 and s_d.status_detail == 0 and s_c.status_detail == 0
+    assert s_d.a_diagonals == 5 and s_c.a_diagonals == 0
+    assert abs(s_d.n_iters - s_c.n_iters) <= max(3, 0.1 * s_c.n_iters)
+    for x in (x_d, x_c):
+        r = b - oracle.csrmv(rp, ci, v, x.astype(np.float64))[0]
+        assert np.abs(r).sum() / np.abs(b).sum() <= tol * (1.5 if dtype == np.float64 else 3.0)
