@@ -142,7 +142,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
                 const uint32_t j = i + static_cast<uint32_t>(off.o[d]);  // wraps below 0: out of range
                 xv[u][d] = buf_load<T>(rx, j * S, 0);
                 if (d < KL) {
-                    v[u][d] = buf_load<T>(rv, i * S, static_cast<uint32_t>(d) * n * S);
+                    // Rows past n read row n - 1: the diagonal's base is a scalar
+                    // offset, which the buffer's range check may not cover.
+                    const uint32_t il = i < n ? i : n - 1;
+                    v[u][d] = buf_load<T>(rv, il * S, static_cast<uint32_t>(d) * n * S);
                 } else {
                     // Mirrored value at row j of the lower diagonal; an offset past
                     // the buffer (no branch) reads 0 where j >= n.
