@@ -536,9 +536,59 @@ def fsai_workload(cls, torch, ctl, n1d=160, tol=1e-8, weights=None, cpu=True):
         out["cpu_baseline"] = cpu_cg_baseline(rp, ci, v, b, out["fp64_jacobi_pcg"]["iterations"],
                                               sample_iters=12)
     f.release()
-    net.release()
     A.meta_delete()
+    out["held_out_sweep"] = fsai_sweep(cls, torch, ctl, net, tol)
+    net.release()
     return out
+
+
+def fsai_sweep(cls, torch, ctl, net, tol):
+    """The same comparison on further held-out 3-D problems (other anisotropy,
+    coefficient spread and seeds; 128^3): learned-FSAI mixed PCG (first and
+    repeat solve) against FP64 Jacobi-PCG from diagonal storage."""
+    lib = cls.load()
+    res = []
+    for az, sigma, seed in ((100.0, 1.0, 61), (10.0, 1.5, 62), (1000.0, 0.5, 63), (1.0, 1.0, 64)):
+        rp, ci, v, m = cls.gen_diffusion3d(128, 128, 128, sigma=sigma, ax=1.0, ay=1.0, az=az, seed=seed)
+        b = torch.tensor(cls.gen_uniform(m, -1.0, 1.0, seed=seed + 100), device="cuda")
+        A = cls.CsrMatrix(rp, ci, v, m)
+        A.meta_create(ctl)
+        f = cls.Fsai(A, ctl)
+        f.apply_net(net, ctl)
+
+        def solve(fn, with_fsai):
+            x = torch.zeros(m, dtype=torch.float64, device="cuda")
+            sc = cls.SolverControl(cls.DIAGONAL, 50000, tol, 0.0)
+            xd, bd = cls.DenseVector(x), cls.DenseVector(b)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if with_fsai:
+                st = fn(ctypes.byref(xd.s), ctypes.byref(A.s), ctypes.byref(bd.s), f.handle, sc.handle,
+                        ctl.handle)
+            else:
+                st = fn(ctypes.byref(xd.s), ctypes.byref(A.s), ctypes.byref(bd.s), sc.handle, ctl.handle)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            stt = sc.stats()
+            sc.release()
+            cls.check("solve", st)
+            return dt, stt.iterations, stt.preconditioner, stt.status_detail == 0
+
+        first = solve(lib.clsparseAmdDcsrcgFsai, True)
+        rep_ = solve(lib.clsparseAmdDcsrcgFsai, True)
+        solve(lib.clsparseAmdDcsrcgRefined, False)  # warm
+        base = solve(lib.clsparseAmdDcsrcgRefined, False)
+        res.append({"az": az, "sigma": sigma, "seed": seed, "n": m,
+                    "learned_s": round(rep_[0], 5), "learned_first_s": round(first[0], 5),
+                    "learned_iterations": rep_[1], "preconditioner": rep_[2],
+                    "fp64_jacobi_s": round(base[0], 5), "fp64_jacobi_iterations": base[1],
+                    "converged": rep_[3] and base[3],
+                    "speedup": round(base[0] / rep_[0], 3),
+                    "speedup_first_solve": round(base[0] / first[0], 3)})
+        log(f"held-out sweep: {res[-1]}")
+        f.release()
+        A.meta_delete()
+    return res
 
 
 def fsai_train(cls, torch, ctl, args):

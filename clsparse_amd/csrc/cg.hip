@@ -889,7 +889,7 @@ extern "C" clsparseStatus clsparseDcsrcg(cldenseVector* x, const clsparseCsrMatr
 namespace {
 
 constexpr int kStagnationChunks = 16;  // learned FSAI must halve ||r|| within this many chunks
-constexpr int kPilotChunks = 3;        // chunks per preconditioner in the pilot race
+constexpr int kPilotChunks = 2;        // chunks per preconditioner in the pilot race
 
 // CLSPARSE_AMD_FSAI_POLICY=always keeps the learned factor unless it breaks
 // down or stagnates; the default ("race") also runs the pilot race.
@@ -1146,7 +1146,8 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
         // of each from the same starting point (FSAI first; its end point is
         // kept, x goes back to the start for Jacobi); the rate
         // -ln(res_end / res_start) / seconds is measured over all chunks but
-        // the first of each (which captures the graph). The faster one goes on
+        // the first of each (which captures the graph), i.e. over the second
+        // chunk of 32 iterations. The faster one goes on
         // from its own end point. (Jacobi started where FSAI stopped would
         // meet a residual whose easily damped components are already gone.)
         enum Phase { kPilotFsai, kPilotJacobi, kMain };

@@ -152,7 +152,8 @@ CLSPARSE_EXPORT clsparseStatus
  * non-finite value or lacks a diagonal entry; when the recurrence residual
  * turns non-finite (x goes back to the last replacement point); when the
  * residual fails to halve within 16 chunks of 32 iterations; or when a pilot
- * race -- 3 chunks with G, then 3 with Jacobi, each timed -- finds Jacobi
+ * race -- 2 chunks with G, then 2 with Jacobi from the same start, each
+ * timed -- finds Jacobi
  * reducing the residual faster per second (CLSPARSE_AMD_FSAI_POLICY=always
  * skips the race). The race's outcome is kept with the factor and reused by
  * later solves until the factor changes (clsparseAmdFsaiApplyNet,
