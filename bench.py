@@ -239,12 +239,13 @@ def next_rows(cls, torch, ctl, stream, cpu=True):
         best = (dt, stats.n_iters + 1, stats.current_residual, stats.a_diagonals)
     nnzb = int(rpb[-1])
     # Bytes of one iteration as executed: two SpMVs (diagonal storage: K values
-    # + x + y per row; CSR: the reference formula), the vector kernels
-    # (s update 40, <AMs,s>/<AMs,AMs> 16, x/r update 64, p update 48 B per row)
-    # and <r*, A Mp> (fused into the first diagonal SpMV: 8, else its own 16).
+    # + x + y per row; CSR: the reference formula), the s, x/r and p updates
+    # (40 + 64 + 48 B per row) and the dots <r*, A Mp>, <A Ms, s>, <A Ms, A Ms>
+    # (fused into the diagonal SpMVs: r* and s read, 16 B; else two dot
+    # kernels, 16 + 16 B).
     K = best[3]
     spmv = (K * 8 + 16) * m if K else spmv_bytes(m, m, nnzb)
-    it_bytes = 2 * spmv + (8 if K else 16) * m + 168 * m
+    it_bytes = 2 * spmv + 152 * m + (16 if K else 32) * m
     r = b - oracle.csrmv(rpb, cib, vb, xt.cpu().numpy())[0]
     res = {"workload": f"convection_diffusion3d_{n1}^3_upwind{conv}", "n": m, "nnz": nnzb,
            "tolerance_l1_relative": tol, "seconds": round(best[0], 5), "iterations": best[1],

@@ -14,9 +14,10 @@
 //   p = beta p + r; p = p - (beta omega) AMp.
 // finished() reads the solver control's nIters, which keeps its value from
 // before the call until the first iteration ends (then = iteration index).
-// One iteration is seven kernels (two SpMVs, two dot kernels, the s update
-// with M s, the x / r update, the p update with M p and the second check;
-// the first check is a one-block kernel) whose scalars stay on the device;
+// One iteration is five kernels for a stencil matrix in diagonal storage (two
+// SpMVs with their dot products fused, the s update with M s, the first check
+// with the x / r update, the p update with M p and the second check), seven
+// with the CSR SpMV (two dot kernels more); the scalars stay on the device;
 // 32 iterations are captured once in a hipGraph and replayed, and kernels
 // return at once after the stop. Reductions are fixed-order (FP64 partials),
 // so solves are reproducible. The reference's function ends without a return
@@ -187,42 +188,40 @@ __global__ __launch_bounds__(256) void bi_s_update(size_t n, const T* __restrict
     if (threadIdx.x == 0) part_s[blockIdx.x] = ns;
 }
 
-// First check (one block): ||s||_1 / ||b||_1.
-__global__ __launch_bounds__(256) void bi_check_s(BiDevState* __restrict__ st,
-                                                  const double* __restrict__ part_s,
-                                                  const double* __restrict__ part_rr_old,
-                                                  const double* __restrict__ part_ra, uint32_t g)
-{
-    __shared__ double sh[4];
-    if (st->done) return;
-    const double res = sum_partials(part_s, g, sh) / st->norm_b;
-    const double alpha = sum_partials(part_rr_old, g, sh) / sum_partials(part_ra, g, sh);
-    if (threadIdx.x == 0) {
-        st->cur_res = res;
-        if (bi_finished(st, res, st->iter)) {
-            st->done = 1;
-            st->stop_kind = 1;
-            st->n_iters = st->iter;
-            st->alpha_stop = alpha;
-            st->converged = (res <= st->rel_tol || res <= st->abs_tol * st->init_res) ? 1 : 0;
-        }
-    }
-}
-
-// omega = <AMs, s> / <AMs, AMs>; x += alpha Mp; x += omega Ms; r = s - omega AMs;
-// partials ||r||_1 and <r*, r>.
+// First check, then omega = <AMs, s> / <AMs, AMs>; x += alpha Mp;
+// x += omega Ms; r = s - omega AMs; partials ||r||_1 and <r*, r>.
+// Every workgroup evaluates the first check, finished(||s||_1 / ||b||_1),
+// from the same partials and so reaches the same decision; on a stop,
+// workgroup 0 records it (x += alpha Mp follows after the loop) and none
+// updates x or r. The A Ms product and its dots, launched before this
+// kernel, are then unused.
 template <typename T>
 __global__ __launch_bounds__(256) void bi_xr_update(
     size_t n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ rs, const T* __restrict__ s,
     const T* __restrict__ Mp, const T* __restrict__ Ms, const T* __restrict__ AMs,
     const double* __restrict__ part_rr_old, const double* __restrict__ part_ra,
     const double* __restrict__ part_us, const double* __restrict__ part_ww,
-    double* __restrict__ part_n, double* __restrict__ part_rr_new, uint32_t g,
-    const BiDevState* __restrict__ st)
+    const double* __restrict__ part_s, double* __restrict__ part_n, double* __restrict__ part_rr_new,
+    uint32_t g, BiDevState* __restrict__ st)
 {
     __shared__ double sh[4];
     if (st->done) return;
-    const T alpha = static_cast<T>(sum_partials(part_rr_old, g, sh) / sum_partials(part_ra, g, sh));
+    const double rr_old = sum_partials(part_rr_old, g, sh);
+    const double ra = sum_partials(part_ra, g, sh);
+    const double res_s = sum_partials(part_s, g, sh) / st->norm_b;
+    const int it = st->iter;
+    if (bi_finished(st, res_s, it)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->cur_res = res_s;
+            st->done = 1;
+            st->stop_kind = 1;
+            st->n_iters = it;
+            st->alpha_stop = rr_old / ra;
+            st->converged = (res_s <= st->rel_tol || res_s <= st->abs_tol * st->init_res) ? 1 : 0;
+        }
+        return;
+    }
+    const T alpha = static_cast<T>(rr_old / ra);
     const T omega = static_cast<T>(sum_partials(part_us, g, sh) / sum_partials(part_ww, g, sh));
     double nr = 0.0, rr = 0.0;
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
@@ -386,15 +385,16 @@ clsparseStatus enqueue_bi_iteration(const clsparseCsrMatrix* A, const DiaMatrix<
     }
     hipLaunchKernelGGL((bi_s_update<T>), dim3(g), dim3(256), 0, s, n, B.r, B.AMp, B.s, B.Ms, B.invD, rr_old,
                        B.part_ra, B.part_s, g, B.st);
-    hipLaunchKernelGGL(bi_check_s, dim3(1), dim3(256), 0, s, B.st, B.part_s, rr_old, B.part_ra, g);
-    if (adia.vals != nullptr)
-        CLS_TRY(launch_dia_spmv<T>(adia, n, g, B.Ms, B.AMs, static_cast<const T*>(nullptr), nullptr, done, s));
-    else
+    if (adia.vals != nullptr) {
+        // <A Ms, s> and <A Ms, A Ms> partials fused (bi_dot's order and values).
+        CLS_TRY(launch_dia_spmv<T>(adia, n, g, B.Ms, B.AMs, B.s, B.part_us, done, s, B.part_ww));
+    } else {
         CLS_TRY(launch_csrmv<T>(B.one, A, B.Ms, B.zero, B.AMs, s, ext, done));
-    hipLaunchKernelGGL((bi_dot<T>), dim3(g), dim3(256), 0, s, n, B.AMs, B.s, B.part_us,
-                       static_cast<const T*>(B.AMs), static_cast<const T*>(B.AMs), B.part_ww, B.st);
+        hipLaunchKernelGGL((bi_dot<T>), dim3(g), dim3(256), 0, s, n, B.AMs, B.s, B.part_us,
+                           static_cast<const T*>(B.AMs), static_cast<const T*>(B.AMs), B.part_ww, B.st);
+    }
     hipLaunchKernelGGL((bi_xr_update<T>), dim3(g), dim3(256), 0, s, n, x, B.r, B.rs, B.s, B.Mp, B.Ms, B.AMs,
-                       rr_old, B.part_ra, B.part_us, B.part_ww, B.part_n, rr_new, g, B.st);
+                       rr_old, B.part_ra, B.part_us, B.part_ww, B.part_s, B.part_n, rr_new, g, B.st);
     hipLaunchKernelGGL((bi_p_update<T>), dim3(g), dim3(256), 0, s, n, B.p, B.Mp, B.r, B.AMp, B.invD, rr_old,
                        rr_new, B.part_ra, B.part_us, B.part_ww, B.part_n, g, B.st);
     CLS_HIP_TRY(hipGetLastError());

@@ -49,13 +49,14 @@ __global__ __launch_bounds__(256) void dia_spmv_dot_kernel(size_t n, DiaOffsets 
                                                            T* __restrict__ y,
                                                            const T* __restrict__ w,
                                                            double* __restrict__ part,
+                                                           double* __restrict__ part2,
                                                            const int* __restrict__ done)
 {
     constexpr int KL = SYM ? (K + 1) / 2 : K;
     __shared__ double sh[4];
     if (done != nullptr && *done != 0) return;
     const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
-    double a = 0.0;
+    double a = 0.0, a2 = 0.0;
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
     for (size_t i0 = static_cast<size_t>(blk) * 256 + threadIdx.x; i0 < n; i0 += kDiaRows * stride) {
         T v[kDiaRows][K], xv[kDiaRows][K], pi[kDiaRows];
@@ -83,11 +84,15 @@ __global__ __launch_bounds__(256) void dia_spmv_dot_kernel(size_t n, DiaOffsets 
                 if (v[u][d] != T(0)) acc += mul_rounded_t(v[u][d], xv[u][d]);
             y[i] = acc;
             a += static_cast<double>(acc) * static_cast<double>(pi[u]);
+            a2 += static_cast<double>(acc) * static_cast<double>(acc);
         }
     }
     if (part == nullptr) return;
     a = block_sum256(a, sh);
     if (threadIdx.x == 0) part[blk] = a;
+    if (part2 == nullptr) return;
+    a2 = block_sum256(a2, sh);
+    if (threadIdx.x == 0) part2[blk] = a2;
 }
 
 // The same product through buffer loads: the value of diagonal d at row i is
@@ -115,6 +120,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
                                                                T* __restrict__ y,
                                                                const T* __restrict__ w,
                                                                double* __restrict__ part,
+                                                               double* __restrict__ part2,
                                                                const int* __restrict__ done)
 {
     constexpr int KL = SYM ? (K + 1) / 2 : K;
@@ -130,7 +136,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(part != nullptr ? w : p), 0, part != nullptr ? n * S : 0u,
                                           0x00020000);
     const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
-    double a = 0.0;
+    double a = 0.0, a2 = 0.0;
     const uint32_t stride = gridDim.x * 256;
     for (uint32_t i0 = blk * 256 + threadIdx.x; i0 < n; i0 += R * stride) {
         T v[R][K], xv[R][K], pi[R];
@@ -167,11 +173,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
                 if (v[u][d] != T(0)) acc += mul_rounded_t(v[u][d], xv[u][d]);
             if (i < n) y[i] = acc;
             a += i < n ? static_cast<double>(acc) * static_cast<double>(pi[u]) : 0.0;
+            a2 += i < n ? static_cast<double>(acc) * static_cast<double>(acc) : 0.0;
         }
     }
     if (part == nullptr) return;
     a = block_sum256(a, sh);
     if (threadIdx.x == 0) part[blk] = a;
+    if (part2 == nullptr) return;
+    a2 = block_sum256(a2, sh);
+    if (threadIdx.x == 0) part2[blk] = a2;
 }
 
 // *bad = 1 unless A(i, i + o) == A(i + o, i) for every upper diagonal o of
@@ -253,14 +263,14 @@ __global__ __launch_bounds__(64) void dia_sample_kernel(uint32_t n, const uint32
 
 template <int K, bool SYM, typename T>
 void launch_dia_spmv_dot(size_t n, uint32_t g, const DiaOffsets& off, const T* vals, const T* p, T* y,
-                         const T* w, double* part, const int* done, hipStream_t s)
+                         const T* w, double* part, double* part2, const int* done, hipStream_t s)
 {
     if (static_cast<uint64_t>(K) * n * sizeof(T) < (uint64_t(1) << 32) - 16)
         hipLaunchKernelGGL((dia_spmv_dot_buf_kernel<K, SYM, T>), dim3(g), dim3(256), 0, s,
-                           static_cast<uint32_t>(n), off, vals, p, y, w, part, done);
+                           static_cast<uint32_t>(n), off, vals, p, y, w, part, part2, done);
     else
         hipLaunchKernelGGL((dia_spmv_dot_kernel<K, SYM, T>), dim3(g), dim3(256), 0, s, n, off, vals, p, y, w,
-                           part, done);
+                           part, part2, done);
 }
 
 // A stencil matrix in diagonal storage for one solve (ndiag = 0: none).
@@ -274,12 +284,13 @@ struct DiaMatrix
 };
 
 // y = A x from diagonal storage on g workgroups; with part, also the
-// partials of <y, w> (grid-stride order, one per workgroup).
+// partials of <y, w>, and with part2 those of <y, y> (grid-stride order, one
+// per workgroup).
 template <typename T>
 clsparseStatus launch_dia_spmv(const DiaMatrix<T>& A, size_t n, uint32_t g, const T* x, T* y, const T* w,
-                               double* part, const int* done, hipStream_t s)
+                               double* part, const int* done, hipStream_t s, double* part2 = nullptr)
 {
-#define CLS_DIA_SPMV(K, S) launch_dia_spmv_dot<K, S, T>(n, g, A.off, A.vals, x, y, w, part, done, s)
+#define CLS_DIA_SPMV(K, S) launch_dia_spmv_dot<K, S, T>(n, g, A.off, A.vals, x, y, w, part, part2, done, s)
     if (A.sym) {
         switch (A.ndiag) {
         case 3: CLS_DIA_SPMV(3, true); break;

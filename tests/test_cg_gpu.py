@@ -388,3 +388,49 @@ def test_nonsymmetric_stencil_uses_full_storage(ctl, monkeypatch):
     _, x_c, s_c = _solve(ctl, rp, ci, v, n, b, np.zeros(n), np.float64, cls.DIAGONAL, 20, 1e-30, 0.0)
     assert s_d.a_diagonals == 5 and s_c.a_diagonals == 0
     assert np.abs(x_d - x_c).max() <= 1e-10 * np.abs(x_c).max()
+
+
+@pytest.mark.parametrize("solver", ["cg", "bicgstab", "mixed"])
+def test_offsets_with_diagonal_storage(ctl, solver):
+    """Matrix arrays, x and b with element offsets (garbage in front): the
+    diagonal-storage detection and fill read through the offsets, and the
+    solve equals the one on offset-free arrays."""
+    import ctypes
+    rp, ci, v, m = cls.gen_poisson2d(48, 40)
+    ov, oc, orp, ox, ob = 5, 9, 4, 7, 3
+    dev = "cuda"
+    vals = torch.cat([torch.full((ov,), 99.0, dtype=torch.float64), torch.tensor(v)]).to(dev)
+    cols = torch.cat([torch.full((oc,), 12345, dtype=torch.int32), torch.tensor(ci.view(np.int32))]).to(dev)
+    rptr = torch.cat([torch.full((orp,), 77, dtype=torch.int32), torch.tensor(rp.view(np.int32))]).to(dev)
+    A = cls.clsparseCsrMatrix()
+    lib = cls.load()
+    lib.clsparseInitCsrMatrix(ctypes.byref(A))
+    A.num_rows, A.num_cols, A.num_nonzeros = m, m, v.size
+    A.values, A.col_indices, A.row_pointer = vals.data_ptr(), cols.data_ptr(), rptr.data_ptr()
+    A.off_values, A.off_col_indices, A.off_row_pointer = ov, oc, orp
+    assert lib.clsparseCsrMetaCreate(ctypes.byref(A), ctl.handle) == 0
+    b = cls.gen_uniform(m, -1, 1, seed=51)
+    say(f"offsets with diagonal storage: {solver}")
+
+    def run(Amat, x_off, b_off):
+        xt = torch.cat([torch.full((x_off,), 1e30, dtype=torch.float64), torch.zeros(m, dtype=torch.float64)]).to(dev)
+        bt = torch.cat([torch.full((b_off,), -7.0, dtype=torch.float64), torch.tensor(b)]).to(dev)
+        xd = cls.cldenseVector(m, xt.data_ptr(), x_off)
+        bd = cls.cldenseVector(m, bt.data_ptr(), b_off)
+        sc = cls.SolverControl(cls.DIAGONAL, 3000, 1e-10, 0.0)
+        fn = {"cg": lib.clsparseDcsrcg, "bicgstab": lib.clsparseDcsrbicgStab,
+              "mixed": lib.clsparseAmdDcsrcgMixed}[solver]
+        assert fn(ctypes.byref(xd), ctypes.byref(Amat), ctypes.byref(bd), sc.handle, ctl.handle) == 0
+        s = sc.stats()
+        sc.release()
+        out = xt.cpu().numpy()
+        assert np.all(out[:x_off] == 1e30)
+        return out[x_off:], s
+    x1, s1 = run(A, ox, ob)
+    B = cls.CsrMatrix(rp, ci, v, m)
+    B.meta_create(ctl)
+    x2, s2 = run(B.s, 0, 0)
+    assert s1.a_diagonals == 5 and s2.a_diagonals == 5
+    assert s1.iterations == s2.iterations
+    assert np.array_equal(x1, x2)
+    lib.clsparseCsrMetaDelete(ctypes.byref(A))
