@@ -39,13 +39,35 @@ __global__ __launch_bounds__(256) void csrmm_kernel(uint32_t num_rows, uint32_t 
     const int base = static_cast<int>(slot * W);  // first lane of this slot
     const T alpha = *pAlpha;
     const T beta = *pBeta;
-    const uint32_t wave = xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
-    const uint32_t waves = gridDim.x * 4;
-    for (uint32_t row0 = wave * kSlots; row0 < num_rows; row0 += waves * kSlots) {
+    // Each XCD (gridDim.x is a multiple of 8; logical workgroups [x G/8,
+    // (x+1) G/8) run on XCD x) owns one contiguous eighth of the rows and its
+    // waves sweep it in steps of waves * kSlots rows: a B row shared by
+    // nearby rows is then fetched into one L2, not one per XCD.
+    const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t per_xcd = gridDim.x / 8;
+    const uint32_t xcd = lb / per_xcd;
+    const uint32_t rows_per_xcd = (num_rows + 7) / 8;
+    const uint32_t first = min(num_rows, xcd * rows_per_xcd);
+    const uint32_t stop = min(num_rows, first + rows_per_xcd);
+    const uint32_t wave = (lb - xcd * per_xcd) * 4 + (threadIdx.x >> 6);
+    const uint32_t waves = per_xcd * 4;
+    // The next step's row bounds are loaded at the top of each step, so their
+    // latency hides behind this step's nonzeros and B rows.
+    uint32_t row0 = first + wave * kSlots;
+    uint32_t nbeg = 0, nend = 0;
+    if (row0 + slot < stop) {
+        nbeg = rp[row0 + slot];
+        nend = rp[row0 + slot + 1];
+    }
+    for (; row0 < stop; row0 += waves * kSlots) {
         const uint32_t row = row0 + slot;
-        const bool live = row < num_rows;
-        const uint32_t beg = live ? rp[row] : 0u;
-        const uint32_t end = live ? rp[row + 1] : 0u;
+        const bool live = row < stop;
+        const uint32_t beg = nbeg, end = nend;
+        const uint32_t next = row0 + waves * kSlots + slot;
+        if (next < stop) {
+            nbeg = rp[next];
+            nend = rp[next + 1];
+        }
         for (uint32_t c0 = 0; c0 < ncols; c0 += W) {
             const uint32_t c = c0 + t;
             const bool colok = c < ncols;
@@ -113,10 +135,14 @@ clsparseStatus csrmm_entry(const clsparseScalar* alpha, const clsparseCsrMatrix*
     T* Cp = dev_ptr<T>(C->values, C->off_values);
     if (rp == nullptr || Bp == nullptr || Cp == nullptr || pa == nullptr || pb == nullptr)
         return clsparseInvalidMemObject;
-    // Enough waves for several per SIMD; each walks rows in steps of the grid.
+    // At most as many waves as the chip holds at once (2048 workgroups of 4
+    // waves at 8 waves per SIMD): the resident waves of an XCD then work on one
+    // compact window of its rows that advances together, so the B rows that
+    // neighbouring rows share stay in its L2.
     const uint32_t slots = ncols > 32 ? 1u : ncols > 16 ? 2u : ncols > 8 ? 4u : ncols > 4 ? 8u
                                        : ncols > 2 ? 16u : ncols > 1 ? 32u : 64u;
-    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((n + 4 * slots - 1) / (4 * slots), 16384));
+    const uint32_t want = (n + 4 * slots - 1) / (4 * slots);
+    const uint32_t grid = std::min<uint32_t>(2048, (want + 7) / 8 * 8);  // a multiple of 8 (one share per XCD)
     hipStream_t s = control->stream;
 #define CLS_CSRMM(W)                                                                                 \
     hipLaunchKernelGGL((csrmm_kernel<T, W>), dim3(grid), dim3(256), 0, s, n, ncols, rp, ci, v, Bp,   \
