@@ -466,7 +466,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(uint32_t nnzG, const uint
                                                        T* __restrict__ gval, T* __restrict__ gtval,
                                                        uint16_t* __restrict__ g16,
                                                        uint16_t* __restrict__ gt16,
-                                                       const size_t* __restrict__ dia_pos,
+                                                       const uint32_t* __restrict__ dia_pos,
                                                        uint16_t* __restrict__ dia16,
                                                        int* __restrict__ bad)
 {
@@ -583,7 +583,7 @@ struct _clsparseAmdFsai
     int ndiag = 0;
     int32_t dia_off[kFsaiMaxDiags] = {};
     uint16_t* dia16 = nullptr;   // [ndiag][n]
-    size_t* dia_pos = nullptr;   // per G nonzero: its index in dia16
+    uint32_t* dia_pos = nullptr; // per G nonzero: its index in dia16
     int* bad = nullptr;
     clsparseCsrMatrix G64{}, GT64{}, G32{}, GT32{};
     clsparseCsrMatrix G16{}, GT16{};  // share G32's / GT32's metadata (not owned)
@@ -782,15 +782,16 @@ extern "C" clsparseStatus clsparseAmdFsaiCreate(clsparseAmdFsai* out, const clsp
         if (std::find(offs.begin(), offs.end(), o) == offs.end()) offs.push_back(o);
     }
     const bool use_dia = nnzG > 0 && offs.size() <= static_cast<size_t>(kFsaiMaxDiags) &&
-                         static_cast<double>(offs.size()) * n <= 1.5 * nnzG;
-    std::vector<size_t> dpos;
+                         static_cast<double>(offs.size()) * n <= 1.5 * nnzG &&
+                         static_cast<uint64_t>(offs.size()) * n < (uint64_t(1) << 32);
+    std::vector<uint32_t> dpos;
     if (use_dia) {
         std::sort(offs.begin(), offs.end());
         dpos.resize(nnzG);
         for (uint32_t k = 0; k < nnzG; ++k) {
             const int64_t o = static_cast<int64_t>(gci[k]) - static_cast<int64_t>(grow[k]);
             const size_t d = static_cast<size_t>(std::lower_bound(offs.begin(), offs.end(), o) - offs.begin());
-            dpos[k] = d * n + grow[k];
+            dpos[k] = static_cast<uint32_t>(d * n + grow[k]);
         }
     }
 
@@ -832,7 +833,7 @@ extern "C" clsparseStatus clsparseAmdFsaiCreate(clsparseAmdFsai* out, const clsp
             e = hipMemsetAsync(f->dia16, 0, offs.size() * n * sizeof(uint16_t), st);  // padding stays 0
         if (e == hipSuccess) e = dmalloc(&f->dia_pos, nnzG);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(f->dia_pos, dpos.data(), nnzG * sizeof(size_t), hipMemcpyHostToDevice, st);
+            e = hipMemcpyAsync(f->dia_pos, dpos.data(), nnzG * sizeof(uint32_t), hipMemcpyHostToDevice, st);
     }
     if (e != hipSuccess) {
         free_fsai(f);
@@ -920,7 +921,7 @@ clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_
     hipLaunchKernelGGL((assemble_kernel<double>), dim3(blocks_for(f->nnzG)), dim3(256), 0, st, f->nnzG,
                        f->g_row, f->g_ci, f->g_tpos, o, f->dhat, f->g64, f->gt64,
                        static_cast<uint16_t*>(nullptr), static_cast<uint16_t*>(nullptr),
-                       static_cast<const size_t*>(nullptr), static_cast<uint16_t*>(nullptr), f->bad);
+                       static_cast<const uint32_t*>(nullptr), static_cast<uint16_t*>(nullptr), f->bad);
     hipLaunchKernelGGL((assemble_kernel<float>), dim3(blocks_for(f->nnzG)), dim3(256), 0, st, f->nnzG,
                        f->g_row, f->g_ci, f->g_tpos, o, f->dhat, f->g32, f->gt32, f->g16, f->gt16,
                        f->dia_pos, f->dia16, f->bad);
