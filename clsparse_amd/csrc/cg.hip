@@ -1063,6 +1063,7 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
         }
         prec = fsai_valid(fsai) ? kPrecFsai : kPrecJacobi;
         used = fsai_valid(fsai) ? 2 : 3;
+        if (!have_dia) CLS_TRY(fsai_ensure_meta(fsai, control));  // CSR SpMVs with G and G^T
     }
     if (prec != kPrecNone) {
         hipLaunchKernelGGL((extract_inv_diag<double, TI>), dim3((n + 255) / 256), dim3(256), 0, s,

@@ -194,6 +194,9 @@ bool fsai_valid(clsparseAmdFsai f);
 int fsai_race(clsparseAmdFsai f, double* rate_fsai, double* rate_jacobi);
 void fsai_set_race(clsparseAmdFsai f, int outcome, double rate_fsai, double rate_jacobi);
 uint32_t fsai_n(clsparseAmdFsai f);
+// Creates the CSR metadata of G and G^T on first use (the solvers that
+// apply the factor from diagonal storage never need it).
+clsparseStatus fsai_ensure_meta(clsparseAmdFsai f, clsparseControl control);
 
 // Diagonal storage of the BF16 factor, present when G's pattern lies on at
 // most kFsaiMaxDiags diagonals (structured-grid stencils): vals[k * n + i] =

@@ -328,12 +328,14 @@ bool dia_enabled()
 // ascending, when it has at most kDiaMax entries and that many diagonals of
 // length n would store at most 1.5 nnz values; *ndiag = 0 otherwise. The
 // fill kernel checks every row against the result.
+// honour_env: CLSPARSE_AMD_DIA=0 turns the detection off (the solvers' A);
+// the learned factor's storage has its own switch at solve time.
 clsparseStatus dia_candidates(const clsparseCsrMatrix* A, int32_t* scratch, hipStream_t s,
-                              DiaOffsets* off, int* ndiag)
+                              DiaOffsets* off, int* ndiag, bool honour_env = true)
 {
     *ndiag = 0;
     const uint32_t n = A->num_rows;
-    if (n < 8 || A->num_nonzeros == 0 || !dia_enabled()) return clsparseSuccess;
+    if (n < 8 || A->num_nonzeros == 0 || (honour_env && !dia_enabled())) return clsparseSuccess;
     hipLaunchKernelGGL(dia_sample_kernel, dim3(1), dim3(64), 0, s, n,
                        dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer),
                        dev_ptr<uint32_t>(A->col_indices, A->off_col_indices), scratch);

@@ -22,7 +22,10 @@
 #include <new>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "common.hpp"
+#include "dia.hpp"
 
 namespace {
 
@@ -586,6 +589,10 @@ struct _clsparseAmdFsai
     uint32_t* dia_pos = nullptr; // per G nonzero: its index in dia16
     int* bad = nullptr;
     clsparseCsrMatrix G64{}, GT64{}, G32{}, GT32{};
+    // The CSR metadata of G and G^T (G32 / G16 share G64's, GT32 / GT16
+    // GT64's) is created on first use: solves from diagonal storage never
+    // need it.
+    bool meta_ready = false;
     clsparseCsrMatrix G16{}, GT16{};  // share G32's / GT32's metadata (not owned)
     bool valid = true;
     // Training scratch (FP64 vectors of length n, and the G gradient).
@@ -626,10 +633,8 @@ clsparseStatus compute_features(_clsparseAmdFsai* f, const clsparseCsrMatrix* A,
 void free_fsai(_clsparseAmdFsai* f)
 {
     if (!f) return;
-    clsparseCsrMetaDelete(&f->G64);
+    clsparseCsrMetaDelete(&f->G64);  // G32 / G16 share it
     clsparseCsrMetaDelete(&f->GT64);
-    clsparseCsrMetaDelete(&f->G32);
-    clsparseCsrMetaDelete(&f->GT32);
     void* ptrs[] = {f->g_rp, f->g_ci, f->gt_rp, f->gt_ci, f->g_row, f->g_src, f->g_tpos,
                     f->dhat, f->stats, f->X, f->o, f->g64, f->gt64, f->g32, f->gt32, f->g16, f->gt16, f->bad,
                     f->dia16, f->dia_pos, f->t, f->u, f->s, f->v, f->t2, f->w, f->z, f->gG, f->sc, f->gout};
@@ -723,6 +728,261 @@ extern "C" clsparseStatus clsparseAmdMlpGetGrad(clsparseAmdMlp m, cl_float* host
 }
 
 // Pattern of G = tril(A) and of G^T, built on the host from A's pattern.
+namespace {
+
+// ---- pattern of G = tril(A) and of G^T, built on the device ---------------------
+
+// Row r of A: entries with column <= r (G's row length); *bad on a row
+// pointer that decreases or leaves [0, nnzA], or a column >= n; *nodiag when
+// the row has no diagonal entry.
+__global__ __launch_bounds__(256) void tril_count_kernel(uint32_t n, uint32_t nnzA, const uint32_t* __restrict__ rp,
+                                                         const uint32_t* __restrict__ ci, uint32_t* __restrict__ cnt,
+                                                         int* __restrict__ bad, int* __restrict__ nodiag)
+{
+    const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= n) return;
+    const uint32_t b = rp[r], e = rp[r + 1];
+    if (e < b || e > nnzA) {
+        cnt[r] = 0;
+        *bad = 1;
+        return;
+    }
+    uint32_t c0 = 0;
+    bool diag = false;
+    for (uint32_t k = b; k < e; ++k) {
+        const uint32_t c = ci[k];
+        if (c >= n) {
+            *bad = 1;
+            continue;
+        }
+        c0 += c <= r;
+        diag = diag || c == r;
+    }
+    cnt[r] = c0;
+    if (!diag) *nodiag = 1;
+}
+
+// G's row r: column, row and source index in A of each entry with column <= r.
+__global__ __launch_bounds__(256) void tril_fill_kernel(uint32_t n, const uint32_t* __restrict__ rp,
+                                                        const uint32_t* __restrict__ ci,
+                                                        const uint32_t* __restrict__ grp, uint32_t* __restrict__ gci,
+                                                        uint32_t* __restrict__ grow, uint32_t* __restrict__ gsrc,
+                                                        uint64_t* __restrict__ key, uint32_t* __restrict__ idx)
+{
+    const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= n) return;
+    uint32_t pos = grp[r];
+    for (uint32_t k = rp[r]; k < rp[r + 1]; ++k) {
+        const uint32_t c = ci[k];
+        if (c > r) continue;
+        gci[pos] = c;
+        grow[pos] = r;
+        gsrc[pos] = k;
+        key[pos] = (static_cast<uint64_t>(c) << 32) | r;  // G^T order: by column, then row
+        idx[pos] = pos;
+        ++pos;
+    }
+}
+
+__global__ __launch_bounds__(256) void column_count_kernel(uint32_t nnz, const uint32_t* __restrict__ gci,
+                                                           uint32_t* __restrict__ tcnt)
+{
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k < nnz) atomicAdd(&tcnt[gci[k]], 1u);
+}
+
+// Sorted (column, row) keys -> G^T's columns and the position of every G entry in G^T.
+__global__ __launch_bounds__(256) void transpose_fill_kernel(uint32_t nnz, const uint64_t* __restrict__ key,
+                                                             const uint32_t* __restrict__ idx,
+                                                             uint32_t* __restrict__ tci, uint32_t* __restrict__ tpos)
+{
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= nnz) return;
+    tci[p] = static_cast<uint32_t>(key[p] & 0xFFFFFFFFull);
+    tpos[idx[p]] = p;
+}
+
+// Position of every G entry in the diagonal storage; *bad when an entry lies
+// off the K candidate diagonals.
+__global__ __launch_bounds__(256) void dia_position_kernel(uint32_t nnz, uint32_t n, const uint32_t* __restrict__ gci,
+                                                           const uint32_t* __restrict__ grow, int K, DiaOffsets off,
+                                                           uint32_t* __restrict__ pos, int* __restrict__ bad)
+{
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nnz) return;
+    const int64_t o = static_cast<int64_t>(gci[k]) - static_cast<int64_t>(grow[k]);
+    int d = -1;
+    for (int j = 0; j < K; ++j)
+        if (off.o[j] == o) d = j;
+    if (d < 0) {
+        *bad = 1;
+        return;
+    }
+    pos[k] = static_cast<uint32_t>(d) * n + grow[k];
+}
+
+__global__ __launch_bounds__(256) void row_pointer_check_kernel(uint32_t n, uint32_t nnz, const uint32_t* __restrict__ rp,
+                                                                int* __restrict__ bad)
+{
+    const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+    if (r < n && (rp[r + 1] < rp[r] || rp[r + 1] > nnz)) *bad = 1;
+}
+
+uint32_t blocks256(uint64_t n) { return static_cast<uint32_t>(std::max<uint64_t>(1, (n + 255) / 256)); }
+
+// clsparseInvalidValue unless rp[0..n] never decreases and stays within nnz.
+clsparseStatus check_row_pointer(const uint32_t* rp, uint32_t n, uint32_t nnz, int* flag, hipStream_t st)
+{
+    CLS_HIP_TRY(hipMemsetAsync(flag, 0, sizeof(int), st));
+    hipLaunchKernelGGL(row_pointer_check_kernel, dim3(blocks256(n)), dim3(256), 0, st, n, nnz, rp, flag);
+    CLS_HIP_TRY(hipGetLastError());
+    int h = 0;
+    CLS_HIP_TRY(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    CLS_HIP_TRY(hipStreamSynchronize(st));
+    return h ? clsparseInvalidValue : clsparseSuccess;
+}
+
+// Device buffers freed at scope exit.
+struct Scratch
+{
+    std::vector<void*> p;
+    template <typename T>
+    hipError_t get(T** out, size_t count)
+    {
+        void* q = nullptr;
+        const hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) p.push_back(q);
+        *out = static_cast<T*>(q);
+        return e;
+    }
+    ~Scratch()
+    {
+        for (void* q : p) (void)hipFree(q);
+    }
+};
+
+// G = tril(A) (CSR, with each entry's row and source index in A), G^T (CSR,
+// rows of G^T in ascending order), and the diagonal storage of G's BF16
+// values when its pattern lies on at most kFsaiMaxDiags diagonals (found from
+// sampled rows, checked on every entry). All on the device; the host reads
+// back only counts and flags.
+clsparseStatus build_pattern(_clsparseAmdFsai* f, const uint32_t* arp, const uint32_t* aci, uint32_t n,
+                             uint32_t nnzA, clsparseControl control)
+{
+    hipStream_t st = control->stream;
+    Scratch tmp;
+    uint32_t *cnt = nullptr, *tcnt = nullptr;
+    int* flags = nullptr;  // [0] bad, [1] no diagonal, [2] off the diagonals
+    CLS_HIP_TRY(tmp.get(&cnt, n));
+    CLS_HIP_TRY(tmp.get(&flags, 4));
+    CLS_HIP_TRY(hipMemsetAsync(flags, 0, 4 * sizeof(int), st));
+    hipLaunchKernelGGL(tril_count_kernel, dim3(blocks256(n)), dim3(256), 0, st, n, nnzA, arp, aci, cnt, flags,
+                       flags + 1);
+    CLS_HIP_TRY(hipGetLastError());
+    if (dmalloc(&f->g_rp, static_cast<size_t>(n) + 1) != hipSuccess) return clsparseOutOfResources;
+    CLS_HIP_TRY(hipMemsetAsync(f->g_rp, 0, sizeof(uint32_t), st));
+    size_t tb = 0;
+    CLS_HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, tb, cnt, f->g_rp + 1, static_cast<int>(n), st));
+    void* tstore = nullptr;
+    CLS_HIP_TRY(tmp.get(reinterpret_cast<char**>(&tstore), tb));
+    CLS_HIP_TRY(hipcub::DeviceScan::InclusiveSum(tstore, tb, cnt, f->g_rp + 1, static_cast<int>(n), st));
+    int hflags[4] = {0, 0, 0, 0};
+    uint32_t nnzG = 0;
+    CLS_HIP_TRY(hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost, st));
+    CLS_HIP_TRY(hipMemcpyAsync(&nnzG, f->g_rp + n, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    CLS_HIP_TRY(hipStreamSynchronize(st));
+    if (hflags[0]) return clsparseInvalidValue;
+    f->nnzG = nnzG;
+    f->full_diag = hflags[1] == 0;
+
+    uint64_t *key = nullptr, *key2 = nullptr;
+    uint32_t *idx = nullptr, *idx2 = nullptr;
+    if (dmalloc(&f->g_ci, nnzG) != hipSuccess || dmalloc(&f->g_row, nnzG) != hipSuccess ||
+        dmalloc(&f->g_src, nnzG) != hipSuccess || dmalloc(&f->gt_rp, static_cast<size_t>(n) + 1) != hipSuccess ||
+        dmalloc(&f->gt_ci, nnzG) != hipSuccess || dmalloc(&f->g_tpos, nnzG) != hipSuccess)
+        return clsparseOutOfResources;
+    CLS_HIP_TRY(tmp.get(&key, nnzG));
+    CLS_HIP_TRY(tmp.get(&key2, nnzG));
+    CLS_HIP_TRY(tmp.get(&idx, nnzG));
+    CLS_HIP_TRY(tmp.get(&idx2, nnzG));
+    CLS_HIP_TRY(tmp.get(&tcnt, n));
+    hipLaunchKernelGGL(tril_fill_kernel, dim3(blocks256(n)), dim3(256), 0, st, n, arp, aci, f->g_rp, f->g_ci,
+                       f->g_row, f->g_src, key, idx);
+    // G^T: row pointer from the column counts, columns and positions from a
+    // radix sort of the (column, row) keys (rows ascend within each column).
+    CLS_HIP_TRY(hipMemsetAsync(tcnt, 0, static_cast<size_t>(n) * sizeof(uint32_t), st));
+    hipLaunchKernelGGL(column_count_kernel, dim3(blocks256(nnzG)), dim3(256), 0, st, nnzG, f->g_ci, tcnt);
+    CLS_HIP_TRY(hipGetLastError());
+    CLS_HIP_TRY(hipMemsetAsync(f->gt_rp, 0, sizeof(uint32_t), st));
+    size_t tb2 = 0;
+    CLS_HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, tb2, tcnt, f->gt_rp + 1, static_cast<int>(n), st));
+    int end_bit = 32;
+    while (end_bit < 64 && (uint64_t(1) << (end_bit - 32)) < n) ++end_bit;
+    size_t tb3 = 0;
+    CLS_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb3, key, key2, idx, idx2, static_cast<int>(nnzG), 0,
+                                                   end_bit, st));
+    void* tstore2 = nullptr;
+    CLS_HIP_TRY(tmp.get(reinterpret_cast<char**>(&tstore2), std::max(tb2, tb3)));
+    CLS_HIP_TRY(hipcub::DeviceScan::InclusiveSum(tstore2, tb2, tcnt, f->gt_rp + 1, static_cast<int>(n), st));
+    if (nnzG > 0) {
+        CLS_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tstore2, tb3, key, key2, idx, idx2, static_cast<int>(nnzG), 0,
+                                                       end_bit, st));
+        hipLaunchKernelGGL(transpose_fill_kernel, dim3(blocks256(nnzG)), dim3(256), 0, st, nnzG, key2, idx2,
+                           f->gt_ci, f->g_tpos);
+        CLS_HIP_TRY(hipGetLastError());
+    }
+
+    // Diagonal storage of the BF16 values.
+    clsparseCsrMatrix Gm;
+    clsparseInitCsrMatrix(&Gm);
+    Gm.num_rows = Gm.num_cols = n;
+    Gm.num_nonzeros = nnzG;
+    Gm.row_pointer = f->g_rp;
+    Gm.col_indices = f->g_ci;
+    DiaOffsets off{};
+    int K = 0;
+    CLS_TRY(dia_candidates(&Gm, reinterpret_cast<int32_t*>(control->red_partials), st, &off, &K, false));
+    if (K > kFsaiMaxDiags || static_cast<uint64_t>(K) * n >= (uint64_t(1) << 32) || (K > 0 && off.o[K - 1] != 0))
+        K = 0;
+    if (K > 0) {
+        if (dmalloc(&f->dia16, static_cast<size_t>(K) * n) != hipSuccess || dmalloc(&f->dia_pos, nnzG) != hipSuccess)
+            return clsparseOutOfResources;
+        CLS_HIP_TRY(hipMemsetAsync(f->dia16, 0, static_cast<size_t>(K) * n * sizeof(uint16_t), st));  // padding stays 0
+        hipLaunchKernelGGL(dia_position_kernel, dim3(blocks256(nnzG)), dim3(256), 0, st, nnzG, n, f->g_ci, f->g_row, K,
+                           off, f->dia_pos, flags + 2);
+        CLS_HIP_TRY(hipGetLastError());
+        int off_diag = 0;
+        CLS_HIP_TRY(hipMemcpyAsync(&off_diag, flags + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+        CLS_HIP_TRY(hipStreamSynchronize(st));
+        if (off_diag) {
+            (void)hipFree(f->dia16);
+            (void)hipFree(f->dia_pos);
+            f->dia16 = nullptr;
+            f->dia_pos = nullptr;
+            K = 0;
+        }
+    }
+    f->ndiag = K;
+    for (int d = 0; d < K; ++d) f->dia_off[d] = off.o[d];
+    CLS_HIP_TRY(hipStreamSynchronize(st));  // before the scratch buffers go
+    return clsparseSuccess;
+}
+
+}  // namespace
+
+// CSR metadata of G and G^T on first use (training, the FP64 solver, and the
+// FP32 solver when the factor is not in diagonal storage).
+clsparseStatus fsai_ensure_meta(clsparseAmdFsai f, clsparseControl control)
+{
+    if (f->meta_ready) return clsparseSuccess;
+    CLS_TRY(clsparseCsrMetaCreate(&f->G64, control));
+    CLS_TRY(clsparseCsrMetaCreate(&f->GT64, control));
+    f->G32.meta = f->G16.meta = f->G64.meta;
+    f->GT32.meta = f->GT16.meta = f->GT64.meta;
+    f->meta_ready = true;
+    return clsparseSuccess;
+}
+
 extern "C" clsparseStatus clsparseAmdFsaiCreate(clsparseAmdFsai* out, const clsparseCsrMatrix* A,
                                                 clsparseControl control)
 {
@@ -731,89 +991,23 @@ extern "C" clsparseStatus clsparseAmdFsaiCreate(clsparseAmdFsai* out, const clsp
     if (A->num_rows != A->num_cols) return clsparseInvalidSystemSize;
     if (A->num_rows == 0) return clsparseInvalidSize;
     const uint32_t n = A->num_rows, nnzA = A->num_nonzeros;
-    std::vector<uint32_t> rp(n + 1), ci(nnzA);
     hipStream_t st = control->stream;
-    CLS_HIP_TRY(hipMemcpyAsync(rp.data(), dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer),
-                               (n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    if (nnzA)
-        CLS_HIP_TRY(hipMemcpyAsync(ci.data(), dev_ptr<uint32_t>(A->col_indices, A->off_col_indices),
-                                   nnzA * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    CLS_HIP_TRY(hipStreamSynchronize(st));
-    for (uint32_t r = 0; r < n; ++r)
-        if (rp[r + 1] < rp[r] || rp[r + 1] > nnzA) return clsparseInvalidValue;
-
-    std::vector<uint32_t> grp(n + 1, 0), gci, grow, gsrc;
-    gci.reserve(nnzA / 2 + n);
-    bool full_diag = true;
-    for (uint32_t r = 0; r < n; ++r) {
-        bool diag = false;
-        for (uint32_t k = rp[r]; k < rp[r + 1]; ++k) {
-            const uint32_t c = ci[k];
-            if (c >= n) return clsparseInvalidValue;
-            diag = diag || (c == r);
-            if (c <= r) {
-                gci.push_back(c);
-                grow.push_back(r);
-                gsrc.push_back(k);
-            }
-        }
-        grp[r + 1] = static_cast<uint32_t>(gci.size());
-        full_diag = full_diag && diag;
-    }
-    const uint32_t nnzG = static_cast<uint32_t>(gci.size());
-    // G^T by counting sort on the column; tpos[k] = position of G's k in G^T.
-    std::vector<uint32_t> trp(n + 1, 0), tci(nnzG), tpos(nnzG);
-    for (uint32_t k = 0; k < nnzG; ++k) ++trp[gci[k] + 1];
-    for (uint32_t r = 0; r < n; ++r) trp[r + 1] += trp[r];
-    {
-        std::vector<uint32_t> fill(trp.begin(), trp.end() - 1);
-        for (uint32_t k = 0; k < nnzG; ++k) {
-            const uint32_t p = fill[gci[k]]++;
-            tci[p] = grow[k];
-            tpos[k] = p;
-        }
-    }
-
-    // Diagonal storage when the pattern lies on few diagonals: distinct
-    // offsets col - row (all <= 0), ascending; at most 1.5 nnzG slots.
-    std::vector<int64_t> offs;
-    for (uint32_t k = 0; k < nnzG && offs.size() <= static_cast<size_t>(kFsaiMaxDiags); ++k) {
-        const int64_t o = static_cast<int64_t>(gci[k]) - static_cast<int64_t>(grow[k]);
-        if (std::find(offs.begin(), offs.end(), o) == offs.end()) offs.push_back(o);
-    }
-    const bool use_dia = nnzG > 0 && offs.size() <= static_cast<size_t>(kFsaiMaxDiags) &&
-                         static_cast<double>(offs.size()) * n <= 1.5 * nnzG &&
-                         static_cast<uint64_t>(offs.size()) * n < (uint64_t(1) << 32);
-    std::vector<uint32_t> dpos;
-    if (use_dia) {
-        std::sort(offs.begin(), offs.end());
-        dpos.resize(nnzG);
-        for (uint32_t k = 0; k < nnzG; ++k) {
-            const int64_t o = static_cast<int64_t>(gci[k]) - static_cast<int64_t>(grow[k]);
-            const size_t d = static_cast<size_t>(std::lower_bound(offs.begin(), offs.end(), o) - offs.begin());
-            dpos[k] = static_cast<uint32_t>(d * n + grow[k]);
-        }
-    }
+    const uint32_t* arp = dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer);
+    const uint32_t* aci = dev_ptr<uint32_t>(A->col_indices, A->off_col_indices);
+    if (arp == nullptr || (nnzA > 0 && aci == nullptr)) return clsparseInvalidMemObject;
 
     _clsparseAmdFsai* f = new (std::nothrow) _clsparseAmdFsai;
     if (!f) return clsparseOutOfHostMemory;
     f->device = control->device;
     f->n = n;
-    f->nnzG = nnzG;
-    f->full_diag = full_diag;
+    f->nnzA = nnzA;
+    clsparseStatus s = build_pattern(f, arp, aci, n, nnzA, control);
+    if (s != clsparseSuccess) {
+        free_fsai(f);
+        return s;
+    }
+    const uint32_t nnzG = f->nnzG;
     hipError_t e = hipSuccess;
-    auto up = [&](uint32_t** d, const std::vector<uint32_t>& h) {
-        if (e == hipSuccess) e = dmalloc(d, h.size());
-        if (e == hipSuccess && !h.empty())
-            e = hipMemcpyAsync(*d, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st);
-    };
-    up(&f->g_rp, grp);
-    up(&f->g_ci, gci);
-    up(&f->gt_rp, trp);
-    up(&f->gt_ci, tci);
-    up(&f->g_row, grow);
-    up(&f->g_src, gsrc);
-    up(&f->g_tpos, tpos);
     if (e == hipSuccess) e = dmalloc(&f->dhat, n);
     if (e == hipSuccess) e = dmalloc(&f->stats, 3 * static_cast<size_t>(n));
     if (e == hipSuccess) e = dmalloc(&f->X, static_cast<size_t>(nnzG) * kFeat);
@@ -825,16 +1019,6 @@ extern "C" clsparseStatus clsparseAmdFsaiCreate(clsparseAmdFsai* out, const clsp
     if (e == hipSuccess) e = dmalloc(&f->g16, nnzG);
     if (e == hipSuccess) e = dmalloc(&f->gt16, nnzG);
     if (e == hipSuccess) e = dmalloc(&f->bad, 1);
-    if (use_dia) {
-        f->ndiag = static_cast<int>(offs.size());
-        for (int d = 0; d < f->ndiag; ++d) f->dia_off[d] = static_cast<int32_t>(offs[d]);
-        if (e == hipSuccess) e = dmalloc(&f->dia16, offs.size() * n);
-        if (e == hipSuccess)
-            e = hipMemsetAsync(f->dia16, 0, offs.size() * n * sizeof(uint16_t), st);  // padding stays 0
-        if (e == hipSuccess) e = dmalloc(&f->dia_pos, nnzG);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(f->dia_pos, dpos.data(), nnzG * sizeof(uint32_t), hipMemcpyHostToDevice, st);
-    }
     if (e != hipSuccess) {
         free_fsai(f);
         return clsparseOutOfResources;
@@ -851,20 +1035,11 @@ extern "C" clsparseStatus clsparseAmdFsaiCreate(clsparseAmdFsai* out, const clsp
     csr(f->GT64, f->gt_rp, f->gt_ci, f->gt64);
     csr(f->G32, f->g_rp, f->g_ci, f->g32);
     csr(f->GT32, f->gt_rp, f->gt_ci, f->gt32);
-    clsparseStatus s = clsparseCsrMetaCreate(&f->G64, control);
-    if (s == clsparseSuccess) s = clsparseCsrMetaCreate(&f->GT64, control);
-    if (s == clsparseSuccess) s = clsparseCsrMetaCreate(&f->G32, control);
-    if (s == clsparseSuccess) s = clsparseCsrMetaCreate(&f->GT32, control);
     f->G16 = f->G32;
     f->G16.values = f->g16;
     f->GT16 = f->GT32;
     f->GT16.values = f->gt16;
-    if (s != clsparseSuccess) {
-        free_fsai(f);
-        return s;
-    }
 
-    f->nnzA = nnzA;
     s = compute_features(f, A, st);
     if (s == clsparseSuccess) s = fsai_assemble(f, nullptr, st);
     if (s != clsparseSuccess) {
@@ -883,12 +1058,8 @@ extern "C" clsparseStatus clsparseAmdFsaiUpdateValues(clsparseAmdFsai f, const c
     if (A->num_rows != f->n || A->num_cols != f->n || A->num_nonzeros != f->nnzA)
         return clsparseInvalidSystemSize;
     // The kernels walk A's rows: the row pointer must stay inside the arrays.
-    std::vector<uint32_t> rp(f->n + 1);
-    CLS_HIP_TRY(hipMemcpyAsync(rp.data(), dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer),
-                               (f->n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, control->stream));
-    CLS_HIP_TRY(hipStreamSynchronize(control->stream));
-    for (uint32_t r = 0; r < f->n; ++r)
-        if (rp[r + 1] < rp[r] || rp[r + 1] > f->nnzA) return clsparseInvalidValue;
+    CLS_TRY(check_row_pointer(dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer), f->n, f->nnzA, f->bad,
+                              control->stream));
     CLS_TRY(compute_features(f, A, control->stream));
     return fsai_assemble(f, nullptr, control->stream);
 }
@@ -1013,6 +1184,7 @@ extern "C" clsparseStatus clsparseAmdFsaiTrainStep(clsparseAmdFsai f, clsparseAm
     if (A->num_rows != f->n || A->num_cols != f->n) return clsparseInvalidSystemSize;
     if (f->nnzG == 0) return clsparseInvalidValue;
     CLS_TRY(ensure_train_scratch(f));
+    CLS_TRY(fsai_ensure_meta(f, control));
     hipStream_t st = control->stream;
     CLS_TRY(fsai_assemble(f, net, st));
     const uint32_t n = f->n, nnzG = f->nnzG;

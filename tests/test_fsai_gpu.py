@@ -444,3 +444,77 @@ def test_diagonal_storage_solves_like_the_csr_factor(ctl, monkeypatch):
     finally:
         f.release()
         net.release()
+
+
+@pytest.mark.parametrize("dia", ["1", "0"])
+def test_factor_from_unsorted_rows_applies_like_the_sorted_one(ctl, monkeypatch, dia):
+    """The pattern of G and G^T is built on the device from A's rows in the
+    order they are stored: with the columns of every row shuffled, G keeps
+    A's order within a row, holds the same entries, and one PCG iteration
+    applies the same G G^T (diagonal storage and the CSR SpMVs, whose G^T
+    is the device-sorted transpose)."""
+    monkeypatch.setenv("CLSPARSE_AMD_FSAI_POLICY", "always")
+    monkeypatch.setenv("CLSPARSE_AMD_FSAI_DIA", dia)
+    from clsparse_amd import fsai as fs
+    params, _ = fs.load_weights()
+    rp, ci, v, m = cls.gen_diffusion3d(13, 11, 9, sigma=1.0, ax=1.0, ay=1.0, az=100.0, seed=8)
+    rng = np.random.default_rng(8)
+    perm = np.concatenate([rp[r] + rng.permutation(rp[r + 1] - rp[r]) for r in range(m)])
+    ci_s, v_s = ci[perm], v[perm]
+    A = cls.CsrMatrix(rp, ci_s, v_s, m)
+    A.meta_create(ctl)
+    net = cls.Mlp(seed=1)
+    net.set_params(params)
+    f = cls.Fsai(A, ctl)
+    try:
+        f.apply_net(net, ctl)
+        assert len(f.diagonals()) == 4
+        grp, gci, gva = f.values()
+        rows = np.repeat(np.arange(m, dtype=np.int64), np.diff(rp.astype(np.int64)))
+        lower = ci_s.astype(np.int64) <= rows
+        np.testing.assert_array_equal(gci, ci_s[lower])  # A's order within each row
+        np.testing.assert_array_equal(np.diff(grp.astype(np.int64)),
+                                      np.bincount(rows[lower], minlength=m))
+        Gs = sp.csr_matrix((gva, gci, grp), shape=(m, m))
+        Ao = cls.CsrMatrix(rp, ci, v, m)
+        fo = cls.Fsai(Ao, ctl)
+        try:
+            fo.apply_net(net, ctl)
+            Go = sp.csr_matrix(tuple(reversed(fo.values())), shape=(m, m))
+            assert abs(Gs - Go).max() == 0.0
+        finally:
+            fo.release()
+        b = cls.gen_uniform(m, -1, 1, seed=21)
+        say(f"fsai: unsorted rows, first iteration, diagonal storage={dia}")
+        st, x = _solve_fsai(ctl, A, f, b, max_iters=1)
+        G = sp.csr_matrix((of.bf16_round(gva.astype(np.float32)).astype(np.float64), gci, grp),
+                          shape=(m, m))
+        z0 = G @ (G.T @ b.astype(np.float32).astype(np.float64))
+        err = np.linalg.norm(x / np.linalg.norm(x) - z0 / np.linalg.norm(z0))
+        assert err <= 1e-5, err
+        assert st.preconditioner == 2 and st.iterations == 1
+    finally:
+        f.release()
+        net.release()
+
+
+def test_factor_rejects_bad_row_pointer_and_columns(ctl):
+    """clsparseAmdFsaiCreate checks A's structure on the device: a row
+    pointer that decreases or passes nnz, or a column index >= n, is
+    clsparseInvalidValue (the kernels read only inside the arrays)."""
+    rp, ci, v, m = cls.gen_diffusion3d(6, 5, 4, sigma=0.0, ax=1.0, ay=1.0, az=1.0, seed=1)
+    bad_rp = rp.copy()
+    bad_rp[7] = bad_rp[8] + 1  # row 6 ends before it starts
+    bad_end = rp.copy()
+    bad_end[-1] = rp[-1] + 3  # last row runs past nnz
+    bad_end[-2] = min(bad_end[-2], rp[-1])
+    bad_ci = ci.copy()
+    bad_ci[10] = m + 5
+    for k, (r, c) in enumerate([(bad_rp, ci), (bad_end, ci), (rp, bad_ci)]):
+        A = cls.CsrMatrix(r, c, v, m)
+        say(f"fsai: bad structure case {k}")
+        with pytest.raises(cls.ClsparseError):
+            cls.Fsai(A, ctl)
+    A = cls.CsrMatrix(rp, ci, v, m)
+    f = cls.Fsai(A, ctl)  # the same matrix with a valid structure
+    f.release()
