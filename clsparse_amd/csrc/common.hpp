@@ -319,6 +319,18 @@ __device__ __forceinline__ T block_sum256(T v, T* scratch)
     return r;
 }
 
+// Raw buffer load of a float or double at byte offset voff + soff of the
+// buffer rs; offsets past the buffer's size read 0 (the hardware's range
+// check), so a 32-bit offset replaces a 64-bit flat address.
+template <typename T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff)
+{
+    if constexpr (sizeof(T) == 4)
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
+    else
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
+}
+
 // XCD-aware block order: hardware hands consecutive workgroups to the 8 XCDs
 // round-robin; remap so that each XCD walks one contiguous range of logical
 // blocks (keeps neighbouring rows -- and their x gathers -- in one L2).

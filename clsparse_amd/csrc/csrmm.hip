@@ -6,11 +6,14 @@
 // element (k, c) of B at off_values + k * lead_dim + c; alpha == 0 gives
 // beta * C without reading B, beta == 0 never reads C, beta == 1 adds C.
 // The reference runs one CSR-Vector SpMV per column of C (A re-read for every
-// column). Here W lanes (the column tile, a power of two >= the columns of C up
-// to 64) form a row slot, and a wave runs 64 / W rows of C at once: each slot
-// loads W nonzeros of its row with one coalesced read, hands each nonzero's
-// value and column to its lanes through lane permutes, and keeps 8 B row
-// segments in flight per step, every one a coalesced read of W columns. Waves
+// column). Here each lane owns V consecutive columns (V = 16 bytes of them
+// when the shapes and addresses allow, else 1), W lanes (a power of two >= the
+// column groups, up to 64) form a row slot, and a wave runs 64 / W rows of C
+// at once: each slot loads W nonzeros of its row with one coalesced read,
+// hands each nonzero's value and column to its lanes through lane permutes,
+// and keeps 8 B row segments in flight per step, every one a coalesced read
+// (16-byte accesses halve the load and permute instructions per byte, which
+// is what bounds the kernel: the texture data path is busy for most of it). Waves
 // walk consecutive rows and workgroups are remapped so each XCD covers one
 // contiguous range of rows (neighbouring rows share B rows in its L2). Each
 // element of C is summed over the row's nonzeros in storage order, so results
@@ -21,9 +24,52 @@
 
 namespace {
 
-constexpr int kBInFlight = 8;
+constexpr int kBInFlight = 8;     // B row segments in flight per lane (scalar columns)
+constexpr int kBInFlightVec = 4;  // the same with 16-byte column groups (twice the bytes each)
 
-template <typename T, int W>
+// V consecutive columns of a row of B or C per lane: one 16-byte access
+// when V * sizeof(T) == 16 (V = 2 doubles or 4 floats), else V scalars.
+template <typename T, int V>
+struct Cols
+{
+    T e[V];
+};
+
+template <typename T, int V>
+__device__ __forceinline__ Cols<T, V> load_cols(const T* p)
+{
+    Cols<T, V> r;
+    if constexpr (V * sizeof(T) == 16) {
+        typedef T vec __attribute__((ext_vector_type(V)));
+        const vec x = *reinterpret_cast<const vec*>(p);
+#pragma unroll
+        for (int k = 0; k < V; ++k) r.e[k] = x[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < V; ++k) r.e[k] = p[k];
+    }
+    return r;
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void store_cols(T* p, const Cols<T, V>& r)
+{
+    if constexpr (V * sizeof(T) == 16) {
+        typedef T vec __attribute__((ext_vector_type(V)));
+        vec x;
+#pragma unroll
+        for (int k = 0; k < V; ++k) x[k] = r.e[k];
+        *reinterpret_cast<vec*>(p) = x;
+    } else {
+#pragma unroll
+        for (int k = 0; k < V; ++k) p[k] = r.e[k];
+    }
+}
+
+// W lanes per row slot, V columns per lane (the host picks V > 1 only when
+// the columns, both leading dimensions and both base addresses are multiples
+// of V elements / 16 bytes, so every access is an aligned 16-byte one).
+template <typename T, int W, int V>
 __global__ __launch_bounds__(256) void csrmm_kernel(uint32_t num_rows, uint32_t ncols,
                                                     const uint32_t* __restrict__ rp,
                                                     const uint32_t* __restrict__ ci,
@@ -33,6 +79,7 @@ __global__ __launch_bounds__(256) void csrmm_kernel(uint32_t num_rows, uint32_t 
                                                     const T* __restrict__ pBeta)
 {
     constexpr uint32_t kSlots = 64 / W;
+    constexpr int kIn = V > 1 ? kBInFlightVec : kBInFlight;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t slot = lane / W;
     const uint32_t t = lane % W;
@@ -68,10 +115,12 @@ __global__ __launch_bounds__(256) void csrmm_kernel(uint32_t num_rows, uint32_t 
             nbeg = rp[next];
             nend = rp[next + 1];
         }
-        for (uint32_t c0 = 0; c0 < ncols; c0 += W) {
-            const uint32_t c = c0 + t;
+        for (uint32_t c0 = 0; c0 < ncols; c0 += W * V) {
+            const uint32_t c = c0 + t * V;
             const bool colok = c < ncols;
-            T sum = T(0);
+            Cols<T, V> sum;
+#pragma unroll
+            for (int k = 0; k < V; ++k) sum.e[k] = T(0);
             if (alpha != T(0)) {
                 for (uint32_t j0 = beg; j0 < end; j0 += W) {
                     // Lane t of the slot holds nonzero j0 + t of the row.
@@ -83,32 +132,61 @@ __global__ __launch_bounds__(256) void csrmm_kernel(uint32_t num_rows, uint32_t 
                         cc = ci[jj];
                     }
                     const uint32_t cnt = min(static_cast<uint32_t>(W), end - j0);
-                    for (uint32_t q = 0; q < cnt; q += kBInFlight) {
-                        T a8[kBInFlight], b8[kBInFlight];
+                    for (uint32_t q = 0; q < cnt; q += kIn) {
+                        T a8[kIn];
+                        Cols<T, V> b8[kIn];
 #pragma unroll
-                        for (int u = 0; u < kBInFlight; ++u) {
+                        for (int u = 0; u < kIn; ++u) {
                             const int src = base + static_cast<int>(min(q + u, static_cast<uint32_t>(W - 1)));
                             a8[u] = __shfl(av, src, 64);
                             const uint32_t col = __shfl(cc, src, 64);
-                            b8[u] = (colok && q + u < cnt) ? B[static_cast<size_t>(col) * ldb + c] : T(0);
+#pragma unroll
+                            for (int k = 0; k < V; ++k) b8[u].e[k] = T(0);
+                            if (colok && q + u < cnt) b8[u] = load_cols<T, V>(B + static_cast<size_t>(col) * ldb + c);
                         }
 #pragma unroll
-                        for (int u = 0; u < kBInFlight; ++u)
-                            if (q + u < cnt) sum = fma(a8[u], b8[u], sum);
+                        for (int u = 0; u < kIn; ++u)
+                            if (q + u < cnt)
+#pragma unroll
+                                for (int k = 0; k < V; ++k) sum.e[k] = fma(a8[u], b8[u].e[k], sum.e[k]);
                     }
                 }
             }
             if (live && colok) {
                 T* out = C + static_cast<size_t>(row) * ldc + c;
-                if (beta == T(0))
-                    *out = sum;
-                else if (beta == T(1))
-                    *out = sum + *out;
-                else
-                    *out = sum + beta * *out;
+                if (beta != T(0)) {
+                    const Cols<T, V> old = load_cols<T, V>(out);
+#pragma unroll
+                    for (int k = 0; k < V; ++k)
+                        sum.e[k] = beta == T(1) ? sum.e[k] + old.e[k] : sum.e[k] + beta * old.e[k];
+                }
+                store_cols<T, V>(out, sum);
             }
         }
     }
+}
+
+// Launches csrmm_kernel<T, W, V> on at most as many workgroups as the chip
+// holds at once: the kernel's resident workgroups per CU (queried once) times
+// the CUs, a multiple of 8 (one equal share per XCD). The resident waves of an
+// XCD then work on one compact window of its rows that advances together, so
+// the B rows that neighbouring rows share stay in its L2.
+template <typename T, int W, int V>
+clsparseStatus launch_csrmm(uint32_t want, int num_cus, hipStream_t s, uint32_t n, uint32_t ncols,
+                            const uint32_t* rp, const uint32_t* ci, const T* v, const T* B, uint32_t ldb, T* C,
+                            uint32_t ldc, const T* pa, const T* pb)
+{
+    static const int per_cu = [] {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, csrmm_kernel<T, W, V>, 256, 0) != hipSuccess) b = 0;
+        return std::max(b, 1);
+    }();
+    const uint32_t resident = std::max<uint32_t>(8u, static_cast<uint32_t>(per_cu * std::max(num_cus, 1)) / 8 * 8);
+    const uint32_t grid = std::min<uint32_t>(resident, (want + 7) / 8 * 8);
+    hipLaunchKernelGGL((csrmm_kernel<T, W, V>), dim3(grid), dim3(256), 0, s, n, ncols, rp, ci, v, B, ldb, C, ldc,
+                       pa, pb);
+    CLS_HIP_TRY(hipGetLastError());
+    return clsparseSuccess;
 }
 
 template <typename T>
@@ -135,32 +213,37 @@ clsparseStatus csrmm_entry(const clsparseScalar* alpha, const clsparseCsrMatrix*
     T* Cp = dev_ptr<T>(C->values, C->off_values);
     if (rp == nullptr || Bp == nullptr || Cp == nullptr || pa == nullptr || pb == nullptr)
         return clsparseInvalidMemObject;
-    // At most as many waves as the chip holds at once (2048 workgroups of 4
-    // waves at 8 waves per SIMD): the resident waves of an XCD then work on one
-    // compact window of its rows that advances together, so the B rows that
-    // neighbouring rows share stay in its L2.
-    const uint32_t slots = ncols > 32 ? 1u : ncols > 16 ? 2u : ncols > 8 ? 4u : ncols > 4 ? 8u
-                                       : ncols > 2 ? 16u : ncols > 1 ? 32u : 64u;
+    // V columns per lane (one 16-byte access) when the columns, both leading
+    // dimensions and both base addresses allow it; W lanes per row slot (a
+    // power of two >= the column groups, at most 64).
+    constexpr uint32_t kVec = 16 / sizeof(T);
+    const bool vec = ncols % kVec == 0 && B->lead_dim % kVec == 0 && C->lead_dim % kVec == 0 &&
+                     reinterpret_cast<uintptr_t>(Bp) % 16 == 0 && reinterpret_cast<uintptr_t>(Cp) % 16 == 0;
+    const uint32_t groups = vec ? ncols / kVec : ncols;
+    const uint32_t W = groups > 32 ? 64u : groups > 16 ? 32u : groups > 8 ? 16u : groups > 4 ? 8u
+                                     : groups > 2 ? 4u : groups > 1 ? 2u : 1u;
+    const uint32_t slots = 64 / W;
     const uint32_t want = (n + 4 * slots - 1) / (4 * slots);
-    const uint32_t grid = std::min<uint32_t>(2048, (want + 7) / 8 * 8);  // a multiple of 8 (one share per XCD)
     hipStream_t s = control->stream;
-#define CLS_CSRMM(W)                                                                                 \
-    hipLaunchKernelGGL((csrmm_kernel<T, W>), dim3(grid), dim3(256), 0, s, n, ncols, rp, ci, v, Bp,   \
-                       B->lead_dim, Cp, C->lead_dim, pa, pb)
-    if (ncols > 32)
-        CLS_CSRMM(64);
-    else if (ncols > 16)
-        CLS_CSRMM(32);
-    else if (ncols > 8)
-        CLS_CSRMM(16);
-    else if (ncols > 4)
-        CLS_CSRMM(8);
-    else if (ncols > 2)
-        CLS_CSRMM(4);
-    else if (ncols > 1)
-        CLS_CSRMM(2);
-    else
-        CLS_CSRMM(1);
+#define CLS_CSRMM(WW, VV)                                                                                \
+    CLS_TRY((launch_csrmm<T, WW, VV>(want, control->num_cus, s, n, ncols, rp, ci, v, Bp, B->lead_dim, Cp, \
+                                     C->lead_dim, pa, pb)))
+#define CLS_CSRMM_W(VV)                  \
+    switch (W) {                         \
+    case 64: CLS_CSRMM(64, VV); break;   \
+    case 32: CLS_CSRMM(32, VV); break;   \
+    case 16: CLS_CSRMM(16, VV); break;   \
+    case 8: CLS_CSRMM(8, VV); break;     \
+    case 4: CLS_CSRMM(4, VV); break;     \
+    case 2: CLS_CSRMM(2, VV); break;     \
+    default: CLS_CSRMM(1, VV); break;    \
+    }
+    if (vec) {
+        CLS_CSRMM_W(kVec);
+    } else {
+        CLS_CSRMM_W(1);
+    }
+#undef CLS_CSRMM_W
 #undef CLS_CSRMM
     CLS_HIP_TRY(hipGetLastError());
     return finish_op(control);

@@ -101,15 +101,6 @@ __global__ __launch_bounds__(256) void dia_spmv_dot_kernel(size_t n, DiaOffsets 
 // took two vector registers per load and left two waves per SIMD; reads of p
 // outside [0, n) return 0 from the buffer's range check. Needs
 // K * n * sizeof(T) < 2^32.
-template <typename T>
-__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff)
-{
-    if constexpr (sizeof(T) == 4)
-        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
-    else
-        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
-}
-
 // The grid (at most kReducePartialsMax workgroups of 4 waves) puts 4 waves on
 // a SIMD: tell the scheduler so, or it trades the loads in flight for an
 // occupancy the launch never reaches.

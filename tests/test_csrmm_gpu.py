@@ -82,3 +82,28 @@ def test_csrmm_size_checks(ctl):
     Bd, Cd = _dense(t, 2, 2, 1), _dense(t, 2, 2, 2)  # lead_dim < columns
     assert lib.clsparseDcsrmm(ctypes.byref(s.s), ctypes.byref(A.s), ctypes.byref(Bd), ctypes.byref(s.s),
                               ctypes.byref(Cd), ctl.handle) == cls.clsparseInvalidSize
+
+
+@pytest.mark.parametrize("ncols", [1, 4, 32, 64])
+def test_csrmm_many_row_steps_matches_oracle(ctl, ncols):
+    """600k rows: every wave runs many pipelined row steps (the next step's
+    nonzeros and the bounds of the step after are loaded ahead), including
+    rows longer than the column tile, against the oracle."""
+    rp, ci, v, m = cls.gen_powerlaw(600_000, 6.0, 1.4, 3000, 4, 2500, seed=40 + ncols)
+    n = rp.size - 1
+    rng = np.random.default_rng(ncols)
+    B = rng.uniform(-1, 1, (m, ncols))
+    C0 = rng.uniform(-1, 1, (n, ncols))
+    A = cls.CsrMatrix(rp, ci, v, m)
+    Bt = torch.tensor(B.ravel(), device="cuda")
+    Ct = torch.tensor(C0.ravel(), device="cuda")
+    Bd, Cd = _dense(Bt, m, ncols, ncols), _dense(Ct, n, ncols, ncols)
+    a, b = cls.Scalar(1.5), cls.Scalar(0.5)
+    say(f"csrmm 600k rows ncols={ncols}")
+    cls.check("csrmm", cls.load().clsparseDcsrmm(ctypes.byref(a.s), ctypes.byref(A.s), ctypes.byref(Bd),
+                                                  ctypes.byref(b.s), ctypes.byref(Cd), ctl.handle))
+    got = Ct.cpu().numpy().reshape(n, ncols)
+    ref, scale = oracle.csrmm(rp, ci, v, B, C0, 1.5, 0.5)
+    lens = np.diff(rp.astype(np.int64))
+    bound = np.maximum(1e-13, (lens + 4) * np.finfo(np.float64).eps)[:, None]
+    assert np.all(np.abs(got - ref) <= bound * scale)
