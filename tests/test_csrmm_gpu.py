@@ -107,3 +107,47 @@ def test_csrmm_many_row_steps_matches_oracle(ctl, ncols):
     lens = np.diff(rp.astype(np.int64))
     bound = np.maximum(1e-13, (lens + 4) * np.finfo(np.float64).eps)[:, None]
     assert np.all(np.abs(got - ref) <= bound * scale)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("ncols,pad,off", [(4, 0, 0), (8, 4, 4), (12, 0, 4), (64, 0, 0), (96, 4, 0),
+                                           (300, 4, 4), (8, 0, 1), (64, 4, 3)])
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (-0.75, 1.0), (2.0, 0.5), (0.0, -1.5)])
+def test_csrmm_column_groups_match_oracle(ctl, dtype, ncols, pad, off, alpha, beta):
+    """Shapes whose columns, leading dimensions and base offsets are multiples
+    of 16 bytes take 16-byte column groups per lane (several column tiles for
+    300 columns); an offset of 1 or 3 elements keeps one column per lane. Both
+    against the oracle, with every alpha / beta case."""
+    rp, ci, v, m = cls.gen_powerlaw(2500, 9.0, 1.2, 1800, 2, 1200, seed=ncols + off)
+    n = rp.size - 1
+    rng = np.random.default_rng(ncols * 7 + off)
+    ldb, ldc = ncols + pad, ncols + pad
+    B = rng.uniform(-1, 1, (m, ldb)).astype(dtype)
+    C0 = rng.uniform(-1, 1, (n, ldc)).astype(dtype)
+    if beta == 0.0:
+        C0[:, :ncols] = np.nan  # beta == 0 must not read C
+    tdt = torch.float32 if dtype == np.float32 else torch.float64
+    A = cls.CsrMatrix(rp, ci, v.astype(dtype), m, dtype=dtype)
+    lead = np.zeros(off, dtype=dtype)
+    Bt = torch.tensor(np.concatenate([lead, B.ravel()]), device="cuda")
+    Ct = torch.tensor(np.concatenate([lead, C0.ravel()]), device="cuda")
+    Bd = _dense(Bt, m, ncols, ldb, off)
+    Cd = _dense(Ct, n, ncols, ldc, off)
+    a, b = cls.Scalar(alpha, tdt), cls.Scalar(beta, tdt)
+    lib = cls.load()
+    fn = lib.clsparseScsrmm if dtype == np.float32 else lib.clsparseDcsrmm
+    say(f"csrmm groups {np.dtype(dtype).name} ncols={ncols} pad={pad} off={off} alpha={alpha} beta={beta}")
+    cls.check("csrmm", fn(ctypes.byref(a.s), ctypes.byref(A.s), ctypes.byref(Bd), ctypes.byref(b.s),
+                          ctypes.byref(Cd), ctl.handle))
+    C = Ct.cpu().numpy()[off:].reshape(n, ldc).astype(np.float64)
+    Cin = np.where(np.isnan(C0), 0, C0)[:, :ncols].astype(np.float64)
+    ref, scale = oracle.csrmm(rp, ci, v.astype(dtype).astype(np.float64),
+                              B[:, :ncols].astype(np.float64), Cin, alpha, beta)
+    got = C[:, :ncols]
+    zero = scale == 0
+    assert np.all(got[zero] == 0)
+    lens = np.diff(rp.astype(np.int64))
+    eps = np.finfo(dtype).eps
+    bound = np.maximum(1e-6 if dtype == np.float32 else 1e-13, (lens + 4) * eps)[:, None].repeat(ncols, 1)
+    assert np.all(np.abs(got - ref)[~zero] <= bound[~zero] * scale[~zero])
+    np.testing.assert_array_equal(C[:, ncols:], C0[:, ncols:].astype(np.float64))
