@@ -990,6 +990,8 @@ extern "C" clsparseStatus clsparseAmdFsaiCreate(clsparseAmdFsai* out, const clsp
     if (out == nullptr || A == nullptr) return clsparseInvalidValue;
     if (A->num_rows != A->num_cols) return clsparseInvalidSystemSize;
     if (A->num_rows == 0) return clsparseInvalidSize;
+    // The device scans and sorts of the pattern count items in int.
+    if (A->num_rows > 0x7FFFFFFFu || A->num_nonzeros > 0x7FFFFFFFu) return clsparseInvalidSize;
     const uint32_t n = A->num_rows, nnzA = A->num_nonzeros;
     hipStream_t st = control->stream;
     const uint32_t* arp = dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer);

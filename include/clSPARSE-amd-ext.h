@@ -110,7 +110,9 @@ CLSPARSE_EXPORT clsparseStatus clsparseAmdMlpGetGrad(clsparseAmdMlp net, cl_floa
 
 /* Builds G's pattern (and that of G^T) from a square FP64 matrix A with
  * metadata-free CSR input, the diagonal scaling and the per-nonzero features
- * (BF16, 16 per nonzero of G). G starts as Jacobi. */
+ * (BF16, 16 per nonzero of G), all on the device. G starts as Jacobi.
+ * clsparseInvalidValue for a row pointer that decreases or passes nnz, or a
+ * column index >= n; clsparseInvalidSize for n or nnz >= 2^31. */
 CLSPARSE_EXPORT clsparseStatus
     clsparseAmdFsaiCreate(clsparseAmdFsai* fsai, const clsparseCsrMatrix* A, clsparseControl control);
 /* Recomputes scaling and features for new values of A on the same pattern. */
