@@ -509,6 +509,10 @@ def fsai_workload(cls, torch, ctl, n1d=160, tol=1e-8, weights=None, cpu=True):
                     "a_diagonals": s.a_diagonals, "g_diagonals": s.g_diagonals}
             if first is None:
                 first = {"seconds": best["seconds"], "iterations": best["iterations"]}
+                if with_fsai and s.pilot_rate_fsai > 0:
+                    # the first solve's pilot race: -ln(residual ratio) per second
+                    first["pilot_rates"] = {"fsai": round(s.pilot_rate_fsai, 1),
+                                            "jacobi": round(s.pilot_rate_jacobi, 1)}
         best["first_solve"] = first
         return best
 
@@ -573,7 +577,8 @@ def fsai_sweep(cls, torch, ctl, net, tol):
             stt = sc.stats()
             sc.release()
             cls.check("solve", st)
-            return dt, stt.iterations, stt.preconditioner, stt.status_detail == 0
+            return (dt, stt.iterations, stt.preconditioner, stt.status_detail == 0,
+                    (round(stt.pilot_rate_fsai, 1), round(stt.pilot_rate_jacobi, 1)))
 
         first = solve(lib.clsparseAmdDcsrcgFsai, True)
         rep_ = solve(lib.clsparseAmdDcsrcgFsai, True)
@@ -585,7 +590,8 @@ def fsai_sweep(cls, torch, ctl, net, tol):
                     "fp64_jacobi_s": round(base[0], 5), "fp64_jacobi_iterations": base[1],
                     "converged": rep_[3] and base[3],
                     "speedup": round(base[0] / rep_[0], 3),
-                    "speedup_first_solve": round(base[0] / first[0], 3)})
+                    "speedup_first_solve": round(base[0] / first[0], 3),
+                    "pilot_rates_fsai_jacobi": first[4]})
         log(f"held-out sweep: {res[-1]}")
         f.release()
         A.meta_delete()

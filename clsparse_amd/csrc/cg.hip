@@ -891,6 +891,22 @@ namespace {
 constexpr int kStagnationChunks = 16;  // learned FSAI must halve ||r|| within this many chunks
 constexpr int kPilotChunks = 2;        // chunks per preconditioner in the pilot race
 
+// Jacobi wins the pilot race only when its rate beats the learned factor's by
+// this factor (CLSPARSE_AMD_RACE_MARGIN overrides): over the pilot's window
+// (iterations 33-64) Jacobi's residual falls faster than over a whole solve
+// -- its first iterations damp the components it handles well -- while the
+// learned factor's rate there is about its whole-solve rate (160^3
+// anisotropic problem: Jacobi 0.0228 per iteration in the window, 0.0183
+// over the solve; learned factor 0.0322 and 0.033). A factor that is exactly
+// Jacobi but dearer per iteration (a zero network) still loses: its rate is
+// below Jacobi's by the cost ratio, about 1.27.
+double race_margin()
+{
+    const char* e = std::getenv("CLSPARSE_AMD_RACE_MARGIN");
+    const double m = e ? std::atof(e) : 1.15;
+    return m > 0 ? m : 1.15;
+}
+
 // CLSPARSE_AMD_FSAI_POLICY=always keeps the learned factor unless it breaks
 // down or stagnates; the default ("race") also runs the pilot race.
 bool race_enabled()
@@ -907,6 +923,18 @@ __global__ __launch_bounds__(256) void rr_cast(size_t n, const double* __restric
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
     for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
         r[i] = static_cast<T>(r64[i]);
+}
+
+// x += (double) xd; xd = 0: folds the low-precision solution increment of
+// the inner iterations into the FP64 solution.
+template <typename T>
+__global__ __launch_bounds__(256) void flush_increment(size_t n, double* __restrict__ x, T* __restrict__ xd)
+{
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
+        x[i] += static_cast<double>(xd[i]);
+        xd[i] = T(0);
+    }
 }
 
 // s = <r', z'> / <r, z>;  p = z' + s (p - z);  z = z'.
@@ -979,11 +1007,15 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     CLS_TRY(enter_solver_stream(control));
     CLS_TRY(dia_candidates(A, reinterpret_cast<int32_t*>(control->red_partials), s, &a_off, &a_ndiag));
 
-    // Workspace: TI values of A (mixed only), FP64 residual, FP64 copy of x
-    // (restored if a preconditioner breaks down), scalars, TI start residual,
-    // then the CG buffers; with a factor also the pilot race's start point
-    // and the learned factor's pilot end point.
-    const size_t bytes = (kMixed ? align_up(nnz * sizeof(TI)) : 0) + 2 * align_up(n * sizeof(double)) +
+    // Workspace: TI values of A (mixed only), FP64 residual, the solution
+    // increment since the last residual replacement (mixed: an FP32 vector
+    // folded into the FP64 x at every true residual, so the inner iterations
+    // move 4-byte x values; FP64: a copy of x restored if a preconditioner
+    // breaks down), scalars, TI start residual, then the CG buffers; with a
+    // factor also the pilot race's start point and the learned factor's
+    // pilot end point.
+    const size_t bytes = (kMixed ? align_up(nnz * sizeof(TI)) : 0) + align_up(n * sizeof(double)) +
+                         align_up(n * (kMixed ? sizeof(TI) : sizeof(double))) +
                          align_up(8 * sizeof(double)) + align_up(n * sizeof(TI)) +
                          CgBuffers<TI>::bytes(n) + (fsai ? 2 * align_up(n * sizeof(double)) : 0) +
                          (a_ndiag ? align_up(a_ndiag * n * sizeof(TI)) + align_up(sizeof(int)) : 0);
@@ -995,7 +1027,8 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     Carve c{static_cast<char*>(work)};
     TI* valsI = kMixed ? c.take<TI>(nnz) : nullptr;
     double* r64 = c.take<double>(n);
-    double* xsave = c.take<double>(n);
+    TI* xd = kMixed ? c.take<TI>(n) : nullptr;              // mixed: increment since the last flush
+    double* xsave = kMixed ? nullptr : c.take<double>(n);  // FP64: last replacement point
     double* sc64 = c.take<double>(8);  // [0]=||r||_2, [1]=||b||_2, [2]=-1, [3]=+1
     TI* rI = c.take<TI>(n);            // TI start residual; FSAI temporary afterwards
     CgBuffers<TI> buf;
@@ -1074,10 +1107,37 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     }
     hipLaunchKernelGGL((fill_kernel<TI>), dim3(1), dim3(256), 0, s, size_t(1), buf.one, TI(1));
     hipLaunchKernelGGL((fill_kernel<TI>), dim3(1), dim3(256), 0, s, size_t(1), buf.zero, TI(0));
+    // The increment starts at zero (every true residual below folds it in).
+    if constexpr (kMixed) hipLaunchKernelGGL((fill_kernel<TI>), dim3(ew_grid(n)), dim3(256), 0, s, n, xd, TI(0));
     CLS_HIP_TRY(hipGetLastError());
 
-    // r64 = b - A x; ||r64||_2 -> sc64[0]; returns the true relative residual.
+    // Mixed: x += xd, xd = 0 (flush) or xd = 0 (discard, back to the last
+    // flush point); FP64 engines update x itself.
+    auto flush = [&]() -> clsparseStatus {
+        if constexpr (kMixed) {
+            hipLaunchKernelGGL((flush_increment<TI>), dim3(ew_grid(n)), dim3(256), 0, s, n, xp, xd);
+            CLS_HIP_TRY(hipGetLastError());
+        }
+        return clsparseSuccess;
+    };
+    auto discard = [&]() -> clsparseStatus {
+        if constexpr (kMixed) {
+            hipLaunchKernelGGL((fill_kernel<TI>), dim3(ew_grid(n)), dim3(256), 0, s, n, xd, TI(0));
+            CLS_HIP_TRY(hipGetLastError());
+        }
+        return clsparseSuccess;
+    };
+    // Back to the last replacement point after a breakdown.
+    auto restore = [&]() -> clsparseStatus {
+        if constexpr (kMixed)
+            return discard();
+        else
+            CLS_HIP_TRY(hipMemcpyAsync(xp, xsave, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        return clsparseSuccess;
+    };
+    // x += xd; r64 = b - A x; ||r64||_2 -> sc64[0]; returns the true relative residual.
     auto true_residual = [&](double* rel) -> clsparseStatus {
+        CLS_TRY(flush());
         CLS_HIP_TRY(hipMemcpyAsync(r64, bp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         CLS_TRY(launch_csrmv<double>(sc64 + 2, A, xp, sc64 + 3, r64, s, control->extended_precision,
                                      nullptr));
@@ -1100,11 +1160,18 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     *hs = init;
     CLS_HIP_TRY(hipMemcpyAsync(buf.st, hs, sizeof(CgDevState), hipMemcpyHostToDevice, s));
 
-    CgEngine<TI, double> eng_main(&AI, xp, rI, n, prec, kCritL2, buf, s, false, G, GT, g_bf16,
-                                  have_dia ? &dia : nullptr, &adia);
-    CgEngine<TI, double> eng_jac(&AI, xp, rI, n, kPrecJacobi, kCritL2, buf, s, false, nullptr,
-                                 nullptr, false, nullptr, &adia);
-    CgEngine<TI, double>* eng = &eng_main;
+    using Engine = CgEngine<TI, std::conditional_t<kMixed, TI, double>>;
+    auto* xe = [&] {
+        if constexpr (kMixed)
+            return xd;
+        else
+            return xp;
+    }();
+    Engine eng_main(&AI, xe, rI, n, prec, kCritL2, buf, s, false, G, GT, g_bf16, have_dia ? &dia : nullptr,
+                    &adia);
+    Engine eng_jac(&AI, xe, rI, n, kPrecJacobi, kCritL2, buf, s, false, nullptr, nullptr, false, nullptr,
+                   &adia);
+    Engine* eng = &eng_main;
     // A pilot race already run for this factor decides without a new one
     // (the factor object forgets it when G changes).
     int race_known = 0;
@@ -1137,7 +1204,8 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     if (rel <= sc->relativeTolerance) {
         detail = 0;
     } else {
-        CLS_HIP_TRY(hipMemcpyAsync(xsave, xp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        if constexpr (!kMixed)
+            CLS_HIP_TRY(hipMemcpyAsync(xsave, xp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         CLS_TRY(restart());
         double res_at_replace = rel;
         double best_res = rel;
@@ -1148,8 +1216,9 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
         // kept, x goes back to the start for Jacobi); the rate
         // -ln(res_end / res_start) / seconds is measured over all chunks but
         // the first of each (which captures the graph), i.e. over the second
-        // chunk of 32 iterations. The faster one goes on
-        // from its own end point. (Jacobi started where FSAI stopped would
+        // chunk of 32 iterations. The learned factor goes on from its own
+        // end point unless Jacobi's rate beats it by race_margin(), then Jacobi
+        // goes on from its own. (Jacobi started where FSAI stopped would
         // meet a residual whose easily damped components are already gone.)
         enum Phase { kPilotFsai, kPilotJacobi, kMain };
         Phase phase = (eng->prec() == kPrecFsai && race_enabled() && race_known == 0) ? kPilotFsai : kMain;
@@ -1160,7 +1229,7 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
         auto phase_t0 = std::chrono::steady_clock::now();
         // Switches to engine e from the current x (true residual, restart);
         // *over is set when the solve has ended (converged or out of iterations).
-        auto switch_to = [&](CgEngine<TI, double>* e, bool* over) -> clsparseStatus {
+        auto switch_to = [&](Engine* e, bool* over) -> clsparseStatus {
             eng = e;
             CLS_TRY(true_residual(&rel));
             *over = true;
@@ -1198,15 +1267,18 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
                         rate_fsai = rate;
                         phase = kPilotJacobi;
                         used = 3;
+                        CLS_TRY(flush());  // the learned factor's end point, whole
                         CLS_HIP_TRY(hipMemcpyAsync(xpilot, xp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
                         CLS_HIP_TRY(hipMemcpyAsync(xp, xstart, n * sizeof(double), hipMemcpyDeviceToDevice, s));
                         CLS_TRY(switch_to(&eng_jac, &over));
                     } else {
                         rate_jac = rate;
                         phase = kMain;
-                        fsai_set_race(fsai, rate_fsai > rate_jac ? 1 : 2, rate_fsai, rate_jac);
-                        if (rate_fsai > rate_jac) {
+                        const bool keep = rate_jac <= race_margin() * rate_fsai;
+                        fsai_set_race(fsai, keep ? 1 : 2, rate_fsai, rate_jac);
+                        if (keep) {
                             used = 2;
+                            CLS_TRY(discard());  // Jacobi's progress is dropped
                             CLS_HIP_TRY(hipMemcpyAsync(xp, xpilot, n * sizeof(double), hipMemcpyDeviceToDevice, s));
                             CLS_TRY(switch_to(&eng_main, &over));
                         }
@@ -1227,9 +1299,7 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
                     ++chunks_since_best;
                 }
                 if (!finite || chunks_since_best >= kStagnationChunks) {
-                    if (!finite)
-                        CLS_HIP_TRY(hipMemcpyAsync(xp, xsave, n * sizeof(double),
-                                                   hipMemcpyDeviceToDevice, s));
+                    if (!finite) CLS_TRY(restore());
                     used = 3;
                     phase = kMain;
                     bool over = false;
@@ -1238,7 +1308,10 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
                     continue;
                 }
             }
-            if (!finite) break;  // Jacobi or none broke down: stop with the last true residual
+            if (!finite) {  // Jacobi or none broke down: stop at the last replacement point
+                CLS_TRY(restore());
+                break;
+            }
             const bool out_of_iters = hs->done && !hs->converged;
             if (!hs->done && cur > delta * res_at_replace) continue;
 
@@ -1250,7 +1323,8 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
                 break;
             }
             if (out_of_iters || iters >= sc->maxIters) break;
-            CLS_HIP_TRY(hipMemcpyAsync(xsave, xp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            if constexpr (!kMixed)
+                CLS_HIP_TRY(hipMemcpyAsync(xsave, xp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
             // r = (TI) r64; z' = M r into y; <r, z'> partials -> part_yp;
             // re-base p and z; the new <r, z> becomes the previous one.
             CLS_TRY(eng->materialize_z(rI));

@@ -155,8 +155,9 @@ CLSPARSE_EXPORT clsparseStatus
  * turns non-finite (x goes back to the last replacement point); when the
  * residual fails to halve within 16 chunks of 32 iterations; or when a pilot
  * race -- 2 chunks with G, then 2 with Jacobi from the same start, each
- * timed -- finds Jacobi
- * reducing the residual faster per second (CLSPARSE_AMD_FSAI_POLICY=always
+ * timed -- finds Jacobi reducing the residual more than 1.15 times faster per
+ * second (CLSPARSE_AMD_RACE_MARGIN sets the factor: Jacobi's first
+ * iterations run ahead of its whole-solve rate; CLSPARSE_AMD_FSAI_POLICY=always
  * skips the race). The race's outcome is kept with the factor and reused by
  * later solves until the factor changes (clsparseAmdFsaiApplyNet,
  * clsparseAmdFsaiUpdateValues). The FP64 true residual decides convergence;

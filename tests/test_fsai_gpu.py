@@ -275,7 +275,8 @@ def test_pilot_race_picks_a_preconditioner_and_converges(ctl, monkeypatch):
         raced = st.iterations > 3 * 32
         if raced:
             assert st.pilot_rate_fsai > 0 and st.pilot_rate_jacobi > 0
-            assert (st.preconditioner == 2) == (st.pilot_rate_fsai > st.pilot_rate_jacobi)
+            # Jacobi must beat the learned factor by the race margin (1.15)
+            assert (st.preconditioner == 2) == (st.pilot_rate_jacobi <= 1.15 * st.pilot_rate_fsai)
         Acpu = sp.csr_matrix((v, ci, rp), shape=(m, m))
         assert np.linalg.norm(b - Acpu @ x) / np.linalg.norm(b) <= 1e-8
         # a second solve with the same factor reuses the outcome without racing
@@ -291,6 +292,29 @@ def test_pilot_race_picks_a_preconditioner_and_converges(ctl, monkeypatch):
         st3, _ = _solve_fsai(ctl, A, f, b)
         if st3.iterations > 3 * 32:
             assert st3.pilot_rate_jacobi > 0 and st3.pilot_rate_jacobi != st.pilot_rate_jacobi
+    finally:
+        f.release()
+
+
+@pytest.mark.parametrize("margin,expect", [("1e-6", 3), ("1e6", 2)])
+def test_race_margin_decides_the_pilot_race(ctl, monkeypatch, margin, expect):
+    """CLSPARSE_AMD_RACE_MARGIN scales Jacobi's required lead: a tiny margin
+    hands any raced solve to Jacobi, a huge one keeps the learned factor;
+    both converge."""
+    monkeypatch.delenv("CLSPARSE_AMD_FSAI_POLICY", raising=False)
+    monkeypatch.setenv("CLSPARSE_AMD_RACE_MARGIN", margin)
+    rp, ci, v, m, A = _sample_matrix(ctl, 64, 64, seed=15)
+    b = cls.gen_uniform(m, -1, 1, seed=8)
+    f = cls.Fsai(A, ctl)
+    try:
+        say(f"fsai: race margin {margin}")
+        st, x = _solve_fsai(ctl, A, f, b)
+        assert st.status_detail == 0
+        if st.iterations > 3 * 32:  # the race ran to its end
+            assert st.pilot_rate_fsai > 0 and st.pilot_rate_jacobi > 0
+            assert st.preconditioner == expect
+        Acpu = sp.csr_matrix((v, ci, rp), shape=(m, m))
+        assert np.linalg.norm(b - Acpu @ x) / np.linalg.norm(b) <= 1e-8
     finally:
         f.release()
 
