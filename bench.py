@@ -433,8 +433,56 @@ def solver_workload(cls, torch, ctl, n1d=128, tol=1e-10, cpu=True):
                                              out["mixed_fp32_inner"]["seconds"], 3)
     if cpu:
         out["cpu_baseline"] = cpu_cg_baseline(rp, ci, v, b, out["fp64_jacobi_pcg"]["iterations"])
+    out["reference_api_cg"] = reference_api_cg(cls, torch, ctl, rp, ci, v, b, m)
     A.meta_delete()
     return out
+
+
+def reference_api_cg(cls, torch, ctl, rp, ci, v, b, m):
+    """The drop-in clsparseDcsrcg / clsparseScsrcg (the reference's CG API:
+    Jacobi, L1 relative residual, x0 = 0) on the same matrix, with A in
+    diagonal storage (default) and in CSR (CLSPARSE_AMD_DIA=0). Bytes of one
+    iteration as executed from diagonal storage: SpMV 4 lower diagonals + p +
+    y, update 8 vectors, p update 3 vectors (S = sizeof(T) per value)."""
+    import numpy as np
+    lib = cls.load()
+    res = {}
+    for name, dtype, fn, tol in (("fp64", np.float64, lib.clsparseDcsrcg, 1e-8),
+                                 ("fp32", np.float32, lib.clsparseScsrcg, 1e-6)):
+        tdt = torch.float64 if dtype == np.float64 else torch.float32
+        A = cls.CsrMatrix(rp, ci, v.astype(dtype), m, dtype=dtype)
+        A.meta_create(ctl)
+        bt = torch.tensor(b.astype(dtype), device="cuda")
+
+        def run():
+            best = None
+            for _ in range(2):  # the second solve is reported
+                x = torch.zeros(m, dtype=tdt, device="cuda")
+                sc = cls.SolverControl(cls.DIAGONAL, 20000, tol, 0.0)
+                xd, bd = cls.DenseVector(x), cls.DenseVector(bt)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                st = fn(ctypes.byref(xd.s), ctypes.byref(A.s), ctypes.byref(bd.s), sc.handle, ctl.handle)
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                s = sc.stats()
+                sc.release()
+                cls.check(name, st)
+                it = s.iterations
+                best = {"seconds": round(dt, 5), "iterations": it, "l1_residual": s.l1_residual,
+                        "us_per_iteration": round(dt / it * 1e6, 2), "a_diagonals": s.a_diagonals}
+            return best
+        S = np.dtype(dtype).itemsize
+        r = run()
+        r["bytes_per_iteration"] = (4 * S + 2 * S + 8 * S + 3 * S) * m
+        r["GBps"] = round(r["bytes_per_iteration"] * r["iterations"] / r["seconds"] / 1e9, 1)
+        with csr_only():
+            r["csr"] = run()
+        r["tolerance_l1_relative"] = tol
+        res[name] = r
+        log(f"reference-API CG {name}: {r}")
+        A.meta_delete()
+    return res
 
 
 class csr_only:
