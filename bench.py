@@ -434,8 +434,38 @@ def solver_workload(cls, torch, ctl, n1d=128, tol=1e-10, cpu=True):
     if cpu:
         out["cpu_baseline"] = cpu_cg_baseline(rp, ci, v, b, out["fp64_jacobi_pcg"]["iterations"])
     out["reference_api_cg"] = reference_api_cg(cls, torch, ctl, rp, ci, v, b, m)
+    out["blas1_fp64"] = blas1_workload(cls, torch, ctl, ctl.stream)
     A.meta_delete()
     return out
+
+
+def blas1_workload(cls, torch, ctl, stream, n=1 << 26):
+    """The reference's dense BLAS-1 entry points on FP64 vectors of 2^26
+    values (512 MiB each): median call time and the rate of the bytes each
+    one must move (reads of x / y, the write of r)."""
+    lib = cls.load()
+    x = torch.rand(n, dtype=torch.float64, device="cuda")
+    y = torch.rand(n, dtype=torch.float64, device="cuda")
+    r = torch.empty_like(x)
+    xv, yv, rv = cls.DenseVector(x), cls.DenseVector(y), cls.DenseVector(r)
+    a, b, out = cls.Scalar(0.5), cls.Scalar(-2.0), cls.Scalar(0.0)
+    B = ctypes.byref
+    ops = {
+        "dot": (lambda: lib.cldenseDdot(B(out.s), B(xv.s), B(yv.s), ctl.handle), 16),
+        "nrm1": (lambda: lib.cldenseDnrm1(B(out.s), B(xv.s), ctl.handle), 8),
+        "nrm2": (lambda: lib.cldenseDnrm2(B(out.s), B(xv.s), ctl.handle), 8),
+        "axpy": (lambda: lib.cldenseDaxpy(B(rv.s), B(a.s), B(xv.s), B(yv.s), ctl.handle), 24),
+        "axpby": (lambda: lib.cldenseDaxpby(B(rv.s), B(a.s), B(xv.s), B(b.s), B(yv.s), ctl.handle), 24),
+        "scale": (lambda: lib.cldenseDscale(B(rv.s), B(a.s), B(xv.s), ctl.handle), 16),
+    }
+    res = {"n": n}
+    for name, (fn, per) in ops.items():
+        def call(fn=fn, name=name):
+            cls.check(f"cldenseD{name}", fn())
+        ms = median_call_ms(torch, stream, call, calls=10)
+        res[name] = {"us": round(ms * 1e3, 1), "GBps": round(per * n / (ms * 1e-3) / 1e9, 1)}
+    log(f"BLAS-1: {res}")
+    return res
 
 
 def reference_api_cg(cls, torch, ctl, rp, ci, v, b, m):
