@@ -46,11 +46,27 @@ typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // FP32 -> BF16, round to nearest even (finite inputs).
+// FP32 -> BF16, round to nearest even, in integer arithmetic. (A native
+// __bf16 conversion of a float that was just rounded from a double may be
+// compiled as one rounding of the double, which differs from the two the
+// features are defined with.)
 __device__ __forceinline__ uint16_t f2bf(float f)
 {
     uint32_t u = __float_as_uint(f);
     u += 0x7FFFu + ((u >> 16) & 1u);
     return static_cast<uint16_t>(u >> 16);
+}
+// Two network activations (FP32 MFMA results) -> BF16 with gfx950's
+// v_cvt_pk_bf16_f32: the same rounding as f2bf for every non-NaN float
+// (checked bit for bit on 4 M values: ties, denormals, random bit patterns),
+// one instruction for two values where f2bf takes four per value -- the
+// activation code is VALU-bound.
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t f2bf2(float lo, float hi)
+{
+    const bf16x2_t v = __builtin_convertvector(f32x2_t{lo, hi}, bf16x2_t);
+    return __builtin_bit_cast(uint32_t, v);
 }
 __device__ __forceinline__ float bf2f(uint16_t h)
 {
@@ -75,8 +91,12 @@ __device__ __forceinline__ bf16x4 load_bf16x4(const uint16_t* p)
 
 __device__ __forceinline__ bf16x4 pack_bf16x4(f32x4 v)
 {
+    const uint32_t lo = f2bf2(v[0], v[1]), hi = f2bf2(v[2], v[3]);
     bf16x4 r;
-    for (int e = 0; e < 4; ++e) r[e] = static_cast<short>(f2bf(v[e]));
+    r[0] = static_cast<short>(lo & 0xFFFFu);
+    r[1] = static_cast<short>(lo >> 16);
+    r[2] = static_cast<short>(hi & 0xFFFFu);
+    r[3] = static_cast<short>(hi >> 16);
     return r;
 }
 
@@ -162,9 +182,12 @@ __device__ __forceinline__ float tile_output(const NetRegs& w, const TileFwd& t)
 {
     float s = 0.f;
     for (int m = 0; m < 4; ++m)
-        for (int e = 0; e < 4; ++e) {
-            const float h = t.a2[m][e] > 0.f ? bf2f(f2bf(t.a2[m][e])) : 0.f;
-            s += w.w3[4 * m + e] * h;
+        for (int e = 0; e < 4; e += 2) {
+            const uint32_t pk = f2bf2(t.a2[m][e], t.a2[m][e + 1]);
+            const float h0 = t.a2[m][e] > 0.f ? bf2f(static_cast<uint16_t>(pk & 0xFFFFu)) : 0.f;
+            const float h1 = t.a2[m][e + 1] > 0.f ? bf2f(static_cast<uint16_t>(pk >> 16)) : 0.f;
+            s += w.w3[4 * m + e] * h0;
+            s += w.w3[4 * m + e + 1] * h1;
         }
     s += __shfl_xor(s, 16, 64);
     s += __shfl_xor(s, 32, 64);
@@ -235,13 +258,17 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_backward_kernel(MlpDev net
         // D2^T = g * w3 * relu'(a2); dw3 += g * h2
         f32x4 d2[4];
         for (int m = 0; m < 4; ++m)
-            for (int e = 0; e < 4; ++e) {
-                const bool on = t.a2[m][e] > 0.f;
-                const float h2 = on ? bf2f(f2bf(t.a2[m][e])) : 0.f;
-                dw3[4 * m + e] += g * h2;
-                const float d = on ? g * w.w3[4 * m + e] : 0.f;
-                d2[m][e] = d;
-                db2[4 * m + e] += d;
+            for (int e0 = 0; e0 < 4; e0 += 2) {
+                const uint32_t pk = f2bf2(t.a2[m][e0], t.a2[m][e0 + 1]);
+                for (int k = 0; k < 2; ++k) {
+                    const int e = e0 + k;
+                    const bool on = t.a2[m][e] > 0.f;
+                    const float h2 = on ? bf2f(static_cast<uint16_t>(k ? pk >> 16 : pk & 0xFFFFu)) : 0.f;
+                    dw3[4 * m + e] += g * h2;
+                    const float d = on ? g * w.w3[4 * m + e] : 0.f;
+                    d2[m][e] = d;
+                    db2[4 * m + e] += d;
+                }
             }
         bf16x4 d2b[4];
         for (int m = 0; m < 4; ++m) d2b[m] = pack_bf16x4(d2[m]);
