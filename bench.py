@@ -261,6 +261,29 @@ def next_rows(cls, torch, ctl, stream, cpu=True):
                                "projected_seconds": round(per * best[1], 3),
                                "sample": f"{k} iterations of the oracle's FP64 BiCGStab on the same "
                                          "matrix, single thread, projected"}
+    # clsparseScsrbicgStab on the same system in FP32, to 1e-6.
+    A32 = cls.CsrMatrix(rpb, cib, vb.astype(np.float32), m, dtype=np.float32)
+    A32.meta_create(ctl)
+    bt32 = bt.float()
+    best32 = None
+    for _ in range(2):
+        xt32 = torch.zeros(m, dtype=torch.float32, device="cuda")
+        sc = cls.SolverControl(cls.DIAGONAL, 20000, 1e-6, 0.0)
+        xd, bd = cls.DenseVector(xt32), cls.DenseVector(bt32)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = lib.clsparseScsrbicgStab(ctypes.byref(xd.s), ctypes.byref(A32.s), ctypes.byref(bd.s),
+                                      sc.handle, ctl.handle)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        stats = sc.stats()
+        sc.release()
+        cls.check("clsparseScsrbicgStab", st)
+        best32 = {"seconds": round(dt, 5), "iterations": stats.n_iters + 1,
+                  "l1_residual": stats.current_residual, "tolerance_l1_relative": 1e-6,
+                  "a_diagonals": stats.a_diagonals}
+    res["fp32"] = best32
+    A32.meta_delete()
     out["bicgstab"] = res
     log(f"bicgstab: {res}")
     A.meta_delete()
@@ -306,9 +329,23 @@ def next_rows(cls, torch, ctl, stream, cpu=True):
         res["cpu_baseline"] = {"kind": "port", "cores": 1, "us": round((time.perf_counter() - t0) * 1e6, 1),
                                "sample": "one full call of the oracle's csrmm (long-double sums), "
                                          "single thread"}
+    # The same product in FP32 (4 columns per lane in 16-byte groups).
+    A32 = cls.CsrMatrix(rp, ci, v.astype(np.float32), m, dtype=np.float32)
+    B32, C32 = B.float(), torch.zeros(m * k, dtype=torch.float32, device="cuda")
+    Bd32, Cd32 = dense(B32, m, k), dense(C32, m, k)
+    one32, zero32 = cls.Scalar(1.0, torch.float32), cls.Scalar(0.0, torch.float32)
+
+    def call32():
+        cls.check("clsparseScsrmm", lib.clsparseScsrmm(ctypes.byref(one32.s), ctypes.byref(A32.s),
+                                                       ctypes.byref(Bd32), ctypes.byref(zero32.s),
+                                                       ctypes.byref(Cd32), ctl.handle))
+    ms32 = median_call_ms(torch, stream, call32)
+    nb32 = 8 * nnz + 4 * (m + 1) + 2 * 4 * m * k
+    res["fp32"] = {"us": round(ms32 * 1e3, 2), "bytes": nb32, "GBps": round(nb32 / (ms32 * 1e-3) / 1e9, 1),
+                   "GFLOPs": round(2 * nnz * k / (ms32 * 1e-3) / 1e9, 1)}
     out["csrmm"] = res
     log(f"csrmm: {res}")
-    del B, C
+    del B, C, B32, C32
 
     # Matrix Market: power-law 2^16 x 2^16 written with 17 significant digits.
     import tempfile
