@@ -784,7 +784,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the path is single-GPU; sharding is described in DESIGN.md)", file=sys.stderr)
+              "(one process drives one GPU)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
