@@ -783,7 +783,7 @@ def main():
                     help="locality mode: power-law column windows (bits) and stencils (s64, s32)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(one process drives one GPU)", file=sys.stderr)
         sys.exit(2)
 

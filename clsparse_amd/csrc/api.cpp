@@ -84,7 +84,7 @@ extern "C" clsparseCreateResult clsparseCreateControl(hipStream_t stream)
     }
     // Kernels are compiled for gfx950 only (wave64, CDNA4).
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0 || prop.warpSize != 64) {
-        std::fprintf(stderr, "clSPARSE: device %d is %s; this build targets gfx950 only\n", dev,
+        std::fprintf(stderr, "clSPARSE: device %d is %s; the library is compiled for gfx950 only\n", dev,
                      prop.gcnArchName);
         r.status = clsparseInvalidDevice;
         return r;

@@ -97,7 +97,7 @@ struct LongRow
 };
 
 // CSR-Adaptive metadata owned by the library (reference matrix_meta,
-// src/library/include/clSPARSE-1x.hpp L45-62, plus what this build derives
+// src/library/include/clSPARSE-1x.hpp L45-62, plus what this library derives
 // from it once at creation time).
 struct matrix_meta
 {

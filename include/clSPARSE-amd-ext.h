@@ -19,7 +19,7 @@ extern "C" {
 #endif
 
 /* Copies the row-block entries written by clsparseCsrMetaCreate (before any
- * SpMV; this build never modifies them) into host memory. *entries receives
+ * SpMV; this library never modifies them) into host memory. *entries receives
  * the number of 64-bit entries (= rowBlockSize / 2); at most `capacity` are
  * copied. clsparseStructInvalid if the matrix has no metadata. */
 CLSPARSE_EXPORT clsparseStatus
