@@ -62,6 +62,17 @@ struct CgDevState
     // workgroups of the same launch have yet to start (they would skip their
     // rows of p, which a residual replacement still uses).
     int stop;
+    // The p update fused into the next iteration (Jacobi / no preconditioner
+    // with A in diagonal storage): `first` = 1 while p already holds the
+    // search direction (after set-up or a residual replacement); the SpMV
+    // kernel forms p' = z + beta_f p (pz = 1) or p' = p (pz = 0) on the fly
+    // and publishes which for the x / r update, which stores p' and clears
+    // `first`; `pending` = 1 from an x / r update to its convergence test.
+    int first;
+    int pending;
+    int pz;
+    int pad_fused;
+    double beta_f;
     double hist[kHist];
 };
 
@@ -111,6 +122,8 @@ __global__ __launch_bounds__(256) void cg_init_finalize(CgDevState* __restrict__
         const double norm = st->crit == kCritL2 ? sqrt(s) : s;
         st->init_res = norm / st->norm_ref;
         st->cur_res = st->init_res;
+        st->first = 1;  // p = z
+        st->pending = 0;
     }
 }
 
@@ -145,9 +158,11 @@ __global__ __launch_bounds__(256) void cg_dot_kernel(size_t n, const T* __restri
 
 // alpha = rz / yp; x += alpha p; r -= alpha y; z = M r; partials <r,z>, ||r||.
 // x may be held in a wider type than the other vectors (FP64 solution update).
-template <typename T, typename TX>
+// PN: p is first brought up to date, p = z + beta_f p as the SpMV kernel of
+// this iteration formed it on the fly (cg_spmv_pn_kernel), and stored.
+template <typename T, typename TX, bool PN = false>
 __global__ __launch_bounds__(256) void cg_update_kernel(
-    size_t n, TX* __restrict__ x, T* __restrict__ r, T* __restrict__ z, const T* __restrict__ p,
+    size_t n, TX* __restrict__ x, T* __restrict__ r, T* __restrict__ z, T* __restrict__ p,
     const T* __restrict__ y, const T* __restrict__ invD, int prec,
     const double* __restrict__ part_yp, const double* __restrict__ part_rz_prev,
     double* __restrict__ part_rz, double* __restrict__ part_nr, uint32_t g,
@@ -162,16 +177,19 @@ __global__ __launch_bounds__(256) void cg_update_kernel(
     const T alpha = static_cast<T>(rz / yp);
     const TX alpha_x = static_cast<TX>(rz / yp);
     const int crit = st->crit;
+    const bool pz = PN && st->pz != 0;
+    const T beta = PN ? static_cast<T>(st->beta_f) : T(0);
     double nr = 0.0, rzn = 0.0;
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
     for (size_t i0 = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i0 < n; i0 += kRows * stride) {
-        T pv[kRows], rv[kRows], yv[kRows], dv[kRows];
+        T pv[kRows], rv[kRows], yv[kRows], dv[kRows], zv[kRows];
         TX xv[kRows];
 #pragma unroll
         for (int u = 0; u < kRows; ++u) {
             const size_t i = i0 + u * stride;
             if (i < n) {
                 pv[u] = p[i];
+                if (pz) zv[u] = z[i];
                 xv[u] = x[i];
                 rv[u] = r[i];
                 yv[u] = y[i];
@@ -182,6 +200,10 @@ __global__ __launch_bounds__(256) void cg_update_kernel(
         for (int u = 0; u < kRows; ++u) {
             const size_t i = i0 + u * stride;
             if (i >= n) continue;
+            if (pz) {
+                pv[u] = fma(beta, pv[u], zv[u]);
+                p[i] = pv[u];
+            }
             x[i] = xv[u] + alpha_x * static_cast<TX>(pv[u]);
             const T ri = rv[u] - alpha * yv[u];
             r[i] = ri;
@@ -197,6 +219,10 @@ __global__ __launch_bounds__(256) void cg_update_kernel(
     if (threadIdx.x == 0) {
         part_nr[blockIdx.x] = nr;
         part_rz[blockIdx.x] = rzn;
+    }
+    if (PN && blockIdx.x == 0 && threadIdx.x == 0) {
+        st->pending = 1;  // the convergence test of this iteration follows
+        st->first = 0;    // p now lags z by one update
     }
 }
 
@@ -256,6 +282,114 @@ __global__ __launch_bounds__(256) void cg_pupdate_kernel(size_t n, T* __restrict
     }
 
     cg_finish(st, part_nr, g, sh);
+}
+
+// The convergence test of the last iteration of a chunk (the fused SpMV
+// kernel of the next iteration would run it otherwise): one workgroup.
+__global__ __launch_bounds__(256) void cg_test_kernel(CgDevState* __restrict__ st,
+                                                      const double* __restrict__ part_nr, uint32_t g)
+{
+    __shared__ double sh[4];
+    if (st->done || !st->pending) return;
+    cg_finish(st, part_nr, g, sh);
+    if (threadIdx.x == 0) st->pending = 0;
+}
+
+// p = z + beta p with beta = rz_new / rz_old unless p is already current
+// (`first`): the p update a fused iteration leaves to the next SpMV, made
+// explicit before a residual replacement re-bases p.
+template <typename T>
+__global__ __launch_bounds__(256) void cg_materialize_p(size_t n, T* __restrict__ p, const T* __restrict__ z,
+                                                        const double* __restrict__ part_rz_new,
+                                                        const double* __restrict__ part_rz_old, uint32_t g,
+                                                        const CgDevState* __restrict__ st)
+{
+    __shared__ double sh[4];
+    if (st->first) return;
+    const double rz_new = sum_partials(part_rz_new, g, sh);
+    const double rz_old = sum_partials(part_rz_old, g, sh);
+    const T beta = static_cast<T>(rz_new / rz_old);
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
+        p[i] = fma(beta, p[i], z[i]);
+}
+
+// y = A p' with p' = z + beta p (p' = p while `first`), formed at every row
+// the stencil reads, from diagonal storage as dia_spmv_dot_buf_kernel does;
+// <y, p'> partials; workgroup 0 first runs the convergence test of the
+// previous iteration and publishes beta and the choice for the x / r update.
+// beta = rz_new / rz_old is what cg_pupdate_kernel would have used, and
+// p' = fma(beta, p, z) here and in the update, so both see the same p'.
+template <int K, bool SYM, typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void cg_spmv_pn_kernel(
+    uint32_t n, DiaOffsets off, const T* __restrict__ vals, const T* __restrict__ p, const T* __restrict__ z,
+    T* __restrict__ y, double* __restrict__ part_yp, const double* __restrict__ part_rz_new,
+    const double* __restrict__ part_rz_old, const double* __restrict__ part_nr, uint32_t g,
+    CgDevState* __restrict__ st)
+{
+    constexpr int KL = SYM ? (K + 1) / 2 : K;
+    constexpr uint32_t S = sizeof(T);
+    constexpr int R = 2;  // rows per step (launched for FP32)
+    __shared__ double sh[4];
+    if (st->done) return;
+    const int first = st->first;
+    const int pending = st->pending;
+    double beta = 0.0;
+    if (!first) beta = sum_partials(part_rz_new, g, sh) / sum_partials(part_rz_old, g, sh);
+    const T b = static_cast<T>(beta);
+    if (blockIdx.x == 0) {
+        if (pending) cg_finish(st, part_nr, g, sh);
+        if (threadIdx.x == 0) {
+            st->pending = 0;
+            st->beta_f = beta;
+            st->pz = first ? 0 : 1;
+        }
+    }
+    const __amdgpu_buffer_rsrc_t rv =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(vals), 0, K * n * S, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(p), 0, n * S, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rz = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(z), 0, n * S, 0x00020000);
+    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+    double a = 0.0;
+    const uint32_t stride = gridDim.x * 256;
+    for (uint32_t i0 = blk * 256 + threadIdx.x; i0 < n; i0 += R * stride) {
+        T v[R][K], xv[R][K], pi[R];
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const uint32_t i = i0 + u * stride;
+#pragma unroll
+            for (int d = 0; d < K; ++d) {
+                const uint32_t j = i + static_cast<uint32_t>(off.o[d]);  // wraps below 0: out of range
+                // (z is loaded unconditionally: a load under the condition
+                // would be issued one at a time)
+                const T pj = buf_load<T>(rp, j * S, 0);
+                const T zj = buf_load<T>(rz, j * S, 0);
+                xv[u][d] = first ? pj : fma(b, pj, zj);
+                if (d < KL) {
+                    const uint32_t il = i < n ? i : n - 1;
+                    v[u][d] = buf_load<T>(rv, il * S, static_cast<uint32_t>(d) * n * S);
+                } else {
+                    const uint32_t at = j < n ? (static_cast<uint32_t>(K - 1 - d) * n + j) * S : 0xFFFFFFFFu;
+                    v[u][d] = buf_load<T>(rv, at, 0);
+                }
+            }
+            const T pii = buf_load<T>(rp, i * S, 0);
+            const T zii = buf_load<T>(rz, i * S, 0);
+            pi[u] = first ? pii : fma(b, pii, zii);
+        }
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const uint32_t i = i0 + u * stride;
+            T acc = T(0);
+#pragma unroll
+            for (int d = 0; d < K; ++d)
+                if (v[u][d] != T(0)) acc += mul_rounded_t(v[u][d], xv[u][d]);
+            if (i < n) y[i] = acc;
+            a += i < n ? static_cast<double>(acc) * static_cast<double>(pi[u]) : 0.0;
+        }
+    }
+    a = block_sum256(a, sh);
+    if (threadIdx.x == 0) part_yp[blk] = a;
 }
 
 // out = M r for the elementwise preconditioners.
@@ -471,6 +605,50 @@ struct CgBuffers
     }
 };
 
+// CLSPARSE_AMD_CG_FUSE=0: the Jacobi / unpreconditioned iteration keeps its
+// separate p update kernel.
+bool cg_fuse_enabled()
+{
+    const char* e = std::getenv("CLSPARSE_AMD_CG_FUSE");
+    return !(e && e[0] == '0');
+}
+
+// cg_spmv_pn_kernel for the stencil's diagonal count and storage.
+template <typename T>
+clsparseStatus launch_spmv_pn(const DiaMatrix<T>& A, size_t n, uint32_t g, const T* p, const T* z, T* y,
+                              double* part_yp, const double* rz_new, const double* rz_old, const double* nr,
+                              CgDevState* st, hipStream_t s)
+{
+#define CLS_PN(K, S)                                                                                     \
+    hipLaunchKernelGGL((cg_spmv_pn_kernel<K, S, T>), dim3(g), dim3(256), 0, s, static_cast<uint32_t>(n), \
+                       A.off, A.vals, p, z, y, part_yp, rz_new, rz_old, nr, g, st)
+    if (A.sym) {
+        switch (A.ndiag) {
+        case 3: CLS_PN(3, true); break;
+        case 5: CLS_PN(5, true); break;
+        case 7: CLS_PN(7, true); break;
+        case 9: CLS_PN(9, true); break;
+        default: return clsparseInvalidValue;
+        }
+    } else {
+        switch (A.ndiag) {
+        case 1: CLS_PN(1, false); break;
+        case 2: CLS_PN(2, false); break;
+        case 3: CLS_PN(3, false); break;
+        case 4: CLS_PN(4, false); break;
+        case 5: CLS_PN(5, false); break;
+        case 6: CLS_PN(6, false); break;
+        case 7: CLS_PN(7, false); break;
+        case 8: CLS_PN(8, false); break;
+        case 9: CLS_PN(9, false); break;
+        default: return clsparseInvalidValue;
+        }
+    }
+#undef CLS_PN
+    CLS_HIP_TRY(hipGetLastError());
+    return clsparseSuccess;
+}
+
 // The device-resident PCG loop on one system. x, b are T vectors of length n.
 template <typename T, typename TX = T>
 class CgEngine
@@ -487,6 +665,11 @@ public:
           g_(grid_for(n)), G_(G), GT_(GT), g_bf16_(g_bf16)
     {
         if (adia != nullptr && adia->vals != nullptr) adia_ = *adia;
+        // FP32 only: in FP64 the fused SpMV (p and z at every stencil offset)
+        // ran slower than the separate p update (128^3 Jacobi-PCG 39.4
+        // against 37.2 ms), in FP32 faster (mixed 22.3 against 23.2 ms,
+        // clsparseScsrcg 13.6 against 14.1 ms, the same iterates).
+        fused_ = std::is_same_v<T, float> && prec_ != kPrecFsai && adia_.vals != nullptr && cg_fuse_enabled();
         if (dia != nullptr && std::is_same_v<T, float> && g_bf16) {
             use_dia_ = true;
             dia_ = *dia;
@@ -534,6 +717,7 @@ public:
             CLS_HIP_TRY(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
             clsparseStatus st = clsparseSuccess;
             for (int k = 0; k < kChunk && st == clsparseSuccess; ++k) st = enqueue_iteration(k & 1);
+            if (st == clsparseSuccess) st = enqueue_chunk_end();
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(s_, &g);
             if (st != clsparseSuccess) {
@@ -548,6 +732,7 @@ public:
             CLS_HIP_TRY(hipGraphLaunch(exec_, s_));
         } else {
             for (int k = 0; k < kChunk; ++k) CLS_TRY(enqueue_iteration(k & 1));
+            CLS_TRY(enqueue_chunk_end());
         }
         CLS_HIP_TRY(hipMemcpyAsync(host_state, buf_.st, sizeof(CgDevState), hipMemcpyDeviceToHost,
                                    s_));
@@ -586,6 +771,19 @@ public:
     }
     int prec() const { return prec_; }
 
+    // The fused iteration leaves p one update behind z: bring it up to date
+    // (before a residual replacement re-bases it). ran: iterations the last
+    // chunk executed.
+    clsparseStatus materialize_p(int ran)
+    {
+        if (!fused_ || ran <= 0) return clsparseSuccess;
+        const int j = (ran - 1) & 1;  // partials of the last executed x / r update
+        hipLaunchKernelGGL((cg_materialize_p<T>), dim3(g_), dim3(256), 0, s_, n_, buf_.p, buf_.z,
+                           buf_.part_rz[j], buf_.part_rz[j ^ 1], g_, buf_.st);
+        CLS_HIP_TRY(hipGetLastError());
+        return clsparseSuccess;
+    }
+
     // z = M r of the current recurrence residual into the z buffer, where the
     // iterations keep z in registers only (the residual replacement re-bases
     // p on it). tmp: n values of scratch.
@@ -596,9 +794,32 @@ public:
     }
 
 private:
+    // After the last iteration of a chunk: its convergence test, which the
+    // next fused SpMV would otherwise run.
+    clsparseStatus enqueue_chunk_end()
+    {
+        if (!fused_) return clsparseSuccess;
+        hipLaunchKernelGGL(cg_test_kernel, dim3(1), dim3(256), 0, s_, buf_.st, buf_.part_nr[(kChunk - 1) & 1], g_);
+        CLS_HIP_TRY(hipGetLastError());
+        return clsparseSuccess;
+    }
+
     clsparseStatus enqueue_iteration(int par)
     {
         const int* done = &buf_.st->done;
+        if constexpr (std::is_same_v<T, float>) {
+        if (fused_) {
+            // y = A p' (p' = z + beta p on the fly) with <y, p'> and the previous
+            // iteration's test; then p = p', x, r, z and their partials.
+            CLS_TRY(launch_spmv_pn<T>(adia_, n_, g_, buf_.p, buf_.z, buf_.y, buf_.part_yp, buf_.part_rz[par ^ 1],
+                                      buf_.part_rz[par], buf_.part_nr[par ^ 1], buf_.st, s_));
+            hipLaunchKernelGGL((cg_update_kernel<T, TX, true>), dim3(g_), dim3(256), 0, s_, n_, x_, buf_.r,
+                               buf_.z, buf_.p, buf_.y, buf_.invD, prec_, buf_.part_yp, buf_.part_rz[par ^ 1],
+                               buf_.part_rz[par], buf_.part_nr[par], g_, buf_.st);
+            CLS_HIP_TRY(hipGetLastError());
+            return clsparseSuccess;
+        }
+        }
         if (adia_.vals != nullptr) {
             CLS_TRY(launch_dia_spmv(adia_, n_, g_, buf_.p, buf_.y, buf_.p, buf_.part_yp, done, s_));
         } else {
@@ -697,6 +918,7 @@ private:
     size_t n_;
     int prec_;
     int crit_;
+    bool fused_ = false;  // Jacobi / none with A in diagonal storage: fused p update
     CgBuffers<T>& buf_;
     hipStream_t s_;
     bool ext_;
@@ -970,6 +1192,8 @@ __global__ void state_resume(CgDevState* st, const double* __restrict__ rnorm)
         st->done = 0;
         st->converged = 0;
         st->cur_res = *rnorm / st->norm_ref;
+        st->first = 1;  // p is current (set up or re-based)
+        st->pending = 0;
     }
 }
 
@@ -1333,6 +1557,7 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
                 CLS_HIP_TRY(hipMemcpyAsync(xsave, xp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
             // r = (TI) r64; z' = M r into y; <r, z'> partials -> part_yp;
             // re-base p and z; the new <r, z> becomes the previous one.
+            CLS_TRY(eng->materialize_p(ran));
             CLS_TRY(eng->materialize_z(rI));
             hipLaunchKernelGGL((rr_cast<TI>), dim3(ew_grid(n)), dim3(256), 0, s, n, r64, buf.r);
             CLS_HIP_TRY(hipGetLastError());

@@ -434,3 +434,36 @@ def test_offsets_with_diagonal_storage(ctl, solver):
     assert s1.iterations == s2.iterations
     assert np.array_equal(x1, x2)
     lib.clsparseCsrMetaDelete(ctypes.byref(A))
+
+
+@pytest.mark.parametrize("solver", ["reference_fp32", "mixed"])
+def test_fused_p_update_matches_separate_kernels(ctl, monkeypatch, solver):
+    """The FP32 Jacobi iteration with A in diagonal storage folds the p update
+    into the next SpMV (p' = z + beta p formed at every row it reads, stored by
+    the x / r update; the chunk's last convergence test in a kernel of its
+    own; p brought up to date before a residual replacement). It must give
+    the iterates of the separate p update kernel bit for bit."""
+    rp, ci, v, m = cls.gen_diffusion3d(28, 24, 20, sigma=1.0, ax=1.0, ay=1.0, az=20.0, seed=17)
+    b = cls.gen_uniform(m, -1, 1, seed=18)
+    out = {}
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("CLSPARSE_AMD_CG_FUSE", fuse)
+        say(f"cg fused p update={fuse} solver={solver}")
+        if solver == "reference_fp32":
+            st, x, s = _solve(ctl, rp, ci, v, m, b, np.zeros(m), np.float32, cls.DIAGONAL, 5000, 1e-6, 0.0)
+            assert st == 0 and s.a_diagonals == 7
+            out[fuse] = (x, s.n_iters, s.l1_residual)
+        else:
+            A = cls.CsrMatrix(rp, ci, v, m)
+            A.meta_create(ctl)
+            x = torch.zeros(m, dtype=torch.float64, device="cuda")
+            xd, bd = cls.DenseVector(x), cls.DenseVector(torch.tensor(b, device="cuda"))
+            sc = cls.SolverControl(cls.DIAGONAL, 20000, 1e-10, 0.0)
+            assert cls.load().clsparseAmdDcsrcgMixed(ctypes.byref(xd.s), ctypes.byref(A.s), ctypes.byref(bd.s),
+                                                     sc.handle, ctl.handle) == 0
+            s = sc.stats()
+            sc.release()
+            assert s.status_detail == 0 and s.outer_iterations >= 2 and s.a_diagonals == 7
+            out[fuse] = (x.cpu().numpy(), s.iterations, s.true_residual)
+    np.testing.assert_array_equal(out["1"][0], out["0"][0])
+    assert out["1"][1:] == out["0"][1:]
