@@ -63,15 +63,14 @@ struct CgDevState
     // rows of p, which a residual replacement still uses).
     int stop;
     // The p update fused into the next iteration (Jacobi / no preconditioner
-    // with A in diagonal storage): `first` = 1 while p already holds the
-    // search direction (after set-up or a residual replacement); the SpMV
-    // kernel forms p' = z + beta_f p (pz = 1) or p' = p (pz = 0) on the fly
-    // and publishes which for the x / r update, which stores p' and clears
-    // `first`; `pending` = 1 from an x / r update to its convergence test.
+    // with A in diagonal storage): the SpMV kernel forms p' = fma(beta_f, p, z)
+    // on the fly and publishes beta_f for the x / r update, which stores p'
+    // and clears `first`. `first` = 1 while p already holds the search
+    // direction (after set-up or a residual replacement): then beta_f = 0 and
+    // z holds p, so p' = p. `pending` = 1 from an x / r update to its
+    // convergence test.
     int first;
     int pending;
-    int pz;
-    int pad_fused;
     double beta_f;
     double hist[kHist];
 };
@@ -177,7 +176,6 @@ __global__ __launch_bounds__(256) void cg_update_kernel(
     const T alpha = static_cast<T>(rz / yp);
     const TX alpha_x = static_cast<TX>(rz / yp);
     const int crit = st->crit;
-    const bool pz = PN && st->pz != 0;
     const T beta = PN ? static_cast<T>(st->beta_f) : T(0);
     double nr = 0.0, rzn = 0.0;
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
@@ -189,7 +187,7 @@ __global__ __launch_bounds__(256) void cg_update_kernel(
             const size_t i = i0 + u * stride;
             if (i < n) {
                 pv[u] = p[i];
-                if (pz) zv[u] = z[i];
+                if (PN) zv[u] = z[i];
                 xv[u] = x[i];
                 rv[u] = r[i];
                 yv[u] = y[i];
@@ -200,8 +198,8 @@ __global__ __launch_bounds__(256) void cg_update_kernel(
         for (int u = 0; u < kRows; ++u) {
             const size_t i = i0 + u * stride;
             if (i >= n) continue;
-            if (pz) {
-                pv[u] = fma(beta, pv[u], zv[u]);
+            if (PN) {
+                pv[u] = fma(beta, pv[u], zv[u]);  // p (beta 0 and z == p while `first`)
                 p[i] = pv[u];
             }
             x[i] = xv[u] + alpha_x * static_cast<TX>(pv[u]);
@@ -314,8 +312,9 @@ __global__ __launch_bounds__(256) void cg_materialize_p(size_t n, T* __restrict_
         p[i] = fma(beta, p[i], z[i]);
 }
 
-// y = A p' with p' = z + beta p (p' = p while `first`), formed at every row
-// the stencil reads, from diagonal storage as dia_spmv_dot_buf_kernel does;
+// y = A p' with p' = fma(beta, p, z), formed at every row the stencil reads,
+// from diagonal storage as dia_spmv_dot_buf_kernel does (while `first`,
+// beta = 0 and z holds p, so p' = p);
 // <y, p'> partials; workgroup 0 first runs the convergence test of the
 // previous iteration and publishes beta and the choice for the x / r update.
 // beta = rz_new / rz_old is what cg_pupdate_kernel would have used, and
@@ -342,7 +341,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         if (threadIdx.x == 0) {
             st->pending = 0;
             st->beta_f = beta;
-            st->pz = first ? 0 : 1;
         }
     }
     const __amdgpu_buffer_rsrc_t rv =
@@ -360,11 +358,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
 #pragma unroll
             for (int d = 0; d < K; ++d) {
                 const uint32_t j = i + static_cast<uint32_t>(off.o[d]);  // wraps below 0: out of range
-                // (z is loaded unconditionally: a load under the condition
-                // would be issued one at a time)
-                const T pj = buf_load<T>(rp, j * S, 0);
-                const T zj = buf_load<T>(rz, j * S, 0);
-                xv[u][d] = first ? pj : fma(b, pj, zj);
+                xv[u][d] = fma(b, buf_load<T>(rp, j * S, 0), buf_load<T>(rz, j * S, 0));
                 if (d < KL) {
                     const uint32_t il = i < n ? i : n - 1;
                     v[u][d] = buf_load<T>(rv, il * S, static_cast<uint32_t>(d) * n * S);
@@ -373,9 +367,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
                     v[u][d] = buf_load<T>(rv, at, 0);
                 }
             }
-            const T pii = buf_load<T>(rp, i * S, 0);
-            const T zii = buf_load<T>(rz, i * S, 0);
-            pi[u] = first ? pii : fma(b, pii, zii);
+            pi[u] = fma(b, buf_load<T>(rp, i * S, 0), buf_load<T>(rz, i * S, 0));
         }
 #pragma unroll
         for (int u = 0; u < R; ++u) {
@@ -666,9 +658,9 @@ public:
     {
         if (adia != nullptr && adia->vals != nullptr) adia_ = *adia;
         // FP32 only: in FP64 the fused SpMV (p and z at every stencil offset)
-        // ran slower than the separate p update (128^3 Jacobi-PCG 39.4
-        // against 37.2 ms), in FP32 faster (mixed 22.3 against 23.2 ms,
-        // clsparseScsrcg 13.6 against 14.1 ms, the same iterates).
+        // ran slower than the separate p update (128^3 Jacobi-PCG 39.4-40.5
+        // against 37.2-37.3 ms), in FP32 faster (mixed 22.3-22.6 against
+        // 23.2 ms, clsparseScsrcg 13.6-13.7 against 14.1 ms, the same iterates).
         fused_ = std::is_same_v<T, float> && prec_ != kPrecFsai && adia_.vals != nullptr && cg_fuse_enabled();
         if (dia != nullptr && std::is_same_v<T, float> && g_bf16) {
             use_dia_ = true;
@@ -770,6 +762,7 @@ public:
         return clsparseSuccess;
     }
     int prec() const { return prec_; }
+    bool fused() const { return fused_; }
 
     // The fused iteration leaves p one update behind z: bring it up to date
     // (before a residual replacement re-bases it). ran: iterations the last
@@ -1160,11 +1153,13 @@ __global__ __launch_bounds__(256) void flush_increment(size_t n, double* __restr
 }
 
 // s = <r', z'> / <r, z>;  p = z' + s (p - z);  z = z'.
+// z_is_p: z takes the new p instead (an engine with the fused p update reads
+// z only to form p' = fma(beta, p, z), with beta = 0 right after this).
 template <typename T>
 __global__ __launch_bounds__(256) void rr_rebase(size_t n, T* __restrict__ p, T* __restrict__ z,
                                                  const T* __restrict__ znew,
                                                  const double* __restrict__ part_old,
-                                                 const double* __restrict__ part_new, uint32_t g)
+                                                 const double* __restrict__ part_new, uint32_t g, int z_is_p)
 {
     __shared__ double sh[4];
     const double rz_old = sum_partials(part_old, g, sh);
@@ -1173,8 +1168,9 @@ __global__ __launch_bounds__(256) void rr_rebase(size_t n, T* __restrict__ p, T*
     const size_t stride = static_cast<size_t>(gridDim.x) * 256;
     for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
         const T zn = znew[i];
-        p[i] = zn + sc * (p[i] - z[i]);
-        z[i] = zn;
+        const T pn = zn + sc * (p[i] - z[i]);
+        p[i] = pn;
+        z[i] = z_is_p ? pn : zn;
     }
 }
 
@@ -1571,7 +1567,7 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
             hipLaunchKernelGGL((cg_dot_kernel<TI>), dim3(g), dim3(256), 0, s, n, buf.r, buf.y,
                                buf.part_yp, buf.st);
             hipLaunchKernelGGL((rr_rebase<TI>), dim3(g), dim3(256), 0, s, n, buf.p, buf.z, buf.y,
-                               old_part, buf.part_yp, g);
+                               old_part, buf.part_yp, g, eng->fused() ? 1 : 0);
             hipLaunchKernelGGL(copy_partials, dim3(1), dim3(256), 0, s, buf.part_yp, buf.part_rz[1], g);
             CLS_HIP_TRY(hipGetLastError());
             res_at_replace = rel;
