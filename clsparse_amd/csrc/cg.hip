@@ -328,7 +328,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
 {
     constexpr int KL = SYM ? (K + 1) / 2 : K;
     constexpr uint32_t S = sizeof(T);
-    constexpr int R = 2;  // rows per step (launched for FP32)
+    constexpr int R = 2;  // rows per step (launched for FP32; 4 ran slower)
     __shared__ double sh[4];
     if (st->done) return;
     const int first = st->first;
