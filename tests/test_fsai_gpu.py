@@ -542,3 +542,17 @@ def test_factor_rejects_bad_row_pointer_and_columns(ctl):
     A = cls.CsrMatrix(rp, ci, v, m)
     f = cls.Fsai(A, ctl)  # the same matrix with a valid structure
     f.release()
+
+
+def test_factor_rejects_sizes_beyond_int_counts(ctl):
+    """The device scans and sorts of the pattern count in int: n or nnz of
+    2^31 or more is clsparseInvalidSize, before any array is touched."""
+    A = cls.clsparseCsrMatrix()
+    cls.load().clsparseInitCsrMatrix(ctypes.byref(A))
+    for rows, nnz in ((1 << 31, 10), (1000, 1 << 31)):
+        A.num_rows = A.num_cols = rows
+        A.num_nonzeros = nnz
+        h = ctypes.c_void_p()
+        say(f"fsai: size guard rows={rows} nnz={nnz}")
+        assert cls.load().clsparseAmdFsaiCreate(ctypes.byref(h), ctypes.byref(A), ctl.handle) == \
+            cls.clsparseInvalidSize
