@@ -104,6 +104,65 @@ __global__ __launch_bounds__(256) void dia_spmv_dot_kernel(size_t n, DiaOffsets 
 // The grid (at most kReducePartialsMax workgroups of 4 waves) puts 4 waves on
 // a SIMD: tell the scheduler so, or it trades the loads in flight for an
 // occupancy the launch never reaches.
+// R rows iu[0..R) of y = A x from diagonal storage through buffer loads,
+// their <y, w> and <y, y> terms added to a and a2 (rows with live[u] false are
+// computed from in-range loads and dropped: a branch around a row lets the
+// compiler sink its loads into it, one row at a time).
+template <int K, bool SYM, typename T, int R>
+__device__ __forceinline__ void dia_rows(uint32_t n, const DiaOffsets& off, __amdgpu_buffer_rsrc_t rv,
+                                         __amdgpu_buffer_rsrc_t rx, __amdgpu_buffer_rsrc_t rw, T* __restrict__ y,
+                                         const uint32_t (&iu)[R], const bool (&live)[R], double& a, double& a2)
+{
+    constexpr int KL = SYM ? (K + 1) / 2 : K;
+    constexpr uint32_t S = sizeof(T);
+    T v[R][K], xv[R][K], pi[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+        const uint32_t i = iu[u];
+#pragma unroll
+        for (int d = 0; d < K; ++d) {
+            const uint32_t j = i + static_cast<uint32_t>(off.o[d]);  // wraps below 0: out of range
+            xv[u][d] = buf_load<T>(rx, j * S, 0);
+            if (d < KL) {
+                // Rows past n read row n - 1: the diagonal's base is a scalar
+                // offset, which the buffer's range check may not cover.
+                const uint32_t il = i < n ? i : n - 1;
+                v[u][d] = buf_load<T>(rv, il * S, static_cast<uint32_t>(d) * n * S);
+            } else {
+                // Mirrored value at row j of the lower diagonal; an offset past
+                // the buffer (no branch) reads 0 where j >= n.
+                const uint32_t at = j < n ? (static_cast<uint32_t>(K - 1 - d) * n + j) * S : 0xFFFFFFFFu;
+                v[u][d] = buf_load<T>(rv, at, 0);
+            }
+        }
+        pi[u] = buf_load<T>(rw, i * S, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+        T acc = T(0);
+#pragma unroll
+        for (int d = 0; d < K; ++d)
+            if (v[u][d] != T(0)) acc += mul_rounded_t(v[u][d], xv[u][d]);
+        if (live[u]) y[iu[u]] = acc;
+        a += live[u] ? static_cast<double>(acc) * static_cast<double>(pi[u]) : 0.0;
+        a2 += live[u] ? static_cast<double>(acc) * static_cast<double>(acc) : 0.0;
+    }
+}
+
+// The same product through buffer loads: the value of diagonal d at row i is
+// the buffer offset d*n*sizeof(T) (one scalar register per diagonal) plus
+// i*sizeof(T) (one vector register per row), where the 64-bit flat addresses
+// took two vector registers per load and left two waves per SIMD; reads of p
+// outside [0, n) return 0 from the buffer's range check. Needs
+// K * n * sizeof(T) < 2^32.
+// The grid (at most kReducePartialsMax workgroups of 4 waves) puts 4 waves on
+// a SIMD: tell the scheduler so, or it trades the loads in flight for an
+// occupancy the launch never reaches.
+// line > 0: each lane walks rows c, c + line, c + 2 line, ... (line = the
+// largest offset: a z line of a 3-D grid), a stretch of them per lane, so the
+// rows +-line away and the mirrored values of the far diagonal are the ones
+// the same lane loads one step before or after; lanes of a wave take
+// consecutive c. line == 0: grid-stride rows.
 template <int K, bool SYM, typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void dia_spmv_dot_buf_kernel(uint32_t n, DiaOffsets off,
                                                                const T* __restrict__ vals,
@@ -112,9 +171,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
                                                                const T* __restrict__ w,
                                                                double* __restrict__ part,
                                                                double* __restrict__ part2,
-                                                               const int* __restrict__ done)
+                                                               const int* __restrict__ done,
+                                                               uint32_t line)
 {
-    constexpr int KL = SYM ? (K + 1) / 2 : K;
     constexpr uint32_t S = sizeof(T);
     constexpr int R = S == 8 ? 2 : kDiaRows;  // FP64: fewer rows per step keep it under 128 VGPRs
     __shared__ double sh[4];
@@ -128,43 +187,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
                                           0x00020000);
     const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
     double a = 0.0, a2 = 0.0;
-    const uint32_t stride = gridDim.x * 256;
-    for (uint32_t i0 = blk * 256 + threadIdx.x; i0 < n; i0 += R * stride) {
-        T v[R][K], xv[R][K], pi[R];
+    if (line == 0) {
+        const uint32_t stride = gridDim.x * 256;
+        for (uint32_t i0 = blk * 256 + threadIdx.x; i0 < n; i0 += R * stride) {
+            uint32_t iu[R];
+            bool live[R];
 #pragma unroll
-        for (int u = 0; u < R; ++u) {
-            const uint32_t i = i0 + u * stride;
-#pragma unroll
-            for (int d = 0; d < K; ++d) {
-                const uint32_t j = i + static_cast<uint32_t>(off.o[d]);  // wraps below 0: out of range
-                xv[u][d] = buf_load<T>(rx, j * S, 0);
-                if (d < KL) {
-                    // Rows past n read row n - 1: the diagonal's base is a scalar
-                    // offset, which the buffer's range check may not cover.
-                    const uint32_t il = i < n ? i : n - 1;
-                    v[u][d] = buf_load<T>(rv, il * S, static_cast<uint32_t>(d) * n * S);
-                } else {
-                    // Mirrored value at row j of the lower diagonal; an offset past
-                    // the buffer (no branch) reads 0 where j >= n.
-                    const uint32_t at = j < n ? (static_cast<uint32_t>(K - 1 - d) * n + j) * S : 0xFFFFFFFFu;
-                    v[u][d] = buf_load<T>(rv, at, 0);
-                }
+            for (int u = 0; u < R; ++u) {
+                iu[u] = i0 + u * stride;
+                live[u] = iu[u] < n;
             }
-            pi[u] = buf_load<T>(rw, i * S, 0);
+            dia_rows<K, SYM, T, R>(n, off, rv, rx, rw, y, iu, live, a, a2);
         }
-        // Rows past n are computed too (from in-range or zero loads) and only
-        // their store and dot term are dropped: a branch around the whole row
-        // lets the compiler sink that row's loads into it, one row at a time.
+    } else {
+        const uint32_t threads = gridDim.x * 256, t = blk * 256 + threadIdx.x;
+        const uint32_t segs = max(1u, threads / line);  // stretches per line
+        const uint32_t c = t % line, sg = t / line;
+        const uint32_t m = (n + line - 1) / line;       // rows per line
+        const uint32_t sl = (m + segs - 1) / segs;
+        const uint32_t k_end = sg < segs ? min(m, (sg + 1) * sl) : 0u;
+        for (uint32_t k0 = sg * sl; k0 < k_end; k0 += R) {
+            uint32_t iu[R];
+            bool live[R];
 #pragma unroll
-        for (int u = 0; u < R; ++u) {
-            const uint32_t i = i0 + u * stride;
-            T acc = T(0);
-#pragma unroll
-            for (int d = 0; d < K; ++d)
-                if (v[u][d] != T(0)) acc += mul_rounded_t(v[u][d], xv[u][d]);
-            if (i < n) y[i] = acc;
-            a += i < n ? static_cast<double>(acc) * static_cast<double>(pi[u]) : 0.0;
-            a2 += i < n ? static_cast<double>(acc) * static_cast<double>(acc) : 0.0;
+            for (int u = 0; u < R; ++u) {
+                iu[u] = c + (k0 + u) * line;
+                live[u] = k0 + u < k_end && iu[u] < n;
+                if (!live[u]) iu[u] = c;  // a row in range, computed and dropped
+            }
+            dia_rows<K, SYM, T, R>(n, off, rv, rx, rw, y, iu, live, a, a2);
         }
     }
     if (part == nullptr) return;
@@ -256,9 +307,19 @@ template <int K, bool SYM, typename T>
 void launch_dia_spmv_dot(size_t n, uint32_t g, const DiaOffsets& off, const T* vals, const T* p, T* y,
                          const T* w, double* part, double* part2, const int* done, hipStream_t s)
 {
-    if (static_cast<uint64_t>(K) * n * sizeof(T) < (uint64_t(1) << 32) - 16)
+    if (static_cast<uint64_t>(K) * n * sizeof(T) < (uint64_t(1) << 32) - 16) {
+        // Line order when the largest offset is long (a 3-D grid's n_x n_y)
+        // and the matrix holds several lines (CLSPARSE_AMD_DIA_LINES=0: off).
+        uint32_t line = 0;
+        const char* e = std::getenv("CLSPARSE_AMD_DIA_LINES");
+        if (!(e && e[0] == '0')) {
+            int64_t far = 0;
+            for (int d = 0; d < K; ++d) far = std::max<int64_t>(far, off.o[d] < 0 ? -int64_t(off.o[d]) : off.o[d]);
+            if (far >= 4096 && static_cast<uint64_t>(far) * 8 <= n) line = static_cast<uint32_t>(far);
+        }
         hipLaunchKernelGGL((dia_spmv_dot_buf_kernel<K, SYM, T>), dim3(g), dim3(256), 0, s,
-                           static_cast<uint32_t>(n), off, vals, p, y, w, part, part2, done);
+                           static_cast<uint32_t>(n), off, vals, p, y, w, part, part2, done, line);
+    }
     else
         hipLaunchKernelGGL((dia_spmv_dot_kernel<K, SYM, T>), dim3(g), dim3(256), 0, s, n, off, vals, p, y, w,
                            part, part2, done);
