@@ -319,14 +319,57 @@ __global__ __launch_bounds__(256) void cg_materialize_p(size_t n, T* __restrict_
 // previous iteration and publishes beta and the choice for the x / r update.
 // beta = rz_new / rz_old is what cg_pupdate_kernel would have used, and
 // p' = fma(beta, p, z) here and in the update, so both see the same p'.
+template <int K, bool SYM, typename T, int R>
+__device__ __forceinline__ void pn_rows(uint32_t n, const DiaOffsets& off, __amdgpu_buffer_rsrc_t rv,
+                                        __amdgpu_buffer_rsrc_t rp, __amdgpu_buffer_rsrc_t rz, T b,
+                                        T* __restrict__ y, const uint32_t (&iu)[R], const bool (&live)[R], double& a)
+{
+    constexpr int KL = SYM ? (K + 1) / 2 : K;
+    constexpr uint32_t S = sizeof(T);
+    T v[R][K], xv[R][K], pi[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+        const uint32_t i = iu[u];
+#pragma unroll
+        for (int d = 0; d < K; ++d) {
+            const uint32_t j = i + static_cast<uint32_t>(off.o[d]);  // wraps below 0: out of range
+            xv[u][d] = fma(b, buf_load<T>(rp, j * S, 0), buf_load<T>(rz, j * S, 0));
+            if (d < KL) {
+                const uint32_t il = i < n ? i : n - 1;
+                v[u][d] = buf_load<T>(rv, il * S, static_cast<uint32_t>(d) * n * S);
+            } else {
+                const uint32_t at = j < n ? (static_cast<uint32_t>(K - 1 - d) * n + j) * S : 0xFFFFFFFFu;
+                v[u][d] = buf_load<T>(rv, at, 0);
+            }
+        }
+        pi[u] = fma(b, buf_load<T>(rp, i * S, 0), buf_load<T>(rz, i * S, 0));
+    }
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+        T acc = T(0);
+#pragma unroll
+        for (int d = 0; d < K; ++d)
+            if (v[u][d] != T(0)) acc += mul_rounded_t(v[u][d], xv[u][d]);
+        if (live[u]) y[iu[u]] = acc;
+        a += live[u] ? static_cast<double>(acc) * static_cast<double>(pi[u]) : 0.0;
+    }
+}
+
+// y = A p' with p' = fma(beta, p, z), formed at every row the stencil reads,
+// from diagonal storage as dia_spmv_dot_buf_kernel does (while `first`,
+// beta = 0 and z holds p, so p' = p), in the same row order (line > 0: lanes
+// walk rows c, c + line, ...);
+// <y, p'> partials; workgroup 0 first runs the convergence test of the
+// previous iteration and publishes beta and the choice for the x / r update.
+// beta = rz_new / rz_old is what cg_pupdate_kernel would have used, and
+// p' = fma(beta, p, z) here and in the update, so both see the same p'.
 template <int K, bool SYM, typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void cg_spmv_pn_kernel(
     uint32_t n, DiaOffsets off, const T* __restrict__ vals, const T* __restrict__ p, const T* __restrict__ z,
     T* __restrict__ y, double* __restrict__ part_yp, const double* __restrict__ part_rz_new,
     const double* __restrict__ part_rz_old, const double* __restrict__ part_nr, uint32_t g,
-    CgDevState* __restrict__ st)
+    CgDevState* __restrict__ st, uint32_t line)
 {
-    constexpr int KL = SYM ? (K + 1) / 2 : K;
     constexpr uint32_t S = sizeof(T);
     constexpr int R = 2;  // rows per step (launched for FP32; 4 ran slower)
     __shared__ double sh[4];
@@ -349,35 +392,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     const __amdgpu_buffer_rsrc_t rz = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(z), 0, n * S, 0x00020000);
     const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
     double a = 0.0;
-    const uint32_t stride = gridDim.x * 256;
-    for (uint32_t i0 = blk * 256 + threadIdx.x; i0 < n; i0 += R * stride) {
-        T v[R][K], xv[R][K], pi[R];
+    if (line == 0) {
+        const uint32_t stride = gridDim.x * 256;
+        for (uint32_t i0 = blk * 256 + threadIdx.x; i0 < n; i0 += R * stride) {
+            uint32_t iu[R];
+            bool live[R];
 #pragma unroll
-        for (int u = 0; u < R; ++u) {
-            const uint32_t i = i0 + u * stride;
-#pragma unroll
-            for (int d = 0; d < K; ++d) {
-                const uint32_t j = i + static_cast<uint32_t>(off.o[d]);  // wraps below 0: out of range
-                xv[u][d] = fma(b, buf_load<T>(rp, j * S, 0), buf_load<T>(rz, j * S, 0));
-                if (d < KL) {
-                    const uint32_t il = i < n ? i : n - 1;
-                    v[u][d] = buf_load<T>(rv, il * S, static_cast<uint32_t>(d) * n * S);
-                } else {
-                    const uint32_t at = j < n ? (static_cast<uint32_t>(K - 1 - d) * n + j) * S : 0xFFFFFFFFu;
-                    v[u][d] = buf_load<T>(rv, at, 0);
-                }
+            for (int u = 0; u < R; ++u) {
+                iu[u] = i0 + u * stride;
+                live[u] = iu[u] < n;
             }
-            pi[u] = fma(b, buf_load<T>(rp, i * S, 0), buf_load<T>(rz, i * S, 0));
+            pn_rows<K, SYM, T, R>(n, off, rv, rp, rz, b, y, iu, live, a);
         }
+    } else {
+        const uint32_t threads = gridDim.x * 256, t = blk * 256 + threadIdx.x;
+        const uint32_t segs = max(1u, threads / line);  // stretches per line
+        const uint32_t c = t % line, sg = t / line;
+        const uint32_t m = (n + line - 1) / line;       // rows per line
+        const uint32_t sl = (m + segs - 1) / segs;
+        const uint32_t k_end = sg < segs ? min(m, (sg + 1) * sl) : 0u;
+        for (uint32_t k0 = sg * sl; k0 < k_end; k0 += R) {
+            uint32_t iu[R];
+            bool live[R];
 #pragma unroll
-        for (int u = 0; u < R; ++u) {
-            const uint32_t i = i0 + u * stride;
-            T acc = T(0);
-#pragma unroll
-            for (int d = 0; d < K; ++d)
-                if (v[u][d] != T(0)) acc += mul_rounded_t(v[u][d], xv[u][d]);
-            if (i < n) y[i] = acc;
-            a += i < n ? static_cast<double>(acc) * static_cast<double>(pi[u]) : 0.0;
+            for (int u = 0; u < R; ++u) {
+                iu[u] = c + (k0 + u) * line;
+                live[u] = k0 + u < k_end && iu[u] < n;
+                if (!live[u]) iu[u] = c;  // a row in range, computed and dropped
+            }
+            pn_rows<K, SYM, T, R>(n, off, rv, rp, rz, b, y, iu, live, a);
         }
     }
     a = block_sum256(a, sh);
@@ -611,9 +654,10 @@ clsparseStatus launch_spmv_pn(const DiaMatrix<T>& A, size_t n, uint32_t g, const
                               double* part_yp, const double* rz_new, const double* rz_old, const double* nr,
                               CgDevState* st, hipStream_t s)
 {
+    const uint32_t line = dia_line(A.off, A.ndiag, n);
 #define CLS_PN(K, S)                                                                                     \
     hipLaunchKernelGGL((cg_spmv_pn_kernel<K, S, T>), dim3(g), dim3(256), 0, s, static_cast<uint32_t>(n), \
-                       A.off, A.vals, p, z, y, part_yp, rz_new, rz_old, nr, g, st)
+                       A.off, A.vals, p, z, y, part_yp, rz_new, rz_old, nr, g, st, line)
     if (A.sym) {
         switch (A.ndiag) {
         case 3: CLS_PN(3, true); break;

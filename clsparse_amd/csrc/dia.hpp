@@ -303,6 +303,18 @@ __global__ __launch_bounds__(64) void dia_sample_kernel(uint32_t n, const uint32
     }
 }
 
+// The line length of the line-order kernels: the largest |offset| when it is
+// at least 4096 (a 3-D grid's n_x n_y) and the matrix holds at least 8 lines;
+// 0 (grid-stride rows) otherwise or when CLSPARSE_AMD_DIA_LINES=0.
+inline uint32_t dia_line(const DiaOffsets& off, int k, size_t n)
+{
+    const char* e = std::getenv("CLSPARSE_AMD_DIA_LINES");
+    if (e && e[0] == '0') return 0;
+    int64_t far = 0;
+    for (int d = 0; d < k; ++d) far = std::max<int64_t>(far, off.o[d] < 0 ? -int64_t(off.o[d]) : off.o[d]);
+    return far >= 4096 && static_cast<uint64_t>(far) * 8 <= n ? static_cast<uint32_t>(far) : 0u;
+}
+
 template <int K, bool SYM, typename T>
 void launch_dia_spmv_dot(size_t n, uint32_t g, const DiaOffsets& off, const T* vals, const T* p, T* y,
                          const T* w, double* part, double* part2, const int* done, hipStream_t s)
@@ -310,13 +322,7 @@ void launch_dia_spmv_dot(size_t n, uint32_t g, const DiaOffsets& off, const T* v
     if (static_cast<uint64_t>(K) * n * sizeof(T) < (uint64_t(1) << 32) - 16) {
         // Line order when the largest offset is long (a 3-D grid's n_x n_y)
         // and the matrix holds several lines (CLSPARSE_AMD_DIA_LINES=0: off).
-        uint32_t line = 0;
-        const char* e = std::getenv("CLSPARSE_AMD_DIA_LINES");
-        if (!(e && e[0] == '0')) {
-            int64_t far = 0;
-            for (int d = 0; d < K; ++d) far = std::max<int64_t>(far, off.o[d] < 0 ? -int64_t(off.o[d]) : off.o[d]);
-            if (far >= 4096 && static_cast<uint64_t>(far) * 8 <= n) line = static_cast<uint32_t>(far);
-        }
+        const uint32_t line = dia_line(off, K, n);
         hipLaunchKernelGGL((dia_spmv_dot_buf_kernel<K, SYM, T>), dim3(g), dim3(256), 0, s,
                            static_cast<uint32_t>(n), off, vals, p, y, w, part, part2, done, line);
     }
