@@ -95,15 +95,6 @@ __global__ __launch_bounds__(256) void dia_spmv_dot_kernel(size_t n, DiaOffsets 
     if (threadIdx.x == 0) part2[blk] = a2;
 }
 
-// The same product through buffer loads: the value of diagonal d at row i is
-// the buffer offset d*n*sizeof(T) (one scalar register per diagonal) plus
-// i*sizeof(T) (one vector register per row), where the 64-bit flat addresses
-// took two vector registers per load and left two waves per SIMD; reads of p
-// outside [0, n) return 0 from the buffer's range check. Needs
-// K * n * sizeof(T) < 2^32.
-// The grid (at most kReducePartialsMax workgroups of 4 waves) puts 4 waves on
-// a SIMD: tell the scheduler so, or it trades the loads in flight for an
-// occupancy the launch never reaches.
 // R rows iu[0..R) of y = A x from diagonal storage through buffer loads,
 // their <y, w> and <y, y> terms added to a and a2 (rows with live[u] false are
 // computed from in-range loads and dropped: a branch around a row lets the
@@ -147,6 +138,72 @@ __device__ __forceinline__ void dia_rows(uint32_t n, const DiaOffsets& off, __am
         a += live[u] ? static_cast<double>(acc) * static_cast<double>(pi[u]) : 0.0;
         a2 += live[u] ? static_cast<double>(acc) * static_cast<double>(acc) : 0.0;
     }
+}
+
+// Two rows i and i + line of a symmetric stencil (offsets ascending, so
+// diagonal 0 is -line and K - 1 is +line) in line order, with the far
+// diagonals' operands carried in registers from row to row: pm = p[i - line],
+// p0 = p[i] and f0 = the lower far value at row i come in, and the same three
+// for row i + 2 line go out in their place. The values and the summation
+// order are those of dia_rows; only the loads differ.
+template <int K, typename T>
+__device__ __forceinline__ void dia_line_pair(uint32_t n, const DiaOffsets& off, __amdgpu_buffer_rsrc_t rv,
+                                              __amdgpu_buffer_rsrc_t rx, __amdgpu_buffer_rsrc_t rw,
+                                              T* __restrict__ y, uint32_t i, uint32_t line, bool live0, bool live1,
+                                              T& pm, T& p0, T& f0, double& a, double& a2)
+{
+    constexpr int KL = (K + 1) / 2;
+    constexpr uint32_t S = sizeof(T);
+    const uint32_t i1 = i + line, i2 = i1 + line;
+    // Lower far values at rows i1 and i2 (0 past n, as the mirrored load reads).
+    const T p1 = buf_load<T>(rx, i1 * S, 0), p2 = buf_load<T>(rx, i2 * S, 0);
+    const T f1 = buf_load<T>(rv, i1 < n ? i1 * S : 0xFFFFFFFFu, 0);
+    const T f2 = buf_load<T>(rv, i2 < n ? i2 * S : 0xFFFFFFFFu, 0);
+    const uint32_t iu[2] = {i, i1};
+    T v[2][K], xv[2][K], pi[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+#pragma unroll
+        for (int d = 1; d < K - 1; ++d) {
+            if (d != KL - 1) {  // p[i] itself is carried
+                const uint32_t j = iu[u] + static_cast<uint32_t>(off.o[d]);  // wraps below 0: out of range
+                xv[u][d] = buf_load<T>(rx, j * S, 0);
+            }
+            if (d < KL) {
+                const uint32_t il = iu[u] < n ? iu[u] : n - 1;
+                v[u][d] = buf_load<T>(rv, il * S, static_cast<uint32_t>(d) * n * S);
+            } else {
+                const uint32_t j = iu[u] + static_cast<uint32_t>(off.o[d]);
+                const uint32_t at = j < n ? (static_cast<uint32_t>(K - 1 - d) * n + j) * S : 0xFFFFFFFFu;
+                v[u][d] = buf_load<T>(rv, at, 0);
+            }
+        }
+        pi[u] = buf_load<T>(rw, iu[u] * S, 0);
+    }
+    xv[0][0] = pm;
+    v[0][0] = f0;
+    xv[0][KL - 1] = p0;
+    xv[0][K - 1] = p1;
+    v[0][K - 1] = f1;
+    xv[1][0] = p0;
+    v[1][0] = f1;
+    xv[1][KL - 1] = p1;
+    xv[1][K - 1] = p2;
+    v[1][K - 1] = f2;
+    const bool live[2] = {live0, live1};
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        T acc = T(0);
+#pragma unroll
+        for (int d = 0; d < K; ++d)
+            if (v[u][d] != T(0)) acc += mul_rounded_t(v[u][d], xv[u][d]);
+        if (live[u]) y[iu[u]] = acc;
+        a += live[u] ? static_cast<double>(acc) * static_cast<double>(pi[u]) : 0.0;
+        a2 += live[u] ? static_cast<double>(acc) * static_cast<double>(acc) : 0.0;
+    }
+    pm = p1;
+    p0 = p2;
+    f0 = f2;
 }
 
 // The same product through buffer loads: the value of diagonal d at row i is
@@ -206,16 +263,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         const uint32_t m = (n + line - 1) / line;       // rows per line
         const uint32_t sl = (m + segs - 1) / segs;
         const uint32_t k_end = sg < segs ? min(m, (sg + 1) * sl) : 0u;
-        for (uint32_t k0 = sg * sl; k0 < k_end; k0 += R) {
-            uint32_t iu[R];
-            bool live[R];
-#pragma unroll
-            for (int u = 0; u < R; ++u) {
-                iu[u] = c + (k0 + u) * line;
-                live[u] = k0 + u < k_end && iu[u] < n;
-                if (!live[u]) iu[u] = c;  // a row in range, computed and dropped
+        if constexpr (SYM && K >= 3) {
+            if (sg * sl < k_end) {
+                const uint32_t i = c + sg * sl * line;
+                T pm = buf_load<T>(rx, (i - line) * S, 0);  // wraps below 0: 0
+                T p0 = buf_load<T>(rx, i * S, 0);
+                T f0 = buf_load<T>(rv, i < n ? i * S : 0xFFFFFFFFu, 0);
+                for (uint32_t k0 = sg * sl; k0 < k_end; k0 += 2) {
+                    const uint32_t i0 = c + k0 * line;
+                    dia_line_pair<K, T>(n, off, rv, rx, rw, y, i0, line, i0 < n,
+                                        k0 + 1 < k_end && i0 + line < n, pm, p0, f0, a, a2);
+                }
             }
-            dia_rows<K, SYM, T, R>(n, off, rv, rx, rw, y, iu, live, a, a2);
+        } else {
+            for (uint32_t k0 = sg * sl; k0 < k_end; k0 += R) {
+                uint32_t iu[R];
+                bool live[R];
+#pragma unroll
+                for (int u = 0; u < R; ++u) {
+                    iu[u] = c + (k0 + u) * line;
+                    live[u] = k0 + u < k_end && iu[u] < n;
+                    if (!live[u]) iu[u] = c;  // a row in range, computed and dropped
+                }
+                dia_rows<K, SYM, T, R>(n, off, rv, rx, rw, y, iu, live, a, a2);
+            }
         }
     }
     if (part == nullptr) return;
