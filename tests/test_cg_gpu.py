@@ -185,6 +185,35 @@ def test_graph_and_stream_paths_identical(ctl, monkeypatch):
     assert np.array_equal(res[0][0], res[1][0])
 
 
+@pytest.mark.parametrize("dtype,rtol", [(np.float64, 1e-11), (np.float32, 1e-4)])
+def test_line_order_matches_grid_stride_and_csr(ctl, monkeypatch, dtype, rtol):
+    """A 64 x 64 x 16 grid (largest offset 4096, 16 lines): the diagonal-storage
+    SpMV walks rows in line order (the symmetric FP64 and FP32 kernels carry
+    the far operands in registers, the fused FP32 Jacobi SpMV forms p' per
+    row). Row sums are the same as in grid-stride order and in the CSR SpMV;
+    only the <y, p> partials are added in another order, so six iterations
+    agree to rounding and the full solves take the same iterations."""
+    say(f"cg line order 64x64x16 {np.dtype(dtype).name}")
+    rp, ci, v, m = cls.gen_diffusion3d(64, 64, 16, sigma=1.0, ax=1.0, ay=1.0, az=10.0, seed=23)
+    b = cls.gen_uniform(m, -1, 1, seed=24)
+    short, full = [], []
+    for dia, lines in (("1", "1"), ("1", "0"), ("0", "1")):
+        monkeypatch.setenv("CLSPARSE_AMD_DIA", dia)
+        monkeypatch.setenv("CLSPARSE_AMD_DIA_LINES", lines)
+        st, x, s = _solve(ctl, rp, ci, v, m, b, np.zeros(m), dtype, cls.DIAGONAL, 6, 1e-30, 0.0)
+        assert 0 < s.n_iters <= 6
+        short.append((s.n_iters, np.asarray(x, dtype=np.float64)))
+        tol = 1e-9 if dtype == np.float64 else 1e-5
+        st, x, s = _solve(ctl, rp, ci, v, m, b, np.zeros(m), dtype, cls.DIAGONAL, 5000, tol, 0.0)
+        assert st == 0
+        full.append(s.n_iters)
+    assert short[0][0] == short[1][0] == short[2][0]
+    scale = np.abs(short[1][1]).max()
+    for _, x in (short[0], short[2]):
+        assert np.allclose(x, short[1][1], rtol=rtol, atol=rtol * scale)
+    assert max(full) - min(full) <= 2, full
+
+
 def test_diffusion_fp64_true_residual(ctl):
     say("cg diffusion 32^3")
     rp, ci, v, m = cls.gen_diffusion3d(32, 32, 32, sigma=1.0, seed=6)

@@ -438,7 +438,8 @@ def test_first_iteration_applies_the_bf16_factor(ctl, monkeypatch, dia):
         net.release()
 
 
-def test_diagonal_storage_solves_like_the_csr_factor(ctl, monkeypatch):
+@pytest.mark.parametrize("dims", [(24, 24, 24), (64, 64, 16)])
+def test_diagonal_storage_solves_like_the_csr_factor(ctl, monkeypatch, dims):
     """The learned factor on a 3-D anisotropic stencil: the solve from
     diagonal storage and the one through the CSR SpMVs of G and G^T take the
     same iterations (the row sums are formed in the same order; only a
@@ -446,7 +447,9 @@ def test_diagonal_storage_solves_like_the_csr_factor(ctl, monkeypatch):
     monkeypatch.setenv("CLSPARSE_AMD_FSAI_POLICY", "always")
     from clsparse_amd import fsai as fs
     params, _ = fs.load_weights()
-    rp, ci, v, m = cls.gen_diffusion3d(24, 24, 24, sigma=1.0, ax=1.0, ay=1.0, az=100.0, seed=5)
+    # 64 x 64 x 16: A's largest offset is 4096 rows, so the FP32 SpMV of A
+    # walks rows in line order with the far operands carried in registers.
+    rp, ci, v, m = cls.gen_diffusion3d(*dims, sigma=1.0, ax=1.0, ay=1.0, az=100.0, seed=5)
     A = cls.CsrMatrix(rp, ci, v, m)
     A.meta_create(ctl)
     net = cls.Mlp(seed=1)
@@ -456,7 +459,7 @@ def test_diagonal_storage_solves_like_the_csr_factor(ctl, monkeypatch):
     try:
         f.apply_net(net, ctl)
         b = cls.gen_uniform(m, -1, 1, seed=13)
-        say("fsai: diagonal storage vs CSR factor, 24^3")
+        say(f"fsai: diagonal storage vs CSR factor, {dims}")
         monkeypatch.setenv("CLSPARSE_AMD_FSAI_DIA", "0")
         st_c, x_c = _solve_fsai(ctl, A, f, b)
         monkeypatch.setenv("CLSPARSE_AMD_FSAI_DIA", "1")
