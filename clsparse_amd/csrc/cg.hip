@@ -312,13 +312,9 @@ __global__ __launch_bounds__(256) void cg_materialize_p(size_t n, T* __restrict_
         p[i] = fma(beta, p[i], z[i]);
 }
 
-// y = A p' with p' = fma(beta, p, z), formed at every row the stencil reads,
-// from diagonal storage as dia_spmv_dot_buf_kernel does (while `first`,
-// beta = 0 and z holds p, so p' = p);
-// <y, p'> partials; workgroup 0 first runs the convergence test of the
-// previous iteration and publishes beta and the choice for the x / r update.
-// beta = rz_new / rz_old is what cg_pupdate_kernel would have used, and
-// p' = fma(beta, p, z) here and in the update, so both see the same p'.
+// R rows iu[0..R) of y = A p' (p' = fma(b, p, z) formed at each operand)
+// from diagonal storage, their <y, p'> terms added to a; rows with live[u]
+// false are computed from in-range loads and dropped, as in dia_rows.
 template <int K, bool SYM, typename T, int R>
 __device__ __forceinline__ void pn_rows(uint32_t n, const DiaOffsets& off, __amdgpu_buffer_rsrc_t rv,
                                         __amdgpu_buffer_rsrc_t rp, __amdgpu_buffer_rsrc_t rz, T b,
