@@ -39,6 +39,40 @@ struct Acc<T, false>
     __device__ __forceinline__ T with_beta(T beta, T y_old) const { return fma(beta, y_old, s); }
 };
 
+// FP32 values: the sum is kept in FP64 (a product of two FP32 values is exact
+// in FP64), so a row's result carries one FP32 rounding of an FP64 sum: per-row
+// error <= 2^-24 sum|a_ij x_j| + 2^-53 (len) sum|a_ij x_j|, within 1e-6 for
+// any row length, where FP32 accumulation is bounded only by ~len 2^-24.
+template <>
+struct Acc<float, false>
+{
+    double s = 0.0;
+    __device__ __forceinline__ void add(float v) { s += static_cast<double>(v); }
+    __device__ __forceinline__ void mul_add(float a, float b)
+    {
+        s = fma(static_cast<double>(a), static_cast<double>(b), s);
+    }
+    __device__ __forceinline__ void merge(const Acc& o) { s += o.s; }
+    __device__ __forceinline__ float value() const { return static_cast<float>(s); }
+    template <int CTRL>
+    __device__ __forceinline__ Acc dpp_move() const
+    {
+        Acc r;
+        r.s = dpp<CTRL>(s);
+        return r;
+    }
+    __device__ __forceinline__ Acc xor_move(int m) const
+    {
+        Acc r;
+        r.s = shfl_xor_t(s, m);
+        return r;
+    }
+    __device__ __forceinline__ float with_beta(float beta, float y_old) const
+    {
+        return static_cast<float>(fma(static_cast<double>(beta), static_cast<double>(y_old), s));
+    }
+};
+
 template <typename T>
 struct Acc<T, true>
 {

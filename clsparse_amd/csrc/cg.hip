@@ -401,22 +401,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             pn_rows<K, SYM, T, R>(n, off, rv, rp, rz, b, y, iu, live, a);
         }
     } else {
-        const uint32_t threads = gridDim.x * 256, t = blk * 256 + threadIdx.x;
-        const uint32_t segs = max(1u, threads / line);  // stretches per line
-        const uint32_t c = t % line, sg = t / line;
-        const uint32_t m = (n + line - 1) / line;       // rows per line
-        const uint32_t sl = (m + segs - 1) / segs;
-        const uint32_t k_end = sg < segs ? min(m, (sg + 1) * sl) : 0u;
-        for (uint32_t k0 = sg * sl; k0 < k_end; k0 += R) {
-            uint32_t iu[R];
-            bool live[R];
+        const uint32_t threads = gridDim.x * 256;
+        const LinePart lp = line_part(n, line, threads);
+        for (uint32_t q = blk * 256 + threadIdx.x; q < lp.items; q += threads) {
+            uint32_t c, k_beg, k_end;
+            line_item(lp, q, c, k_beg, k_end);
+            for (uint32_t k0 = k_beg; k0 < k_end; k0 += R) {
+                uint32_t iu[R];
+                bool live[R];
 #pragma unroll
-            for (int u = 0; u < R; ++u) {
-                iu[u] = c + (k0 + u) * line;
-                live[u] = k0 + u < k_end && iu[u] < n;
-                if (!live[u]) iu[u] = c;  // a row in range, computed and dropped
+                for (int u = 0; u < R; ++u) {
+                    iu[u] = c + (k0 + u) * line;
+                    live[u] = k0 + u < k_end && iu[u] < n;
+                    if (!live[u]) iu[u] = c;  // a row in range, computed and dropped
+                }
+                pn_rows<K, SYM, T, R>(n, off, rv, rp, rz, b, y, iu, live, a);
             }
-            pn_rows<K, SYM, T, R>(n, off, rv, rp, rz, b, y, iu, live, a);
         }
     }
     a = block_sum256(a, sh);
@@ -701,7 +701,10 @@ public:
         // ran slower than the separate p update (128^3 Jacobi-PCG 39.4-40.5
         // against 37.2-37.3 ms), in FP32 faster (mixed 22.3-22.6 against
         // 23.2 ms, clsparseScsrcg 13.6-13.7 against 14.1 ms, the same iterates).
-        fused_ = std::is_same_v<T, float> && prec_ != kPrecFsai && adia_.vals != nullptr && cg_fuse_enabled();
+        // Its buffer resources take K n sizeof(T) bytes in 32 bits; larger
+        // systems keep the separate kernels (whose SpMV then takes flat loads).
+        fused_ = std::is_same_v<T, float> && prec_ != kPrecFsai && adia_.vals != nullptr && cg_fuse_enabled() &&
+                 dia_buf_fits(adia_.ndiag, n, sizeof(T));
         if (dia != nullptr && std::is_same_v<T, float> && g_bf16) {
             use_dia_ = true;
             dia_ = *dia;
@@ -1106,7 +1109,72 @@ clsparseStatus cg_entry(cldenseVector* x, const clsparseCsrMatrix* A, const clde
     return status;
 }
 
+// y = A x through the diagonal storage the CG solvers move a stencil matrix's
+// values to: the plain SpMV (fused == 0) or the FP32-solver kernel that forms
+// its operand as p' = fma(0, x, x) on the first iteration (fused != 0).
+template <typename T>
+clsparseStatus csrmv_diag(const clsparseCsrMatrix* A, const cldenseVector* x, cldenseVector* y, int fused,
+                          cl_int* ndiag, clsparseControl control)
+{
+    CLS_TRY(check_init_control(control));
+    if (A == nullptr || x == nullptr || y == nullptr || ndiag == nullptr) return clsparseInvalidValue;
+    *ndiag = 0;
+    if (A->num_rows != A->num_cols || x->num_values < A->num_cols || y->num_values < A->num_rows)
+        return clsparseInvalidSize;
+    const size_t n = A->num_rows;
+    const T* xp = dev_ptr<T>(x->values, x->off_values);
+    T* yp = dev_ptr<T>(y->values, y->off_values);
+    if (n > 0 && (xp == nullptr || yp == nullptr)) return clsparseInvalidMemObject;
+    hipStream_t s = control->solver_stream;
+    CLS_TRY(enter_solver_stream(control));
+    DiaOffsets off{};
+    int nd = 0;
+    CLS_TRY(dia_candidates(A, reinterpret_cast<int32_t*>(control->red_partials), s, &off, &nd));
+    if (nd == 0) return leave_solver_stream(control);
+    void* work = nullptr;
+    CLS_TRY(control_workspace(control,
+                              align_up(nd * n * sizeof(T)) + align_up(sizeof(int)) +
+                                  align_up(sizeof(CgDevState)) + align_up(kReducePartialsMax * sizeof(double)),
+                              &work));
+    Carve c{static_cast<char*>(work)};
+    T* vals = c.take<T>(nd * n);
+    int* bad = c.take<int>(1);
+    CgDevState* st = c.take<CgDevState>(1);
+    double* part = c.take<double>(kReducePartialsMax);
+    bool sym = false;
+    CLS_TRY((dia_fill<T, T>(A, n, &nd, off, vals, bad, s, &sym)));
+    if (nd == 0) return leave_solver_stream(control);
+    const DiaMatrix<T> D{vals, nd, sym, off};
+    const uint32_t g = grid_for(n);
+    if (fused && dia_buf_fits(nd, n, sizeof(T))) {
+        CgDevState init{};
+        init.first = 1;  // beta = 0 and z = p: p' = p
+        CLS_HIP_TRY(hipMemcpyAsync(st, &init, sizeof(CgDevState), hipMemcpyHostToDevice, s));
+        CLS_TRY(launch_spmv_pn<T>(D, n, g, xp, xp, yp, part, part, part, part, st, s));
+    } else {
+        CLS_TRY(launch_dia_spmv<T>(D, n, g, xp, yp, nullptr, nullptr, nullptr, s));
+    }
+    CLS_HIP_TRY(hipStreamSynchronize(s));
+    *ndiag = nd;
+    CLS_TRY(leave_solver_stream(control));
+    return finish_op(control);
+}
+
 }  // namespace
+
+extern "C" clsparseStatus clsparseAmdScsrmvDiag(const clsparseCsrMatrix* A, const cldenseVector* x,
+                                                cldenseVector* y, cl_int fused, cl_int* ndiag,
+                                                const clsparseControl control)
+{
+    return csrmv_diag<float>(A, x, y, fused, ndiag, control);
+}
+
+extern "C" clsparseStatus clsparseAmdDcsrmvDiag(const clsparseCsrMatrix* A, const cldenseVector* x,
+                                                cldenseVector* y, cl_int fused, cl_int* ndiag,
+                                                const clsparseControl control)
+{
+    return csrmv_diag<double>(A, x, y, fused, ndiag, control);
+}
 
 // src/library/solvers/conjugate-gradients.cpp L18-64
 extern "C" clsparseStatus clsparseScsrcg(cldenseVector* x, const clsparseCsrMatrix* A,

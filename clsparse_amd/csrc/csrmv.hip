@@ -399,7 +399,7 @@ __global__ __launch_bounds__(64) void csrmv_longrow_combine(const LongRow* __res
         acc.mul_add(static_cast<double>(beta), static_cast<double>(y[L.row]));
         y[L.row] = static_cast<T>(acc.value());
     } else {
-        y[L.row] = fma(beta, y[L.row], static_cast<T>(acc.value()));
+        y[L.row] = static_cast<T>(fma(static_cast<double>(beta), static_cast<double>(y[L.row]), acc.value()));
     }
 }
 

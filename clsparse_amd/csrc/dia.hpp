@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "line_part.hpp"
 #include "solver_common.hpp"
 
 namespace {
@@ -257,35 +258,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             dia_rows<K, SYM, T, R>(n, off, rv, rx, rw, y, iu, live, a, a2);
         }
     } else {
-        const uint32_t threads = gridDim.x * 256, t = blk * 256 + threadIdx.x;
-        const uint32_t segs = max(1u, threads / line);  // stretches per line
-        const uint32_t c = t % line, sg = t / line;
-        const uint32_t m = (n + line - 1) / line;       // rows per line
-        const uint32_t sl = (m + segs - 1) / segs;
-        const uint32_t k_end = sg < segs ? min(m, (sg + 1) * sl) : 0u;
-        if constexpr (SYM && K >= 3) {
-            if (sg * sl < k_end) {
-                const uint32_t i = c + sg * sl * line;
-                T pm = buf_load<T>(rx, (i - line) * S, 0);  // wraps below 0: 0
-                T p0 = buf_load<T>(rx, i * S, 0);
-                T f0 = buf_load<T>(rv, i < n ? i * S : 0xFFFFFFFFu, 0);
-                for (uint32_t k0 = sg * sl; k0 < k_end; k0 += 2) {
-                    const uint32_t i0 = c + k0 * line;
-                    dia_line_pair<K, T>(n, off, rv, rx, rw, y, i0, line, i0 < n,
-                                        k0 + 1 < k_end && i0 + line < n, pm, p0, f0, a, a2);
+        const uint32_t threads = gridDim.x * 256;
+        const LinePart lp = line_part(n, line, threads);
+        for (uint32_t q = blk * 256 + threadIdx.x; q < lp.items; q += threads) {
+            uint32_t c, k_beg, k_end;
+            line_item(lp, q, c, k_beg, k_end);
+            if constexpr (SYM && K >= 3) {
+                if (k_beg < k_end) {
+                    const uint32_t i = c + k_beg * line;
+                    T pm = buf_load<T>(rx, (i - line) * S, 0);  // wraps below 0: 0
+                    T p0 = buf_load<T>(rx, i * S, 0);
+                    T f0 = buf_load<T>(rv, i < n ? i * S : 0xFFFFFFFFu, 0);
+                    for (uint32_t k0 = k_beg; k0 < k_end; k0 += 2) {
+                        const uint32_t i0 = c + k0 * line;
+                        dia_line_pair<K, T>(n, off, rv, rx, rw, y, i0, line, i0 < n,
+                                            k0 + 1 < k_end && i0 + line < n, pm, p0, f0, a, a2);
+                    }
                 }
-            }
-        } else {
-            for (uint32_t k0 = sg * sl; k0 < k_end; k0 += R) {
-                uint32_t iu[R];
-                bool live[R];
+            } else {
+                for (uint32_t k0 = k_beg; k0 < k_end; k0 += R) {
+                    uint32_t iu[R];
+                    bool live[R];
 #pragma unroll
-                for (int u = 0; u < R; ++u) {
-                    iu[u] = c + (k0 + u) * line;
-                    live[u] = k0 + u < k_end && iu[u] < n;
-                    if (!live[u]) iu[u] = c;  // a row in range, computed and dropped
+                    for (int u = 0; u < R; ++u) {
+                        iu[u] = c + (k0 + u) * line;
+                        live[u] = k0 + u < k_end && iu[u] < n;
+                        if (!live[u]) iu[u] = c;  // a row in range, computed and dropped
+                    }
+                    dia_rows<K, SYM, T, R>(n, off, rv, rx, rw, y, iu, live, a, a2);
                 }
-                dia_rows<K, SYM, T, R>(n, off, rv, rx, rw, y, iu, live, a, a2);
             }
         }
     }
@@ -390,7 +391,7 @@ template <int K, bool SYM, typename T>
 void launch_dia_spmv_dot(size_t n, uint32_t g, const DiaOffsets& off, const T* vals, const T* p, T* y,
                          const T* w, double* part, double* part2, const int* done, hipStream_t s)
 {
-    if (static_cast<uint64_t>(K) * n * sizeof(T) < (uint64_t(1) << 32) - 16) {
+    if (dia_buf_fits(K, n, sizeof(T))) {
         // Line order when the largest offset is long (a 3-D grid's n_x n_y)
         // and the matrix holds several lines (CLSPARSE_AMD_DIA_LINES=0: off).
         const uint32_t line = dia_line(off, K, n);

@@ -190,6 +190,21 @@ CLSPARSE_EXPORT clsparseStatus clsparseAmdHostCsrFree(clsparseAmdHostCsr* m);
 CLSPARSE_EXPORT clsparseStatus
     clsparseAmdGenPoisson2D(cl_uint nx, cl_uint ny, clsparseAmdHostCsr* out);
 
+/* y = A x for a square stencil matrix through the diagonal storage the CG
+ * solvers (clsparse[S/D]csrcg, the refinement, learned-FSAI and BiCGStab
+ * solvers) move its values to: the same detection, fill and SpMV kernels, so
+ * that the solvers' SpMV can be checked against clsparse[S/D]csrmv. fused != 0
+ * runs the FP32 solvers' kernel that forms its operand on the fly (first
+ * iteration: the operand is x itself). *ndiag receives the number of
+ * diagonals; 0 when A is not a stencil of at most 9 diagonals, and then y is
+ * not written. No reference counterpart. */
+CLSPARSE_EXPORT clsparseStatus clsparseAmdScsrmvDiag(const clsparseCsrMatrix* A, const cldenseVector* x,
+                                                     cldenseVector* y, cl_int fused, cl_int* ndiag,
+                                                     const clsparseControl control);
+CLSPARSE_EXPORT clsparseStatus clsparseAmdDcsrmvDiag(const clsparseCsrMatrix* A, const cldenseVector* x,
+                                                     cldenseVector* y, cl_int fused, cl_int* ndiag,
+                                                     const clsparseControl control);
+
 /* sum = sum over the stored entries of values[j] * x[col_indices[j]] =
  * 1^T (A x), accumulated in FP64 in a fixed order and written to the device
  * scalar in the matrix's precision. One pass in storage order that ignores the

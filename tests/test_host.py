@@ -260,3 +260,54 @@ def test_committed_fsai_weights_load_and_checksum():
     assert p.shape == (cls.load().clsparseAmdMlpNumParams(),) and np.all(np.isfinite(p))
     assert rec["held_out_loss_learned_mean"] < rec["held_out_loss_jacobi_mean"]
     assert rec["train"]["matrices"] == 1800
+
+
+def test_line_order_partition_covers_every_row_once(tmp_path):
+    """The line-order row partition of the diagonal-storage kernels
+    (clsparse_amd/csrc/line_part.hpp), walked on the CPU exactly as the
+    kernels walk it: every row of the matrix is visited once, for grids with
+    more threads than a line has columns (several stretches per line) and
+    with fewer (a 600 x 600 x 10 grid's line of 360,000 rows against 262,144
+    threads), odd line counts and n not a multiple of the line; and the
+    32-bit buffer predicate."""
+    src = tmp_path / "part.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include <vector>
+#include "line_part.hpp"
+static int check(uint32_t n, uint32_t line, uint32_t threads) {
+  std::vector<unsigned char> seen(n, 0);
+  const LinePart p = line_part(n, line, threads);
+  for (uint32_t t = 0; t < threads; ++t)
+    for (uint32_t q = t; q < p.items; q += threads) {
+      uint32_t c, kb, ke;
+      line_item(p, q, c, kb, ke);
+      for (uint32_t k = kb; k < ke; ++k) {
+        const uint64_t i = uint64_t(c) + uint64_t(k) * line;
+        if (i < n && seen[i]++) { printf("twice %u %u %u row %llu\n", n, line, threads, (unsigned long long)i); return 1; }
+      }
+    }
+  for (uint32_t i = 0; i < n; ++i)
+    if (!seen[i]) { printf("missed %u %u %u row %u\n", n, line, threads, i); return 1; }
+  return 0;
+}
+int main() {
+  const uint32_t cases[][3] = {
+    {64u * 64 * 16, 4096, 1024 * 256}, {64u * 64 * 16, 4096, 16 * 256},
+    {600u * 600 * 10, 360000, 1024 * 256}, {640u * 640 * 16, 409600, 1024 * 256},
+    {4096u * 9 + 77, 4096, 256}, {4096u * 9 + 77, 4096, 3 * 256}, {5000u * 15, 5000, 1024 * 256},
+    {5000u * 15 - 1, 5000, 7 * 256}, {160u * 160 * 160, 25600, 1024 * 256}, {300001u, 37500, 1000 * 256},
+  };
+  int bad = 0;
+  for (const auto& c : cases) bad |= check(c[0], c[1], c[2]);
+  bad |= !dia_buf_fits(7, 153391689u, 4) ? 0 : 1;       // 7 n 4 bytes passes 2^32 - 16
+  bad |= dia_buf_fits(7, 153391688u, 4) ? 0 : 1;        // just fits
+  bad |= dia_buf_fits(4, 4096000u, 8) ? 0 : 1;
+  printf(bad ? "FAIL\n" : "OK\n");
+  return bad;
+}''')
+    exe = tmp_path / "part"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "clsparse_amd", "csrc"),
+                    str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.strip().endswith("OK"), out.stdout
