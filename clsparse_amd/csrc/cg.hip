@@ -1301,7 +1301,18 @@ __global__ void state_resume(CgDevState* st, const double* __restrict__ rnorm)
     }
 }
 
-template <typename TI>
+// CLSPARSE_AMD_MIXED_X64=1: the mixed solvers keep the solution in FP64 and
+// update it every iteration (default: an FP32 increment, folded into the FP64
+// x at each residual replacement).
+bool mixed_x64()
+{
+    const char* e = std::getenv("CLSPARSE_AMD_MIXED_X64");
+    return e && e[0] == '1';
+}
+
+// X64 (mixed only): the inner iterations update the caller's FP64 x in every
+// iteration instead of an FP32 increment folded in at each replacement.
+template <typename TI, bool X64 = false>
 clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, const cldenseVector* b,
                               clSParseSolverControl sc, clsparseControl control,
                               clsparseAmdFsai fsai = nullptr)
@@ -1315,6 +1326,7 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
         return clsparseInvalidSystemSize;
 
     constexpr bool kMixed = sizeof(TI) == sizeof(float);
+    constexpr bool kIncr = kMixed && !X64;  // FP32 solution increment
     const size_t n = A->num_rows;
     const size_t nnz = A->num_nonzeros;
     if (fsai && fsai_n(fsai) != n) return clsparseInvalidSystemSize;
@@ -1349,7 +1361,7 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     // factor also the pilot race's start point and the learned factor's
     // pilot end point.
     const size_t bytes = (kMixed ? align_up(nnz * sizeof(TI)) : 0) + align_up(n * sizeof(double)) +
-                         align_up(n * (kMixed ? sizeof(TI) : sizeof(double))) +
+                         align_up(n * (kIncr ? sizeof(TI) : sizeof(double))) +
                          align_up(8 * sizeof(double)) + align_up(n * sizeof(TI)) +
                          CgBuffers<TI>::bytes(n) + (fsai ? 2 * align_up(n * sizeof(double)) : 0) +
                          (a_ndiag ? align_up(a_ndiag * n * sizeof(TI)) + align_up(sizeof(int)) : 0);
@@ -1361,8 +1373,8 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     Carve c{static_cast<char*>(work)};
     TI* valsI = kMixed ? c.take<TI>(nnz) : nullptr;
     double* r64 = c.take<double>(n);
-    TI* xd = kMixed ? c.take<TI>(n) : nullptr;              // mixed: increment since the last flush
-    double* xsave = kMixed ? nullptr : c.take<double>(n);  // FP64: last replacement point
+    TI* xd = kIncr ? c.take<TI>(n) : nullptr;              // mixed: increment since the last flush
+    double* xsave = kIncr ? nullptr : c.take<double>(n);  // FP64 x: last replacement point
     double* sc64 = c.take<double>(8);  // [0]=||r||_2, [1]=||b||_2, [2]=-1, [3]=+1
     TI* rI = c.take<TI>(n);            // TI start residual; FSAI temporary afterwards
     CgBuffers<TI> buf;
@@ -1442,20 +1454,20 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     hipLaunchKernelGGL((fill_kernel<TI>), dim3(1), dim3(256), 0, s, size_t(1), buf.one, TI(1));
     hipLaunchKernelGGL((fill_kernel<TI>), dim3(1), dim3(256), 0, s, size_t(1), buf.zero, TI(0));
     // The increment starts at zero (every true residual below folds it in).
-    if constexpr (kMixed) hipLaunchKernelGGL((fill_kernel<TI>), dim3(ew_grid(n)), dim3(256), 0, s, n, xd, TI(0));
+    if constexpr (kIncr) hipLaunchKernelGGL((fill_kernel<TI>), dim3(ew_grid(n)), dim3(256), 0, s, n, xd, TI(0));
     CLS_HIP_TRY(hipGetLastError());
 
     // Mixed: x += xd, xd = 0 (flush) or xd = 0 (discard, back to the last
     // flush point); FP64 engines update x itself.
     auto flush = [&]() -> clsparseStatus {
-        if constexpr (kMixed) {
+        if constexpr (kIncr) {
             hipLaunchKernelGGL((flush_increment<TI>), dim3(ew_grid(n)), dim3(256), 0, s, n, xp, xd);
             CLS_HIP_TRY(hipGetLastError());
         }
         return clsparseSuccess;
     };
     auto discard = [&]() -> clsparseStatus {
-        if constexpr (kMixed) {
+        if constexpr (kIncr) {
             hipLaunchKernelGGL((fill_kernel<TI>), dim3(ew_grid(n)), dim3(256), 0, s, n, xd, TI(0));
             CLS_HIP_TRY(hipGetLastError());
         }
@@ -1463,7 +1475,7 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     };
     // Back to the last replacement point after a breakdown.
     auto restore = [&]() -> clsparseStatus {
-        if constexpr (kMixed)
+        if constexpr (kIncr)
             return discard();
         else
             CLS_HIP_TRY(hipMemcpyAsync(xp, xsave, n * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -1494,9 +1506,9 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     *hs = init;
     CLS_HIP_TRY(hipMemcpyAsync(buf.st, hs, sizeof(CgDevState), hipMemcpyHostToDevice, s));
 
-    using Engine = CgEngine<TI, std::conditional_t<kMixed, TI, double>>;
+    using Engine = CgEngine<TI, std::conditional_t<kIncr, TI, double>>;
     auto* xe = [&] {
-        if constexpr (kMixed)
+        if constexpr (kIncr)
             return xd;
         else
             return xp;
@@ -1538,7 +1550,7 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     if (rel <= sc->relativeTolerance) {
         detail = 0;
     } else {
-        if constexpr (!kMixed)
+        if constexpr (!kIncr)
             CLS_HIP_TRY(hipMemcpyAsync(xsave, xp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         CLS_TRY(restart());
         double res_at_replace = rel;
@@ -1657,7 +1669,7 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
                 break;
             }
             if (out_of_iters || iters >= sc->maxIters) break;
-            if constexpr (!kMixed)
+            if constexpr (!kIncr)
                 CLS_HIP_TRY(hipMemcpyAsync(xsave, xp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
             // r = (TI) r64; z' = M r into y; <r, z'> partials -> part_yp;
             // re-base p and z; the new <r, z> becomes the previous one.
@@ -1708,6 +1720,7 @@ extern "C" clsparseStatus clsparseAmdDcsrcgMixed(cldenseVector* x, const clspars
                                                  clSParseSolverControl sc,
                                                  clsparseControl control)
 {
+    if (mixed_x64()) return reliable_solve<float, true>(x, A, b, sc, control);
     return reliable_solve<float>(x, A, b, sc, control);
 }
 
@@ -1724,6 +1737,7 @@ extern "C" clsparseStatus clsparseAmdDcsrcgFsai(cldenseVector* x, const clsparse
                                                 clSParseSolverControl sc, clsparseControl control)
 {
     if (fsai == nullptr) return clsparseInvalidValue;
+    if (mixed_x64()) return reliable_solve<float, true>(x, A, b, sc, control, fsai);
     return reliable_solve<float>(x, A, b, sc, control, fsai);
 }
 

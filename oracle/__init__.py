@@ -43,6 +43,11 @@ def load():
                                     ctypes.c_float, f32p, f64p, f64p]
         lib.orc_spmv_omp_d.restype = None
         lib.orc_spmv_omp_d.argtypes = [c_uint32, u32p, u32p, f64p, f64p, f64p, c_int]
+        lib.orc_spmv_omp_s.restype = None
+        lib.orc_spmv_omp_s.argtypes = [c_uint32, u32p, u32p, f32p, f32p, f32p, c_int]
+        lib.orc_cg_omp_d.restype = c_int
+        lib.orc_cg_omp_d.argtypes = [c_uint32, u32p, u32p, f64p, f64p, f64p, c_int, c_double, c_int,
+                                     c_int, f64p]
         lib.orc_max_threads.restype = c_int
         for nm in ("orc_nrm1_d", "orc_nrm2_d"):
             getattr(lib, nm).restype = c_double
@@ -110,6 +115,27 @@ def csrmv(rp, ci, v, x, alpha=1.0, beta=0.0, y=None):
 def spmv_omp(rp, ci, v, x, y, threads):
     load().orc_spmv_omp_d(rp.size - 1, _p(rp, c_uint32), _p(ci, c_uint32), _p(v, c_double),
                           _p(x, c_double), _p(y, c_double), threads)
+
+
+def spmv_omp_s(rp, ci, v, x, y, threads):
+    load().orc_spmv_omp_s(rp.size - 1, _p(rp, c_uint32), _p(ci, c_uint32), _p(v, ctypes.c_float),
+                          _p(x, ctypes.c_float), _p(y, ctypes.c_float), threads)
+
+
+def cg_omp(rp, ci, v, b, max_iters, rel_tol, crit=1, threads=1):
+    """FP64 Jacobi-PCG from x = 0 on `threads` host threads until the residual
+    (crit 1: L2, 0: the reference's L1) relative to b is <= rel_tol
+    -> (x, iterations, final residual). A timing baseline."""
+    rp = np.ascontiguousarray(rp, dtype=np.uint32)
+    ci = np.ascontiguousarray(ci, dtype=np.uint32)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(rp.size - 1)
+    res = c_double(0.0)
+    it = load().orc_cg_omp_d(rp.size - 1, _p(rp, c_uint32), _p(ci, c_uint32), _p(v, c_double),
+                             _p(b, c_double), _p(x, c_double), max_iters, rel_tol, crit, threads,
+                             ctypes.byref(res))
+    return x, int(it), res.value
 
 
 def max_threads() -> int:

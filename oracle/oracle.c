@@ -16,8 +16,10 @@
  *                    solver-control.hpp L47-64 (L1 residual, nIters quirk),
  *                    Jacobi of preconditioners/diagonal.hpp L45-77; dot and
  *                    norm reductions in double.
- *   orc_spmv_omp_d   plain FP64 row-parallel CSR SpMV (OpenMP): the CPU
- *                    baseline timed beside the GPU.
+ *   orc_spmv_omp_d   plain FP64 / FP32 row-parallel CSR SpMV (OpenMP): the CPU
+ *   orc_spmv_omp_s   baselines timed beside the GPU.
+ *   orc_cg_omp_d     the FP64 Jacobi-PCG recurrence above on all host cores
+ *                    (OpenMP), run to a tolerance: the solvers' CPU baseline.
  *
  * Parity pins (which checks fix each function) are listed in the header of
  * tests/test_oracle.py.
@@ -235,6 +237,22 @@ void orc_spmv_omp_d(uint32_t n, const uint32_t* rp, const uint32_t* ci, const do
     (void)threads;
 }
 
+/* Plain row-parallel FP32 CSR SpMV with FP32 values and sums (CPU baseline of
+ * the FP32 workloads). */
+void orc_spmv_omp_s(uint32_t n, const uint32_t* rp, const uint32_t* ci, const float* v,
+                    const float* x, float* y, int threads)
+{
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 512) num_threads(threads)
+#endif
+    for (int64_t r = 0; r < (int64_t)n; ++r) {
+        float s = 0.0f;
+        for (uint32_t j = rp[r]; j < rp[r + 1]; ++j) s += v[j] * x[ci[j]];
+        y[r] = s;
+    }
+    (void)threads;
+}
+
 int orc_max_threads(void)
 {
 #ifdef _OPENMP
@@ -387,6 +405,90 @@ orc_cg_result orc_cg_s(uint32_t n, const uint32_t* rp, const uint32_t* ci, const
                        double absTol, int nIters_prior)
 {
     ORC_CG_BODY(float)
+}
+
+/* ---- CG on all host cores (CPU baseline) ----------------------------------------
+ * The FP64 Jacobi-PCG recurrence of conjugate-gradients.hpp L57-227 with every
+ * vector loop and reduction split over `threads` OpenMP threads (loops fused
+ * where the recurrence allows: SpMV with <y,p>, the x / r / z update with <r,z>
+ * and the norm). crit 0: the reference's ||r||_1 / ||b||_1 <= relTol; crit 1:
+ * ||r||_2 / ||b||_2 <= relTol (the stopping rule of the FP64 refinement solver
+ * it is timed against). Returns the iterations run; x in/out (x0 = x). The
+ * reduction order depends on the thread count: a timing baseline, not a parity
+ * oracle (orc_cg_d is that). */
+int orc_cg_omp_d(uint32_t n, const uint32_t* rp, const uint32_t* ci, const double* v,
+                 const double* b, double* x, int maxIters, double relTol, int crit, int threads,
+                 double* final_residual)
+{
+    double* y = (double*)malloc((size_t)n * sizeof(double));
+    double* z = (double*)malloc((size_t)n * sizeof(double));
+    double* r = (double*)malloc((size_t)n * sizeof(double));
+    double* p = (double*)malloc((size_t)n * sizeof(double));
+    double* invD = (double*)malloc((size_t)n * sizeof(double));
+    double norm_b = 0, norm_r = 0, rz = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(threads) reduction(+ : norm_b, norm_r, rz)
+#endif
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        double d = 1.0, s = 0.0;
+        for (uint32_t j = rp[i]; j < rp[i + 1]; ++j) {
+            if (ci[j] == (uint32_t)i && d == 1.0) d = 1.0 / v[j];
+            s = fma(v[j], x[ci[j]], s);
+        }
+        invD[i] = d;
+        r[i] = b[i] - s;
+        z[i] = r[i] * d;
+        p[i] = z[i];
+        rz += r[i] * z[i];
+        norm_b += crit ? b[i] * b[i] : fabs(b[i]);
+        norm_r += crit ? r[i] * r[i] : fabs(r[i]);
+    }
+    if (crit) {
+        norm_b = sqrt(norm_b);
+        norm_r = sqrt(norm_r);
+    }
+    double res = norm_b > 0 ? norm_r / norm_b : 0.0;
+    int it = 0;
+    while (res > relTol && it < maxIters) {
+        double yp = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(threads) reduction(+ : yp)
+#endif
+        for (int64_t i = 0; i < (int64_t)n; ++i) {
+            double s = 0.0;
+            for (uint32_t j = rp[i]; j < rp[i + 1]; ++j) s = fma(v[j], p[ci[j]], s);
+            y[i] = s;
+            yp += s * p[i];
+        }
+        const double alpha = rz / yp;
+        double rz_new = 0, nr = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(threads) reduction(+ : rz_new, nr)
+#endif
+        for (int64_t i = 0; i < (int64_t)n; ++i) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * y[i];
+            z[i] = r[i] * invD[i];
+            rz_new += r[i] * z[i];
+            nr += crit ? r[i] * r[i] : fabs(r[i]);
+        }
+        const double beta = rz_new / rz;
+        rz = rz_new;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(threads)
+#endif
+        for (int64_t i = 0; i < (int64_t)n; ++i) p[i] = z[i] + beta * p[i];
+        res = (crit ? sqrt(nr) : nr) / norm_b;
+        ++it;
+    }
+    if (final_residual) *final_residual = res;
+    free(y);
+    free(z);
+    free(r);
+    free(p);
+    free(invD);
+    (void)threads;
+    return it;
 }
 
 /* ---- BiCGStab ------------------------------------------------------------------

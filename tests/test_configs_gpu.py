@@ -68,11 +68,15 @@ def diffusion128():
     return rp, ci, v, m, b
 
 
+@pytest.mark.parametrize("x64", ["0", "1"], ids=["fp32-increment", "fp64-x"])
 @pytest.mark.parametrize("dia", ["1", "0"], ids=["diagonal-storage", "csr-adaptive"])
-def test_mixed_solver_128cubed_to_1e10(ctl, monkeypatch, diffusion128, dia):
+def test_mixed_solver_128cubed_to_1e10(ctl, monkeypatch, diffusion128, dia, x64):
+    """x64 = 1: the solution is updated in FP64 every iteration
+    (CLSPARSE_AMD_MIXED_X64) instead of through an FP32 increment."""
     monkeypatch.setenv("CLSPARSE_AMD_DIA", dia)
+    monkeypatch.setenv("CLSPARSE_AMD_MIXED_X64", x64)
     rp, ci, v, m, b = diffusion128
-    say(f"128^3 mixed solver to 1e-10, dia={dia}")
+    say(f"128^3 mixed solver to 1e-10, dia={dia} x64={x64}")
     A = cls.CsrMatrix(rp, ci, v, m)
     A.meta_create(ctl)
     st, x = _solve(ctl, cls.load().clsparseAmdDcsrcgMixed, A, b, 1e-10)
@@ -130,11 +134,15 @@ def test_learned_fsai_and_fp64_jacobi_160cubed_to_1e8(ctl, monkeypatch, target16
     assert st_l.iterations < 0.75 * st_j.iterations, (st_l.iterations, st_j.iterations)
 
 
-def test_learned_fsai_160cubed_on_csr_adaptive(ctl, monkeypatch, target160, net):
-    """A, G and G^T through CSR-Adaptive SpMVs (no diagonal storage)."""
+@pytest.mark.parametrize("bf16,x64", [("1", "0"), ("0", "1")], ids=["bf16-g", "fp32-g-fp64-x"])
+def test_learned_fsai_160cubed_on_csr_adaptive(ctl, monkeypatch, target160, net, bf16, x64):
+    """A, G and G^T through CSR-Adaptive SpMVs (no diagonal storage); G's
+    values in BF16 or FP32, the solution as an FP32 increment or in FP64."""
     monkeypatch.setenv("CLSPARSE_AMD_DIA", "0")
     monkeypatch.setenv("CLSPARSE_AMD_FSAI_DIA", "0")
     monkeypatch.setenv("CLSPARSE_AMD_FSAI_POLICY", "always")
+    monkeypatch.setenv("CLSPARSE_AMD_FSAI_BF16", bf16)
+    monkeypatch.setenv("CLSPARSE_AMD_MIXED_X64", x64)
     rp, ci, v, m, b = target160
     A = cls.CsrMatrix(rp, ci, v, m)
     A.meta_create(ctl)

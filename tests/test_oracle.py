@@ -357,3 +357,32 @@ def test_bicgstab_iteration_count_depends_on_summation_order():
     counts = {run(seq), run(lambda u, w: float(np.sum(u * w))), run(lambda u, w: seq(u[::-1], w[::-1]))}
     assert ro.n_iters in counts or min(abs(ro.n_iters - k) for k in counts) <= 2
     assert max(counts) - min(counts) <= 0.15 * max(counts)
+
+
+def test_oracle_cg_omp_baseline_runs_the_reference_recurrence():
+    """The all-core CPU baseline (orc_cg_omp_d) runs the same Jacobi-PCG
+    recurrence as orc_cg_d: on the reference's L1 criterion it stops at the
+    same iteration (reductions differ only in order), on any thread count; on
+    the L2 criterion its solution's true residual meets the tolerance; the
+    diagonal closed form converges in one iteration; FP32 SpMV baseline =
+    FP32 products of the FP64 oracle to rounding."""
+    import clsparse_amd as cls
+    rp, ci, v, m = cls.gen_diffusion3d(18, 16, 14, sigma=1.0, seed=5)
+    b = cls.gen_uniform(m, -1, 1, seed=6)
+    _, ref = oracle.cg(rp, ci, v, b, np.zeros(m), 1, 5000, 1e-8, 0.0)
+    for th in (1, 3, 8):
+        x, it, res = oracle.cg_omp(rp, ci, v, b, 5000, 1e-8, crit=0, threads=th)
+        assert abs(it - (ref.n_iters + 1)) <= 1, (th, it, ref.n_iters)
+        assert res <= 1e-8
+        x, it, res = oracle.cg_omp(rp, ci, v, b, 5000, 1e-10, crit=1, threads=th)
+        ax, _ = oracle.csrmv(rp, ci, v, x)
+        assert np.linalg.norm(b - ax) / np.linalg.norm(b) <= 1.01e-10
+    d = np.array([2.0, 5.0, 0.5, 4.0])
+    rpd = np.arange(5, dtype=np.uint32)
+    x, it, _ = oracle.cg_omp(rpd, np.arange(4, dtype=np.uint32), d, np.ones(4), 10, 1e-14, 1, 2)
+    assert it == 1 and np.allclose(x, 1 / d, rtol=1e-15)
+    xs = cls.gen_uniform(m, -1, 1, seed=7).astype(np.float32)
+    y32 = np.zeros(m, np.float32)
+    oracle.spmv_omp_s(rp, ci, v.astype(np.float32), xs, y32, 4)
+    yref, scale = oracle.csrmv(rp, ci, v.astype(np.float32), xs)
+    assert np.all(np.abs(y32 - yref) <= 8 * 2.0 ** -24 * scale)
