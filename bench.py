@@ -50,21 +50,50 @@ def spmv_bytes(n, ncols, nnz, value_bytes=8):
     return 4 * (nnz + n) + value_bytes * (nnz + ncols + n)
 
 
-def cpu_baseline(rp, ci, v, x, budget_s=3.0, max_calls=30):
+def cpu_baseline(rp, ci, v, x, budget_s=3.0, max_calls=30, fp32=False):
+    """Median time of the oracle's OpenMP CSR SpMV on all host cores (FP64, or
+    FP32 values and sums with fp32)."""
     import numpy as np
 
     import oracle
     threads = oracle.max_threads()
-    y = np.empty(rp.size - 1)
-    oracle.spmv_omp(rp, ci, v, x, y, threads)  # warm-up
+    if fp32:
+        v, x = np.ascontiguousarray(v, np.float32), np.ascontiguousarray(x, np.float32)
+        y = np.empty(rp.size - 1, np.float32)
+        fn = oracle.spmv_omp_s
+    else:
+        y = np.empty(rp.size - 1)
+        fn = oracle.spmv_omp
+    fn(rp, ci, v, x, y, threads)  # warm-up
     times = []
     t_end = time.perf_counter() + budget_s
     while len(times) < max_calls and (len(times) < 3 or time.perf_counter() < t_end):
         t0 = time.perf_counter()
-        oracle.spmv_omp(rp, ci, v, x, y, threads)
+        fn(rp, ci, v, x, y, threads)
         times.append(time.perf_counter() - t0)
     times.sort()
     return times[len(times) // 2], threads, len(times)
+
+
+def reference_methodology(call, torch, samples=20, calls=20):
+    """The reference's SpMV timing (docs/performance/clSPARSE-0.8.0.0/W9100/README.md
+    L15-20; clsparseTimer-device.cpp L377-431): `samples` host-timed batches of
+    `calls` back-to-back calls (synchronised after each batch), samples more
+    than one standard deviation from the mean dropped, mean of the rest per call."""
+    import numpy as np
+    for _ in range(3):
+        call()
+    torch.cuda.synchronize()
+    per = []
+    for _ in range(samples):
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            call()
+        torch.cuda.synchronize()
+        per.append((time.perf_counter() - t0) / calls)
+    a = np.array(per)
+    keep = a[np.abs(a - a.mean()) <= a.std()]
+    return float(keep.mean()), int(keep.size)
 
 
 def load_traffic():
@@ -406,32 +435,119 @@ def poisson_workload(cls, torch, ctl, stream, cpu=True):
            "note": "3.4 MB: L2/Infinity-Cache resident, so not an HBM figure"}
     run.close()
     if cpu:
-        med, threads, calls = cpu_baseline(rp, ci, v, x, budget_s=1.0, max_calls=50)
+        med, threads, calls = cpu_baseline(rp, ci, v, x, budget_s=1.0, max_calls=50, fp32=True)
         out["cpu_baseline"] = {"us_per_call": round(med * 1e6, 1), "cores": threads, "kind": "port",
-                               "sample": f"{calls} calls, oracle OpenMP CSR SpMV (FP64 values)"}
+                               "sample": f"median of {calls} calls, oracle OpenMP CSR SpMV with FP32 "
+                                         "values and sums"}
+        med1, _, calls1 = cpu_baseline_1t(rp, ci, v, x)
+        out["cpu_baseline"]["single_thread_us_per_call"] = round(med1 * 1e6, 1)
     return out
 
 
-def cpu_cg_baseline(rp, ci, v, b, iters_needed, sample_iters=24):
-    """The oracle's FP64 Jacobi-CG (single thread, the reference's algorithm) on
-    `sample_iters` iterations of the same system; seconds per iteration and the
-    projection to the GPU solve's iteration count."""
+def cpu_baseline_1t(rp, ci, v, x, calls=20):
+    """The FP32 oracle SpMV on one thread (median of `calls`)."""
     import numpy as np
 
     import oracle
-    t0 = time.perf_counter()
-    _, r = oracle.cg(rp, ci, v, b, np.zeros(rp.size - 1), 1, sample_iters, 1e-300, 0.0)
-    dt = time.perf_counter() - t0
-    per = dt / max(r.executed, 1)
-    return {"kind": "port", "cores": 1, "seconds_per_iteration": round(per, 5),
-            "projected_seconds": round(per * iters_needed, 3),
-            "sample": f"{r.executed} iterations of the oracle's FP64 Jacobi-CG on the same matrix, "
-                      "single thread, projected to the FP64 GPU solve's iteration count"}
+    v, x = np.ascontiguousarray(v, np.float32), np.ascontiguousarray(x, np.float32)
+    y = np.empty(rp.size - 1, np.float32)
+    ts = []
+    for _ in range(calls + 1):
+        t0 = time.perf_counter()
+        oracle.spmv_omp_s(rp, ci, v, x, y, 1)
+        ts.append(time.perf_counter() - t0)
+    ts = sorted(ts[1:])
+    return ts[len(ts) // 2], 1, calls
+
+
+def cpu_cg_baseline(rp, ci, v, b, tol, reps=3, single_thread=False):
+    """The oracle's FP64 Jacobi-PCG (the reference's recurrence,
+    conjugate-gradients.hpp L57-227) run to tolerance from x = 0 on all host
+    cores (OpenMP), stopping on ||r||_2 / ||b||_2 <= tol like the FP64 GPU
+    solver it sits beside: median of `reps` solves; optionally one
+    single-threaded solve as well."""
+    import oracle
+    threads = oracle.max_threads()
+    times, its = [], 0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        _, its, res = oracle.cg_omp(rp, ci, v, b, 100000, tol, crit=1, threads=threads)
+        times.append(time.perf_counter() - t0)
+    out = {"kind": "port", "cores": threads, "seconds": round(float(sorted(times)[len(times) // 2]), 3),
+           "seconds_all": [round(t, 3) for t in times], "iterations": its, "final_rel_residual": res,
+           "sample": f"median of {reps} full solves to ||r||_2/||b||_2 <= {tol:g}: oracle FP64 "
+                     f"Jacobi-PCG (orc_cg_omp_d), {threads} OpenMP threads"}
+    if single_thread:
+        t0 = time.perf_counter()
+        _, its1, _ = oracle.cg_omp(rp, ci, v, b, 100000, tol, crit=1, threads=1)
+        out["single_thread"] = {"cores": 1, "seconds": round(time.perf_counter() - t0, 3),
+                                "iterations": its1, "sample": "one full solve, one thread"}
+    return out
+
+
+class env:
+    """Sets library switches (environment variables) inside the block."""
+
+    def __init__(self, **kv):
+        self.kv = kv
+
+    def __enter__(self):
+        self.old = {k: os.environ.get(k) for k in self.kv}
+        os.environ.update(self.kv)
+
+    def __exit__(self, *exc):
+        for k, v in self.old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def csr_only():
+    """Within the block the solvers keep A in CSR (CLSPARSE_AMD_DIA=0): the
+    solvers on the reference's CSR-Adaptive SpMV."""
+    return env(CLSPARSE_AMD_DIA="0")
+
+
+def timed_solves(cls, torch, ctl, fn, A, bt, m, tol, reps=5, fsai=None, max_iters=50000):
+    """reps + 1 solves from x = 0; the first is reported on its own (it
+    captures the solver's graphs and, for a new learned factor, runs the
+    pilot race), the median and spread of the other reps are the figure."""
+    rows = []
+    for _ in range(reps + 1):
+        x = torch.zeros(m, dtype=torch.float64, device="cuda")
+        sc = cls.SolverControl(cls.DIAGONAL, max_iters, tol, 0.0)
+        xd, bd = cls.DenseVector(x), cls.DenseVector(bt)
+        args = (ctypes.byref(xd.s), ctypes.byref(A.s), ctypes.byref(bd.s)) + \
+            ((fsai.handle,) if fsai is not None else ()) + (sc.handle, ctl.handle)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = fn(*args)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        s = sc.stats()
+        sc.release()
+        cls.check("solve", st)
+        rows.append((dt, s))
+    first, rest = rows[0], rows[1:]
+    ts = sorted(t for t, _ in rest)
+    s = rest[-1][1]
+    out = {"seconds": round(ts[len(ts) // 2], 5), "seconds_min_max": [round(ts[0], 5), round(ts[-1], 5)],
+           "solves": len(rest), "iterations": s.iterations, "replacements": s.outer_iterations,
+           "true_rel_residual": s.true_residual, "converged": s.status_detail == 0,
+           "preconditioner": s.preconditioner, "a_diagonals": s.a_diagonals, "g_diagonals": s.g_diagonals,
+           "first_solve": {"seconds": round(first[0], 5), "iterations": first[1].iterations}}
+    if fsai is not None and first[1].pilot_rate_fsai > 0:
+        # the first solve's pilot race: -ln(residual ratio) per second
+        out["first_solve"]["pilot_rates"] = {"fsai": round(first[1].pilot_rate_fsai, 1),
+                                             "jacobi": round(first[1].pilot_rate_jacobi, 1)}
+    return out
 
 
 def solver_workload(cls, torch, ctl, n1d=128, tol=1e-10, cpu=True):
-    """Mixed-precision refinement vs FP64 Jacobi-PCG to an FP64 true relative residual."""
-    import numpy as np
+    """Mixed-precision refinement vs FP64 Jacobi-PCG to an FP64 true relative
+    residual, with A's SpMV from diagonal storage and from CSR-Adaptive, and
+    with the solution kept as an FP32 increment or updated in FP64."""
     rp, ci, v, m = cls.gen_diffusion3d(n1d, n1d, n1d, sigma=1.0, seed=3)
     b = cls.gen_uniform(m, -1.0, 1.0, seed=4)
     A = cls.CsrMatrix(rp, ci, v, m)
@@ -439,41 +555,46 @@ def solver_workload(cls, torch, ctl, n1d=128, tol=1e-10, cpu=True):
     bt = torch.tensor(b, device="cuda")
     lib = cls.load()
     out = {"workload": f"diffusion3d_{n1d}^3_lognormal1_jacobi", "n": m, "nnz": int(rp[-1]),
-           "tolerance": tol}
-    def run(name, fn):
-        best = None
-        for _ in range(2):  # second solve is the timed one (first builds graphs, warms caches)
-            x = torch.zeros(m, dtype=torch.float64, device="cuda")
-            sc = cls.SolverControl(cls.DIAGONAL, 20000, tol, 0.0)
-            xd, bd = cls.DenseVector(x), cls.DenseVector(bt)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            st = fn(ctypes.byref(xd.s), ctypes.byref(A.s), ctypes.byref(bd.s), sc.handle,
-                    ctl.handle)
-            torch.cuda.synchronize()
-            dt = time.perf_counter() - t0
-            s = sc.stats()
-            sc.release()
-            cls.check(name, st)
-            best = {"seconds": round(dt, 5), "iterations": s.iterations,
-                    "refinements": s.outer_iterations, "true_rel_residual": s.true_residual,
-                    "a_diagonals": s.a_diagonals}
-        out[name] = best
-
-    run("mixed_fp32_inner", lib.clsparseAmdDcsrcgMixed)
-    run("fp64_jacobi_pcg", lib.clsparseAmdDcsrcgRefined)
-    with csr_only():
-        run("fp64_jacobi_pcg_csr", lib.clsparseAmdDcsrcgRefined)
-    out["speedup_mixed_vs_fp64"] = round(out["fp64_jacobi_pcg"]["seconds"] /
-                                         out["mixed_fp32_inner"]["seconds"], 3)
-    out["speedup_mixed_vs_fp64_csr"] = round(out["fp64_jacobi_pcg_csr"]["seconds"] /
-                                             out["mixed_fp32_inner"]["seconds"], 3)
+           "tolerance": tol, "timing": "median of 5 solves after a first one (reported apart)"}
+    runs = [("mixed_fp32_inner", lib.clsparseAmdDcsrcgMixed, {}),
+            ("mixed_fp32_inner_csr", lib.clsparseAmdDcsrcgMixed, {"CLSPARSE_AMD_DIA": "0"}),
+            ("mixed_fp64_x", lib.clsparseAmdDcsrcgMixed, {"CLSPARSE_AMD_MIXED_X64": "1"}),
+            ("mixed_fp64_x_csr", lib.clsparseAmdDcsrcgMixed,
+             {"CLSPARSE_AMD_MIXED_X64": "1", "CLSPARSE_AMD_DIA": "0"}),
+            ("fp64_jacobi_pcg", lib.clsparseAmdDcsrcgRefined, {}),
+            ("fp64_jacobi_pcg_csr", lib.clsparseAmdDcsrcgRefined, {"CLSPARSE_AMD_DIA": "0"})]
+    for name, fn, kv in runs:
+        with env(**kv):
+            out[name] = timed_solves(cls, torch, ctl, fn, A, bt, m, tol)
+        log(f"{name}: {out[name]}")
+    sec = {k: out[k]["seconds"] for k, _, _ in runs}
+    out["speedup_mixed_vs_fp64"] = round(sec["fp64_jacobi_pcg"] / sec["mixed_fp32_inner"], 3)
+    out["speedup_mixed_vs_fp64_csr_adaptive_both"] = round(sec["fp64_jacobi_pcg_csr"] /
+                                                           sec["mixed_fp32_inner_csr"], 3)
+    out["speedup_mixed_fp64_x_vs_fp64"] = round(sec["fp64_jacobi_pcg"] / sec["mixed_fp64_x"], 3)
+    out["metadata_create_s"] = meta_create_seconds(cls, torch, ctl, rp, ci, v, m)
     if cpu:
-        out["cpu_baseline"] = cpu_cg_baseline(rp, ci, v, b, out["fp64_jacobi_pcg"]["iterations"])
+        out["cpu_baseline"] = cpu_cg_baseline(rp, ci, v, b, tol, reps=3, single_thread=True)
+        log(f"CPU baseline: {out['cpu_baseline']}")
     out["reference_api_cg"] = reference_api_cg(cls, torch, ctl, rp, ci, v, b, m)
     out["blas1_fp64"] = blas1_workload(cls, torch, ctl, ctl.stream)
     A.meta_delete()
     return out
+
+
+def meta_create_seconds(cls, torch, ctl, rp, ci, v, m, reps=3):
+    """Median wall time of clsparseCsrMetaCreate (row pointer read back, binning
+    of csr-meta.hpp on the host, descriptors and long-row tables uploaded)."""
+    A = cls.CsrMatrix(rp, ci, v, m)
+    ts = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        A.meta_create(ctl)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+        A.meta_delete()
+    return round(sorted(ts)[len(ts) // 2], 5)
 
 
 def blas1_workload(cls, torch, ctl, stream, n=1 << 26):
@@ -552,21 +673,6 @@ def reference_api_cg(cls, torch, ctl, rp, ci, v, b, m):
     return res
 
 
-class csr_only:
-    """Within the block the solvers keep A in CSR (CLSPARSE_AMD_DIA=0): the
-    FP64 baseline with the reference's SpMV."""
-
-    def __enter__(self):
-        self.old = os.environ.get("CLSPARSE_AMD_DIA")
-        os.environ["CLSPARSE_AMD_DIA"] = "0"
-
-    def __exit__(self, *exc):
-        if self.old is None:
-            del os.environ["CLSPARSE_AMD_DIA"]
-        else:
-            os.environ["CLSPARSE_AMD_DIA"] = self.old
-
-
 def warm_solvers(cls, torch, ctl, net, fs, n1d=20):
     """One small solve of every kind (factor build, network, pilot race,
     Jacobi mixed, FP64), so that the timed first solves measure the factor's
@@ -593,9 +699,11 @@ def warm_solvers(cls, torch, ctl, net, fs, n1d=20):
 
 
 def fsai_workload(cls, torch, ctl, n1d=160, tol=1e-8, weights=None, cpu=True):
-    """Learned-FSAI mixed PCG vs Jacobi (mixed, FP64) on the held-out 3-D problem."""
-    import numpy as np
-
+    """Learned-FSAI mixed PCG vs Jacobi (mixed, FP64) on the held-out 3-D
+    problem: from diagonal storage (A and the factor) and with A, G and G^T on
+    CSR-Adaptive; G's values in BF16 and in FP32; the solution as an FP32
+    increment and in FP64. Median of 5 repeat solves each, the first solve
+    and the factor's set-up reported beside them."""
     from clsparse_amd import fsai as fs
     path = weights or fs.DEFAULT_WEIGHTS
     if not os.path.exists(path):
@@ -619,68 +727,56 @@ def fsai_workload(cls, torch, ctl, n1d=160, tol=1e-8, weights=None, cpu=True):
     f.apply_net(net, ctl)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
+    setup = t2 - t0
     out = {"workload": f"diffusion3d_{n1d}^3_az100_lognormal1", "n": m, "nnz": int(rp[-1]),
            "nnz_G": f.nnz(), "tolerance": tol, "weights_sha256": rec["sha256_float32_le"][:16],
-           "fsai_pattern_setup_s": round(t1 - t0, 4), "fsai_network_apply_s": round(t2 - t1, 5)}
-
-    def solve(fn, with_fsai):
-        best = None
-        first = None
-        for _ in range(2):  # the second solve is the reported one (the first captures graphs
-            # and, for the learned factor, runs the pilot race whose outcome the factor keeps)
-            x = torch.zeros(m, dtype=torch.float64, device="cuda")
-            sc = cls.SolverControl(cls.DIAGONAL, 50000, tol, 0.0)
-            xd, bd = cls.DenseVector(x), cls.DenseVector(bt)
-            torch.cuda.synchronize()
-            ts = time.perf_counter()
-            if with_fsai:
-                st = fn(ctypes.byref(xd.s), ctypes.byref(A.s), ctypes.byref(bd.s), f.handle,
-                        sc.handle, ctl.handle)
-            else:
-                st = fn(ctypes.byref(xd.s), ctypes.byref(A.s), ctypes.byref(bd.s), sc.handle,
-                        ctl.handle)
-            torch.cuda.synchronize()
-            dt = time.perf_counter() - ts
-            s = sc.stats()
-            sc.release()
-            cls.check("solve", st)
-            best = {"seconds": round(dt, 5), "iterations": s.iterations,
-                    "replacements": s.outer_iterations, "true_rel_residual": s.true_residual,
-                    "converged": s.status_detail == 0, "preconditioner": s.preconditioner,
-                    "a_diagonals": s.a_diagonals, "g_diagonals": s.g_diagonals}
-            if first is None:
-                first = {"seconds": best["seconds"], "iterations": best["iterations"]}
-                if with_fsai and s.pilot_rate_fsai > 0:
-                    # the first solve's pilot race: -ln(residual ratio) per second
-                    first["pilot_rates"] = {"fsai": round(s.pilot_rate_fsai, 1),
-                                            "jacobi": round(s.pilot_rate_jacobi, 1)}
-        best["first_solve"] = first
-        return best
-
-    out["learned_fsai_mixed"] = solve(lib.clsparseAmdDcsrcgFsai, True)
-    log(f"learned FSAI mixed: {out['learned_fsai_mixed']}")
-    out["jacobi_mixed"] = solve(lib.clsparseAmdDcsrcgMixed, False)
-    log(f"Jacobi mixed: {out['jacobi_mixed']}")
-    out["fp64_jacobi_pcg"] = solve(lib.clsparseAmdDcsrcgRefined, False)
-    log(f"FP64 Jacobi: {out['fp64_jacobi_pcg']}")
-    with csr_only():
-        out["fp64_jacobi_pcg_csr"] = solve(lib.clsparseAmdDcsrcgRefined, False)
-    log(f"FP64 Jacobi, CSR-Adaptive SpMV: {out['fp64_jacobi_pcg_csr']}")
+           "fsai_pattern_setup_s": round(t1 - t0, 4), "fsai_network_apply_s": round(t2 - t1, 5),
+           "timing": "median of 5 solves after a first one (reported apart)",
+           "metadata_create_s": meta_create_seconds(cls, torch, ctl, rp, ci, v, m)}
+    F = lib.clsparseAmdDcsrcgFsai
+    csr = {"CLSPARSE_AMD_DIA": "0", "CLSPARSE_AMD_FSAI_DIA": "0"}
+    runs = [("learned_fsai_mixed", F, True, {}),
+            ("learned_fsai_mixed_csr", F, True, csr),
+            ("learned_fsai_mixed_fp32_g_csr", F, True, dict(csr, CLSPARSE_AMD_FSAI_BF16="0")),
+            ("learned_fsai_mixed_fp64_x", F, True, {"CLSPARSE_AMD_MIXED_X64": "1"}),
+            ("learned_fsai_mixed_fp32_g_fp64_x_csr", F, True,
+             dict(csr, CLSPARSE_AMD_FSAI_BF16="0", CLSPARSE_AMD_MIXED_X64="1")),
+            ("jacobi_mixed", lib.clsparseAmdDcsrcgMixed, False, {}),
+            ("fp64_jacobi_pcg", lib.clsparseAmdDcsrcgRefined, False, {}),
+            ("fp64_jacobi_pcg_csr", lib.clsparseAmdDcsrcgRefined, False, {"CLSPARSE_AMD_DIA": "0"})]
+    for name, fn, with_f, kv in runs:
+        with env(**kv):
+            if with_f:
+                f.apply_net(net, ctl)  # a fresh factor: the first solve runs its pilot race
+            out[name] = timed_solves(cls, torch, ctl, fn, A, bt, m, tol, fsai=f if with_f else None)
+        log(f"{name}: {out[name]}")
     f.apply_net(None, ctl)  # zero network: the same solver with G = D^-1/2
-    out["untrained_fsai_mixed"] = solve(lib.clsparseAmdDcsrcgFsai, True)
+    out["untrained_fsai_mixed"] = timed_solves(cls, torch, ctl, F, A, bt, m, tol, fsai=f)
     log(f"untrained FSAI mixed: {out['untrained_fsai_mixed']}")
-    base = out["fp64_jacobi_pcg"]["seconds"]
-    out["speedup_learned_fsai_vs_fp64_jacobi"] = round(base / out["learned_fsai_mixed"]["seconds"], 3)
-    out["speedup_learned_fsai_vs_fp64_jacobi_csr"] = round(
-        out["fp64_jacobi_pcg_csr"]["seconds"] / out["learned_fsai_mixed"]["seconds"], 3)
-    out["speedup_learned_fsai_first_solve_vs_fp64_jacobi"] = round(
-        base / out["learned_fsai_mixed"]["first_solve"]["seconds"], 3)
-    out["speedup_jacobi_mixed_vs_fp64_jacobi"] = round(base / out["jacobi_mixed"]["seconds"], 3)
+    sec = {k: out[k]["seconds"] for k, _, _, _ in runs}
+    base, base_csr = sec["fp64_jacobi_pcg"], sec["fp64_jacobi_pcg_csr"]
+    first = out["learned_fsai_mixed"]["first_solve"]["seconds"]
+    first_csr = out["learned_fsai_mixed_csr"]["first_solve"]["seconds"]
+    out["speedup"] = {
+        "learned_vs_fp64_jacobi_diagonal_storage": round(base / sec["learned_fsai_mixed"], 3),
+        "learned_vs_fp64_jacobi_csr_adaptive_both": round(base_csr / sec["learned_fsai_mixed_csr"], 3),
+        "learned_diagonal_storage_vs_fp64_jacobi_csr_adaptive": round(base_csr / sec["learned_fsai_mixed"], 3),
+        "learned_fp32_g_vs_fp64_jacobi_csr_adaptive_both": round(base_csr / sec["learned_fsai_mixed_fp32_g_csr"], 3),
+        "learned_fp64_x_vs_fp64_jacobi_diagonal_storage": round(base / sec["learned_fsai_mixed_fp64_x"], 3),
+        "learned_fp32_g_fp64_x_vs_fp64_jacobi_csr_adaptive_both":
+            round(base_csr / sec["learned_fsai_mixed_fp32_g_fp64_x_csr"], 3),
+        "first_solve_with_setup_vs_fp64_jacobi_diagonal_storage": round(base / (first + setup), 3),
+        "first_solve_with_setup_vs_fp64_jacobi_csr_adaptive_both": round(base_csr / (first_csr + setup), 3),
+        "jacobi_mixed_vs_fp64_jacobi": round(base / sec["jacobi_mixed"], 3),
+        "setup_s": round(setup, 5),
+        "what": "FP64 Jacobi-PCG median solve time over learned-FSAI mixed PCG median solve time; "
+                "first_solve_with_setup adds the factor's pattern build and network evaluation to "
+                "the learned solver's first solve (pilot race included)"}
     out["iteration_ratio_learned_vs_untrained"] = round(
         out["learned_fsai_mixed"]["iterations"] / out["untrained_fsai_mixed"]["iterations"], 3)
     if cpu:
-        out["cpu_baseline"] = cpu_cg_baseline(rp, ci, v, b, out["fp64_jacobi_pcg"]["iterations"],
-                                              sample_iters=12)
+        out["cpu_baseline"] = cpu_cg_baseline(rp, ci, v, b, tol, reps=3)
+        log(f"CPU baseline: {out['cpu_baseline']}")
     f.release()
     A.meta_delete()
     out["held_out_sweep"] = fsai_sweep(cls, torch, ctl, net, tol)
@@ -690,8 +786,9 @@ def fsai_workload(cls, torch, ctl, n1d=160, tol=1e-8, weights=None, cpu=True):
 
 def fsai_sweep(cls, torch, ctl, net, tol):
     """The same comparison on further held-out 3-D problems (other anisotropy,
-    coefficient spread and seeds; 128^3): learned-FSAI mixed PCG (first and
-    repeat solve) against FP64 Jacobi-PCG from diagonal storage."""
+    coefficient spread and seeds; 128^3): learned-FSAI mixed PCG (first solve
+    with the factor's set-up, and the median of 3 repeat solves) against FP64
+    Jacobi-PCG, both from diagonal storage."""
     lib = cls.load()
     res = []
     for az, sigma, seed in ((100.0, 1.0, 61), (10.0, 1.5, 62), (1000.0, 0.5, 63), (1.0, 1.0, 64)):
@@ -699,44 +796,93 @@ def fsai_sweep(cls, torch, ctl, net, tol):
         b = torch.tensor(cls.gen_uniform(m, -1.0, 1.0, seed=seed + 100), device="cuda")
         A = cls.CsrMatrix(rp, ci, v, m)
         A.meta_create(ctl)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         f = cls.Fsai(A, ctl)
         f.apply_net(net, ctl)
-
-        def solve(fn, with_fsai):
-            x = torch.zeros(m, dtype=torch.float64, device="cuda")
-            sc = cls.SolverControl(cls.DIAGONAL, 50000, tol, 0.0)
-            xd, bd = cls.DenseVector(x), cls.DenseVector(b)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            if with_fsai:
-                st = fn(ctypes.byref(xd.s), ctypes.byref(A.s), ctypes.byref(bd.s), f.handle, sc.handle,
-                        ctl.handle)
-            else:
-                st = fn(ctypes.byref(xd.s), ctypes.byref(A.s), ctypes.byref(bd.s), sc.handle, ctl.handle)
-            torch.cuda.synchronize()
-            dt = time.perf_counter() - t0
-            stt = sc.stats()
-            sc.release()
-            cls.check("solve", st)
-            return (dt, stt.iterations, stt.preconditioner, stt.status_detail == 0,
-                    (round(stt.pilot_rate_fsai, 1), round(stt.pilot_rate_jacobi, 1)))
-
-        first = solve(lib.clsparseAmdDcsrcgFsai, True)
-        rep_ = solve(lib.clsparseAmdDcsrcgFsai, True)
-        solve(lib.clsparseAmdDcsrcgRefined, False)  # warm
-        base = solve(lib.clsparseAmdDcsrcgRefined, False)
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        lf = timed_solves(cls, torch, ctl, lib.clsparseAmdDcsrcgFsai, A, b, m, tol, reps=3, fsai=f)
+        jb = timed_solves(cls, torch, ctl, lib.clsparseAmdDcsrcgRefined, A, b, m, tol, reps=3)
         res.append({"az": az, "sigma": sigma, "seed": seed, "n": m,
-                    "learned_s": round(rep_[0], 5), "learned_first_s": round(first[0], 5),
-                    "learned_iterations": rep_[1], "preconditioner": rep_[2],
-                    "fp64_jacobi_s": round(base[0], 5), "fp64_jacobi_iterations": base[1],
-                    "converged": rep_[3] and base[3],
-                    "speedup": round(base[0] / rep_[0], 3),
-                    "speedup_first_solve": round(base[0] / first[0], 3),
-                    "pilot_rates_fsai_jacobi": first[4]})
+                    "learned_s": lf["seconds"], "learned_first_s": lf["first_solve"]["seconds"],
+                    "setup_s": round(setup, 5), "learned_iterations": lf["iterations"],
+                    "preconditioner": lf["preconditioner"],
+                    "fp64_jacobi_s": jb["seconds"], "fp64_jacobi_iterations": jb["iterations"],
+                    "converged": lf["converged"] and jb["converged"],
+                    "speedup": round(jb["seconds"] / lf["seconds"], 3),
+                    "speedup_first_solve_with_setup": round(jb["seconds"] / (lf["first_solve"]["seconds"] + setup), 3),
+                    "pilot_rates": lf["first_solve"].get("pilot_rates")})
         log(f"held-out sweep: {res[-1]}")
         f.release()
         A.meta_delete()
     return res
+
+
+def train_slice(cls, torch, ctl, stream, cpu=True, matrices=24, steps_per_matrix=4, held=4):
+    """A bounded slice of the learned-FSAI training on the 512^2 family: a new
+    network trained on `matrices` family members, the held-out loss before
+    (zero network = Jacobi) and after, steps per second, the network's forward
+    MFMA rate, and one training step checked against the CPU oracle."""
+    import numpy as np
+    import scipy.sparse as sp
+
+    from clsparse_amd import fsai as fs
+    held_ids = list(range(2000 - held, 2000))
+    net = cls.Mlp(seed=5)
+    loss0 = fs.evaluate(None, ctl, held_ids)
+    rec = fs.train(net, ctl, list(range(matrices)), steps_per_matrix=steps_per_matrix, lr=2e-3)
+    loss1 = fs.evaluate(net, ctl, held_ids)
+    out = {"family": "aniso2d 512^2 (n = 262,144), corr 16, sigma 1, log10 ratio U[0,3], theta U[0,pi)",
+           "train_matrices": matrices, "steps": rec["steps"],
+           "steps_per_s": round(rec["steps"] / max(rec["train_step_seconds"], 1e-9), 1),
+           "held_out_loss_jacobi_mean": round(float(np.mean(loss0)), 5),
+           "held_out_loss_after_mean": round(float(np.mean(loss1)), 5), "held_out_ids": held_ids}
+    # Network forward (BF16 MFMA) with G's assembly on one family matrix.
+    params, _ = fs.load_weights()
+    netw = cls.Mlp(seed=1)
+    netw.set_params(params)
+    rp, ci, v, m = fs.gen_family(7)
+    A = cls.CsrMatrix(rp, ci, v, m)
+    A.meta_create(ctl)
+    f = cls.Fsai(A, ctl)
+    ms = median_call_ms(torch, stream, lambda: f.apply_net(netw, ctl), calls=10)
+    flops = 10240 * f.nnz()
+    out["forward_with_assembly_us"] = round(ms * 1e3, 1)
+    out["forward_TFLOPs"] = round(flops / (ms * 1e-3) / 1e12, 1)
+    out["forward_flops_rule"] = "2 (64*16 + 64*64) per nonzero of G; the time includes G's assembly"
+    f.release()
+    A.meta_delete()
+    # One step against the oracle on a 128^2 member-sized matrix of the family's generator.
+    if cpu:
+        from oracle import fsai as of
+        p = fs.aniso2d_params(11)
+        rp, ci, v, m = cls.gen_aniso2d(128, 128, corr_len=p["corr_len"], sigma=p["sigma"],
+                                       ratio=p["ratio"], theta=p["theta"], seed=p["seed"])
+        A = cls.CsrMatrix(rp, ci, v, m)
+        A.meta_create(ctl)
+        f = cls.Fsai(A, ctl)
+        loss, gG = f.train_step(netw, A, ctl, probes=8, seed=99, lr=0.0, want_grad=True)
+        o = f.output().astype(np.float64)
+        X = of.bf16_bits_to_f32(f.features()).reshape(-1, 16)
+        g_dev = netw.grad().astype(np.float64)
+        f.release()
+        A.meta_delete()
+        G = of.assemble_g(rp, ci, v, o)
+        Z = np.stack([of.rademacher(m, of.probe_seed(99, k)) for k in range(8)], axis=1)
+        loss_ref, grad_ref = of.hutchinson(sp.csr_matrix((v, ci, rp), shape=(m, m)), G, Z)
+        gi, gj, _ = of.tril_pattern(rp, ci)
+        gref = np.asarray(grad_ref[gi, gj]).ravel()
+        pg = of.mlp_backward(params, X, (gG * of.dhat(rp, ci, v)[gi]).astype(np.float32))
+        out["gradient_check_128"] = {
+            "loss_rel_err": abs(loss - loss_ref) / abs(loss_ref),
+            "dL_dG_max_rel_err": float(np.max(np.abs(gG - gref)) / np.max(np.abs(gref))),
+            "param_grad_max_err_over_max": float(np.max(np.abs(g_dev - pg)) / np.max(np.abs(pg))),
+            "what": "one training step on a 128^2 matrix of the family's generator vs oracle/fsai.py"}
+    netw.release()
+    net.release()
+    log(f"training slice: {out}")
+    return out
 
 
 def fsai_train(cls, torch, ctl, args):
@@ -902,6 +1048,12 @@ def main():
     # Floor of any CSR SpMV that gathers x directly: the same value, column and
     # x loads in storage order, with no rows and no y (1^T A x, which also
     # checks the sum of y).
+    ref_s, ref_kept = reference_methodology(run.step, torch)
+    reference_method = {"us_per_call": round(ref_s * 1e6, 2), "GBps": round(nbytes / ref_s / 1e9, 1),
+                        "samples_kept": ref_kept,
+                        "what": "20 host-timed samples of 20 back-to-back clsparseDcsrmv calls, "
+                                "samples beyond 1 sigma dropped (the reference's benchmark method)"}
+    meta_s = meta_create_seconds(cls, torch, ctl, rp, ci, v, m)
     floor_ms, sum_ax = run.sum_ax(20, stream)
     ysum = float(np.sum(yg))
     gather_floor = {"sum_ax_us": round(floor_ms * 1e3, 2), "spmv_launch_us": round(launch_ms * 1e3, 2),
@@ -919,12 +1071,16 @@ def main():
                "sample": f"{calls} calls of the same FP64 power-law SpMV (full matrix), median; "
                          f"oracle OpenMP CSR SpMV, {threads} threads"}
 
-    extra = {"gather_floor": gather_floor, "copy_peak": copy_peak(cls, torch, ctl, stream),
+    extra = {"reference_methodology": reference_method,
+             "metadata_create": {"seconds": meta_s, "spmv_calls_equivalent": round(meta_s / (launch_ms * 1e-3), 1),
+                                 "what": "clsparseCsrMetaCreate on the power-law matrix, median of 3"},
+             "gather_floor": gather_floor, "copy_peak": copy_peak(cls, torch, ctl, stream),
              "poisson_fp32": poisson_workload(cls, torch, ctl, stream, cpu=not args.no_cpu)}
     if not args.no_solver:
         extra["solver"] = solver_workload(cls, torch, ctl, cpu=not args.no_cpu)
     if not args.no_fsai:
         extra["fsai"] = fsai_workload(cls, torch, ctl, weights=args.weights, cpu=not args.no_cpu)
+        extra["fsai_train"] = train_slice(cls, torch, ctl, stream, cpu=not args.no_cpu)
 
     out = {
         "metric": METRIC,

@@ -13,7 +13,11 @@ restates the arithmetic of clsparse_amd/csrc/fsai.hip in numpy:
 * ``mlp_backward``   -- its parameter gradient with the kernel's BF16 roundings;
 * ``assemble_g``     -- G_ij = (delta_ij + o_ij) / sqrt(a_ii);
 * ``rademacher``     -- the counter-based probe vectors;
-* ``hutchinson``     -- L = ||(G^T A G - I) Z||_F^2 / (k n) and dL/dG.
+* ``hutchinson``     -- L = ||(G^T A G - I) Z||_F^2 / (k n) and dL/dG;
+* ``mlp_forward_bound`` / ``mlp_backward_bound`` -- the same network with a
+                        per-output / per-parameter bound on what FP32
+                        accumulation and the BF16 rounding flips it can cause
+                        may change (the GPU parity bars).
 
 How each is pinned (tests/test_oracle_fsai.py): ``hutchinson``'s gradient
 against central finite differences of its own loss (a quadratic in G, so the
@@ -23,7 +27,10 @@ rounding switched off against finite differences of ``mlp_forward`` in FP64;
 ``features`` and ``assemble_g`` against hand-computed values on a 3x3
 matrix; ``rademacher`` against values computed by hand from the SplitMix64
 finaliser. ``mlp_forward`` with rounding is pinned only through
-``mlp_backward``'s checks and the closed form o = b3 for w3 = 0.
+``mlp_backward``'s checks and the closed form o = b3 for w3 = 0. The two
+bounds are pinned by an independent FP32 evaluation of the network in numpy
+(other summation order, FP32 accumulators) that must stay inside them, and by
+a planted error (a gate or a rounding step left out) that must not.
 """
 from __future__ import annotations
 
@@ -167,6 +174,116 @@ def mlp_backward(p, X, gout, rounding: bool = True) -> np.ndarray:
     grad[OFF_W3:OFF_B3] = (g[:, None] * h2).sum(axis=0)
     grad[OFF_B3] = g.sum()
     return grad
+
+
+_U32 = 2.0 ** -24
+
+
+def _relu_bf16(a) -> np.ndarray:
+    """h = BF16(max(a, 0)) elementwise, as float64 (monotone in a)."""
+    return bf16_round(np.maximum(a, 0.0)).astype(np.float64)
+
+
+def _spread(f, lo, hi, mid):
+    """max |f(t) - f(mid)| over t in [lo, hi] for a monotone f: the ends bound it."""
+    fm = f(mid)
+    return np.maximum(np.abs(f(hi) - fm), np.abs(f(lo) - fm))
+
+
+def mlp_forward_bound(p, X):
+    """(o, bound, flips): the oracle's BF16 forward output and, per row, a bound
+    on |o_device - o| for any evaluation that accumulates each layer in FP32
+    (in any order) where this one accumulates in FP64. A hidden activation is
+    rounded to BF16 (after the ReLU) on both sides; it can land on another BF16
+    value only when its pre-activation interval [a - e, a + e] (e: FP32
+    accumulation bound of the layer, plus what flips of the layer below can
+    move it) straddles a BF16 rounding boundary or zero. Those flips are
+    counted (`flips`: rows with at least one) and their largest effect carried
+    through the next layers; everything else differs only by FP32 rounding."""
+    W1, b1, W2, b2, w3, b3 = split_params(p)
+    W1q = bf16_round(W1).astype(np.float64)
+    W2q = bf16_round(W2).astype(np.float64)
+    Xq = bf16_round(X).astype(np.float64)
+    a1 = Xq @ W1q.T + b1
+    e1 = 2 * 17 * _U32 * (np.abs(Xq) @ np.abs(W1q).T + np.abs(b1))
+    d1 = _spread(_relu_bf16, a1 - e1, a1 + e1, a1)
+    h1 = _relu_bf16(a1)
+    a2 = h1 @ W2q.T + b2
+    E2 = d1 @ np.abs(W2q).T + 2 * 65 * _U32 * ((h1 + d1) @ np.abs(W2q).T + np.abs(b2))
+    d2 = _spread(_relu_bf16, a2 - E2, a2 + E2, a2)
+    h2 = _relu_bf16(a2)
+    o = h2 @ w3 + float(b3)
+    bound = d2 @ np.abs(w3) + 2 * 66 * _U32 * ((h2 + d2) @ np.abs(w3) + abs(float(b3)))
+    flips = int(np.count_nonzero((d1 > 0).any(axis=1) | (d2 > 0).any(axis=1)))
+    return o, bound, flips
+
+
+def mlp_backward_bound(p, X, gout):
+    """(grad, bound, flips): ``mlp_backward`` with rounding and a per-parameter
+    bound on |grad_device - grad| for an evaluation that accumulates in FP32
+    over all nonzeros (any order) and may round each BF16 value (hidden
+    activations, deltas) to the neighbouring BF16 value wherever the FP32 /
+    FP64 difference of its argument, and the flips below it, allow. gout may
+    differ by one FP32 rounding. `flips`: nonzeros with any possible flip."""
+    W1, b1, W2, b2, w3, b3 = split_params(p)
+    W1q = bf16_round(W1).astype(np.float64)
+    W2q = bf16_round(W2).astype(np.float64)
+    Xq = bf16_round(X).astype(np.float64)
+    g = np.asarray(gout, dtype=np.float64)
+    N = g.size
+    gam = 2 * max(N, 1) * _U32
+    q = lambda t: bf16_round(t).astype(np.float64)
+    # forward, with the activation deviations of mlp_forward_bound
+    a1 = Xq @ W1q.T + b1
+    e1 = 2 * 17 * _U32 * (np.abs(Xq) @ np.abs(W1q).T + np.abs(b1))
+    dh1 = _spread(_relu_bf16, a1 - e1, a1 + e1, a1)
+    h1 = _relu_bf16(a1)
+    a2 = h1 @ W2q.T + b2
+    E2 = dh1 @ np.abs(W2q).T + 2 * 65 * _U32 * ((h1 + dh1) @ np.abs(W2q).T + np.abs(b2))
+    dh2 = _spread(_relu_bf16, a2 - E2, a2 + E2, a2)
+    h2 = _relu_bf16(a2)
+    dg = 2 * _U32 * np.abs(g)
+    # D2 = relu'(a2) g w3 (FP32 product on the device), then BF16
+    v2 = g[:, None] * w3[None, :]
+    ev2 = _U32 * np.abs(v2) + dg[:, None] * np.abs(w3)[None, :]
+    on2 = a2 > 0
+    unc2 = (a2 - E2 <= 0) & (a2 + E2 > 0)
+    d2 = np.where(on2, v2, 0.0)
+    lo2 = np.where(unc2, np.minimum(0.0, v2 - ev2), np.where(on2, v2 - ev2, 0.0))
+    hi2 = np.where(unc2, np.maximum(0.0, v2 + ev2), np.where(on2, v2 + ev2, 0.0))
+    d2q = q(d2)
+    dd2q = _spread(q, lo2, hi2, d2)
+    dd2 = np.maximum(np.abs(hi2 - d2), np.abs(lo2 - d2))
+    # D1 = relu'(a1) (D2q W2q) (FP32 MFMA), then BF16
+    c1 = d2q @ W2q
+    ec1 = dd2q @ np.abs(W2q) + 2 * 65 * _U32 * ((np.abs(d2q) + dd2q) @ np.abs(W2q))
+    on1 = a1 > 0
+    unc1 = (a1 - e1 <= 0) & (a1 + e1 > 0)
+    d1 = np.where(on1, c1, 0.0)
+    lo1 = np.where(unc1, np.minimum(0.0, c1 - ec1), np.where(on1, c1 - ec1, 0.0))
+    hi1 = np.where(unc1, np.maximum(0.0, c1 + ec1), np.where(on1, c1 + ec1, 0.0))
+    d1q = q(d1)
+    dd1q = _spread(q, lo1, hi1, d1)
+    dd1 = np.maximum(np.abs(hi1 - d1), np.abs(lo1 - d1))
+    grad = np.zeros(N_PARAMS)
+    bound = np.zeros(N_PARAMS)
+    grad[OFF_W2:OFF_B2] = (d2q.T @ h1).ravel()
+    bound[OFF_W2:OFF_B2] = (dd2q.T @ (h1 + dh1) + np.abs(d2q).T @ dh1 +
+                            gam * ((np.abs(d2q) + dd2q).T @ (h1 + dh1))).ravel()
+    grad[OFF_W1:OFF_B1] = (d1q.T @ Xq).ravel()
+    bound[OFF_W1:OFF_B1] = (dd1q.T @ np.abs(Xq) + gam * ((np.abs(d1q) + dd1q).T @ np.abs(Xq))).ravel()
+    grad[OFF_B1:OFF_W2] = d1.sum(axis=0)
+    bound[OFF_B1:OFF_W2] = dd1.sum(axis=0) + gam * (np.abs(d1) + dd1).sum(axis=0)
+    grad[OFF_B2:OFF_W3] = d2.sum(axis=0)
+    bound[OFF_B2:OFF_W3] = dd2.sum(axis=0) + gam * (np.abs(d2) + dd2).sum(axis=0)
+    grad[OFF_W3:OFF_B3] = (g[:, None] * h2).sum(axis=0)
+    bound[OFF_W3:OFF_B3] = (np.abs(g)[:, None] * dh2 + dg[:, None] * (h2 + dh2)).sum(axis=0) + \
+        gam * (np.abs(g)[:, None] * (h2 + dh2)).sum(axis=0)
+    grad[OFF_B3] = g.sum()
+    bound[OFF_B3] = dg.sum() + gam * np.abs(g).sum()
+    flips = int(np.count_nonzero((dh1 > 0).any(axis=1) | (dh2 > 0).any(axis=1) | unc2.any(axis=1) |
+                                 (dd2q > 0).any(axis=1) | (dd1q > 0).any(axis=1)))
+    return grad, bound, flips
 
 
 def assemble_g(rp, ci, v, o=None):

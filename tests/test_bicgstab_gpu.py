@@ -38,6 +38,59 @@ def _convection_diffusion(nx, ny, c=0.4):
     return a.indptr.astype(np.uint32), a.indices.astype(np.uint32), a.data, n
 
 
+def _order_window(rp, ci, v, b, precond, tol, dtype):
+    """The iteration counts of the same BiCGStab recurrence (right
+    preconditioning, the two L1 checks of biconjugate-gradients-stabilized.hpp)
+    in numpy with the dot products summed in three orders (sequential,
+    numpy's pairwise, reversed) -- the spread that summation order alone gives
+    on this system. Vectors in `dtype`, dots and norms in FP64, as the library
+    forms them; in FP32 also with the SpMV rows summed in FP32 and in FP64
+    (the CSR kernels sum in FP64, the diagonal-storage ones in FP32)."""
+    n = rp.size - 1
+    a = sp.csr_matrix((np.asarray(v, dtype=dtype), ci, rp), shape=(n, n))
+    a64 = sp.csr_matrix((np.asarray(v, dtype=dtype).astype(np.float64), ci, rp), shape=(n, n))
+    inv = (1 / a.diagonal()).astype(dtype) if precond == cls.DIAGONAL else np.ones(n, dtype)
+    bb = np.asarray(b, dtype=dtype)
+    nb = np.abs(bb.astype(np.float64)).sum()
+
+    def run(dot, wide=False):
+        mv = (lambda t: (a64 @ t.astype(np.float64)).astype(dtype)) if wide else (lambda t: a @ t)
+        r = bb.copy()
+        p, rs = r.copy(), r.copy()
+        rr = dot(rs, r)
+        for it in range(5000):
+            mp = p * inv
+            amp = mv(mp)
+            alpha = dtype(rr / dot(rs, amp))
+            s_ = r - alpha * amp
+            if np.abs(s_.astype(np.float64)).sum() / nb <= tol:
+                return it
+            ms = s_ * inv
+            ams = mv(ms)
+            omega = dtype(dot(ams, s_) / dot(ams, ams))
+            r = s_ - omega * ams
+            if np.abs(r.astype(np.float64)).sum() / nb <= tol:
+                return it
+            rrn = dot(rs, r)
+            beta = dtype(rrn / rr * float(alpha) / float(omega))
+            rr = rrn
+            p = beta * p + r - beta * omega * amp
+        return 5000
+
+    def seq(u, w):
+        acc = 0.0
+        for t in (u.astype(np.float64) * w.astype(np.float64)).tolist():
+            acc += t
+        return acc
+
+    dots = [seq, lambda u, w: float(np.dot(u.astype(np.float64), w.astype(np.float64))),
+            lambda u, w: seq(u[::-1], w[::-1])]
+    counts = [run(d) for d in dots]
+    if dtype == np.float32:
+        counts += [run(d, wide=True) for d in dots]
+    return min(counts), max(counts)
+
+
 def _solve(ctl, rp, ci, v, n, b, dtype, precond, max_iters, rel, absol, n_prior=None):
     lib = cls.load()
     tdt = torch.float32 if dtype == np.float32 else torch.float64
@@ -62,10 +115,14 @@ def test_nonsymmetric_fp64_matches_oracle(ctl, precond):
     st, s, x = _solve(ctl, rp, ci, v, n, b, np.float64, precond, 2000, 1e-10, 0.0)
     assert st == 0
     xo, ro = oracle.bicgstab(rp, ci, v, b, np.zeros(n), precond, 2000, 1e-10, 0.0)
-    # BiCGStab's residual history is erratic: the same FP64 algorithm takes 90
-    # or 96 iterations on this system depending only on the order in which the
-    # dot products are summed, so the count is compared within 10 %.
-    assert abs(s.n_iters - ro.n_iters) <= max(2, 0.1 * ro.n_iters), (s.n_iters, ro.n_iters)
+    # BiCGStab's residual history is erratic: the iteration count moves with
+    # the order in which the dot products are summed alone, so the bar is the
+    # spread of the same recurrence over three summation orders on this
+    # system (and the oracle's own count), one iteration either side.
+    lo, hi = _order_window(rp, ci, v, b, precond, 1e-10, np.float64)
+    lo, hi = min(lo, ro.n_iters), max(hi, ro.n_iters)
+    say(f"  iterations gpu {s.n_iters}, oracle {ro.n_iters}, summation-order window [{lo}, {hi}]")
+    assert lo - 1 <= s.n_iters <= hi + 1, (s.n_iters, lo, hi)
     r = b - oracle.csrmv(rp, ci, v, x)[0]
     assert np.abs(r).sum() / np.abs(b).sum() <= 1e-10 * 1.5
     assert np.abs(x - xo).max() <= 1e-6 * np.abs(xo).max()
@@ -78,7 +135,10 @@ def test_fp32_converges(ctl):
     assert st == 0 and s.status_detail == 0
     _, ro = oracle.bicgstab(rp, ci, v.astype(np.float32), b.astype(np.float32), np.zeros(n, np.float32),
                             1, 1000, 1e-5, 0.0)
-    assert abs(s.n_iters - ro.n_iters) <= max(3, 0.1 * ro.n_iters)
+    lo, hi = _order_window(rp, ci, v, b, cls.DIAGONAL, 1e-5, np.float32)
+    lo, hi = min(lo, ro.n_iters), max(hi, ro.n_iters)
+    say(f"  fp32 iterations gpu {s.n_iters}, oracle {ro.n_iters}, summation-order window [{lo}, {hi}]")
+    assert lo - 1 <= s.n_iters <= hi + 1, (s.n_iters, lo, hi)
     r = b - oracle.csrmv(rp, ci, v, x.astype(np.float64))[0]
     assert np.abs(r).sum() / np.abs(b).sum() <= 2e-5
 

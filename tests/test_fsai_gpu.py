@@ -87,11 +87,14 @@ def test_mlp_forward_mfma_matches_oracle(ctl):
         f.apply_net(net, ctl)
         o = f.output().astype(np.float64)
         X = of.bf16_bits_to_f32(f.features()).reshape(-1, 16)
-        o_ref = of.mlp_forward(p, X)
-        scale = np.abs(o_ref).max()
+        # Per nonzero: within what FP32 accumulation and the BF16 rounding
+        # flips it can cause allow (flips counted by the oracle).
+        o_ref, bound, flips = of.mlp_forward_bound(p, X)
         err = np.abs(o - o_ref)
-        assert err.max() <= 2e-2 * scale
-        assert np.median(err) <= 1e-3 * scale
+        bad = err > bound
+        say(f"  forward: {flips} of {o.size} nonzeros with a possible BF16 flip, "
+            f"max err / bound {float((err / np.maximum(bound, 1e-300)).max()):.3f}")
+        assert not bad.any(), (int(np.argmax(bad)), float(err[bad][0]), float(bound[bad][0]))
         # G assembled from the network output
         _, _, gv = f.values()
         G = of.assemble_g(rp, ci, v, o)
@@ -139,12 +142,15 @@ def test_hutchinson_loss_gradient_and_backward_match_oracle(ctl):
         np.testing.assert_allclose(gG, gref, rtol=1e-9, atol=1e-12 * np.abs(gref).max())
         # parameter gradient through the network: gout = dL/dG * dhat_i
         gout = gG * of.dhat(rp, ci, v)[gi]
-        g_ref = of.mlp_backward(p, X, gout.astype(np.float32))
+        g_ref, bound, flips = of.mlp_backward_bound(p, X, gout.astype(np.float32))
         g_dev = net.grad().astype(np.float64)
-        for a, b in [(of.OFF_W1, of.OFF_B1), (of.OFF_B1, of.OFF_W2), (of.OFF_W2, of.OFF_B2),
-                     (of.OFF_B2, of.OFF_W3), (of.OFF_W3, of.OFF_B3), (of.OFF_B3, of.N_PARAMS)]:
-            rel = np.linalg.norm(g_dev[a:b] - g_ref[a:b]) / np.linalg.norm(g_ref[a:b])
-            assert rel < 2e-2, (a, rel)
+        # Per parameter: within what FP32 accumulation over the nonzeros and
+        # the BF16 rounding flips (activations, deltas) can change.
+        err = np.abs(g_dev - g_ref)
+        bad = err > bound
+        say(f"  backward: {flips} nonzeros with a possible flip; max err / bound "
+            f"{float((err / np.maximum(bound, 1e-300)).max()):.3f}")
+        assert not bad.any(), (int(np.argmax(bad)), float(err[bad][0]), float(bound[bad][0]))
     finally:
         f.release()
         net.release()

@@ -163,3 +163,59 @@ def test_mlp_backward_rounding_close_to_exact():
     for a, b in [(of.OFF_W1, of.OFF_B1), (of.OFF_W2, of.OFF_B2), (of.OFF_W3, of.OFF_B3)]:
         err = np.linalg.norm(exact[a:b] - rounded[a:b]) / np.linalg.norm(exact[a:b])
         assert err < 0.06  # ReLU masks flip where BF16 moves a pre-activation across 0
+
+
+def _fp32_network(p, X, g, round_h1=True, round_d2=True):
+    """The network in numpy FP32 (BLAS accumulation order, FP32 accumulators),
+    the device's arithmetic restated independently of the oracle's FP64 one:
+    (o, parameter gradient of sum g o)."""
+    f = np.float32
+    W1, b1, W2, b2, w3, b3 = (np.asarray(t, dtype=f) for t in of.split_params(p))
+    W1q, W2q, Xq = of.bf16_round(W1), of.bf16_round(W2), of.bf16_round(X)
+    a1 = (Xq @ W1q.T + b1).astype(f)
+    h1 = of.bf16_round(np.maximum(a1, 0)) if round_h1 else np.maximum(a1, 0)
+    a2 = (h1 @ W2q.T + b2).astype(f)
+    h2 = of.bf16_round(np.maximum(a2, 0))
+    o = (h2 @ w3).astype(f) + f(b3)
+    g = np.asarray(g, dtype=f)
+    d2 = np.where(a2 > 0, g[:, None] * w3[None, :], f(0)).astype(f)
+    d2q = of.bf16_round(d2) if round_d2 else d2
+    d1 = np.where(a1 > 0, (d2q @ W2q).astype(f), f(0)).astype(f)
+    d1q = of.bf16_round(d1)
+    grad = np.zeros(of.N_PARAMS)
+    grad[of.OFF_W2:of.OFF_B2] = (d2q.T @ h1).ravel()
+    grad[of.OFF_W1:of.OFF_B1] = (d1q.T @ Xq).ravel()
+    grad[of.OFF_B1:of.OFF_W2] = d1.sum(axis=0, dtype=f)
+    grad[of.OFF_B2:of.OFF_W3] = d2.sum(axis=0, dtype=f)
+    grad[of.OFF_W3:of.OFF_B3] = (g[:, None] * h2).sum(axis=0, dtype=f)
+    grad[of.OFF_B3] = g.sum(dtype=f)
+    return o.astype(np.float64), grad
+
+
+def test_mlp_bounds_hold_for_an_fp32_evaluation_and_catch_planted_errors():
+    """The forward and backward parity bounds (what FP32 accumulation and the
+    BF16 rounding flips it causes may change) contain an independent FP32
+    evaluation of the network on real features, and are tight enough that
+    leaving out one rounding step breaks them."""
+    rng = np.random.default_rng(3)
+    rp, ci, v, m = cls.gen_aniso2d(40, 36, corr_len=4.0, sigma=1.0, ratio=100.0, theta=0.6, seed=3)
+    X = of.bf16_round(of.features(rp, ci, v))
+    p = np.zeros(of.N_PARAMS, dtype=np.float32)
+    p[:] = rng.normal(scale=0.3, size=of.N_PARAMS)
+    g = rng.normal(scale=1e-3, size=X.shape[0]).astype(np.float32)
+    o_ref, ob, fflips = of.mlp_forward_bound(p, X)
+    np.testing.assert_array_equal(o_ref, of.mlp_forward(p, X))
+    g_ref, gb, bflips = of.mlp_backward_bound(p, X, g)
+    np.testing.assert_allclose(g_ref, of.mlp_backward(p, X, g), rtol=1e-12, atol=1e-300)
+    assert 0 < fflips < X.shape[0] and bflips > 0
+    o32, g32 = _fp32_network(p, X, g)
+    assert np.all(np.abs(o32 - o_ref) <= ob)
+    assert np.all(np.abs(g32 - g_ref) <= gb)
+    # far tighter than a fixed 2 % of the output scale on most rows
+    scale = np.abs(o_ref).max()
+    assert np.median(ob) <= 1e-3 * scale and ob.max() <= 5e-2 * scale
+    # planted: h1 left unrounded / d2 left unrounded
+    o_bad, g_bad = _fp32_network(p, X, g, round_h1=False)
+    assert np.any(np.abs(o_bad - o_ref) > ob)
+    _, g_bad = _fp32_network(p, X, g, round_d2=False)
+    assert np.any(np.abs(g_bad - g_ref) > gb)
