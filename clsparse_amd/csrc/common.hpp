@@ -217,6 +217,18 @@ clsparseStatus launch_csrmv_bf16(const float* alpha, const clsparseCsrMatrix* A,
                                  const float* beta, float* y, hipStream_t stream,
                                  const int* done_flag);
 
+// 1^T (A x) in storage order (clsparseAmd[S|D]csrSumAx): argument checks, the
+// number of per-workgroup partials for nnz entries (at most kSumAxMaxGrid),
+// and the launch of both kernels on a partials buffer of that many doubles.
+constexpr uint32_t kSumAxMaxGrid = 4096;
+uint32_t sum_ax_grid(uint64_t nnz);
+template <typename T>
+clsparseStatus sum_ax_check(const clsparseCsrMatrix* A, const cldenseVector* x, const clsparseScalar* sum,
+                            clsparseControl control);
+template <typename T>
+clsparseStatus launch_sum_ax(const clsparseCsrMatrix* A, const cldenseVector* x, clsparseScalar* sum,
+                             double* part, hipStream_t s);
+
 // Deterministic reductions: result written to a device scalar of type R.
 enum class RedOp { Sum, Abs, Sqr, Dot };
 template <typename T, typename R>

@@ -616,7 +616,6 @@ extern "C" clsparseStatus clsparseDcsrmv(const clsparseScalar* alpha,
 namespace {
 
 constexpr int kSumAxUnroll = 8;
-constexpr uint32_t kSumAxMaxGrid = 4096;
 
 template <typename T>
 __global__ __launch_bounds__(256) void sum_ax_kernel(uint64_t nnz, const T* __restrict__ vals,
@@ -663,19 +662,43 @@ template <typename T>
 clsparseStatus sum_ax_entry(const clsparseCsrMatrix* A, const cldenseVector* x, clsparseScalar* sum,
                             clsparseControl control)
 {
+    CLS_TRY(sum_ax_check<T>(A, x, sum, control));
+    const uint32_t g = sum_ax_grid(A->num_nonzeros);
+    void* work = nullptr;
+    CLS_TRY(control_workspace(control, g * sizeof(double), &work));
+    CLS_TRY(launch_sum_ax<T>(A, x, sum, static_cast<double*>(work), control->stream));
+    return finish_op(control);
+}
+
+}  // namespace
+
+uint32_t sum_ax_grid(uint64_t nnz)
+{
+    return static_cast<uint32_t>(std::max<uint64_t>(
+        1, std::min<uint64_t>(kSumAxMaxGrid, (nnz + 256 * kSumAxUnroll - 1) / (256 * kSumAxUnroll))));
+}
+
+template <typename T>
+clsparseStatus sum_ax_check(const clsparseCsrMatrix* A, const cldenseVector* x, const clsparseScalar* sum,
+                            clsparseControl control)
+{
     CLS_TRY(check_init_control(control));
     if (A == nullptr || x == nullptr || sum == nullptr) return clsparseInvalidValue;
     if (x->num_values < A->num_cols) return clsparseInvalidSize;
     if (sum->value == nullptr || (A->num_nonzeros > 0 &&
                                   (A->values == nullptr || A->col_indices == nullptr || x->values == nullptr)))
         return clsparseInvalidMemObject;
+    return clsparseSuccess;
+}
+
+// Both kernels of 1^T (A x) on `part`, which holds sum_ax_grid(nnz) doubles:
+// every workgroup writes its partial before the finishing kernel reads them.
+template <typename T>
+clsparseStatus launch_sum_ax(const clsparseCsrMatrix* A, const cldenseVector* x, clsparseScalar* sum,
+                             double* part, hipStream_t s)
+{
     const uint64_t nnz = A->num_nonzeros;
-    const uint32_t g = static_cast<uint32_t>(std::max<uint64_t>(
-        1, std::min<uint64_t>(kSumAxMaxGrid, (nnz + 256 * kSumAxUnroll - 1) / (256 * kSumAxUnroll))));
-    void* work = nullptr;
-    CLS_TRY(control_workspace(control, g * sizeof(double), &work));
-    double* part = static_cast<double*>(work);
-    hipStream_t s = control->stream;
+    const uint32_t g = sum_ax_grid(nnz);
     hipLaunchKernelGGL((sum_ax_kernel<T>), dim3(g), dim3(256), 0, s, nnz,
                        dev_ptr<T>(A->values, A->off_values),
                        dev_ptr<uint32_t>(A->col_indices, A->off_col_indices),
@@ -683,10 +706,17 @@ clsparseStatus sum_ax_entry(const clsparseCsrMatrix* A, const cldenseVector* x, 
     hipLaunchKernelGGL((sum_ax_finish<T>), dim3(1), dim3(256), 0, s, part, g,
                        dev_ptr<T>(sum->value, sum->off_value));
     CLS_HIP_TRY(hipGetLastError());
-    return finish_op(control);
+    return clsparseSuccess;
 }
 
-}  // namespace
+template clsparseStatus sum_ax_check<float>(const clsparseCsrMatrix*, const cldenseVector*,
+                                            const clsparseScalar*, clsparseControl);
+template clsparseStatus sum_ax_check<double>(const clsparseCsrMatrix*, const cldenseVector*,
+                                             const clsparseScalar*, clsparseControl);
+template clsparseStatus launch_sum_ax<float>(const clsparseCsrMatrix*, const cldenseVector*, clsparseScalar*,
+                                             double*, hipStream_t);
+template clsparseStatus launch_sum_ax<double>(const clsparseCsrMatrix*, const cldenseVector*, clsparseScalar*,
+                                              double*, hipStream_t);
 
 extern "C" clsparseStatus clsparseAmdScsrSumAx(const clsparseCsrMatrix* A, const cldenseVector* x,
                                                clsparseScalar* sum, const clsparseControl control)
