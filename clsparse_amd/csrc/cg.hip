@@ -342,10 +342,11 @@ __device__ __forceinline__ void pn_rows(uint32_t n, const DiaOffsets& off, __amd
     }
 #pragma unroll
     for (int u = 0; u < R; ++u) {
-        T acc = T(0);
+        double sum = 0.0;  // FP32 products summed in FP64, as the CSR SpMV sums them
 #pragma unroll
         for (int d = 0; d < K; ++d)
-            if (v[u][d] != T(0)) acc += mul_rounded_t(v[u][d], xv[u][d]);
+            if (v[u][d] != T(0)) sum += static_cast<double>(mul_rounded_t(v[u][d], xv[u][d]));
+        const T acc = static_cast<T>(sum);
         if (live[u]) y[iu[u]] = acc;
         a += live[u] ? static_cast<double>(acc) * static_cast<double>(pi[u]) : 0.0;
     }
@@ -482,10 +483,11 @@ __global__ __launch_bounds__(256) void fsai_gt_dia_kernel(size_t n, DiaOffsets o
         for (int u = 0; u < kRows; ++u) {
             const size_t i = i0 + u * stride;
             if (i >= n) continue;
-            float acc = 0.f;
+            double sum = 0.0;  // products summed in FP64, as the CSR SpMV sums them
 #pragma unroll
             for (int d = K - 1; d >= 0; --d)
-                if (h[u][d] != 0) acc += mul_rounded_f(bf16_to_f32(h[u][d]), rv[u][d]);
+                if (h[u][d] != 0) sum += static_cast<double>(mul_rounded_f(bf16_to_f32(h[u][d]), rv[u][d]));
+            const float acc = static_cast<float>(sum);
             y[i] = acc;
             a += static_cast<double>(acc) * static_cast<double>(acc);
         }
@@ -522,11 +524,11 @@ __global__ __launch_bounds__(256) void fsai_g_dia_kernel(size_t n, DiaOffsets of
         for (int u = 0; u < kRows; ++u) {
             const size_t i = i0 + u * stride;
             if (i >= n) continue;
-            float acc = 0.f;
+            double sum = 0.0;  // products summed in FP64, as the CSR SpMV sums them
 #pragma unroll
             for (int d = 0; d < K; ++d)
-                if (h[u][d] != 0) acc += mul_rounded_f(bf16_to_f32(h[u][d]), yv[u][d]);
-            z[i] = acc;
+                if (h[u][d] != 0) sum += static_cast<double>(mul_rounded_f(bf16_to_f32(h[u][d]), yv[u][d]));
+            z[i] = static_cast<float>(sum);
         }
     }
 }
@@ -566,11 +568,11 @@ __global__ __launch_bounds__(256) void fsai_g_pupdate_dia_kernel(
         for (int u = 0; u < kRows; ++u) {
             const size_t i = i0 + u * stride;
             if (i >= n) continue;
-            float zi = 0.f;
+            double sum = 0.0;  // products summed in FP64, as the CSR SpMV sums them
 #pragma unroll
             for (int d = 0; d < K; ++d)
-                if (h[u][d] != 0) zi += mul_rounded_f(bf16_to_f32(h[u][d]), yv[u][d]);
-            p[i] = zi + beta * pv[u];
+                if (h[u][d] != 0) sum += static_cast<double>(mul_rounded_f(bf16_to_f32(h[u][d]), yv[u][d]));
+            p[i] = static_cast<float>(sum) + beta * pv[u];
         }
     }
     cg_finish(st, part_nr, g, sh);

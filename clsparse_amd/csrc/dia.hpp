@@ -79,10 +79,11 @@ __global__ __launch_bounds__(256) void dia_spmv_dot_kernel(size_t n, DiaOffsets 
         for (int u = 0; u < kDiaRows; ++u) {
             const size_t i = i0 + u * stride;
             if (i >= n) continue;
-            T acc = T(0);
+            double sum = 0.0;  // FP32 products summed in FP64, as the CSR SpMV sums them
 #pragma unroll
             for (int d = 0; d < K; ++d)
-                if (v[u][d] != T(0)) acc += mul_rounded_t(v[u][d], xv[u][d]);
+                if (v[u][d] != T(0)) sum += static_cast<double>(mul_rounded_t(v[u][d], xv[u][d]));
+            const T acc = static_cast<T>(sum);
             y[i] = acc;
             a += static_cast<double>(acc) * static_cast<double>(pi[u]);
             a2 += static_cast<double>(acc) * static_cast<double>(acc);
@@ -131,10 +132,11 @@ __device__ __forceinline__ void dia_rows(uint32_t n, const DiaOffsets& off, __am
     }
 #pragma unroll
     for (int u = 0; u < R; ++u) {
-        T acc = T(0);
+        double sum = 0.0;  // FP32 products summed in FP64, as the CSR SpMV sums them
 #pragma unroll
         for (int d = 0; d < K; ++d)
-            if (v[u][d] != T(0)) acc += mul_rounded_t(v[u][d], xv[u][d]);
+            if (v[u][d] != T(0)) sum += static_cast<double>(mul_rounded_t(v[u][d], xv[u][d]));
+        const T acc = static_cast<T>(sum);
         if (live[u]) y[iu[u]] = acc;
         a += live[u] ? static_cast<double>(acc) * static_cast<double>(pi[u]) : 0.0;
         a2 += live[u] ? static_cast<double>(acc) * static_cast<double>(acc) : 0.0;
@@ -194,10 +196,11 @@ __device__ __forceinline__ void dia_line_pair(uint32_t n, const DiaOffsets& off,
     const bool live[2] = {live0, live1};
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
-        T acc = T(0);
+        double sum = 0.0;  // FP32 products summed in FP64, as the CSR SpMV sums them
 #pragma unroll
         for (int d = 0; d < K; ++d)
-            if (v[u][d] != T(0)) acc += mul_rounded_t(v[u][d], xv[u][d]);
+            if (v[u][d] != T(0)) sum += static_cast<double>(mul_rounded_t(v[u][d], xv[u][d]));
+        const T acc = static_cast<T>(sum);
         if (live[u]) y[iu[u]] = acc;
         a += live[u] ? static_cast<double>(acc) * static_cast<double>(pi[u]) : 0.0;
         a2 += live[u] ? static_cast<double>(acc) * static_cast<double>(acc) : 0.0;
