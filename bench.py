@@ -995,9 +995,11 @@ def main():
             cif = (ci & np.uint32((1 << bits) - 1)) if bits < 21 else ci
             run = SpmvRun(cls, torch, rp, cif, v, m, x, np.float64, ctl)
             wall, launch = run.time(args.steps, args.warmup, stream)
+            floor_ms, _ = run.sum_ax(args.steps, stream)
             res.append({"matrix": f"powerlaw cols folded to 2^{bits}",
                         "x_window_bytes": 8 << bits, "launch_us": round(launch * 1e3, 2),
-                        "GBps": round(run.nbytes / (launch * 1e-3) / 1e9, 1)})
+                        "GBps": round(run.nbytes / (launch * 1e-3) / 1e9, 1),
+                        "sum_ax_floor_us": round(floor_ms * 1e3, 2)})
             run.close()
             log(str(res[-1]))
         # Stencils: "s64" / "s32" (FP64 / FP32, 128^3), "s32:160" (size), and

@@ -1,8 +1,8 @@
 """CSR SpMV on the GPU (clsparseScsrmv / clsparseDcsrmv) against the FP64 oracle.
 
 Per-row relative error = |y - y*| / (|alpha| sum_j |a_ij x_j| + |beta y_i|), with
-y* from the oracle (long-double compensated). Bars: 1e-6 FP32 (raised to the
-worst-case (len+8)*2^-24 for rows longer than 8), 1e-13 FP64.
+y* from the oracle (long-double compensated). Bars: 1e-6 FP32 and 1e-13 FP64 on
+every row, whatever its length (the FP32 kernels sum each row in FP64).
 """
 import numpy as np
 import pytest
@@ -70,6 +70,46 @@ def test_edge_cases(ctl, dtype, name):
         for beta in (0.0, 2.0):
             spmv_check(rp, ci, v, ncols, dtype, alpha=-1.25, beta=beta, seed=11,
                        adaptive=adaptive, control=ctl, tol=row_tolerance(rp, dtype))
+
+
+MORE_CASES = {"single_entry": [1], "two_rows": [0, 1], "long_run_of_empty_rows": [0] * 5000 + [3],
+              "row_of_40k_between_singles": [1, 40000, 1],
+              "rows_at_block_edges": [511, 1, 512, 0, 0, 1023, 2, 4096]}
+
+
+@pytest.mark.parametrize("name", sorted(MORE_CASES))
+def test_more_edge_cases(ctl, name):
+    """FP32 and FP64, beta 0 and 2: a single entry, runs of empty rows, a
+    40k-entry row between single entries, rows ending on block edges."""
+    say(f"edge {name}")
+    lens = MORE_CASES[name]
+    rng = np.random.default_rng(abs(hash(name)) % 1000 + 7)
+    ncols = max(60000, max(lens) + 1)
+    rp, ci, v = random_csr(rng, lens, ncols)
+    for dtype in (np.float32, np.float64):
+        for beta in (0.0, 2.0):
+            spmv_check(rp, ci, v, ncols, dtype, alpha=0.75, beta=beta, seed=5, control=ctl,
+                       tol=row_tolerance(rp, dtype))
+
+
+def test_row_pointer_not_starting_at_zero(ctl):
+    """row_pointer[0] = 5: rows index values and columns from entry 5 on."""
+    say("row pointer base 5")
+    rng = np.random.default_rng(3)
+    rp, ci, v = random_csr(rng, rng.integers(0, 60, 3000), 4000)
+    rp5 = (rp + 5).astype(np.uint32)
+    ci5 = np.concatenate([np.full(5, 3999, np.uint32), ci])
+    v5 = np.concatenate([np.full(5, 1e30), v])
+    x = cls.gen_uniform(4000, -1, 1, seed=4)
+    A = cls.CsrMatrix(rp5, ci5, v5, 4000)
+    A.meta_create(ctl)
+    xd = cls.DenseVector(torch.tensor(x, device="cuda"))
+    yd = cls.DenseVector(torch.zeros(3000, dtype=torch.float64, device="cuda"))
+    cls.check("csrmv", A.spmv(cls.Scalar(1.0), xd, cls.Scalar(0.0), yd, ctl))
+    yref, scale = oracle.csrmv(rp, ci, v, x)
+    err = np.abs(yd.t.cpu().numpy() - yref) / np.maximum(scale, 1e-300)
+    assert err.max() <= 1e-13
+    A.meta_delete()
 
 
 def test_rectangular_and_empty_nnz(ctl):
