@@ -215,6 +215,8 @@ for _s in ("S", "D"):
     _SIGS[f"clsparseAmd{_s}csrmvDiag"] = (c_int32, [POINTER(clsparseCsrMatrix), POINTER(cldenseVector),
                                                     POINTER(cldenseVector), c_int32, POINTER(c_int32), _V])
 _SIGS["cldenseIreduce"] = (c_int32, [POINTER(clsparseScalar), POINTER(cldenseVector), _V])
+_SIGS["clsparseAmdDcsrSumAxNoAlloc"] = (c_int32, [POINTER(clsparseCsrMatrix), POINTER(cldenseVector),
+                                                 POINTER(clsparseScalar), _V])
 
 EXPORTED = tuple(sorted(_SIGS))
 

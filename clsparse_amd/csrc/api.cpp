@@ -116,6 +116,14 @@ extern "C" clsparseCreateResult clsparseCreateControl(hipStream_t stream)
         r.status = clsparseOutOfResources;
         return r;
     }
+    if (const clsparseStatus st = sum_ax_scratch_setup(dev); st != clsparseSuccess) {
+        (void)hipStreamDestroy(c->solver_stream);
+        (void)hipFree(c->red_partials);
+        (void)hipEventDestroy(c->event);
+        delete c;
+        r.status = st;
+        return r;
+    }
     r.control = c;
     return r;
 }

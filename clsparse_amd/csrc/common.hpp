@@ -119,6 +119,7 @@ struct matrix_meta
     uint32_t num_long_rows = 0;
     LongRow* d_long = nullptr;
     double* d_partials = nullptr;     // per row-block entry: segment sum, segment error
+
 };
 
 // ---------------------------------------------------------------------------
@@ -228,6 +229,9 @@ clsparseStatus sum_ax_check(const clsparseCsrMatrix* A, const cldenseVector* x, 
 template <typename T>
 clsparseStatus launch_sum_ax(const clsparseCsrMatrix* A, const cldenseVector* x, clsparseScalar* sum,
                              double* part, hipStream_t s);
+// Takes the device address of the no-allocation SumAx's partials array on
+// `device` (which must be current); once per device, at control creation.
+clsparseStatus sum_ax_scratch_setup(int device);
 
 // Deterministic reductions: result written to a device scalar of type R.
 enum class RedOp { Sum, Abs, Sqr, Dot };

@@ -21,7 +21,9 @@
 //   * CSR-Vector (meta == NULL): one sub-wave of 2..64 lanes per row, the
 //     sub-wave width chosen from nnz/row as csrmv-vector.hpp L35-49.
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
+#include <mutex>
 
 #include "accum.hpp"
 #include "common.hpp"
@@ -722,6 +724,53 @@ extern "C" clsparseStatus clsparseAmdScsrSumAx(const clsparseCsrMatrix* A, const
                                                clsparseScalar* sum, const clsparseControl control)
 {
     return sum_ax_entry<float>(A, x, sum, control);
+}
+
+// ---- 1^T (A x) without allocation ----------------------------------------------
+// The FP64 variant's per-workgroup partials live in a file-scope device array
+// (kSumAxMaxGrid doubles, 32 KiB; one copy per GPU, held until the process
+// ends). Its name is a host-side stand-in: host code uses only the device
+// address that hipGetSymbolAddress returns, taken once per device while a
+// control object is created on it (the device is current then).
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) double scratch[kSumAxMaxGrid];
+std::atomic<double*> g_scratch_ptr[kScratchDevices] = {};
+std::mutex g_scratch_mu;
+
+}  // namespace
+
+clsparseStatus sum_ax_scratch_setup(int device)
+{
+    if (device < 0 || device >= kScratchDevices) return clsparseInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device].load() != nullptr) return clsparseSuccess;
+    void* p = nullptr;
+    CLS_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device].store(static_cast<double*>(p));
+    return clsparseSuccess;
+}
+
+extern "C" clsparseStatus clsparseAmdDcsrSumAxNoAlloc(const clsparseCsrMatrix* A, const cldenseVector* x,
+                                                      clsparseScalar* sum, const clsparseControl control)
+{
+    CLS_TRY(sum_ax_check<double>(A, x, sum, control));
+    // The grid is capped at kSumAxMaxGrid workgroups, so every nnz fits the
+    // scratch; the check guards that invariant.
+    if (sum_ax_grid(A->num_nonzeros) > kSumAxMaxGrid) {
+        std::fprintf(stderr, "clSPARSE: clsparseAmdDcsrSumAxNoAlloc needs %u partials, scratch holds %u\n",
+                     sum_ax_grid(A->num_nonzeros), kSumAxMaxGrid);
+        return clsparseInvalidSize;
+    }
+    if (control->device < 0 || control->device >= kScratchDevices) return clsparseInvalidDevice;
+    double* part = g_scratch_ptr[control->device].load();
+    if (part == nullptr) return clsparseInvalidDevice;
+    // Every workgroup writes its partial before the finishing kernel reads
+    // them, so nothing in the scratch has to start at zero.
+    CLS_TRY(launch_sum_ax<double>(A, x, sum, part, control->stream));
+    return finish_op(control);
 }
 
 extern "C" clsparseStatus clsparseAmdDcsrSumAx(const clsparseCsrMatrix* A, const cldenseVector* x,

@@ -215,6 +215,18 @@ CLSPARSE_EXPORT clsparseStatus clsparseAmdScsrSumAx(const clsparseCsrMatrix* A, 
 CLSPARSE_EXPORT clsparseStatus clsparseAmdDcsrSumAx(const clsparseCsrMatrix* A, const cldenseVector* x,
                                                     clsparseScalar* sum, const clsparseControl control);
 
+/* clsparseAmdDcsrSumAx without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). No counterpart in the reference. Its kernels keep their
+ * per-workgroup partials in a library-owned device array of 4096 doubles per
+ * GPU (created with the first control object on that GPU and held until the
+ * process ends); the grid never needs more, so no size is refused for it.
+ * Every call on one GPU shares that array: calls that overlap on one GPU (on
+ * different streams, or from different threads) race and must be ordered by
+ * the caller. Same arguments, checks and result as clsparseAmdDcsrSumAx. */
+CLSPARSE_EXPORT clsparseStatus clsparseAmdDcsrSumAxNoAlloc(const clsparseCsrMatrix* A, const cldenseVector* x,
+                                                           clsparseScalar* sum, const clsparseControl control);
+
 /* Power-law rows: length = clip(round(LogNormal(ln(mean_len) - sigma^2/2, sigma)),
  * 1, max_len); then n_dense rows at seeded positions get dense_len entries.
  * Columns uniform without replacement over [0, n), sorted; values U[-1, 1]. */
