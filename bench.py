@@ -15,6 +15,8 @@ and by FP64 Jacobi-PCG.
 
 Other modes (diagnostics, not the bench line):
   --mode copy      plain streaming kernel of known size, to calibrate rocprof counters
+  --mode gather    independent gathers at hashed positions in windows of x (32 KiB .. 512 MiB),
+                   the gather rate of each cache level (clsparseAmd[D|S]gatherSum)
   --mode locality  the same power-law rows with columns folded into smaller windows
                    (x-gather locality sweep), and stencil SpMVs: s64 / s32 (128^3),
                    s32:160 (160^3, z faces x100), t32:160 (its lower triangle)
@@ -189,6 +191,32 @@ def median_call_ms(torch, stream, call, calls=20, warmup=3):
         per.append((a, b))
     torch.cuda.synchronize()
     return sorted(a.elapsed_time(b) for a, b in per)[len(per) // 2]
+
+
+def gather_floors(cls, torch, ctl, stream, count, calls=15):
+    """clsparseAmd[D|S]gatherSum: `count` independent gathers at hashed
+    positions inside windows of x of growing size (FP64 32 KiB .. 16 MiB,
+    512 MiB; FP32 8 MiB): the gather rate each level of the memory hierarchy
+    gives, the floor under any CSR SpMV whose columns are scattered over that
+    much of x."""
+    lib = cls.load()
+    out = []
+    for dt, bits in ((torch.float64, 12), (torch.float64, 15), (torch.float64, 18), (torch.float64, 19),
+                     (torch.float64, 20), (torch.float64, 21), (torch.float64, 26), (torch.float32, 21)):
+        n = 1 << bits
+        x = torch.rand(n, dtype=dt, device="cuda")
+        xd, res = cls.DenseVector(x), cls.Scalar(0.0, dt)
+        fn = lib.clsparseAmdDgatherSum if dt == torch.float64 else lib.clsparseAmdSgatherSum
+
+        def call(fn=fn, xd=xd, res=res, n=n):
+            cls.check("gatherSum", fn(ctypes.byref(xd.s), n, count, 7, ctypes.byref(res.s), ctl.handle))
+        ms = median_call_ms(torch, stream, call, calls=calls)
+        item = 8 if dt == torch.float64 else 4
+        out.append({"window_bytes": n * item, "dtype": "f64" if item == 8 else "f32", "gathers": count,
+                    "us": round(ms * 1e3, 1), "Ggathers_per_s": round(count / (ms * 1e-3) / 1e9, 1)})
+        log(f"gather floor: {out[-1]}")
+        del x
+    return out
 
 
 def copy_peak(cls, torch, ctl, stream, n=1 << 26):
@@ -916,7 +944,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-solver", action="store_true", help="skip the solver workload")
     ap.add_argument("--mode", default="spmv",
-                    choices=["spmv", "copy", "locality", "cg", "fsai", "fsai-train", "next"])
+                    choices=["spmv", "copy", "gather", "locality", "cg", "fsai", "fsai-train", "next"])
     ap.add_argument("--no-fsai", action="store_true", help="skip the learned-FSAI workload")
     ap.add_argument("--weights", default=None, help="learned-FSAI weights (.npy)")
     ap.add_argument("--train-matrices", type=int, default=1800)
@@ -942,6 +970,11 @@ def main():
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
 
+    if args.mode == "gather":
+        with cls.Control(stream=stream) as c:
+            print(json.dumps({"mode": "gather", "floors": gather_floors(cls, torch, c, stream, 34467246),
+                              "copy_peak": copy_peak(cls, torch, c, stream)}))
+        return
     if args.mode == "copy":
         n = 64 << 20
         src = torch.rand(n, dtype=torch.float64, device="cuda")

@@ -212,6 +212,8 @@ for _s in ("S", "D"):
                                                 POINTER(cldenseVector), _V])
     _SIGS[f"clsparseAmd{_s}csrSumAx"] = (c_int32, [POINTER(clsparseCsrMatrix), POINTER(cldenseVector),
                                                    POINTER(clsparseScalar), _V])
+    _SIGS[f"clsparseAmd{_s}gatherSum"] = (c_int32, [POINTER(cldenseVector), c_uint32, c_uint64, c_uint32,
+                                                    POINTER(clsparseScalar), _V])
     _SIGS[f"clsparseAmd{_s}csrmvDiag"] = (c_int32, [POINTER(clsparseCsrMatrix), POINTER(cldenseVector),
                                                     POINTER(cldenseVector), c_int32, POINTER(c_int32), _V])
 _SIGS["cldenseIreduce"] = (c_int32, [POINTER(clsparseScalar), POINTER(cldenseVector), _V])

@@ -726,6 +726,85 @@ extern "C" clsparseStatus clsparseAmdScsrSumAx(const clsparseCsrMatrix* A, const
     return sum_ax_entry<float>(A, x, sum, control);
 }
 
+// ---- random-gather floor ------------------------------------------------------
+// sum of x[h(seed, i) mod window] over i < count, window a power of two: the
+// rate of independent 8-byte (or 4-byte) gathers into a window of x, the part
+// of an SpMV with scattered columns that no row structure can avoid. h is a
+// 32-bit integer hash (xor-shift-multiply, the finaliser of MurmurHash3 with
+// Wang's constants) of i + seed.
+
+namespace {
+
+__device__ __forceinline__ uint32_t gather_hash(uint32_t a)
+{
+    a ^= a >> 16;
+    a *= 0x7feb352du;
+    a ^= a >> 15;
+    a *= 0x846ca68bu;
+    a ^= a >> 16;
+    return a;
+}
+
+constexpr int kGatherUnroll = 8;
+
+template <typename T>
+__global__ __launch_bounds__(256) void gather_sum_kernel(uint64_t count, const T* __restrict__ x, uint32_t mask,
+                                                         uint32_t seed, double* __restrict__ part)
+{
+    __shared__ double sh[4];
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256;
+    double acc = 0.0;
+    for (uint64_t j0 = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x; j0 < count;
+         j0 += kGatherUnroll * stride) {
+        T v[kGatherUnroll];
+#pragma unroll
+        for (int u = 0; u < kGatherUnroll; ++u) {
+            const uint64_t j = j0 + u * stride;
+            v[u] = j < count ? x[gather_hash(static_cast<uint32_t>(j) + seed) & mask] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kGatherUnroll; ++u) acc += static_cast<double>(v[u]);
+    }
+    acc = block_sum256(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+template <typename T>
+clsparseStatus gather_sum_entry(const cldenseVector* x, cl_uint window, cl_ulong count, cl_uint seed,
+                                clsparseScalar* sum, clsparseControl control)
+{
+    CLS_TRY(check_init_control(control));
+    if (x == nullptr || sum == nullptr) return clsparseInvalidValue;
+    if (window == 0 || (window & (window - 1)) != 0 || window > x->num_values || count >= (1ull << 32))
+        return clsparseInvalidSize;
+    if (x->values == nullptr || sum->value == nullptr) return clsparseInvalidMemObject;
+    const uint32_t g = static_cast<uint32_t>(std::max<uint64_t>(
+        1, std::min<uint64_t>(kSumAxMaxGrid, (count + 256 * kGatherUnroll - 1) / (256 * kGatherUnroll))));
+    void* work = nullptr;
+    CLS_TRY(control_workspace(control, g * sizeof(double), &work));
+    double* part = static_cast<double*>(work);
+    hipLaunchKernelGGL((gather_sum_kernel<T>), dim3(g), dim3(256), 0, control->stream, count,
+                       dev_ptr<T>(x->values, x->off_values), window - 1, seed, part);
+    hipLaunchKernelGGL((sum_ax_finish<T>), dim3(1), dim3(256), 0, control->stream, part, g,
+                       dev_ptr<T>(sum->value, sum->off_value));
+    CLS_HIP_TRY(hipGetLastError());
+    return finish_op(control);
+}
+
+}  // namespace
+
+extern "C" clsparseStatus clsparseAmdSgatherSum(const cldenseVector* x, cl_uint window, cl_ulong count, cl_uint seed,
+                                                clsparseScalar* sum, const clsparseControl control)
+{
+    return gather_sum_entry<float>(x, window, count, seed, sum, control);
+}
+
+extern "C" clsparseStatus clsparseAmdDgatherSum(const cldenseVector* x, cl_uint window, cl_ulong count, cl_uint seed,
+                                                clsparseScalar* sum, const clsparseControl control)
+{
+    return gather_sum_entry<double>(x, window, count, seed, sum, control);
+}
+
 // ---- 1^T (A x) without allocation ----------------------------------------------
 // The FP64 variant's per-workgroup partials live in a file-scope device array
 // (kSumAxMaxGrid doubles, 32 KiB; one copy per GPU, held until the process

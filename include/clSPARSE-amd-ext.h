@@ -215,6 +215,20 @@ CLSPARSE_EXPORT clsparseStatus clsparseAmdScsrSumAx(const clsparseCsrMatrix* A, 
 CLSPARSE_EXPORT clsparseStatus clsparseAmdDcsrSumAx(const clsparseCsrMatrix* A, const cldenseVector* x,
                                                     clsparseScalar* sum, const clsparseControl control);
 
+/* sum = sum over i < count of x[h(i + seed) mod window]: independent gathers
+ * of x at hashed positions inside its first `window` values (a power of two,
+ * at most x's length; count < 2^32), accumulated in FP64. h is the 32-bit
+ * xor-shift-multiply hash (MurmurHash3 finaliser, constants 0x7feb352d and
+ * 0x846ca68b). The rate at which the device gathers into a window of a given
+ * size: the floor that scattered columns set for any CSR SpMV. No reference
+ * counterpart. */
+CLSPARSE_EXPORT clsparseStatus clsparseAmdSgatherSum(const cldenseVector* x, cl_uint window, cl_ulong count,
+                                                     cl_uint seed, clsparseScalar* sum,
+                                                     const clsparseControl control);
+CLSPARSE_EXPORT clsparseStatus clsparseAmdDgatherSum(const cldenseVector* x, cl_uint window, cl_ulong count,
+                                                     cl_uint seed, clsparseScalar* sum,
+                                                     const clsparseControl control);
+
 /* clsparseAmdDcsrSumAx without any device allocation, for callers that cannot
  * allocate (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc). No counterpart in the reference. Its kernels keep their

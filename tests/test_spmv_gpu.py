@@ -268,3 +268,38 @@ def test_sum_ax_is_the_spmv_checksum(ctl, dtype):
     short = cls.DenseVector(torch.ones(2, dtype=tdt, device="cuda"))
     assert fn(ctypes.byref(empty.s), ctypes.byref(short.s), ctypes.byref(out.s),
               ctl.handle) == cls.clsparseInvalidSize
+
+
+def _hash32(a):
+    a = a & 0xFFFFFFFF
+    a ^= a >> 16
+    a = (a * 0x7FEB352D) & 0xFFFFFFFF
+    a ^= a >> 15
+    a = (a * 0x846CA68B) & 0xFFFFFFFF
+    a ^= a >> 16
+    return a
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["f32", "f64"])
+def test_gather_sum(ctl, dtype):
+    """clsparseAmd[S|D]gatherSum = sum of x[h(i + seed) mod window] over i <
+    count, against the same positions formed with numpy's integer arithmetic;
+    a window that is not a power of two or exceeds x is refused."""
+    import ctypes
+    lib = cls.load()
+    fn = lib.clsparseAmdSgatherSum if dtype == np.float32 else lib.clsparseAmdDgatherSum
+    tdt = torch.float32 if dtype == np.float32 else torch.float64
+    x = cls.gen_uniform(1 << 16, -1, 1, seed=12).astype(dtype)
+    xd = cls.DenseVector(torch.tensor(x, device="cuda"))
+    for window, count, seed in ((1 << 16, 1_000_003, 7), (1 << 10, 300_000, 4_000_000_000), (1, 5, 0)):
+        say(f"gather sum {dtype.__name__} window {window} count {count}")
+        out = cls.Scalar(0.0, tdt)
+        assert fn(ctypes.byref(xd.s), window, count, seed, ctypes.byref(out.s), ctl.handle) == 0
+        idx = _hash32(np.arange(count, dtype=np.uint64) + np.uint64(seed)) & np.uint64(window - 1)
+        vals = x[idx.astype(np.int64)].astype(np.float64)
+        ref = float(np.sum(vals.astype(np.longdouble)))
+        bound = 1e-13 * float(np.sum(np.abs(vals))) + (2.0 ** -24 if dtype == np.float32 else 0.0) * abs(ref)
+        assert abs(out.item() - ref) <= bound, (out.item(), ref)
+    out = cls.Scalar(0.0, tdt)
+    assert fn(ctypes.byref(xd.s), 3, 10, 0, ctypes.byref(out.s), ctl.handle) == cls.clsparseInvalidSize
+    assert fn(ctypes.byref(xd.s), 1 << 17, 10, 0, ctypes.byref(out.s), ctl.handle) == cls.clsparseInvalidSize
