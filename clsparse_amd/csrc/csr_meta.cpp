@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <new>
 
 #include "common.hpp"
@@ -32,8 +33,9 @@ inline uint64_t threads_for_reduction(uint64_t num_rows)
 
 // One pass over the row pointer. With out == nullptr it only counts entries
 // (ComputeRowBlocksSize); otherwise it writes them. Returns the reference's
-// rowBlockSize: 2 * entries written, or 2 * entries counted.
-uint64_t compute_row_blocks(const uint32_t* rp, uint32_t nRows, uint64_t* out)
+// rowBlockSize: 2 * entries written, or 2 * entries counted; a writing pass
+// also stores the counting pass's value in *counted when given.
+uint64_t compute_row_blocks(const uint32_t* rp, uint32_t nRows, uint64_t* out, uint64_t* counted = nullptr)
 {
     const bool write = out != nullptr;
     uint64_t* w = out;
@@ -55,6 +57,23 @@ uint64_t compute_row_blocks(const uint32_t* rp, uint32_t nRows, uint64_t* out)
     };
 
     for (i = 1; i <= nRows; ++i) {
+        // Outside a long-row region a row of at most 128 nonzeros only adds
+        // to the block's sum until the sum reaches the block size (at every
+        // iteration boundary sum == rp[i - 1] - rp[last_i]): skip such rows
+        // and take up the rules again at the first row that can change the
+        // state.
+        if (long_run == 0 && sum == static_cast<uint64_t>(rp[i - 1] - rp[last_i])) {
+            const uint32_t base = rp[last_i];
+            uint64_t r = i;
+            while (r <= nRows && rp[r] - rp[r - 1] <= 128u && rp[r] - base < blk) ++r;
+            if (r > nRows) {
+                i = nRows + 1;
+                sum = rp[nRows] - base;
+                break;
+            }
+            i = r;
+            sum = rp[i - 1] - base;
+        }
         const uint32_t len = rp[i] - rp[i - 1];
         sum += len;
 
@@ -125,8 +144,17 @@ uint64_t compute_row_blocks(const uint32_t* rp, uint32_t nRows, uint64_t* out)
     }
     ++total;
 
+    if (counted) *counted = 2 * total;
     if (write) return 2 * static_cast<uint64_t>(w - out);
     return 2 * total;
+}
+
+// Upper bound on the entries of compute_row_blocks: the first, at most one
+// per row that ends a block or is shorter than a segment, one more per
+// 3072-nonzero segment, and the tail.
+uint64_t row_blocks_bound(uint32_t nRows, uint64_t nnz)
+{
+    return 2 * static_cast<uint64_t>(nRows) + nnz / kLongSeg + 4;
 }
 
 clsparseStatus read_row_pointer(const clsparseCsrMatrix* A, clsparseControl control,
@@ -198,14 +226,22 @@ extern "C" clsparseStatus clsparseCsrMetaCreate(clsparseCsrMatrix* csrMatx,
     if (csrMatx == nullptr) return clsparseInvalidValue;
     if (control == nullptr) return clsparseInvalidControlObject;
 
-    std::vector<uint32_t> rp;
-    CLS_TRY(read_row_pointer(csrMatx, control, rp));
+    // The row pointer comes to the host through the control's pinned buffer.
+    const uint32_t n = csrMatx->num_rows;
+    if (csrMatx->row_pointer == nullptr) return clsparseInvalidMemObject;
+    void* pinned = nullptr;
+    CLS_TRY(control_pinned(control, (static_cast<size_t>(n) + 1) * sizeof(uint32_t), &pinned));
+    const uint32_t* rp = static_cast<const uint32_t*>(pinned);
+    CLS_HIP_TRY(hipMemcpyAsync(pinned, dev_ptr<uint32_t>(csrMatx->row_pointer, csrMatx->off_row_pointer),
+                               (static_cast<size_t>(n) + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               control->stream));
+    CLS_HIP_TRY(hipStreamSynchronize(control->stream));
 
     // The kernels index vals/cols through this row pointer; a malformed one
     // would send them out of bounds, so check it once here on the host.
-    const uint32_t n = csrMatx->num_rows;
-    for (uint32_t r = 0; r < n; ++r)
-        if (rp[r + 1] < rp[r]) return clsparseInvalidValue;
+    bool decreasing = false;
+    for (uint32_t r = 0; r < n; ++r) decreasing |= rp[r + 1] < rp[r];
+    if (decreasing) return clsparseInvalidValue;
     if (rp[n] > csrMatx->num_nonzeros) return clsparseInvalidValue;
 
     if (csrMatx->meta) {
@@ -219,12 +255,25 @@ extern "C" clsparseStatus clsparseCsrMetaCreate(clsparseCsrMatrix* csrMatx,
     m->num_rows = n;
     m->num_nonzeros = csrMatx->num_nonzeros;
 
-    const uint64_t size_words = compute_row_blocks(rp.data(), n, nullptr);
-    std::vector<uint64_t> host(size_words, 0);
-    const uint64_t created = compute_row_blocks(rp.data(), n, host.data());
+    // One writing pass into a buffer of twice the bound on the entries: the
+    // device array has the counting pass's size, two words per entry (the
+    // second half zero, as the reference's).
+    const uint64_t bound = 2 * row_blocks_bound(n, rp[n] - rp[0]);
+    std::unique_ptr<uint64_t[]> host(new (std::nothrow) uint64_t[bound]);
+    if (!host) {
+        delete m;
+        return clsparseOutOfHostMemory;
+    }
+    uint64_t size_words = 0;
+    const uint64_t created = compute_row_blocks(rp, n, host.get(), &size_words);
+    if (size_words > bound) {
+        delete m;
+        return clsparseOutOfHostMemory;
+    }
+    std::fill(host.get() + created / 2, host.get() + size_words, uint64_t(0));
     m->rowBlockSize = static_cast<clsparseIdx_t>(created);
     m->entries = static_cast<uint32_t>(created / 2);
-    m->h_rowBlocks.assign(host.begin(), host.begin() + m->entries);
+    m->h_rowBlocks.assign(host.get(), host.get() + m->entries);
 
     // Long rows: entry f with an empty row range starts a row split into
     // segments f, f+1, ... whose low bits count 1, 2, ...; the last segment
@@ -275,7 +324,7 @@ extern "C" clsparseStatus clsparseCsrMetaCreate(clsparseCsrMatrix* csrMatx,
 
     hipError_t err = hipMalloc(&m->d_rowBlocks, size_words * sizeof(uint64_t));
     if (err == hipSuccess)
-        err = hipMemcpyAsync(m->d_rowBlocks, host.data(), size_words * sizeof(uint64_t),
+        err = hipMemcpyAsync(m->d_rowBlocks, host.get(), size_words * sizeof(uint64_t),
                              hipMemcpyHostToDevice, control->stream);
     if (err == hipSuccess && !desc.empty()) {
         err = hipMalloc(&m->d_desc, desc.size() * sizeof(BlockDesc));

@@ -111,6 +111,28 @@ def test_library_binning_matches_oracle_random(seed):
     assert cls.row_blocks_host(rp, count_only=True) == oracle.row_blocks(rp, count_only=True)
 
 
+def test_library_binning_skips_match_oracle():
+    """The binning skips runs of rows of at most 128 nonzeros outside long-row
+    regions; patterns that stress where the skip stops: the full power-law
+    workload, regular stencil rows, long runs of empty rows, long-row regions
+    continued by 32..128-nonzero rows and left by shorter ones."""
+    rng = np.random.default_rng(77)
+    pats = [cls.gen_powerlaw(1 << 21, 16.0, 1.5, 200000, 8, 100000, seed=1)[0],
+            cls.gen_diffusion3d(96, 96, 96)[0],
+            np.concatenate([[0], np.cumsum(np.tile([200, 40, 129, 31, 0, 0, 5, 128, 1024, 3073, 32, 33], 5000))]),
+            np.concatenate([[0], np.cumsum(np.where(rng.random(300000) < 0.01, rng.integers(0, 3000, 300000),
+                                                    0))]),
+            np.concatenate([[0], np.cumsum(rng.choice([0, 1, 127, 128, 129, 1000, 1023, 1024, 1025], 400000,
+                                                      p=[.3, .3, .1, .1, .05, .05, .04, .03, .03]))])]
+    for rp in pats:
+        rp = np.ascontiguousarray(rp, dtype=np.uint32)
+        e1, s1 = cls.row_blocks_host(rp)
+        e2, s2 = oracle.row_blocks(rp)
+        assert s1 == s2
+        np.testing.assert_array_equal(e1, e2)
+        assert cls.row_blocks_host(rp, count_only=True) == oracle.row_blocks(rp, count_only=True)
+
+
 def test_library_binning_config_patterns_match_oracle():
     # The three structured workloads at full size (pattern only).
     for rp in (cls.gen_poisson2d(256, 256)[0], cls.gen_diffusion3d(64, 64, 64)[0]):
