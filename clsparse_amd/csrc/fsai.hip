@@ -40,7 +40,7 @@ constexpr int kOffW3 = kOffB2 + kHid;            // w3[64]
 constexpr int kOffB3 = kOffW3 + kHid;            // b3
 constexpr int kNumParams = kOffB3 + 1;           // 5313
 constexpr int kMlpWaves = 4;
-constexpr int kMlpGrid = 256;                    // workgroups of the MLP kernels
+constexpr int kMlpGrid = 512;                    // workgroups of the backward kernel
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -89,15 +89,27 @@ __device__ __forceinline__ bf16x4 load_bf16x4(const uint16_t* p)
     return r;
 }
 
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+
 __device__ __forceinline__ bf16x4 pack_bf16x4(f32x4 v)
 {
-    const uint32_t lo = f2bf2(v[0], v[1]), hi = f2bf2(v[2], v[3]);
-    bf16x4 r;
-    r[0] = static_cast<short>(lo & 0xFFFFu);
-    r[1] = static_cast<short>(lo >> 16);
-    r[2] = static_cast<short>(hi & 0xFFFFu);
-    r[3] = static_cast<short>(hi >> 16);
-    return r;
+    return __builtin_bit_cast(bf16x4, u32x2{f2bf2(v[0], v[1]), f2bf2(v[2], v[3])});
+}
+
+// ReLU of a pair of BF16 values: a BF16 value is negative exactly when its
+// bits, read as an int16, are negative, so the 16-bit integer max with 0 is
+// the ReLU (-0 becomes +0). Rounding keeps the sign, so relu(round(a)) =
+// round(relu(a)).
+__device__ __forceinline__ uint32_t relu_bf16x2(uint32_t p)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(i16x2, p), i16x2{0, 0}));
+}
+
+// round_bf16(relu(v)) for 4 activations.
+__device__ __forceinline__ bf16x4 pack_relu_bf16x4(f32x4 v)
+{
+    return __builtin_bit_cast(bf16x4, u32x2{relu_bf16x2(f2bf2(v[0], v[1])), relu_bf16x2(f2bf2(v[2], v[3]))});
 }
 
 // Weights as the MFMA reads them (BF16). W1: [64][16]; W2: [64][64] row =
@@ -149,6 +161,10 @@ struct TileFwd
     bf16x4 x;  // B operand X^T (features 4(l/16)..+3 of nonzero l%16)
 };
 
+// BIAS_INIT: the biases are the MFMA accumulator's initial value (fewer VALU
+// instructions); otherwise they are added to the products' sum (fewer live
+// registers across the MFMA chains, which the backward kernel needs).
+template <bool BIAS_INIT>
 __device__ __forceinline__ void tile_forward(const NetRegs& w, const uint16_t* __restrict__ X,
                                              uint32_t r0, uint32_t nnz, uint32_t lane, TileFwd& t)
 {
@@ -158,36 +174,39 @@ __device__ __forceinline__ void tile_forward(const NetRegs& w, const uint16_t* _
         t.x = load_bf16x4(X + static_cast<size_t>(nz) * kFeat + 4 * q);
     else
         t.x = bf16x4{0, 0, 0, 0};
-    // Layer 1: A1^T (64 x 16nnz) = W1 (64 x 16) . X^T (16 x 16nnz)
+    // Layer 1: A1^T (64 x 16nnz) = W1 (64 x 16) . X^T (16 x 16nnz) + b1
     for (int m = 0; m < 4; ++m) {
         f32x4 c = {0.f, 0.f, 0.f, 0.f};
+        if constexpr (BIAS_INIT)
+            for (int e = 0; e < 4; ++e) c[e] = w.b1[4 * m + e];
         c = mfma16(w.w1[m], t.x, c);
-        for (int e = 0; e < 4; ++e) c[e] += w.b1[4 * m + e];
+        if constexpr (!BIAS_INIT)
+            for (int e = 0; e < 4; ++e) c[e] += w.b1[4 * m + e];
         t.a1[m] = c;
-        f32x4 h;
-        for (int e = 0; e < 4; ++e) h[e] = c[e] > 0.f ? c[e] : 0.f;
-        t.h1[m] = pack_bf16x4(h);
+        t.h1[m] = pack_relu_bf16x4(c);
     }
-    // Layer 2: A2^T (64 x 16) = W2 (64 x 64) . H1^T (64 x 16)
+    // Layer 2: A2^T (64 x 16) = W2 (64 x 64) . H1^T (64 x 16) + b2
     for (int m = 0; m < 4; ++m) {
         f32x4 c = {0.f, 0.f, 0.f, 0.f};
+        if constexpr (BIAS_INIT)
+            for (int e = 0; e < 4; ++e) c[e] = w.b2[4 * m + e];
         for (int kb = 0; kb < 4; ++kb) c = mfma16(w.w2[m][kb], t.h1[kb], c);
-        for (int e = 0; e < 4; ++e) c[e] += w.b2[4 * m + e];
+        if constexpr (!BIAS_INIT)
+            for (int e = 0; e < 4; ++e) c[e] += w.b2[4 * m + e];
         t.a2[m] = c;
     }
 }
 
-// o = w3 . relu(a2) + b3 for the lane's nonzero (all 4 lanes of a nonzero agree).
+// o = w3 . round_bf16(relu(a2)) + b3 for the lane's nonzero (all 4 lanes of a
+// nonzero agree).
 __device__ __forceinline__ float tile_output(const NetRegs& w, const TileFwd& t)
 {
     float s = 0.f;
     for (int m = 0; m < 4; ++m)
         for (int e = 0; e < 4; e += 2) {
-            const uint32_t pk = f2bf2(t.a2[m][e], t.a2[m][e + 1]);
-            const float h0 = t.a2[m][e] > 0.f ? bf2f(static_cast<uint16_t>(pk & 0xFFFFu)) : 0.f;
-            const float h1 = t.a2[m][e + 1] > 0.f ? bf2f(static_cast<uint16_t>(pk >> 16)) : 0.f;
-            s += w.w3[4 * m + e] * h0;
-            s += w.w3[4 * m + e + 1] * h1;
+            const uint32_t pk = relu_bf16x2(f2bf2(t.a2[m][e], t.a2[m][e + 1]));
+            s += w.w3[4 * m + e] * __uint_as_float(pk << 16);
+            s += w.w3[4 * m + e + 1] * __uint_as_float(pk & 0xFFFF0000u);
         }
     s += __shfl_xor(s, 16, 64);
     s += __shfl_xor(s, 32, 64);
@@ -206,120 +225,162 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_forward_kernel(MlpDev net,
     load_net(net, lane, w);
     for (uint32_t tile = blockIdx.x * kMlpWaves + wave; tile < tiles; tile += gridDim.x * kMlpWaves) {
         TileFwd t;
-        tile_forward(w, X, tile * 16, nnz, lane, t);
+        tile_forward<true>(w, X, tile * 16, nnz, lane, t);
         const float o = tile_output(w, t);
         const uint32_t nz = tile * 16 + (lane & 15);
         if (lane < 16 && nz < nnz) out[nz] = o;
     }
 }
 
-// Backward: recompute the tile's forward pass, then accumulate the wave's
-// parameter gradients; the 4 waves of a workgroup add theirs into one LDS
-// vector in wave order, and each workgroup writes one partial gradient vector
-// (reduced later in a fixed order).
+// Backward. Per tile of 16 nonzeros the forward pass is recomputed in two
+// layouts from the same register fragments: the forward kernel's unit x nnz
+// layout ("T": lane (q, c) holds units 16m + 4q + e of nonzero c) and the
+// nnz x unit layout ("N": lane (q, c) holds nonzeros 4q + e of unit 16n + c;
+// A1 = X W1^T with X as the A operand and W1's fragments as the B operand,
+// A2 = H1 W2^T likewise). Every product then reads its operands in the layout
+// the MFMA wants straight from registers, with no pass through LDS:
+//   D1 = relu'(A1) (D2 W2)   A = D2 in T, B = W2^T fragments   -> N
+//   dW2 += D2^T H1           A = D2 in N, B = H1 in N
+//   dW1 += D1^T X            A = D1 in N, B = X in N (loaded so)
+// In N each lane holds one unit per block for 4 nonzeros, so the bias-like
+// sums (db1, db2, dw3) take 4 accumulators per lane, reduced over the 4 lane
+// groups at the end. The 4 waves of a workgroup add their gradients into one
+// LDS vector in wave order, and each workgroup writes one partial gradient
+// vector (reduced later in a fixed order).
 __global__ __launch_bounds__(64 * kMlpWaves) void mlp_backward_kernel(MlpDev net,
                                                                        const uint16_t* __restrict__ X,
                                                                        const float* __restrict__ gout,
                                                                        uint32_t nnz,
                                                                        float* __restrict__ partials)
 {
-    __shared__ uint16_t sh_t[kMlpWaves][kHid][16];   // a 64 x 16 tile, nnz contiguous
-    __shared__ uint16_t sh_x[kMlpWaves][kFeat][16];  // X^T tile, nnz contiguous
-    __shared__ float sh_g[kNumParams];               // the workgroup's gradient
+    __shared__ float sh_vec[3][kHid];  // b1, b2, w3 for the T-layout reads
+    __shared__ float sh_g[kNumParams];  // the workgroup's gradient
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t q = lane >> 4, c16 = lane & 15;
 
-    NetRegs w;
-    load_net(net, lane, w);
-    bf16x4 w2t[4][4];  // A fragments of W2^T: rows = input unit, K = output unit
-    for (int m = 0; m < 4; ++m)
-        for (int kb = 0; kb < 4; ++kb)
+    for (uint32_t i = threadIdx.x; i < static_cast<uint32_t>(kHid); i += 64 * kMlpWaves) {
+        sh_vec[0][i] = net.p[kOffB1 + i];
+        sh_vec[1][i] = net.p[kOffB2 + i];
+        sh_vec[2][i] = net.p[kOffW3 + i];
+    }
+    bf16x4 w1[4], w2[4][4], w2t[4][4];
+    float b1n[4], b2n[4], w3n[4];  // unit 16n + c16
+    for (int m = 0; m < 4; ++m) {
+        w1[m] = load_bf16x4(net.W1 + (16 * m + c16) * kFeat + 4 * q);
+        for (int kb = 0; kb < 4; ++kb) {
+            w2[m][kb] = load_bf16x4(net.W2 + (16 * m + c16) * kHid + 16 * kb + 4 * q);
             w2t[m][kb] = load_bf16x4(net.W2T + (16 * m + c16) * kHid + 16 * kb + 4 * q);
+        }
+        b1n[m] = net.p[kOffB1 + 16 * m + c16];
+        b2n[m] = net.p[kOffB2 + 16 * m + c16];
+        w3n[m] = net.p[kOffW3 + 16 * m + c16];
+    }
+    __syncthreads();
 
     f32x4 dW2[4][4];  // block (m = output unit, n = input unit)
     f32x4 dW1[4];     // block m (rows = hidden), 16 features
-    float db1[16], db2[16], dw3[16];  // per lane: units 16m + 4q + e
+    float db1[4], db2[4], dw3[4];  // per lane: unit 16n + c16
     float db3 = 0.f;
     for (int m = 0; m < 4; ++m) {
         dW1[m] = f32x4{0.f, 0.f, 0.f, 0.f};
         for (int n = 0; n < 4; ++n) dW2[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+        db1[m] = db2[m] = dw3[m] = 0.f;
     }
-    for (int i = 0; i < 16; ++i) db1[i] = db2[i] = dw3[i] = 0.f;
 
     const uint32_t tiles = (nnz + 15) / 16;
     for (uint32_t tile = blockIdx.x * kMlpWaves + wave; tile < tiles; tile += gridDim.x * kMlpWaves) {
-        TileFwd t;
-        tile_forward(w, X, tile * 16, nnz, lane, t);
-        const uint32_t nz = tile * 16 + c16;
-        const float g = (nz < nnz) ? gout[nz] : 0.f;
-        if (q == 0) db3 += g;
+        const uint32_t r0 = tile * 16;
+        const uint32_t nzT = r0 + c16;
+        // X: the T layout's B operand (= the N layout's A operand) holds
+        // features 4q..4q+3 of nonzero c16; the N layout's dW1 B operand
+        // holds feature c16 of nonzeros 4q + e.
+        const bf16x4 x = nzT < nnz ? load_bf16x4(X + static_cast<size_t>(nzT) * kFeat + 4 * q) : bf16x4{0, 0, 0, 0};
+        bf16x4 xn;
+        float gN[4];
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t nz = r0 + 4 * q + e;
+            xn[e] = nz < nnz ? static_cast<short>(X[static_cast<size_t>(nz) * kFeat + c16]) : short(0);
+            gN[e] = nz < nnz ? gout[nz] : 0.f;
+        }
+        const float gT = nzT < nnz ? gout[nzT] : 0.f;
+        if (q == 0) db3 += gT;
 
-        // D2^T = g * w3 * relu'(a2); dw3 += g * h2
-        f32x4 d2[4];
-        for (int m = 0; m < 4; ++m)
-            for (int e0 = 0; e0 < 4; e0 += 2) {
-                const uint32_t pk = f2bf2(t.a2[m][e0], t.a2[m][e0 + 1]);
-                for (int k = 0; k < 2; ++k) {
-                    const int e = e0 + k;
-                    const bool on = t.a2[m][e] > 0.f;
-                    const float h2 = on ? bf2f(static_cast<uint16_t>(k ? pk >> 16 : pk & 0xFFFFu)) : 0.f;
-                    dw3[4 * m + e] += g * h2;
-                    const float d = on ? g * w.w3[4 * m + e] : 0.f;
-                    d2[m][e] = d;
-                    db2[4 * m + e] += d;
-                }
-            }
-        bf16x4 d2b[4];
-        for (int m = 0; m < 4; ++m) d2b[m] = pack_bf16x4(d2[m]);
-
-        // D1^T = (W2^T D2^T) * relu'(a1)
-        f32x4 d1[4];
+        // Forward, T layout: H1^T, A2^T.
+        bf16x4 h1T[4];
         for (int m = 0; m < 4; ++m) {
-            f32x4 c = {0.f, 0.f, 0.f, 0.f};
-            for (int kb = 0; kb < 4; ++kb) c = mfma16(w2t[m][kb], d2b[kb], c);
+            const f32x4 b = *reinterpret_cast<const f32x4*>(&sh_vec[0][16 * m + 4 * q]);
+            f32x4 a = mfma16(w1[m], x, f32x4{0.f, 0.f, 0.f, 0.f});
+            for (int e = 0; e < 4; ++e) a[e] += b[e];
+            h1T[m] = pack_relu_bf16x4(a);
+        }
+        // D2 in T layout from A2^T: relu'(a2) g w3 (FP32), then BF16.
+        bf16x4 d2T[4];
+        for (int m = 0; m < 4; ++m) {
+            const f32x4 b = *reinterpret_cast<const f32x4*>(&sh_vec[1][16 * m + 4 * q]);
+            const f32x4 w = *reinterpret_cast<const f32x4*>(&sh_vec[2][16 * m + 4 * q]);
+            f32x4 a = {0.f, 0.f, 0.f, 0.f};
+            for (int kb = 0; kb < 4; ++kb) a = mfma16(w2[m][kb], h1T[kb], a);
+            f32x4 d;
+            for (int e = 0; e < 4; ++e) d[e] = a[e] + b[e] > 0.f ? gT * w[e] : 0.f;
+            d2T[m] = pack_bf16x4(d);
+        }
+        // Forward, N layout: A1 = X W1^T, A2 = H1 W2^T; D2 and H2 in N.
+        bf16x4 h1n[4];
+        uint32_t a1pos = 0;  // relu'(A1) in N: bit 4n + e
+        for (int n = 0; n < 4; ++n) {
+            f32x4 a = mfma16(x, w1[n], f32x4{0.f, 0.f, 0.f, 0.f});
             for (int e = 0; e < 4; ++e) {
-                const float d = t.a1[m][e] > 0.f ? c[e] : 0.f;
-                d1[m][e] = d;
-                db1[4 * m + e] += d;
+                a[e] += b1n[n];
+                a1pos |= (a[e] > 0.f ? 1u : 0u) << (4 * n + e);
             }
+            h1n[n] = pack_relu_bf16x4(a);
         }
-
-        // dW2 += D2^T H1 and dW1 += D1^T X: the nonzeros go in the K slot,
-        // through LDS ([unit][nnz] BF16 tiles).
-        for (int m = 0; m < 4; ++m)
-            for (int e = 0; e < 4; ++e) sh_t[wave][16 * m + 4 * q + e][c16] = static_cast<uint16_t>(d2b[m][e]);
-        wave_sync_lds();
-        bf16x4 aD2[4];
-        for (int m = 0; m < 4; ++m) aD2[m] = load_bf16x4(&sh_t[wave][16 * m + c16][4 * q]);
-        wave_sync_lds();
-        for (int m = 0; m < 4; ++m)
-            for (int e = 0; e < 4; ++e) sh_t[wave][16 * m + 4 * q + e][c16] = static_cast<uint16_t>(t.h1[m][e]);
-        wave_sync_lds();
-        bf16x4 bH1[4];
-        for (int n = 0; n < 4; ++n) bH1[n] = load_bf16x4(&sh_t[wave][16 * n + c16][4 * q]);
-        wave_sync_lds();
-        for (int m = 0; m < 4; ++m) {
-            const bf16x4 d1b = pack_bf16x4(d1[m]);
-            for (int e = 0; e < 4; ++e) sh_t[wave][16 * m + 4 * q + e][c16] = static_cast<uint16_t>(d1b[e]);
+        bf16x4 d2n[4];
+        for (int n = 0; n < 4; ++n) {
+            f32x4 a = {0.f, 0.f, 0.f, 0.f};
+            for (int kb = 0; kb < 4; ++kb) a = mfma16(h1T[kb], w2[n][kb], a);
+            f32x4 d;
+            for (int e = 0; e < 4; ++e) {
+                a[e] += b2n[n];
+                d[e] = a[e] > 0.f ? gN[e] * w3n[n] : 0.f;
+                db2[n] += d[e];
+            }
+            d2n[n] = pack_bf16x4(d);
+            // dw3 += g * round_bf16(relu(a2))
+            const uint32_t lo = relu_bf16x2(f2bf2(a[0], a[1])), hi = relu_bf16x2(f2bf2(a[2], a[3]));
+            dw3[n] += gN[0] * __uint_as_float(lo << 16);
+            dw3[n] += gN[1] * __uint_as_float(lo & 0xFFFF0000u);
+            dw3[n] += gN[2] * __uint_as_float(hi << 16);
+            dw3[n] += gN[3] * __uint_as_float(hi & 0xFFFF0000u);
         }
-        for (int e = 0; e < 4; ++e) sh_x[wave][4 * q + e][c16] = static_cast<uint16_t>(t.x[e]);
-        wave_sync_lds();
-        const bf16x4 bX = load_bf16x4(&sh_x[wave][c16][4 * q]);
-        bf16x4 aD1[4];
-        for (int m = 0; m < 4; ++m) aD1[m] = load_bf16x4(&sh_t[wave][16 * m + c16][4 * q]);
-        wave_sync_lds();
-        for (int n = 0; n < 4; ++n)
-            for (int m = 0; m < 4; ++m) dW2[m][n] = mfma16(aD2[m], bH1[n], dW2[m][n]);
-        for (int m = 0; m < 4; ++m) dW1[m] = mfma16(aD1[m], bX, dW1[m]);
+        // D1 in N: relu'(A1) (D2 W2), A = D2 in T, B = W2^T fragments.
+        bf16x4 d1n[4];
+        for (int n = 0; n < 4; ++n) {
+            f32x4 a = {0.f, 0.f, 0.f, 0.f};
+            for (int kb = 0; kb < 4; ++kb) a = mfma16(d2T[kb], w2t[n][kb], a);
+            f32x4 d;
+            for (int e = 0; e < 4; ++e) {
+                d[e] = (a1pos >> (4 * n + e)) & 1u ? a[e] : 0.f;
+                db1[n] += d[e];
+            }
+            d1n[n] = pack_bf16x4(d);
+        }
+        // Weight gradients, the nonzeros in the K slot.
+        for (int m = 0; m < 4; ++m)
+            for (int n = 0; n < 4; ++n) dW2[m][n] = mfma16(d2n[m], h1n[n], dW2[m][n]);
+        for (int m = 0; m < 4; ++m) dW1[m] = mfma16(d1n[m], xn, dW1[m]);
     }
 
-    // Bias-like sums: lanes with the same q hold the same units for 16
-    // different nonzeros -> reduce over the 16 lanes of a row (DPP).
-    for (int i = 0; i < 16; ++i) {
-        db1[i] = group_sum(db1[i], 16);
-        db2[i] = group_sum(db2[i], 16);
-        dw3[i] = group_sum(dw3[i], 16);
+    // Bias-like sums: the 4 lane groups hold the same units for different
+    // nonzeros.
+    for (int n = 0; n < 4; ++n) {
+        db1[n] += __shfl_xor(db1[n], 16, 64);
+        db1[n] += __shfl_xor(db1[n], 32, 64);
+        db2[n] += __shfl_xor(db2[n], 16, 64);
+        db2[n] += __shfl_xor(db2[n], 32, 64);
+        dw3[n] += __shfl_xor(dw3[n], 16, 64);
+        dw3[n] += __shfl_xor(dw3[n], 32, 64);
     }
     db3 = wave_sum(db3);
 
@@ -335,13 +396,12 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_backward_kernel(MlpDev net
                         put(kOffW2 + (16 * m + 4 * q + e) * kHid + 16 * n + c16, dW2[m][n][e]);
             for (int m = 0; m < 4; ++m)
                 for (int e = 0; e < 4; ++e) put(kOffW1 + (16 * m + 4 * q + e) * kFeat + c16, dW1[m][e]);
-            if (c16 == 0)
-                for (int m = 0; m < 4; ++m)
-                    for (int e = 0; e < 4; ++e) {
-                        put(kOffB1 + 16 * m + 4 * q + e, db1[4 * m + e]);
-                        put(kOffB2 + 16 * m + 4 * q + e, db2[4 * m + e]);
-                        put(kOffW3 + 16 * m + 4 * q + e, dw3[4 * m + e]);
-                    }
+            if (q == 0)
+                for (int n = 0; n < 4; ++n) {
+                    put(kOffB1 + 16 * n + c16, db1[n]);
+                    put(kOffB2 + 16 * n + c16, db2[n]);
+                    put(kOffW3 + 16 * n + c16, dw3[n]);
+                }
             if (lane == 0) put(kOffB3, db3);
         }
         __syncthreads();
