@@ -682,6 +682,10 @@ struct _clsparseAmdFsai
     bool meta_ready = false;
     clsparseCsrMatrix G16{}, GT16{};  // share G32's / GT32's metadata (not owned)
     bool valid = true;
+    // Device memory of the arrays above: two allocations (the n-sized
+    // arrays, then the nnz(G)-sized ones), carved in 256-byte pieces.
+    void* arena_n = nullptr;
+    void* arena_g = nullptr;
     // Training scratch (FP64 vectors of length n, and the G gradient).
     double *t = nullptr, *u = nullptr, *s = nullptr, *v = nullptr, *t2 = nullptr, *w = nullptr,
            *z = nullptr, *gG = nullptr, *sc = nullptr;
@@ -722,9 +726,7 @@ void free_fsai(_clsparseAmdFsai* f)
     if (!f) return;
     clsparseCsrMetaDelete(&f->G64);  // G32 / G16 share it
     clsparseCsrMetaDelete(&f->GT64);
-    void* ptrs[] = {f->g_rp, f->g_ci, f->gt_rp, f->gt_ci, f->g_row, f->g_src, f->g_tpos,
-                    f->dhat, f->stats, f->X, f->o, f->g64, f->gt64, f->g32, f->gt32, f->g16, f->gt16, f->bad,
-                    f->dia16, f->dia_pos, f->t, f->u, f->s, f->v, f->t2, f->w, f->z, f->gG, f->sc, f->gout};
+    void* ptrs[] = {f->arena_n, f->arena_g, f->t, f->u, f->s, f->v, f->t2, f->w, f->z, f->gG, f->sc, f->gout};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete f;
@@ -929,24 +931,24 @@ clsparseStatus check_row_pointer(const uint32_t* rp, uint32_t n, uint32_t nnz, i
     return h ? clsparseInvalidValue : clsparseSuccess;
 }
 
-// Device buffers freed at scope exit.
-struct Scratch
+// Byte offsets of arrays to be carved from one device allocation, each
+// 256-byte aligned (planned before the allocation is made).
+struct ByteLayout
 {
-    std::vector<void*> p;
+    size_t bytes = 0;
     template <typename T>
-    hipError_t get(T** out, size_t count)
+    size_t add(size_t count)
     {
-        void* q = nullptr;
-        const hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
-        if (e == hipSuccess) p.push_back(q);
-        *out = static_cast<T*>(q);
-        return e;
-    }
-    ~Scratch()
-    {
-        for (void* q : p) (void)hipFree(q);
+        const size_t at = bytes;
+        bytes += (std::max<size_t>(count, 1) * sizeof(T) + 255) & ~static_cast<size_t>(255);
+        return at;
     }
 };
+template <typename T>
+T* at(void* base, size_t offset)
+{
+    return reinterpret_cast<T*>(static_cast<char*>(base) + offset);
+}
 
 // G = tril(A) (CSR, with each entry's row and source index in A), G^T (CSR,
 // rows of G^T in ascending order), and the diagonal storage of G's BF16
@@ -957,21 +959,42 @@ clsparseStatus build_pattern(_clsparseAmdFsai* f, const uint32_t* arp, const uin
                              uint32_t nnzA, clsparseControl control)
 {
     hipStream_t st = control->stream;
-    Scratch tmp;
-    uint32_t *cnt = nullptr, *tcnt = nullptr;
-    int* flags = nullptr;  // [0] bad, [1] no diagonal, [2] off the diagonals
-    CLS_HIP_TRY(tmp.get(&cnt, n));
-    CLS_HIP_TRY(tmp.get(&flags, 4));
+    // The n-sized arrays of the factor in one allocation.
+    {
+        ByteLayout c;
+        const size_t o_grp = c.add<uint32_t>(static_cast<size_t>(n) + 1), o_gtrp = c.add<uint32_t>(static_cast<size_t>(n) + 1);
+        const size_t o_dhat = c.add<double>(n), o_stats = c.add<float>(3 * static_cast<size_t>(n));
+        if (hipMalloc(&f->arena_n, c.bytes) != hipSuccess) {
+            f->arena_n = nullptr;
+            return clsparseOutOfResources;
+        }
+        f->g_rp = at<uint32_t>(f->arena_n, o_grp);
+        f->gt_rp = at<uint32_t>(f->arena_n, o_gtrp);
+        f->dhat = at<double>(f->arena_n, o_dhat);
+        f->stats = at<float>(f->arena_n, o_stats);
+    }
+    // Build scratch in the control's grow-only workspace: the row counts, the
+    // flags and the scan's temporary storage first.
+    size_t tb = 0;
+    CLS_HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, tb, static_cast<uint32_t*>(nullptr), f->g_rp + 1,
+                                                 static_cast<int>(n), st));
+    uint32_t* cnt = nullptr;
+    int* flags = nullptr;  // [0] bad, [1] no diagonal
+    void* tstore = nullptr;
+    {
+        ByteLayout c;
+        const size_t o_cnt = c.add<uint32_t>(n), o_flags = c.add<int>(4), o_t = c.add<char>(tb);
+        void* work = nullptr;
+        CLS_TRY(control_workspace(control, c.bytes, &work));
+        cnt = at<uint32_t>(work, o_cnt);
+        flags = at<int>(work, o_flags);
+        tstore = at<void>(work, o_t);
+    }
     CLS_HIP_TRY(hipMemsetAsync(flags, 0, 4 * sizeof(int), st));
     hipLaunchKernelGGL(tril_count_kernel, dim3(blocks256(n)), dim3(256), 0, st, n, nnzA, arp, aci, cnt, flags,
                        flags + 1);
     CLS_HIP_TRY(hipGetLastError());
-    if (dmalloc(&f->g_rp, static_cast<size_t>(n) + 1) != hipSuccess) return clsparseOutOfResources;
     CLS_HIP_TRY(hipMemsetAsync(f->g_rp, 0, sizeof(uint32_t), st));
-    size_t tb = 0;
-    CLS_HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, tb, cnt, f->g_rp + 1, static_cast<int>(n), st));
-    void* tstore = nullptr;
-    CLS_HIP_TRY(tmp.get(reinterpret_cast<char**>(&tstore), tb));
     CLS_HIP_TRY(hipcub::DeviceScan::InclusiveSum(tstore, tb, cnt, f->g_rp + 1, static_cast<int>(n), st));
     int hflags[4] = {0, 0, 0, 0};
     uint32_t nnzG = 0;
@@ -982,17 +1005,71 @@ clsparseStatus build_pattern(_clsparseAmdFsai* f, const uint32_t* arp, const uin
     f->nnzG = nnzG;
     f->full_diag = hflags[1] == 0;
 
+    // The nnz(G)-sized arrays of the factor in one allocation (the diagonal
+    // storage sized for the most diagonals it may hold).
+    {
+        ByteLayout c;
+        const size_t o_ci = c.add<uint32_t>(nnzG), o_row = c.add<uint32_t>(nnzG), o_src = c.add<uint32_t>(nnzG);
+        const size_t o_gtci = c.add<uint32_t>(nnzG), o_tpos = c.add<uint32_t>(nnzG);
+        const size_t o_X = c.add<uint16_t>(static_cast<size_t>(nnzG) * kFeat), o_o = c.add<float>(nnzG);
+        const size_t o_g64 = c.add<double>(nnzG), o_gt64 = c.add<double>(nnzG);
+        const size_t o_g32 = c.add<float>(nnzG), o_gt32 = c.add<float>(nnzG);
+        const size_t o_g16 = c.add<uint16_t>(nnzG), o_gt16 = c.add<uint16_t>(nnzG);
+        const size_t o_dia = c.add<uint16_t>(static_cast<size_t>(kFsaiMaxDiags) * n), o_dpos = c.add<uint32_t>(nnzG);
+        const size_t o_bad = c.add<int>(1);
+        if (hipMalloc(&f->arena_g, c.bytes) != hipSuccess) {
+            f->arena_g = nullptr;
+            return clsparseOutOfResources;
+        }
+        void* b = f->arena_g;
+        f->g_ci = at<uint32_t>(b, o_ci);
+        f->g_row = at<uint32_t>(b, o_row);
+        f->g_src = at<uint32_t>(b, o_src);
+        f->gt_ci = at<uint32_t>(b, o_gtci);
+        f->g_tpos = at<uint32_t>(b, o_tpos);
+        f->X = at<uint16_t>(b, o_X);
+        f->o = at<float>(b, o_o);
+        f->g64 = at<double>(b, o_g64);
+        f->gt64 = at<double>(b, o_gt64);
+        f->g32 = at<float>(b, o_g32);
+        f->gt32 = at<float>(b, o_gt32);
+        f->g16 = at<uint16_t>(b, o_g16);
+        f->gt16 = at<uint16_t>(b, o_gt16);
+        f->dia16 = at<uint16_t>(b, o_dia);
+        f->dia_pos = at<uint32_t>(b, o_dpos);
+        f->bad = at<int>(b, o_bad);
+    }
+    // Scratch for G^T: the sort's keys and indices, the column counts, the
+    // scan's and the sort's temporary storage, a flag (the workspace is
+    // taken again: nothing of the first scratch is needed any more).
+    int end_bit = 32;
+    while (end_bit < 64 && (uint64_t(1) << (end_bit - 32)) < n) ++end_bit;
     uint64_t *key = nullptr, *key2 = nullptr;
-    uint32_t *idx = nullptr, *idx2 = nullptr;
-    if (dmalloc(&f->g_ci, nnzG) != hipSuccess || dmalloc(&f->g_row, nnzG) != hipSuccess ||
-        dmalloc(&f->g_src, nnzG) != hipSuccess || dmalloc(&f->gt_rp, static_cast<size_t>(n) + 1) != hipSuccess ||
-        dmalloc(&f->gt_ci, nnzG) != hipSuccess || dmalloc(&f->g_tpos, nnzG) != hipSuccess)
-        return clsparseOutOfResources;
-    CLS_HIP_TRY(tmp.get(&key, nnzG));
-    CLS_HIP_TRY(tmp.get(&key2, nnzG));
-    CLS_HIP_TRY(tmp.get(&idx, nnzG));
-    CLS_HIP_TRY(tmp.get(&idx2, nnzG));
-    CLS_HIP_TRY(tmp.get(&tcnt, n));
+    uint32_t *idx = nullptr, *idx2 = nullptr, *tcnt = nullptr;
+    size_t tb2 = 0, tb3 = 0;
+    CLS_HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, tb2, static_cast<uint32_t*>(nullptr), f->gt_rp + 1,
+                                                 static_cast<int>(n), st));
+    CLS_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb3, static_cast<uint64_t*>(nullptr),
+                                                   static_cast<uint64_t*>(nullptr), static_cast<uint32_t*>(nullptr),
+                                                   static_cast<uint32_t*>(nullptr), static_cast<int>(nnzG), 0,
+                                                   end_bit, st));
+    void* tstore2 = nullptr;
+    {
+        ByteLayout c;
+        const size_t o_k = c.add<uint64_t>(nnzG), o_k2 = c.add<uint64_t>(nnzG);
+        const size_t o_i = c.add<uint32_t>(nnzG), o_i2 = c.add<uint32_t>(nnzG), o_tc = c.add<uint32_t>(n);
+        const size_t o_t = c.add<char>(std::max(tb2, tb3)), o_f = c.add<int>(4);
+        void* work = nullptr;
+        CLS_TRY(control_workspace(control, c.bytes, &work));
+        key = at<uint64_t>(work, o_k);
+        key2 = at<uint64_t>(work, o_k2);
+        idx = at<uint32_t>(work, o_i);
+        idx2 = at<uint32_t>(work, o_i2);
+        tcnt = at<uint32_t>(work, o_tc);
+        tstore2 = at<void>(work, o_t);
+        flags = at<int>(work, o_f);
+    }
+    CLS_HIP_TRY(hipMemsetAsync(flags, 0, 4 * sizeof(int), st));
     hipLaunchKernelGGL(tril_fill_kernel, dim3(blocks256(n)), dim3(256), 0, st, n, arp, aci, f->g_rp, f->g_ci,
                        f->g_row, f->g_src, key, idx);
     // G^T: row pointer from the column counts, columns and positions from a
@@ -1001,15 +1078,6 @@ clsparseStatus build_pattern(_clsparseAmdFsai* f, const uint32_t* arp, const uin
     hipLaunchKernelGGL(column_count_kernel, dim3(blocks256(nnzG)), dim3(256), 0, st, nnzG, f->g_ci, tcnt);
     CLS_HIP_TRY(hipGetLastError());
     CLS_HIP_TRY(hipMemsetAsync(f->gt_rp, 0, sizeof(uint32_t), st));
-    size_t tb2 = 0;
-    CLS_HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, tb2, tcnt, f->gt_rp + 1, static_cast<int>(n), st));
-    int end_bit = 32;
-    while (end_bit < 64 && (uint64_t(1) << (end_bit - 32)) < n) ++end_bit;
-    size_t tb3 = 0;
-    CLS_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb3, key, key2, idx, idx2, static_cast<int>(nnzG), 0,
-                                                   end_bit, st));
-    void* tstore2 = nullptr;
-    CLS_HIP_TRY(tmp.get(reinterpret_cast<char**>(&tstore2), std::max(tb2, tb3)));
     CLS_HIP_TRY(hipcub::DeviceScan::InclusiveSum(tstore2, tb2, tcnt, f->gt_rp + 1, static_cast<int>(n), st));
     if (nnzG > 0) {
         CLS_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tstore2, tb3, key, key2, idx, idx2, static_cast<int>(nnzG), 0,
@@ -1032,8 +1100,6 @@ clsparseStatus build_pattern(_clsparseAmdFsai* f, const uint32_t* arp, const uin
     if (K > kFsaiMaxDiags || static_cast<uint64_t>(K) * n >= (uint64_t(1) << 32) || (K > 0 && off.o[K - 1] != 0))
         K = 0;
     if (K > 0) {
-        if (dmalloc(&f->dia16, static_cast<size_t>(K) * n) != hipSuccess || dmalloc(&f->dia_pos, nnzG) != hipSuccess)
-            return clsparseOutOfResources;
         CLS_HIP_TRY(hipMemsetAsync(f->dia16, 0, static_cast<size_t>(K) * n * sizeof(uint16_t), st));  // padding stays 0
         hipLaunchKernelGGL(dia_position_kernel, dim3(blocks256(nnzG)), dim3(256), 0, st, nnzG, n, f->g_ci, f->g_row, K,
                            off, f->dia_pos, flags + 2);
@@ -1041,13 +1107,11 @@ clsparseStatus build_pattern(_clsparseAmdFsai* f, const uint32_t* arp, const uin
         int off_diag = 0;
         CLS_HIP_TRY(hipMemcpyAsync(&off_diag, flags + 2, sizeof(int), hipMemcpyDeviceToHost, st));
         CLS_HIP_TRY(hipStreamSynchronize(st));
-        if (off_diag) {
-            (void)hipFree(f->dia16);
-            (void)hipFree(f->dia_pos);
-            f->dia16 = nullptr;
-            f->dia_pos = nullptr;
-            K = 0;
-        }
+        if (off_diag) K = 0;
+    }
+    if (K == 0) {
+        f->dia16 = nullptr;
+        f->dia_pos = nullptr;
     }
     f->ndiag = K;
     for (int d = 0; d < K; ++d) f->dia_off[d] = off.o[d];
@@ -1096,22 +1160,6 @@ extern "C" clsparseStatus clsparseAmdFsaiCreate(clsparseAmdFsai* out, const clsp
         return s;
     }
     const uint32_t nnzG = f->nnzG;
-    hipError_t e = hipSuccess;
-    if (e == hipSuccess) e = dmalloc(&f->dhat, n);
-    if (e == hipSuccess) e = dmalloc(&f->stats, 3 * static_cast<size_t>(n));
-    if (e == hipSuccess) e = dmalloc(&f->X, static_cast<size_t>(nnzG) * kFeat);
-    if (e == hipSuccess) e = dmalloc(&f->o, nnzG);
-    if (e == hipSuccess) e = dmalloc(&f->g64, nnzG);
-    if (e == hipSuccess) e = dmalloc(&f->gt64, nnzG);
-    if (e == hipSuccess) e = dmalloc(&f->g32, nnzG);
-    if (e == hipSuccess) e = dmalloc(&f->gt32, nnzG);
-    if (e == hipSuccess) e = dmalloc(&f->g16, nnzG);
-    if (e == hipSuccess) e = dmalloc(&f->gt16, nnzG);
-    if (e == hipSuccess) e = dmalloc(&f->bad, 1);
-    if (e != hipSuccess) {
-        free_fsai(f);
-        return clsparseOutOfResources;
-    }
     auto csr = [&](clsparseCsrMatrix& M, uint32_t* prp, uint32_t* pci, void* vals) {
         clsparseInitCsrMatrix(&M);
         M.num_rows = M.num_cols = n;
