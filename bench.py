@@ -726,7 +726,7 @@ def warm_solvers(cls, torch, ctl, net, fs, n1d=20):
     A.meta_delete()
 
 
-def fsai_workload(cls, torch, ctl, n1d=160, tol=1e-8, weights=None, cpu=True):
+def fsai_workload(cls, torch, ctl, n1d=160, tol=1e-8, weights=None, cpu=True, sweep=True):
     """Learned-FSAI mixed PCG vs Jacobi (mixed, FP64) on the held-out 3-D
     problem: from diagonal storage (A and the factor) and with A, G and G^T on
     CSR-Adaptive; G's values in BF16 and in FP32; the solution as an FP32
@@ -807,7 +807,8 @@ def fsai_workload(cls, torch, ctl, n1d=160, tol=1e-8, weights=None, cpu=True):
         log(f"CPU baseline: {out['cpu_baseline']}")
     f.release()
     A.meta_delete()
-    out["held_out_sweep"] = fsai_sweep(cls, torch, ctl, net, tol)
+    if sweep:
+        out["held_out_sweep"] = fsai_sweep(cls, torch, ctl, net, tol)
     net.release()
     return out
 
@@ -946,6 +947,7 @@ def main():
     ap.add_argument("--mode", default="spmv",
                     choices=["spmv", "copy", "gather", "locality", "cg", "fsai", "fsai-train", "next"])
     ap.add_argument("--no-fsai", action="store_true", help="skip the learned-FSAI workload")
+    ap.add_argument("--no-sweep", action="store_true", help="--mode fsai: skip the held-out 128^3 sweep")
     ap.add_argument("--weights", default=None, help="learned-FSAI weights (.npy)")
     ap.add_argument("--train-matrices", type=int, default=1800)
     ap.add_argument("--eval-matrices", type=int, default=200)
@@ -1003,8 +1005,9 @@ def main():
         ctl.release()
         return
     if args.mode == "fsai":
-        print(json.dumps({"mode": "fsai", "fsai": fsai_workload(cls, torch, ctl,
-                                                                 weights=args.weights)}))
+        print(json.dumps({"mode": "fsai", "fsai": fsai_workload(cls, torch, ctl, weights=args.weights,
+                                                                 cpu=not args.no_cpu,
+                                                                 sweep=not args.no_sweep)}))
         ctl.release()
         return
     if args.mode == "fsai-train":
