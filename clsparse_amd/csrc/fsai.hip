@@ -12,10 +12,9 @@
 // t = G z, u = A t, v = G^T u - z, w = A G v.
 //
 // MLP kernels compute every layer transposed, Z^T = W H^T (hidden x nnz):
-// the 16x16x16 BF16 MFMA writes its result in exactly the lane layout it
-// reads its B operand in, so each layer's activations feed the next MFMA
-// straight from registers. Only the weight-gradient products (summed over
-// nonzeros) transpose a 64x16 tile through LDS.
+// the 16x16 BF16 MFMAs write their result in exactly the lane layout they
+// read their B operand in, so each layer's activations feed the next MFMA
+// straight from registers.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -122,14 +121,48 @@ struct MlpDev
     const float* p;  // FP32 master parameters (biases, w3, b3 are read from here)
 };
 
-// Per-lane register copy of the network, loaded once per wave: the MFMA A
-// fragments of W1, W2 (and W2^T for the backward pass) and the lane's slice
-// of b1, b2, w3 (units 16m + 4(l/16) + e).
+// The 16x16x32 BF16 MFMA: lane group q = l/16 holds 8 K values of row (A) or
+// column (B) l%16. Which 8 does not matter as long as A and B agree, so a K=32
+// operand is two K=16 fragments side by side (units of blocks 2j and 2j+1).
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ f32x4 mfma32(s16x8 a, s16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b),
+                                                   c, 0, 0, 0);
+}
+
+__device__ __forceinline__ s16x8 cat8(bf16x4 lo, bf16x4 hi)
+{
+    return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
+// Piece `which` (0, 1, 2) of the exact split v = hi + mid + lo into BF16 values:
+// each piece takes the next 8 significant bits (truncation), so both
+// remainders are exact in FP32 and the last one fits a BF16 exactly.
+__device__ __forceinline__ short bf16_piece(float v, uint32_t which)
+{
+    const float hi = __uint_as_float(__float_as_uint(v) & 0xFFFF0000u);
+    const float r = v - hi;
+    const float mid = __uint_as_float(__float_as_uint(r) & 0xFFFF0000u);
+    const float lo = r - mid;
+    const float piece = which == 0 ? hi : (which == 1 ? mid : lo);
+    return static_cast<short>(__float_as_uint(piece) >> 16);
+}
+
+// Per-lane register copy of the network for the forward kernel, loaded once
+// per wave. Lane l = 16q + c: the MFMA A fragments of W1 (rows 16m + c,
+// features 4q..4q+3) and W2 (rows 16m + c, K pair j), the output weights as
+// an A operand whose rows 0, 1, 2 hold the three BF16 pieces of w3 (so the MFMA
+// forms w3 . h2 with w3 exact), and b1, b2 in the accumulator layout (units
+// 16m + 4q + e), the MFMA chains' initial values.
 struct NetRegs
 {
     bf16x4 w1[4];
-    bf16x4 w2[4][4];  // [m][kb]
-    float b1[16], b2[16], w3[16];
+    s16x8 w2[4][2];  // [m][j]
+    s16x8 w3[2];     // [j]
+    f32x4 b1[4], b2[4];
     float b3;
 };
 
@@ -138,81 +171,54 @@ __device__ __forceinline__ void load_net(const MlpDev& net, uint32_t lane, NetRe
     const uint32_t q = lane >> 4, c16 = lane & 15;
     for (int m = 0; m < 4; ++m) {
         w.w1[m] = load_bf16x4(net.W1 + (16 * m + c16) * kFeat + 4 * q);
-        for (int kb = 0; kb < 4; ++kb)
-            w.w2[m][kb] = load_bf16x4(net.W2 + (16 * m + c16) * kHid + 16 * kb + 4 * q);
+        for (int j = 0; j < 2; ++j)
+            w.w2[m][j] = cat8(load_bf16x4(net.W2 + (16 * m + c16) * kHid + 16 * (2 * j) + 4 * q),
+                              load_bf16x4(net.W2 + (16 * m + c16) * kHid + 16 * (2 * j + 1) + 4 * q));
         for (int e = 0; e < 4; ++e) {
-            const int u = 16 * m + 4 * q + e;
-            w.b1[4 * m + e] = net.p[kOffB1 + u];
-            w.b2[4 * m + e] = net.p[kOffB2 + u];
-            w.w3[4 * m + e] = net.p[kOffW3 + u];
+            w.b1[m][e] = net.p[kOffB1 + 16 * m + 4 * q + e];
+            w.b2[m][e] = net.p[kOffB2 + 16 * m + 4 * q + e];
         }
     }
+    for (int j = 0; j < 2; ++j)
+        for (int t = 0; t < 8; ++t) {
+            const int u = 16 * (2 * j + t / 4) + 4 * q + t % 4;
+            w.w3[j][t] = c16 < 3 ? bf16_piece(net.p[kOffW3 + u], c16) : short(0);
+        }
     w.b3 = net.p[kOffB3];
 }
 
-// Forward for 16 nonzeros [r0, r0+16) held by one wave. Lane l works on
-// nonzero l % 16 and hidden units 16m + 4(l/16) + e of block m (e = 0..3).
-// Keeps pre-activations a1, a2 and the BF16 activations h1 (for backward).
-struct TileFwd
-{
-    f32x4 a1[4];
-    f32x4 a2[4];
-    bf16x4 h1[4];
-    bf16x4 x;  // B operand X^T (features 4(l/16)..+3 of nonzero l%16)
-};
-
-// BIAS_INIT: the biases are the MFMA accumulator's initial value (fewer VALU
-// instructions); otherwise they are added to the products' sum (fewer live
-// registers across the MFMA chains, which the backward kernel needs).
-template <bool BIAS_INIT>
-__device__ __forceinline__ void tile_forward(const NetRegs& w, const uint16_t* __restrict__ X,
-                                             uint32_t r0, uint32_t nnz, uint32_t lane, TileFwd& t)
+// Features of nonzeros r0 .. r0+15 as the layer-1 B operand: lane l holds
+// features 4(l/16)..+3 of nonzero r0 + l%16 (zeros past nnz).
+__device__ __forceinline__ bf16x4 load_tile_x(const uint16_t* __restrict__ X, uint32_t r0, uint32_t nnz, uint32_t lane)
 {
     const uint32_t nz = r0 + (lane & 15);
-    const uint32_t q = lane >> 4;
-    if (nz < nnz)
-        t.x = load_bf16x4(X + static_cast<size_t>(nz) * kFeat + 4 * q);
-    else
-        t.x = bf16x4{0, 0, 0, 0};
-    // Layer 1: A1^T (64 x 16nnz) = W1 (64 x 16) . X^T (16 x 16nnz) + b1
-    for (int m = 0; m < 4; ++m) {
-        f32x4 c = {0.f, 0.f, 0.f, 0.f};
-        if constexpr (BIAS_INIT)
-            for (int e = 0; e < 4; ++e) c[e] = w.b1[4 * m + e];
-        c = mfma16(w.w1[m], t.x, c);
-        if constexpr (!BIAS_INIT)
-            for (int e = 0; e < 4; ++e) c[e] += w.b1[4 * m + e];
-        t.a1[m] = c;
-        t.h1[m] = pack_relu_bf16x4(c);
-    }
-    // Layer 2: A2^T (64 x 16) = W2 (64 x 64) . H1^T (64 x 16) + b2
-    for (int m = 0; m < 4; ++m) {
-        f32x4 c = {0.f, 0.f, 0.f, 0.f};
-        if constexpr (BIAS_INIT)
-            for (int e = 0; e < 4; ++e) c[e] = w.b2[4 * m + e];
-        for (int kb = 0; kb < 4; ++kb) c = mfma16(w.w2[m][kb], t.h1[kb], c);
-        if constexpr (!BIAS_INIT)
-            for (int e = 0; e < 4; ++e) c[e] += w.b2[4 * m + e];
-        t.a2[m] = c;
-    }
+    return nz < nnz ? load_bf16x4(X + static_cast<size_t>(nz) * kFeat + 4 * (lane >> 4)) : bf16x4{0, 0, 0, 0};
 }
 
-// o = w3 . round_bf16(relu(a2)) + b3 for the lane's nonzero (all 4 lanes of a
-// nonzero agree).
-__device__ __forceinline__ float tile_output(const NetRegs& w, const TileFwd& t)
+// One tile of 16 nonzeros, every layer transposed (hidden x nnz): lane l works
+// on nonzero l % 16. Returns o = w3 . round_bf16(relu(a2)) + b3 in lanes 0..15.
+__device__ __forceinline__ float tile_forward(const NetRegs& w, bf16x4 x)
 {
-    float s = 0.f;
-    for (int m = 0; m < 4; ++m)
-        for (int e = 0; e < 4; e += 2) {
-            const uint32_t pk = relu_bf16x2(f2bf2(t.a2[m][e], t.a2[m][e + 1]));
-            s += w.w3[4 * m + e] * __uint_as_float(pk << 16);
-            s += w.w3[4 * m + e + 1] * __uint_as_float(pk & 0xFFFF0000u);
-        }
-    s += __shfl_xor(s, 16, 64);
-    s += __shfl_xor(s, 32, 64);
-    return s + w.b3;
+    // Layer 1: A1^T (64 x 16nnz) = W1 (64 x 16) . X^T (16 x 16nnz) + b1
+    bf16x4 h1[4];
+    for (int m = 0; m < 4; ++m) h1[m] = pack_relu_bf16x4(mfma16(w.w1[m], x, w.b1[m]));
+    const s16x8 h1p[2] = {cat8(h1[0], h1[1]), cat8(h1[2], h1[3])};
+    // Layer 2: A2^T (64 x 16) = W2 (64 x 64) . H1^T (64 x 16) + b2
+    bf16x4 h2[4];
+    for (int m = 0; m < 4; ++m) {
+        f32x4 c = w.b2[m];
+        for (int j = 0; j < 2; ++j) c = mfma32(w.w2[m][j], h1p[j], c);
+        h2[m] = pack_relu_bf16x4(c);
+    }
+    // Output: rows 0, 1, 2 of W3pieces . H2^T; b3 rides in row 0's initial value.
+    f32x4 s = {w.b3, 0.f, 0.f, 0.f};
+    s = mfma32(w.w3[0], cat8(h2[0], h2[1]), s);
+    s = mfma32(w.w3[1], cat8(h2[2], h2[3]), s);
+    return (s[0] + s[1]) + s[2];
 }
 
+// Persistent waves (the grid is what the GPU holds at once): each wave loads
+// the network once and walks tiles with the next tile's features in flight.
 __global__ __launch_bounds__(64 * kMlpWaves) void mlp_forward_kernel(MlpDev net,
                                                                       const uint16_t* __restrict__ X,
                                                                       uint32_t nnz,
@@ -221,13 +227,17 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_forward_kernel(MlpDev net,
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t tiles = (nnz + 15) / 16;
+    const uint32_t stride = gridDim.x * kMlpWaves;
+    uint32_t tile = blockIdx.x * kMlpWaves + wave;
+    if (tile >= tiles) return;
     NetRegs w;
     load_net(net, lane, w);
-    for (uint32_t tile = blockIdx.x * kMlpWaves + wave; tile < tiles; tile += gridDim.x * kMlpWaves) {
-        TileFwd t;
-        tile_forward<true>(w, X, tile * 16, nnz, lane, t);
-        const float o = tile_output(w, t);
-        const uint32_t nz = tile * 16 + (lane & 15);
+    bf16x4 xnext = load_tile_x(X, tile * 16, nnz, lane);
+    for (; tile < tiles; tile += stride) {
+        const bf16x4 x = xnext;
+        if (tile + stride < tiles) xnext = load_tile_x(X, (tile + stride) * 16, nnz, lane);
+        const float o = tile_forward(w, x);
+        const uint32_t nz = tile * 16 + lane;
         if (lane < 16 && nz < nnz) out[nz] = o;
     }
 }
@@ -244,36 +254,49 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_forward_kernel(MlpDev net,
 //   dW1 += D1^T X            A = D1 in N, B = X in N (loaded so)
 // In N each lane holds one unit per block for 4 nonzeros, so the bias-like
 // sums (db1, db2, dw3) take 4 accumulators per lane, reduced over the 4 lane
-// groups at the end. The 4 waves of a workgroup add their gradients into one
-// LDS vector in wave order, and each workgroup writes one partial gradient
-// vector (reduced later in a fixed order).
-__global__ __launch_bounds__(64 * kMlpWaves) void mlp_backward_kernel(MlpDev net,
+// groups at the end. The products over the 64 hidden units run on the K=32
+// MFMA; the biases are the MFMA chains' initial values (T layout from the
+// bias vectors, N layout from copies with each value 4 times). The W2 and W2^T
+// fragments are read from LDS for every tile, so a wave holds the 80 gradient
+// accumulators, the next tile's inputs and the tile's activations in 225
+// registers: two waves per SIMD. The 4 waves of a workgroup add their
+// gradients into one LDS vector in wave order, and each workgroup writes one
+// partial gradient vector (reduced later in a fixed order).
+__global__ __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_per_eu(2, 2))) void mlp_backward_kernel(MlpDev net,
                                                                        const uint16_t* __restrict__ X,
                                                                        const float* __restrict__ gout,
                                                                        uint32_t nnz,
                                                                        float* __restrict__ partials)
 {
-    __shared__ float sh_vec[3][kHid];  // b1, b2, w3 for the T-layout reads
-    __shared__ float sh_g[kNumParams];  // the workgroup's gradient
+    __shared__ float sh_vec[3][kHid];     // b1, b2, w3 for the T-layout reads
+    __shared__ f32x4 sh_splat[2][kHid];   // b1, b2: each value 4 times (N-layout initial values)
+    __shared__ s16x8 sh_w2[4][2][64];     // W2 fragments [block][K pair][lane]
+    __shared__ s16x8 sh_w2t[4][2][64];    // W2^T fragments
+    __shared__ float sh_g[kNumParams];    // the workgroup's gradient
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t q = lane >> 4, c16 = lane & 15;
 
     for (uint32_t i = threadIdx.x; i < static_cast<uint32_t>(kHid); i += 64 * kMlpWaves) {
-        sh_vec[0][i] = net.p[kOffB1 + i];
-        sh_vec[1][i] = net.p[kOffB2 + i];
+        const float b1 = net.p[kOffB1 + i], b2 = net.p[kOffB2 + i];
+        sh_vec[0][i] = b1;
+        sh_vec[1][i] = b2;
         sh_vec[2][i] = net.p[kOffW3 + i];
+        sh_splat[0][i] = f32x4{b1, b1, b1, b1};
+        sh_splat[1][i] = f32x4{b2, b2, b2, b2};
     }
-    bf16x4 w1[4], w2[4][4], w2t[4][4];
-    float b1n[4], b2n[4], w3n[4];  // unit 16n + c16
+    // The W2 / W2^T fragments of K pair j (blocks 2j, 2j+1) in lane order, so
+    // each read is one conflict-free 16-byte load per lane.
+    for (uint32_t i = threadIdx.x; i < 4 * 2 * 64; i += 64 * kMlpWaves) {
+        const uint32_t m = i / 128, j = (i / 64) % 2, l = i % 64;
+        const uint32_t row = 16 * m + (l & 15), col = 32 * j + 4 * (l >> 4);
+        sh_w2[m][j][l] = cat8(load_bf16x4(net.W2 + row * kHid + col), load_bf16x4(net.W2 + row * kHid + col + 16));
+        sh_w2t[m][j][l] = cat8(load_bf16x4(net.W2T + row * kHid + col), load_bf16x4(net.W2T + row * kHid + col + 16));
+    }
+    bf16x4 w1[4];
+    float w3n[4];  // unit 16n + c16
     for (int m = 0; m < 4; ++m) {
         w1[m] = load_bf16x4(net.W1 + (16 * m + c16) * kFeat + 4 * q);
-        for (int kb = 0; kb < 4; ++kb) {
-            w2[m][kb] = load_bf16x4(net.W2 + (16 * m + c16) * kHid + 16 * kb + 4 * q);
-            w2t[m][kb] = load_bf16x4(net.W2T + (16 * m + c16) * kHid + 16 * kb + 4 * q);
-        }
-        b1n[m] = net.p[kOffB1 + 16 * m + c16];
-        b2n[m] = net.p[kOffB2 + 16 * m + c16];
         w3n[m] = net.p[kOffW3 + 16 * m + c16];
     }
     __syncthreads();
@@ -288,61 +311,74 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_backward_kernel(MlpDev net
         db1[m] = db2[m] = dw3[m] = 0.f;
     }
 
-    const uint32_t tiles = (nnz + 15) / 16;
-    for (uint32_t tile = blockIdx.x * kMlpWaves + wave; tile < tiles; tile += gridDim.x * kMlpWaves) {
+    // A tile's inputs, loaded one tile ahead: X as the T layout's B operand
+    // (= the N layout's A operand: features 4q..4q+3 of nonzero c16) and as
+    // the dW1 B operand (feature c16 of nonzeros 4q + e), gout in both layouts.
+    struct TileIn
+    {
+        bf16x4 x, xn;
+        float gN[4];
+        float gT;
+    };
+    auto load_in = [&](uint32_t tile, TileIn& t) {
         const uint32_t r0 = tile * 16;
         const uint32_t nzT = r0 + c16;
-        // X: the T layout's B operand (= the N layout's A operand) holds
-        // features 4q..4q+3 of nonzero c16; the N layout's dW1 B operand
-        // holds feature c16 of nonzeros 4q + e.
-        const bf16x4 x = nzT < nnz ? load_bf16x4(X + static_cast<size_t>(nzT) * kFeat + 4 * q) : bf16x4{0, 0, 0, 0};
-        bf16x4 xn;
-        float gN[4];
+        t.x = nzT < nnz ? load_bf16x4(X + static_cast<size_t>(nzT) * kFeat + 4 * q) : bf16x4{0, 0, 0, 0};
         for (int e = 0; e < 4; ++e) {
             const uint32_t nz = r0 + 4 * q + e;
-            xn[e] = nz < nnz ? static_cast<short>(X[static_cast<size_t>(nz) * kFeat + c16]) : short(0);
-            gN[e] = nz < nnz ? gout[nz] : 0.f;
+            t.xn[e] = nz < nnz ? static_cast<short>(X[static_cast<size_t>(nz) * kFeat + c16]) : short(0);
+            t.gN[e] = nz < nnz ? gout[nz] : 0.f;
         }
-        const float gT = nzT < nnz ? gout[nzT] : 0.f;
+        t.gT = nzT < nnz ? gout[nzT] : 0.f;
+    };
+    const uint32_t tiles = (nnz + 15) / 16;
+    const uint32_t stride = gridDim.x * kMlpWaves;
+    uint32_t tile = blockIdx.x * kMlpWaves + wave;
+    TileIn next;
+    if (tile < tiles) load_in(tile, next);
+    for (; tile < tiles; tile += stride) {
+        // Keeps the W2 fragment reads inside the loop (hoisted, they would
+        // hold 96 registers across it).
+        asm volatile("" ::: "memory");
+        const TileIn in = next;
+        if (tile + stride < tiles) load_in(tile + stride, next);
+        const bf16x4 x = in.x, xn = in.xn;
+        const float* gN = in.gN;
+        const float gT = in.gT;
         if (q == 0) db3 += gT;
 
         // Forward, T layout: H1^T, A2^T.
         bf16x4 h1T[4];
         for (int m = 0; m < 4; ++m) {
             const f32x4 b = *reinterpret_cast<const f32x4*>(&sh_vec[0][16 * m + 4 * q]);
-            f32x4 a = mfma16(w1[m], x, f32x4{0.f, 0.f, 0.f, 0.f});
-            for (int e = 0; e < 4; ++e) a[e] += b[e];
-            h1T[m] = pack_relu_bf16x4(a);
+            h1T[m] = pack_relu_bf16x4(mfma16(w1[m], x, b));
         }
+        const s16x8 h1Tp[2] = {cat8(h1T[0], h1T[1]), cat8(h1T[2], h1T[3])};
         // D2 in T layout from A2^T: relu'(a2) g w3 (FP32), then BF16.
         bf16x4 d2T[4];
         for (int m = 0; m < 4; ++m) {
             const f32x4 b = *reinterpret_cast<const f32x4*>(&sh_vec[1][16 * m + 4 * q]);
             const f32x4 w = *reinterpret_cast<const f32x4*>(&sh_vec[2][16 * m + 4 * q]);
-            f32x4 a = {0.f, 0.f, 0.f, 0.f};
-            for (int kb = 0; kb < 4; ++kb) a = mfma16(w2[m][kb], h1T[kb], a);
+            f32x4 a = b;
+            for (int j = 0; j < 2; ++j) a = mfma32(sh_w2[m][j][lane], h1Tp[j], a);
             f32x4 d;
-            for (int e = 0; e < 4; ++e) d[e] = a[e] + b[e] > 0.f ? gT * w[e] : 0.f;
+            for (int e = 0; e < 4; ++e) d[e] = a[e] > 0.f ? gT * w[e] : 0.f;
             d2T[m] = pack_bf16x4(d);
         }
         // Forward, N layout: A1 = X W1^T, A2 = H1 W2^T; D2 and H2 in N.
         bf16x4 h1n[4];
         uint32_t a1pos = 0;  // relu'(A1) in N: bit 4n + e
         for (int n = 0; n < 4; ++n) {
-            f32x4 a = mfma16(x, w1[n], f32x4{0.f, 0.f, 0.f, 0.f});
-            for (int e = 0; e < 4; ++e) {
-                a[e] += b1n[n];
-                a1pos |= (a[e] > 0.f ? 1u : 0u) << (4 * n + e);
-            }
+            const f32x4 a = mfma16(x, w1[n], sh_splat[0][16 * n + c16]);
+            for (int e = 0; e < 4; ++e) a1pos |= (a[e] > 0.f ? 1u : 0u) << (4 * n + e);
             h1n[n] = pack_relu_bf16x4(a);
         }
         bf16x4 d2n[4];
         for (int n = 0; n < 4; ++n) {
-            f32x4 a = {0.f, 0.f, 0.f, 0.f};
-            for (int kb = 0; kb < 4; ++kb) a = mfma16(h1T[kb], w2[n][kb], a);
+            f32x4 a = sh_splat[1][16 * n + c16];
+            for (int j = 0; j < 2; ++j) a = mfma32(h1Tp[j], sh_w2[n][j][lane], a);
             f32x4 d;
             for (int e = 0; e < 4; ++e) {
-                a[e] += b2n[n];
                 d[e] = a[e] > 0.f ? gN[e] * w3n[n] : 0.f;
                 db2[n] += d[e];
             }
@@ -355,10 +391,11 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_backward_kernel(MlpDev net
             dw3[n] += gN[3] * __uint_as_float(hi & 0xFFFF0000u);
         }
         // D1 in N: relu'(A1) (D2 W2), A = D2 in T, B = W2^T fragments.
+        const s16x8 d2Tp[2] = {cat8(d2T[0], d2T[1]), cat8(d2T[2], d2T[3])};
         bf16x4 d1n[4];
         for (int n = 0; n < 4; ++n) {
             f32x4 a = {0.f, 0.f, 0.f, 0.f};
-            for (int kb = 0; kb < 4; ++kb) a = mfma16(d2T[kb], w2t[n][kb], a);
+            for (int j = 0; j < 2; ++j) a = mfma32(d2Tp[j], sh_w2t[n][j][lane], a);
             f32x4 d;
             for (int e = 0; e < 4; ++e) {
                 d[e] = (a1pos >> (4 * n + e)) & 1u ? a[e] : 0.f;
@@ -700,7 +737,7 @@ hipError_t dmalloc(T** p, size_t count)
     return hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T));
 }
 
-clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_t st);
+clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_t st, int num_cus = 0);
 
 // Diagonal scaling, row statistics and features from A's values.
 clsparseStatus compute_features(_clsparseAmdFsai* f, const clsparseCsrMatrix* A, hipStream_t st)
@@ -1210,17 +1247,25 @@ extern "C" clsparseStatus clsparseAmdFsaiDestroy(clsparseAmdFsai f)
 
 namespace {
 
-clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_t st)
+// num_cus: the device's compute units (sizes the forward kernel's grid).
+clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_t st, int num_cus)
 {
     if (f->nnzG == 0) {
         f->valid = false;
         return clsparseSuccess;
     }
     if (net) {
-        // Enough workgroups for several waves per SIMD (the forward kernel
-        // keeps no per-workgroup state).
+        // As many workgroups as the GPU holds at once (queried once), or one
+        // per 4 tiles if fewer.
+        static const int per_cu = [] {
+            int b = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, mlp_forward_kernel, 64 * kMlpWaves, 0) != hipSuccess)
+                b = 0;
+            return std::max(b, 1);
+        }();
         const uint32_t tiles = (f->nnzG + 15) / 16;
-        const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((tiles + kMlpWaves - 1) / kMlpWaves, 4096));
+        const uint32_t resident = static_cast<uint32_t>(per_cu) * static_cast<uint32_t>(std::max(num_cus, 1));
+        const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((tiles + kMlpWaves - 1) / kMlpWaves, resident));
         hipLaunchKernelGGL(mlp_forward_kernel, dim3(grid), dim3(64 * kMlpWaves), 0, st, net->dev(), f->X,
                            f->nnzG, f->o);
     }
@@ -1266,7 +1311,7 @@ extern "C" clsparseStatus clsparseAmdFsaiApplyNet(clsparseAmdFsai f, clsparseAmd
 {
     CLS_TRY(check_init_control(control));
     if (f == nullptr) return clsparseInvalidValue;
-    return fsai_assemble(f, net, control->stream);
+    return fsai_assemble(f, net, control->stream, control->num_cus);
 }
 
 extern "C" clsparseStatus clsparseAmdFsaiGetValues(clsparseAmdFsai f, cl_uint* g_row_pointer,
@@ -1323,7 +1368,7 @@ extern "C" clsparseStatus clsparseAmdFsaiTrainStep(clsparseAmdFsai f, clsparseAm
     CLS_TRY(ensure_train_scratch(f));
     CLS_TRY(fsai_ensure_meta(f, control));
     hipStream_t st = control->stream;
-    CLS_TRY(fsai_assemble(f, net, st));
+    CLS_TRY(fsai_assemble(f, net, st, control->num_cus));
     const uint32_t n = f->n, nnzG = f->nnzG;
     double* one = f->sc;
     double* zero = f->sc + 1;
