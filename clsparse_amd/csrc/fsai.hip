@@ -189,10 +189,27 @@ __device__ __forceinline__ void load_net(const MlpDev& net, uint32_t lane, NetRe
 
 // Features of nonzeros r0 .. r0+15 as the layer-1 B operand: lane l holds
 // features 4(l/16)..+3 of nonzero r0 + l%16 (zeros past nnz).
+// Features of nonzeros r0 .. r0+15 as the layer-1 B operand: lane l holds
+// features 4(l/16)..+3 of nonzero r0 + l%16. A buffer load over the tile's
+// own rows: past nnz the hardware's range check reads zeros, so the load has
+// no branch (and the loop's wait for it can stay a tile behind).
 __device__ __forceinline__ bf16x4 load_tile_x(const uint16_t* __restrict__ X, uint32_t r0, uint32_t nnz, uint32_t lane)
 {
-    const uint32_t nz = r0 + (lane & 15);
-    return nz < nnz ? load_bf16x4(X + static_cast<size_t>(nz) * kFeat + 4 * (lane >> 4)) : bf16x4{0, 0, 0, 0};
+    const uint32_t rows = std::min(nnz - r0, 16u);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint16_t*>(X + static_cast<size_t>(r0) * kFeat), 0, rows * kFeat * 2, 0x00020000);
+    const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(rs, (lane & 15) * kFeat * 2 + (lane >> 4) * 8, 0, 0);
+    return __builtin_bit_cast(bf16x4, v);
+}
+
+// out[r0 + l] = o for lanes l < 16 and r0 + l < nnz (a buffer store: the
+// other lanes' offsets are past the tile's range and the hardware drops them).
+__device__ __forceinline__ void store_tile_out(float* __restrict__ out, uint32_t r0, uint32_t nnz, uint32_t lane,
+                                               float o)
+{
+    const uint32_t rows = std::min(nnz - r0, 16u);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(out + r0, 0, rows * 4, 0x00020000);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(o), rs, lane * 4, 0, 0);
 }
 
 // One tile of 16 nonzeros, every layer transposed (hidden x nnz): lane l works
@@ -225,65 +242,75 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_forward_kernel(MlpDev net,
                                                                       float* __restrict__ out)
 {
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar loop control
     const uint32_t tiles = (nnz + 15) / 16;
     const uint32_t stride = gridDim.x * kMlpWaves;
     uint32_t tile = blockIdx.x * kMlpWaves + wave;
     if (tile >= tiles) return;
     NetRegs w;
     load_net(net, lane, w);
-    bf16x4 xnext = load_tile_x(X, tile * 16, nnz, lane);
-    for (; tile < tiles; tile += stride) {
-        const bf16x4 x = xnext;
-        if (tile + stride < tiles) xnext = load_tile_x(X, (tile + stride) * 16, nnz, lane);
-        const float o = tile_forward(w, x);
-        const uint32_t nz = tile * 16 + lane;
-        if (lane < 16 && nz < nnz) out[nz] = o;
+    // Two feature buffers in turn, each refilled (from a clamped tile index,
+    // so the load is always issued) right after its tile has used it: the
+    // loads run two tiles ahead.
+    bf16x4 xa = load_tile_x(X, tile * 16, nnz, lane);
+    bf16x4 xb = load_tile_x(X, std::min(tile + stride, tiles - 1) * 16, nnz, lane);
+    while (true) {
+        float o = tile_forward(w, xa);
+        xa = load_tile_x(X, std::min(tile + 2 * stride, tiles - 1) * 16, nnz, lane);
+        store_tile_out(out, tile * 16, nnz, lane, o);
+        tile += stride;
+        if (tile >= tiles) break;
+        o = tile_forward(w, xb);
+        xb = load_tile_x(X, std::min(tile + 2 * stride, tiles - 1) * 16, nnz, lane);
+        store_tile_out(out, tile * 16, nnz, lane, o);
+        tile += stride;
+        if (tile >= tiles) break;
     }
 }
 
-// Backward. Per tile of 16 nonzeros the forward pass is recomputed in two
-// layouts from the same register fragments: the forward kernel's unit x nnz
-// layout ("T": lane (q, c) holds units 16m + 4q + e of nonzero c) and the
-// nnz x unit layout ("N": lane (q, c) holds nonzeros 4q + e of unit 16n + c;
-// A1 = X W1^T with X as the A operand and W1's fragments as the B operand,
-// A2 = H1 W2^T likewise). Every product then reads its operands in the layout
-// the MFMA wants straight from registers, with no pass through LDS:
-//   D1 = relu'(A1) (D2 W2)   A = D2 in T, B = W2^T fragments   -> N
-//   dW2 += D2^T H1           A = D2 in N, B = H1 in N
-//   dW1 += D1^T X            A = D1 in N, B = X in N (loaded so)
-// In N each lane holds one unit per block for 4 nonzeros, so the bias-like
-// sums (db1, db2, dw3) take 4 accumulators per lane, reduced over the 4 lane
-// groups at the end. The products over the 64 hidden units run on the K=32
-// MFMA; the biases are the MFMA chains' initial values (T layout from the
-// bias vectors, N layout from copies with each value 4 times). The W2 and W2^T
-// fragments are read from LDS for every tile, so a wave holds the 80 gradient
-// accumulators, the next tile's inputs and the tile's activations in 225
-// registers: two waves per SIMD. The 4 waves of a workgroup add their
-// gradients into one LDS vector in wave order, and each workgroup writes one
-// partial gradient vector (reduced later in a fixed order).
+// Backward. Two lane layouts of a 16-nonzero tile: the forward kernel's
+// unit x nnz layout ("T": lane (q, c) holds units 16m + 4q + e of nonzero c)
+// and the nnz x unit layout ("N": lane (q, c) holds nonzeros 4q + e of unit
+// 16n + c). A T-layout fragment read as an MFMA A operand and multiplied by the
+// 16x16 identity comes out in N layout, and an N-layout one in T layout, each
+// value exactly (one product by 1 plus zeros). So per tile:
+//   H1^T = relu(W1 X^T + b1)           T    (forward kernel's layer 1)
+//   H1   = transpose(H1^T)             N
+//   A2   = H1 W2^T + b2                N    A = H1^T in T, B = W2 fragments
+//   D2   = relu'(A2) g w3 (FP32)       N    -> db2; BF16 -> dW2 operand
+//   D2^T = transpose(D2)               T
+//   D1   = relu'(A1) (D2 W2)           N    A = D2 in T, B = W2^T fragments
+//   dW2 += D2^T H1                     A = D2 in N, B = H1 in N
+//   dW1 += D1^T X                      A = D1 in N, B = X in N (loaded so)
+// relu'(A1) is read off H1 (a nonzero BF16 activation). In N each lane holds
+// one unit per block for 4 nonzeros, so the bias-like sums (db1, db2, dw3) take
+// 4 accumulators per lane, reduced over the 4 lane groups at the end. The
+// products over the 64 hidden units run on the K=32 MFMA; the biases are the
+// MFMA chains' initial values. The W2 and W2^T fragments are read from LDS for
+// every tile, so a wave holds the 80 gradient accumulators, the next tile's
+// inputs and the tile's activations in registers for two waves per SIMD. The
+// 4 waves of a workgroup add their gradients into one LDS vector in wave
+// order, and each workgroup writes one partial gradient vector (reduced later
+// in a fixed order).
 __global__ __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_per_eu(2, 2))) void mlp_backward_kernel(MlpDev net,
                                                                        const uint16_t* __restrict__ X,
                                                                        const float* __restrict__ gout,
                                                                        uint32_t nnz,
                                                                        float* __restrict__ partials)
 {
-    __shared__ float sh_vec[3][kHid];     // b1, b2, w3 for the T-layout reads
-    __shared__ f32x4 sh_splat[2][kHid];   // b1, b2: each value 4 times (N-layout initial values)
+    __shared__ float sh_b1[kHid];         // b1 for the T-layout reads
+    __shared__ f32x4 sh_b2[kHid];         // b2, each value 4 times (N-layout initial values)
     __shared__ s16x8 sh_w2[4][2][64];     // W2 fragments [block][K pair][lane]
     __shared__ s16x8 sh_w2t[4][2][64];    // W2^T fragments
     __shared__ float sh_g[kNumParams];    // the workgroup's gradient
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar loop control
     const uint32_t q = lane >> 4, c16 = lane & 15;
 
     for (uint32_t i = threadIdx.x; i < static_cast<uint32_t>(kHid); i += 64 * kMlpWaves) {
-        const float b1 = net.p[kOffB1 + i], b2 = net.p[kOffB2 + i];
-        sh_vec[0][i] = b1;
-        sh_vec[1][i] = b2;
-        sh_vec[2][i] = net.p[kOffW3 + i];
-        sh_splat[0][i] = f32x4{b1, b1, b1, b1};
-        sh_splat[1][i] = f32x4{b2, b2, b2, b2};
+        sh_b1[i] = net.p[kOffB1 + i];
+        const float b2 = net.p[kOffB2 + i];
+        sh_b2[i] = f32x4{b2, b2, b2, b2};
     }
     // The W2 / W2^T fragments of K pair j (blocks 2j, 2j+1) in lane order, so
     // each read is one conflict-free 16-byte load per lane.
@@ -299,6 +326,10 @@ __global__ __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_per_eu(
         w1[m] = load_bf16x4(net.W1 + (16 * m + c16) * kFeat + 4 * q);
         w3n[m] = net.p[kOffW3 + 16 * m + c16];
     }
+    // The identity as a B operand: lane (q, c) holds rows 4q..4q+3 of column c.
+    bf16x4 ident;
+    for (int e = 0; e < 4; ++e) ident[e] = (4 * q + e == c16) ? short(0x3F80) : short(0);
+    const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
     __syncthreads();
 
     f32x4 dW2[4][4];  // block (m = output unit, n = input unit)
@@ -306,76 +337,56 @@ __global__ __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_per_eu(
     float db1[4], db2[4], dw3[4];  // per lane: unit 16n + c16
     float db3 = 0.f;
     for (int m = 0; m < 4; ++m) {
-        dW1[m] = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int n = 0; n < 4; ++n) dW2[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+        dW1[m] = zero4;
+        for (int n = 0; n < 4; ++n) dW2[m][n] = zero4;
         db1[m] = db2[m] = dw3[m] = 0.f;
     }
 
     // A tile's inputs, loaded one tile ahead: X as the T layout's B operand
-    // (= the N layout's A operand: features 4q..4q+3 of nonzero c16) and as
-    // the dW1 B operand (feature c16 of nonzeros 4q + e), gout in both layouts.
+    // (features 4q..4q+3 of nonzero c16) and as the dW1 B operand (feature c16
+    // of nonzeros 4q + e), gout in both layouts.
     struct TileIn
     {
         bf16x4 x, xn;
         float gN[4];
         float gT;
     };
+    // Buffer loads over the tile's own rows: past nnz the hardware's range
+    // check reads zeros (zero features and zero gout add nothing), so the
+    // loads have no branch and the loop's waits for them stay a tile behind.
     auto load_in = [&](uint32_t tile, TileIn& t) {
         const uint32_t r0 = tile * 16;
-        const uint32_t nzT = r0 + c16;
-        t.x = nzT < nnz ? load_bf16x4(X + static_cast<size_t>(nzT) * kFeat + 4 * q) : bf16x4{0, 0, 0, 0};
+        const uint32_t rows = std::min(nnz - r0, 16u);
+        const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<uint16_t*>(X + static_cast<size_t>(r0) * kFeat), 0, rows * kFeat * 2, 0x00020000);
+        const __amdgpu_buffer_rsrc_t rg =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(gout + r0), 0, rows * 4, 0x00020000);
+        t.x = __builtin_bit_cast(bf16x4, __builtin_amdgcn_raw_buffer_load_b64(rx, c16 * kFeat * 2 + q * 8, 0, 0));
         for (int e = 0; e < 4; ++e) {
-            const uint32_t nz = r0 + 4 * q + e;
-            t.xn[e] = nz < nnz ? static_cast<short>(X[static_cast<size_t>(nz) * kFeat + c16]) : short(0);
-            t.gN[e] = nz < nnz ? gout[nz] : 0.f;
+            t.xn[e] = static_cast<short>(__builtin_amdgcn_raw_buffer_load_b16(rx, (4 * q + e) * kFeat * 2 + c16 * 2, 0, 0));
+            t.gN[e] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rg, (4 * q + e) * 4, 0, 0));
         }
-        t.gT = nzT < nnz ? gout[nzT] : 0.f;
+        t.gT = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rg, c16 * 4, 0, 0));
     };
     const uint32_t tiles = (nnz + 15) / 16;
     const uint32_t stride = gridDim.x * kMlpWaves;
-    uint32_t tile = blockIdx.x * kMlpWaves + wave;
-    TileIn next;
-    if (tile < tiles) load_in(tile, next);
-    for (; tile < tiles; tile += stride) {
-        // Keeps the W2 fragment reads inside the loop (hoisted, they would
-        // hold 96 registers across it).
+    auto process = [&](const TileIn& in) {
+        // Keeps the W2 fragment reads with the tile (hoisted out of the loop,
+        // they would hold 64 registers across it).
         asm volatile("" ::: "memory");
-        const TileIn in = next;
-        if (tile + stride < tiles) load_in(tile + stride, next);
-        const bf16x4 x = in.x, xn = in.xn;
         const float* gN = in.gN;
-        const float gT = in.gT;
-        if (q == 0) db3 += gT;
+        if (q == 0) db3 += in.gT;
 
-        // Forward, T layout: H1^T, A2^T.
-        bf16x4 h1T[4];
-        for (int m = 0; m < 4; ++m) {
-            const f32x4 b = *reinterpret_cast<const f32x4*>(&sh_vec[0][16 * m + 4 * q]);
-            h1T[m] = pack_relu_bf16x4(mfma16(w1[m], x, b));
-        }
+        // H1^T (T), then H1 (N) by the identity product.
+        bf16x4 h1T[4], h1n[4];
+        for (int m = 0; m < 4; ++m)
+            h1T[m] = pack_relu_bf16x4(mfma16(w1[m], in.x, *reinterpret_cast<const f32x4*>(&sh_b1[16 * m + 4 * q])));
+        for (int n = 0; n < 4; ++n) h1n[n] = pack_bf16x4(mfma16(h1T[n], ident, zero4));
         const s16x8 h1Tp[2] = {cat8(h1T[0], h1T[1]), cat8(h1T[2], h1T[3])};
-        // D2 in T layout from A2^T: relu'(a2) g w3 (FP32), then BF16.
-        bf16x4 d2T[4];
-        for (int m = 0; m < 4; ++m) {
-            const f32x4 b = *reinterpret_cast<const f32x4*>(&sh_vec[1][16 * m + 4 * q]);
-            const f32x4 w = *reinterpret_cast<const f32x4*>(&sh_vec[2][16 * m + 4 * q]);
-            f32x4 a = b;
-            for (int j = 0; j < 2; ++j) a = mfma32(sh_w2[m][j][lane], h1Tp[j], a);
-            f32x4 d;
-            for (int e = 0; e < 4; ++e) d[e] = a[e] > 0.f ? gT * w[e] : 0.f;
-            d2T[m] = pack_bf16x4(d);
-        }
-        // Forward, N layout: A1 = X W1^T, A2 = H1 W2^T; D2 and H2 in N.
-        bf16x4 h1n[4];
-        uint32_t a1pos = 0;  // relu'(A1) in N: bit 4n + e
-        for (int n = 0; n < 4; ++n) {
-            const f32x4 a = mfma16(x, w1[n], sh_splat[0][16 * n + c16]);
-            for (int e = 0; e < 4; ++e) a1pos |= (a[e] > 0.f ? 1u : 0u) << (4 * n + e);
-            h1n[n] = pack_relu_bf16x4(a);
-        }
+        // A2 and D2 in N; db2, dw3.
         bf16x4 d2n[4];
         for (int n = 0; n < 4; ++n) {
-            f32x4 a = sh_splat[1][16 * n + c16];
+            f32x4 a = sh_b2[16 * n + c16];
             for (int j = 0; j < 2; ++j) a = mfma32(h1Tp[j], sh_w2[n][j][lane], a);
             f32x4 d;
             for (int e = 0; e < 4; ++e) {
@@ -390,15 +401,17 @@ __global__ __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_per_eu(
             dw3[n] += gN[2] * __uint_as_float(hi << 16);
             dw3[n] += gN[3] * __uint_as_float(hi & 0xFFFF0000u);
         }
-        // D1 in N: relu'(A1) (D2 W2), A = D2 in T, B = W2^T fragments.
+        // D2 in T by the identity product; D1 = relu'(A1) (D2 W2) in N.
+        bf16x4 d2T[4];
+        for (int m = 0; m < 4; ++m) d2T[m] = pack_bf16x4(mfma16(d2n[m], ident, zero4));
         const s16x8 d2Tp[2] = {cat8(d2T[0], d2T[1]), cat8(d2T[2], d2T[3])};
         bf16x4 d1n[4];
         for (int n = 0; n < 4; ++n) {
-            f32x4 a = {0.f, 0.f, 0.f, 0.f};
+            f32x4 a = zero4;
             for (int j = 0; j < 2; ++j) a = mfma32(d2Tp[j], sh_w2t[n][j][lane], a);
             f32x4 d;
             for (int e = 0; e < 4; ++e) {
-                d[e] = (a1pos >> (4 * n + e)) & 1u ? a[e] : 0.f;
+                d[e] = h1n[n][e] != 0 ? a[e] : 0.f;
                 db1[n] += d[e];
             }
             d1n[n] = pack_bf16x4(d);
@@ -406,7 +419,22 @@ __global__ __launch_bounds__(64 * kMlpWaves) __attribute__((amdgpu_waves_per_eu(
         // Weight gradients, the nonzeros in the K slot.
         for (int m = 0; m < 4; ++m)
             for (int n = 0; n < 4; ++n) dW2[m][n] = mfma16(d2n[m], h1n[n], dW2[m][n]);
-        for (int m = 0; m < 4; ++m) dW1[m] = mfma16(d1n[m], xn, dW1[m]);
+        for (int m = 0; m < 4; ++m) dW1[m] = mfma16(d1n[m], in.xn, dW1[m]);
+    };
+    // Two input buffers in turn: the next tile's loads are in flight while a
+    // tile is processed.
+    // (The index of the next tile is clamped, so a load is always issued.)
+    TileIn in0, in1;
+    uint32_t tile = blockIdx.x * kMlpWaves + wave;
+    if (tile < tiles) load_in(tile, in0);
+    while (tile < tiles) {
+        load_in(std::min(tile + stride, tiles - 1), in1);
+        process(in0);
+        tile += stride;
+        if (tile >= tiles) break;
+        load_in(std::min(tile + stride, tiles - 1), in0);
+        process(in1);
+        tile += stride;
     }
 
     // Bias-like sums: the 4 lane groups hold the same units for different
