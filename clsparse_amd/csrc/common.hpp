@@ -183,6 +183,12 @@ clsparseStatus launch_csrmv(const T* alpha, const clsparseCsrMatrix* A, const T*
                             const T* beta, T* y, hipStream_t stream, bool extended,
                             const int* done_flag);
 
+// Sparse times dense, row-major B (ldb) and C (ldc), ncols columns of C:
+// C = alpha A B + beta C with alpha, beta on the device (csrmm.hip).
+template <typename T>
+clsparseStatus launch_csrmm_rows(const clsparseCsrMatrix* A, uint32_t ncols, const T* B, uint32_t ldb, T* C,
+                                 uint32_t ldc, const T* alpha, const T* beta, int num_cus, hipStream_t stream);
+
 // Learned FSAI factor (fsai.hip): G and G^T in FP32 / FP64 with metadata.
 const clsparseCsrMatrix* fsai_G32(clsparseAmdFsai f);
 const clsparseCsrMatrix* fsai_GT32(clsparseAmdFsai f);

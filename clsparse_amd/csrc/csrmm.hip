@@ -206,28 +206,39 @@ clsparseStatus csrmm_entry(const clsparseScalar* alpha, const clsparseCsrMatrix*
     if (n == 0 || ncols == 0) return finish_op(control);
     const T* pa = dev_ptr<T>(alpha->value, alpha->off_value);
     const T* pb = dev_ptr<T>(beta->value, beta->off_value);
+    const T* Bp = dev_ptr<T>(B->values, B->off_values);
+    T* Cp = dev_ptr<T>(C->values, C->off_values);
+    if (Bp == nullptr || Cp == nullptr || pa == nullptr || pb == nullptr) return clsparseInvalidMemObject;
+    CLS_TRY(launch_csrmm_rows<T>(A, ncols, Bp, B->lead_dim, Cp, C->lead_dim, pa, pb, control->num_cus,
+                                 control->stream));
+    return finish_op(control);
+}
+
+}  // namespace
+
+template <typename T>
+clsparseStatus launch_csrmm_rows(const clsparseCsrMatrix* A, uint32_t ncols, const T* Bp, uint32_t ldb, T* Cp,
+                                 uint32_t ldc, const T* pa, const T* pb, int num_cus, hipStream_t s)
+{
+    const uint32_t n = A->num_rows;
+    if (n == 0 || ncols == 0) return clsparseSuccess;
     const uint32_t* rp = dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer);
     const uint32_t* ci = dev_ptr<uint32_t>(A->col_indices, A->off_col_indices);
     const T* v = dev_ptr<T>(A->values, A->off_values);
-    const T* Bp = dev_ptr<T>(B->values, B->off_values);
-    T* Cp = dev_ptr<T>(C->values, C->off_values);
-    if (rp == nullptr || Bp == nullptr || Cp == nullptr || pa == nullptr || pb == nullptr)
-        return clsparseInvalidMemObject;
+    if (rp == nullptr) return clsparseInvalidMemObject;
     // V columns per lane (one 16-byte access) when the columns, both leading
     // dimensions and both base addresses allow it; W lanes per row slot (a
     // power of two >= the column groups, at most 64).
     constexpr uint32_t kVec = 16 / sizeof(T);
-    const bool vec = ncols % kVec == 0 && B->lead_dim % kVec == 0 && C->lead_dim % kVec == 0 &&
+    const bool vec = ncols % kVec == 0 && ldb % kVec == 0 && ldc % kVec == 0 &&
                      reinterpret_cast<uintptr_t>(Bp) % 16 == 0 && reinterpret_cast<uintptr_t>(Cp) % 16 == 0;
     const uint32_t groups = vec ? ncols / kVec : ncols;
     const uint32_t W = groups > 32 ? 64u : groups > 16 ? 32u : groups > 8 ? 16u : groups > 4 ? 8u
                                      : groups > 2 ? 4u : groups > 1 ? 2u : 1u;
     const uint32_t slots = 64 / W;
     const uint32_t want = (n + 4 * slots - 1) / (4 * slots);
-    hipStream_t s = control->stream;
-#define CLS_CSRMM(WW, VV)                                                                                \
-    CLS_TRY((launch_csrmm<T, WW, VV>(want, control->num_cus, s, n, ncols, rp, ci, v, Bp, B->lead_dim, Cp, \
-                                     C->lead_dim, pa, pb)))
+#define CLS_CSRMM(WW, VV) \
+    CLS_TRY((launch_csrmm<T, WW, VV>(want, num_cus, s, n, ncols, rp, ci, v, Bp, ldb, Cp, ldc, pa, pb)))
 #define CLS_CSRMM_W(VV)                  \
     switch (W) {                         \
     case 64: CLS_CSRMM(64, VV); break;   \
@@ -246,10 +257,13 @@ clsparseStatus csrmm_entry(const clsparseScalar* alpha, const clsparseCsrMatrix*
 #undef CLS_CSRMM_W
 #undef CLS_CSRMM
     CLS_HIP_TRY(hipGetLastError());
-    return finish_op(control);
+    return clsparseSuccess;
 }
 
-}  // namespace
+template clsparseStatus launch_csrmm_rows<float>(const clsparseCsrMatrix*, uint32_t, const float*, uint32_t, float*,
+                                                 uint32_t, const float*, const float*, int, hipStream_t);
+template clsparseStatus launch_csrmm_rows<double>(const clsparseCsrMatrix*, uint32_t, const double*, uint32_t,
+                                                  double*, uint32_t, const double*, const double*, int, hipStream_t);
 
 // src/library/blas3/clsparse-csrmm.cpp L25-49
 extern "C" clsparseStatus clsparseScsrmm(const clsparseScalar* alpha, const clsparseCsrMatrix* sparseMatA,

@@ -641,7 +641,10 @@ __global__ __launch_bounds__(256) void assemble_kernel(uint32_t nnzG, const uint
 }
 
 // Hutchinson gradient for one probe: gG[k] += c (u_i v_j + w_i z_j).
-__global__ __launch_bounds__(256) void hutch_grad_kernel(uint32_t nnzG, const uint32_t* __restrict__ grow,
+// dL/dG_k = sum over the probes p, in order, of c (u_ip v_jp + w_ip z_jp) for
+// the k-th nonzero (i, j) of G; the probe blocks are row-major n x np.
+__global__ __launch_bounds__(256) void hutch_grad_kernel(uint32_t nnzG, uint32_t np,
+                                                         const uint32_t* __restrict__ grow,
                                                          const uint32_t* __restrict__ gcol,
                                                          const double* __restrict__ u,
                                                          const double* __restrict__ v,
@@ -651,8 +654,10 @@ __global__ __launch_bounds__(256) void hutch_grad_kernel(uint32_t nnzG, const ui
 {
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
     if (k >= nnzG) return;
-    const uint32_t i = grow[k], j = gcol[k];
-    gG[k] += c * (u[i] * v[j] + w[i] * z[j]);
+    const size_t i = static_cast<size_t>(grow[k]) * np, j = static_cast<size_t>(gcol[k]) * np;
+    double g = 0.0;
+    for (uint32_t p = 0; p < np; ++p) g += c * (u[i + p] * v[j + p] + w[i + p] * z[j + p]);
+    gG[k] = g;
 }
 
 // dL/do_k = dL/dG_k * dhat_i
@@ -666,22 +671,25 @@ __global__ __launch_bounds__(256) void chain_kernel(uint32_t nnzG, const uint32_
     gout[k] = static_cast<float>(gG[k] * dhat[grow[k]]);
 }
 
-// Rademacher probe from a counter-based hash of (seed, i).
-__global__ __launch_bounds__(256) void rademacher_kernel(uint32_t n, uint64_t seed, double* __restrict__ z)
+// Rademacher probes from a counter-based hash of (seed_p, i): z[i * np + p] with
+// seed_p = seed * 0x100000001B3 + p.
+__global__ __launch_bounds__(256) void rademacher_kernel(uint32_t n, uint32_t np, uint64_t seed,
+                                                         double* __restrict__ z)
 {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    uint64_t x = seed ^ (0x9E3779B97F4A7C15ull * (static_cast<uint64_t>(i) + 1));
+    const size_t e = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= static_cast<size_t>(n) * np) return;
+    const uint64_t i = e / np, p = e % np;
+    uint64_t x = (seed * 0x100000001B3ull + p) ^ (0x9E3779B97F4A7C15ull * (i + 1));
     x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
     x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
     x ^= x >> 31;
-    z[i] = (x & 1) ? 1.0 : -1.0;
+    z[e] = (x & 1) ? 1.0 : -1.0;
 }
 
-__global__ __launch_bounds__(256) void sub_kernel(uint32_t n, const double* __restrict__ a,
+__global__ __launch_bounds__(256) void sub_kernel(size_t n, const double* __restrict__ a,
                                                   const double* __restrict__ b, double* __restrict__ out)
 {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
     if (i < n) out[i] = a[i] - b[i];
 }
 
@@ -751,9 +759,11 @@ struct _clsparseAmdFsai
     // arrays, then the nnz(G)-sized ones), carved in 256-byte pieces.
     void* arena_n = nullptr;
     void* arena_g = nullptr;
-    // Training scratch (FP64 vectors of length n, and the G gradient).
-    double *t = nullptr, *u = nullptr, *s = nullptr, *v = nullptr, *t2 = nullptr, *w = nullptr,
-           *z = nullptr, *gG = nullptr, *sc = nullptr;
+    // Training scratch: the probe blocks Z, T, U, S, V, T2, W (n x probes,
+    // row-major, one allocation for up to `probe_cap` probes), the G gradient.
+    void* probe_buf = nullptr;
+    uint32_t probe_cap = 0;
+    double *gG = nullptr, *sc = nullptr;
     float* gout = nullptr;
 };
 
@@ -791,7 +801,7 @@ void free_fsai(_clsparseAmdFsai* f)
     if (!f) return;
     clsparseCsrMetaDelete(&f->G64);  // G32 / G16 share it
     clsparseCsrMetaDelete(&f->GT64);
-    void* ptrs[] = {f->arena_n, f->arena_g, f->t, f->u, f->s, f->v, f->t2, f->w, f->z, f->gG, f->sc, f->gout};
+    void* ptrs[] = {f->arena_n, f->arena_g, f->probe_buf, f->gG, f->sc, f->gout};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete f;
@@ -1315,21 +1325,23 @@ clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_
     return clsparseSuccess;
 }
 
-clsparseStatus ensure_train_scratch(_clsparseAmdFsai* f)
+clsparseStatus ensure_train_scratch(_clsparseAmdFsai* f, uint32_t probes)
 {
-    if (f->t) return clsparseSuccess;
-    const size_t n = f->n;
-    hipError_t e = dmalloc(&f->t, n);
-    if (e == hipSuccess) e = dmalloc(&f->u, n);
-    if (e == hipSuccess) e = dmalloc(&f->s, n);
-    if (e == hipSuccess) e = dmalloc(&f->v, n);
-    if (e == hipSuccess) e = dmalloc(&f->t2, n);
-    if (e == hipSuccess) e = dmalloc(&f->w, n);
-    if (e == hipSuccess) e = dmalloc(&f->z, n);
-    if (e == hipSuccess) e = dmalloc(&f->gG, f->nnzG);
-    if (e == hipSuccess) e = dmalloc(&f->sc, 8);
-    if (e == hipSuccess) e = dmalloc(&f->gout, f->nnzG);
-    return e == hipSuccess ? clsparseSuccess : clsparseOutOfResources;
+    if (!f->gG) {
+        hipError_t e = dmalloc(&f->gG, f->nnzG);
+        if (e == hipSuccess) e = dmalloc(&f->sc, 8);
+        if (e == hipSuccess) e = dmalloc(&f->gout, f->nnzG);
+        if (e != hipSuccess) return clsparseOutOfResources;
+    }
+    if (probes > f->probe_cap) {
+        if (f->probe_buf) (void)hipFree(f->probe_buf);
+        f->probe_buf = nullptr;
+        f->probe_cap = 0;
+        const size_t block = (static_cast<size_t>(f->n) * probes * sizeof(double) + 255) / 256 * 256;
+        if (hipMalloc(&f->probe_buf, std::max<size_t>(7 * block, 1)) != hipSuccess) return clsparseOutOfResources;
+        f->probe_cap = probes;
+    }
+    return clsparseSuccess;
 }
 
 }  // namespace
@@ -1393,8 +1405,7 @@ extern "C" clsparseStatus clsparseAmdFsaiTrainStep(clsparseAmdFsai f, clsparseAm
     if (f == nullptr || net == nullptr || A == nullptr || probes == 0) return clsparseInvalidValue;
     if (A->num_rows != f->n || A->num_cols != f->n) return clsparseInvalidSystemSize;
     if (f->nnzG == 0) return clsparseInvalidValue;
-    CLS_TRY(ensure_train_scratch(f));
-    CLS_TRY(fsai_ensure_meta(f, control));
+    CLS_TRY(ensure_train_scratch(f, probes));
     hipStream_t st = control->stream;
     CLS_TRY(fsai_assemble(f, net, st, control->num_cus));
     const uint32_t n = f->n, nnzG = f->nnzG;
@@ -1402,27 +1413,28 @@ extern "C" clsparseStatus clsparseAmdFsaiTrainStep(clsparseAmdFsai f, clsparseAm
     double* zero = f->sc + 1;
     hipLaunchKernelGGL((set_scalar<double>), dim3(1), dim3(64), 0, st, one, 1.0);
     hipLaunchKernelGGL((set_scalar<double>), dim3(1), dim3(64), 0, st, zero, 0.0);
-    CLS_HIP_TRY(hipMemsetAsync(f->gG, 0, nnzG * sizeof(double), st));
+    // All probes at once: each product is one sparse x dense (n x probes)
+    // product, reading G, G^T and A once for every probe.
+    const size_t np = probes, elems = static_cast<size_t>(n) * np;
+    const size_t block = (elems * sizeof(double) + 255) / 256 * 256;
+    auto pb = [&](int k) { return reinterpret_cast<double*>(static_cast<char*>(f->probe_buf) + k * block); };
+    double *Z = pb(0), *T = pb(1), *U = pb(2), *S = pb(3), *V = pb(4), *T2 = pb(5), *W = pb(6);
+    const int cus = control->num_cus;
+    hipLaunchKernelGGL(rademacher_kernel, dim3(blocks_for(elems)), dim3(256), 0, st, n, probes, seed, Z);
+    CLS_TRY(launch_csrmm_rows<double>(&f->G64, probes, Z, probes, T, probes, one, zero, cus, st));   // T = G Z
+    CLS_TRY(launch_csrmm_rows<double>(A, probes, T, probes, U, probes, one, zero, cus, st));         // U = A T
+    CLS_TRY(launch_csrmm_rows<double>(&f->GT64, probes, U, probes, S, probes, one, zero, cus, st));  // S = G^T U
+    hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(elems)), dim3(256), 0, st, elems, S, Z, V);       // V = S - Z
+    CLS_TRY(launch_csrmm_rows<double>(&f->G64, probes, V, probes, T2, probes, one, zero, cus, st));  // T2 = G V
+    CLS_TRY(launch_csrmm_rows<double>(A, probes, T2, probes, W, probes, one, zero, cus, st));        // W = A T2
     const double c = 2.0 / (static_cast<double>(probes) * n);
+    hipLaunchKernelGGL(hutch_grad_kernel, dim3(blocks_for(nnzG)), dim3(256), 0, st, nnzG, probes, f->g_row,
+                       f->g_ci, U, V, W, Z, c, f->gG);
+    CLS_TRY(launch_reduce<double, double>(RedOp::Sqr, V, nullptr, elems, f->sc + 2, false, control->red_partials,
+                                          st));
     double total = 0.0;
-    for (uint32_t pi = 0; pi < probes; ++pi) {
-        hipLaunchKernelGGL(rademacher_kernel, dim3(blocks_for(n)), dim3(256), 0, st, n,
-                           seed * 0x100000001B3ull + pi, f->z);
-        CLS_TRY(launch_csrmv<double>(one, &f->G64, f->z, zero, f->t, st, false, nullptr));   // t = G z
-        CLS_TRY(launch_csrmv<double>(one, A, f->t, zero, f->u, st, false, nullptr));         // u = A t
-        CLS_TRY(launch_csrmv<double>(one, &f->GT64, f->u, zero, f->s, st, false, nullptr));  // s = G^T u
-        hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(n)), dim3(256), 0, st, n, f->s, f->z, f->v);
-        CLS_TRY(launch_csrmv<double>(one, &f->G64, f->v, zero, f->t2, st, false, nullptr));  // t2 = G v
-        CLS_TRY(launch_csrmv<double>(one, A, f->t2, zero, f->w, st, false, nullptr));        // w = A t2
-        hipLaunchKernelGGL(hutch_grad_kernel, dim3(blocks_for(nnzG)), dim3(256), 0, st, nnzG,
-                           f->g_row, f->g_ci, f->u, f->v, f->w, f->z, c, f->gG);
-        CLS_TRY(launch_reduce<double, double>(RedOp::Sqr, f->v, nullptr, n, f->sc + 2, false,
-                                              control->red_partials, st));
-        double vv = 0.0;
-        CLS_HIP_TRY(hipMemcpyAsync(&vv, f->sc + 2, sizeof(double), hipMemcpyDeviceToHost, st));
-        CLS_HIP_TRY(hipStreamSynchronize(st));
-        total += vv;
-    }
+    CLS_HIP_TRY(hipMemcpyAsync(&total, f->sc + 2, sizeof(double), hipMemcpyDeviceToHost, st));
+    CLS_HIP_TRY(hipStreamSynchronize(st));
     if (loss) *loss = total / (static_cast<double>(probes) * n);
     if (grad_g_host)
         CLS_HIP_TRY(hipMemcpyAsync(grad_g_host, f->gG, nnzG * sizeof(double), hipMemcpyDeviceToHost, st));
