@@ -656,7 +656,17 @@ __global__ __launch_bounds__(256) void hutch_grad_kernel(uint32_t nnzG, uint32_t
     if (k >= nnzG) return;
     const size_t i = static_cast<size_t>(grow[k]) * np, j = static_cast<size_t>(gcol[k]) * np;
     double g = 0.0;
-    for (uint32_t p = 0; p < np; ++p) g += c * (u[i + p] * v[j + p] + w[i + p] * z[j + p]);
+    uint32_t p = 0;
+    if ((np & 1) == 0) {  // rows of the blocks are 16-byte aligned: two probes per load
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        for (; p < np; p += 2) {
+            const d2 ui = *reinterpret_cast<const d2*>(u + i + p), vj = *reinterpret_cast<const d2*>(v + j + p);
+            const d2 wi = *reinterpret_cast<const d2*>(w + i + p), zj = *reinterpret_cast<const d2*>(z + j + p);
+            g += c * (ui[0] * vj[0] + wi[0] * zj[0]);
+            g += c * (ui[1] * vj[1] + wi[1] * zj[1]);
+        }
+    }
+    for (; p < np; ++p) g += c * (u[i + p] * v[j + p] + w[i + p] * z[j + p]);
     gG[k] = g;
 }
 

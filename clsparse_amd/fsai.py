@@ -135,7 +135,7 @@ def train(net: Mlp, ctl: Control, train_ids, steps_per_matrix: int = 4, lr: floa
             "loss_first": float(np.mean(losses[:n_first])) if losses else None,
             "loss_last": float(np.mean(losses[-n_first:])) if losses else None,
             "seconds": round(time.perf_counter() - t0, 2),
-            "train_step_seconds": round(gpu_s, 2)}
+            "train_step_seconds": round(gpu_s, 5)}
 
 
 def evaluate(net: Mlp | None, ctl: Control, ids, probes: int = 8, grid: int = 512) -> list:
