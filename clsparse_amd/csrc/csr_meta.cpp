@@ -14,8 +14,14 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <atomic>
+#include <condition_variable>
+#include <functional>
 #include <memory>
+#include <mutex>
 #include <new>
+#include <thread>
+#include <vector>
 
 #include "common.hpp"
 
@@ -31,44 +37,158 @@ inline uint64_t threads_for_reduction(uint64_t num_rows)
     return static_cast<uint64_t>(256 >> shift);
 }
 
-// One pass over the row pointer. With out == nullptr it only counts entries
-// (ComputeRowBlocksSize); otherwise it writes them. Returns the reference's
-// rowBlockSize: 2 * entries written, or 2 * entries counted; a writing pass
-// also stores the counting pass's value in *counted when given.
-uint64_t compute_row_blocks(const uint32_t* rp, uint32_t nRows, uint64_t* out, uint64_t* counted = nullptr)
+// A small pool of host threads kept for the life of the process (starting
+// threads costs about as much as the binning itself), used for one parallel
+// pass at a time.
+class HostPool
 {
-    const bool write = out != nullptr;
-    uint64_t* w = out;
-    uint64_t total = 1;  // rowBlocks[0]
-    if (write) *w++ = 0;
+public:
+    static HostPool& get()
+    {
+        static HostPool pool;
+        return pool;
+    }
+    // Threads the pass may use (the calling thread included).
+    static unsigned width()
+    {
+        const unsigned hw = std::thread::hardware_concurrency();
+        return std::max(1u, std::min(16u, hw == 0 ? 1u : hw));
+    }
+    // Runs fn(t) for t = 0 .. T-1 on T threads at once (the caller takes
+    // t = 0), so fn may wait for the others; returns when all have finished.
+    // Falls back to fewer workers if threads cannot be started: returns the
+    // count actually used, and fn must then be called with that T.
+    unsigned start(unsigned T)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        try {
+            while (workers_.size() + 1 < T) {
+                const unsigned idx = static_cast<unsigned>(workers_.size()) + 1;
+                workers_.emplace_back([this, idx] { loop(idx); });
+            }
+        } catch (...) {
+        }
+        return std::min<unsigned>(T, static_cast<unsigned>(workers_.size()) + 1);
+    }
+    void run(unsigned T, const std::function<void(unsigned)>& fn)
+    {
+        std::lock_guard<std::mutex> call(call_mu_);
+        if (T > 1) {
+            std::unique_lock<std::mutex> lk(mu_);
+            job_ = &fn;
+            job_T_ = T;
+            pending_ = T - 1;
+            ++gen_;
+            cv_.notify_all();
+        }
+        fn(0);
+        if (T > 1) {
+            std::unique_lock<std::mutex> lk(mu_);
+            done_.wait(lk, [this] { return pending_ == 0; });
+            job_ = nullptr;
+        }
+    }
+    ~HostPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+            cv_.notify_all();
+        }
+        for (auto& w : workers_) w.join();
+    }
 
-    uint64_t sum = 0;
-    uint64_t i = 0, last_i = 0;
-    uint32_t long_run = 0;  // consecutive rows longer than 128 nnz; UINT32_MAX = leaving
+private:
+    void loop(unsigned idx)
+    {
+        uint64_t seen = 0;
+        for (;;) {
+            const std::function<void(unsigned)>* job;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+                if (idx >= job_T_) continue;
+                job = job_;
+            }
+            (*job)(idx);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--pending_ == 0) done_.notify_all();
+        }
+    }
+    std::mutex call_mu_, mu_;
+    std::condition_variable cv_, done_;
+    std::vector<std::thread> workers_;
+    const std::function<void(unsigned)>* job_ = nullptr;
+    unsigned job_T_ = 0, pending_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
+// The binning state between rows: the current block starts after row
+// last_i, holds sum nonzeros, and long_run counts the consecutive rows of a
+// long-row region (UINT32_MAX: leaving one).
+struct BinState
+{
+    uint64_t last_i = 0, sum = 0;
+    uint32_t long_run = 0;
+};
+
+// Processes rows [i_begin, i_end] of the binning (the loop of csr-meta.hpp
+// L93-238) from state st. Entries go to out (when not null) after its last
+// element, which is the entry a block closed here ORs its reduction-thread
+// count into; total counts them. longs: the rows longer than 128 nonzeros,
+// from which the rows of a block below the budget are skipped in one search
+// (needs a non-decreasing row pointer). Returns the row after the last one
+// processed (nRows + 1 when the rows ran out inside a block).
+uint64_t bin_rows(const uint32_t* rp, uint32_t nRows, uint64_t i_begin, uint64_t i_end, BinState& st,
+                  std::vector<uint64_t>* out, uint64_t& total, const std::vector<uint32_t>& longs)
+{
     const uint64_t blk = kBlkSize;
+    uint64_t last_i = st.last_i, sum = st.sum;
+    uint32_t long_run = st.long_run;
+    size_t lp = std::lower_bound(longs.begin(), longs.end(), static_cast<uint32_t>(i_begin)) - longs.begin();
 
     auto close_block = [&](uint64_t end_row) {
         // New entry starts at end_row; the finished block [last_i, end_row)
         // gets its reduction-thread count when it has more than one row.
-        *w = end_row << (64 - kRowBits);
-        if (end_row - last_i > static_cast<uint64_t>(kRowsForVector))
-            *(w - 1) |= threads_for_reduction(end_row - last_i);
-        ++w;
+        if (end_row - last_i > static_cast<uint64_t>(kRowsForVector)) out->back() |= threads_for_reduction(end_row - last_i);
+        out->push_back(end_row << (64 - kRowBits));
     };
 
-    for (i = 1; i <= nRows; ++i) {
+    uint64_t i;
+    for (i = i_begin; i <= i_end; ++i) {
         // Outside a long-row region a row of at most 128 nonzeros only adds
         // to the block's sum until the sum reaches the block size (at every
         // iteration boundary sum == rp[i - 1] - rp[last_i]): skip such rows
         // and take up the rules again at the first row that can change the
-        // state.
+        // state -- the next long row, or the first row whose end reaches the
+        // budget (a galloping search; the row pointer is non-decreasing).
         if (long_run == 0 && sum == static_cast<uint64_t>(rp[i - 1] - rp[last_i])) {
             const uint32_t base = rp[last_i];
+            while (lp < longs.size() && longs[lp] < i) ++lp;
+            const uint64_t lim = lp < longs.size() ? longs[lp] : static_cast<uint64_t>(nRows) + 1;
             uint64_t r = i;
-            while (r <= nRows && rp[r] - rp[r - 1] <= 128u && rp[r] - base < blk) ++r;
+            if (r < lim && rp[r] - base < blk) {
+                uint64_t lo = r, step = 1;
+                while (lo + step < lim && rp[lo + step] - base < blk) {
+                    lo += step;
+                    step *= 2;
+                }
+                uint64_t hi = std::min(lo + step, lim);  // rp[hi] - base >= blk, or hi == lim
+                while (hi - lo > 1) {
+                    const uint64_t mid = lo + (hi - lo) / 2;
+                    if (rp[mid] - base < blk)
+                        lo = mid;
+                    else
+                        hi = mid;
+                }
+                r = hi;
+            }
             if (r > nRows) {
-                i = nRows + 1;
                 sum = rp[nRows] - base;
+                i = static_cast<uint64_t>(nRows) + 1;
                 break;
             }
             i = r;
@@ -89,14 +209,14 @@ uint64_t compute_row_blocks(const uint32_t* rp, uint32_t nRows, uint64_t* out, u
         if (long_run == 1) {
             // Entering a long-row region: cut off the pending short rows.
             if (i - last_i > 1) {
-                if (write) close_block(i - 1);
+                if (out) close_block(i - 1);
                 ++total;
                 last_i = i - 1;
                 sum = len;
             }
         } else if (long_run == UINT32_MAX) {
             // First short row after a long-row region.
-            if (write) close_block(i - 1);
+            if (out) close_block(i - 1);
             ++total;
             last_i = i - 1;
             sum = len;
@@ -108,10 +228,9 @@ uint64_t compute_row_blocks(const uint32_t* rp, uint32_t nRows, uint64_t* out, u
             // (ceil(len / 3072) entries, capped at 2^24).
             int k = static_cast<int>(std::ceil(static_cast<double>(len) / kLongSeg));
             k = k < (1 << kWgBits) ? k : (1 << kWgBits);
-            if (write) {
-                for (int s = 1; s < k; ++s)
-                    *w++ = ((i - 1) << (64 - kRowBits)) | static_cast<uint64_t>(s);
-                *w++ = i << (64 - kRowBits);
+            if (out) {
+                for (int sg = 1; sg < k; ++sg) out->push_back(((i - 1) << (64 - kRowBits)) | static_cast<uint64_t>(sg));
+                out->push_back(i << (64 - kRowBits));
             }
             total += static_cast<uint64_t>(k);
             last_i = i;
@@ -120,22 +239,137 @@ uint64_t compute_row_blocks(const uint32_t* rp, uint32_t nRows, uint64_t* out, u
         } else if (i - last_i > 1 && sum > blk) {
             // Several rows over budget: back off the row that did not fit.
             --i;
-            if (write) close_block(i);
+            if (out) close_block(i);
             ++total;
             last_i = i;
             sum = 0;
             long_run = 0;
         } else if (sum == blk) {
-            if (write) close_block(i);
+            if (out) close_block(i);
             ++total;
             last_i = i;
             sum = 0;
             long_run = 0;
         }
     }
+    st.last_i = last_i;
+    st.sum = sum;
+    st.long_run = long_run;
+    return i;
+}
+
+// One pass over the row pointer. With out == nullptr it only counts entries
+// (ComputeRowBlocksSize); otherwise it writes them. Returns the reference's
+// rowBlockSize: 2 * entries written, or 2 * entries counted; a writing pass
+// also stores the counting pass's value in *counted when given. Returns 0
+// (no entries) for a row pointer that decreases when `reject_decreasing`.
+//
+// Work split: a row longer than 128 nonzeros that follows a row shorter than
+// 32 leaves the same state behind whatever came before it (a long-row region
+// cannot be running at the short row, so the long row starts one and the
+// block before it is cut at it). The rows are split into T chunks; thread t
+// lists its chunk's long rows and first such "sync" row S_t, then (after a
+// barrier) bins the rows (S_t, S_{t+1}] from the state that S_t leaves
+// (thread 0 from the start), where S_{t+1} is the next published sync row.
+// The ranges' entries are joined in order (a range's first cut ORs its
+// thread count into the previous range's last entry). The search of bin_rows
+// needs a non-decreasing row pointer; a decreasing one is binned by the
+// reference's sequential pass instead (clsparseCsrMetaSize counts it).
+uint64_t compute_row_blocks(const uint32_t* rp, uint32_t nRows, uint64_t* out, uint64_t* counted = nullptr,
+                            bool reject_decreasing = false)
+{
+    const bool write = out != nullptr;
+    HostPool& pool = HostPool::get();
+    unsigned T = std::min(HostPool::width(), std::max(1u, nRows / (1u << 17)));
+    T = pool.start(T);
+
+    struct Part
+    {
+        std::vector<uint32_t> longs;  // rows of the chunk longer than 128 nonzeros
+        uint64_t sync = 0;            // first sync row of the chunk, 0 if none
+        bool decreasing = false;
+        std::vector<uint64_t> ent;    // ent[0]: the seam (ORed into the previous range's last entry)
+        uint64_t total = 0, stop = 0;
+        BinState st;
+        bool ran = false;
+    };
+    std::vector<Part> part(T);
+    std::atomic<unsigned> arrived{0};
+    pool.run(T, [&](unsigned t) {
+        Part& pt = part[t];
+        const uint64_t a = 1 + static_cast<uint64_t>(nRows) * t / T, b = 1 + static_cast<uint64_t>(nRows) * (t + 1) / T;
+        bool dec = false;
+        for (uint64_t r = a; r < b; ++r) {
+            dec |= rp[r] < rp[r - 1];
+            if (rp[r] - rp[r - 1] > 128u) {
+                pt.longs.push_back(static_cast<uint32_t>(r));
+                if (pt.sync == 0 && r >= 2 && rp[r - 1] - rp[r - 2] < 32u) pt.sync = r;
+            }
+        }
+        pt.decreasing = dec;
+        arrived.fetch_add(1);
+        while (arrived.load() < T) std::this_thread::yield();
+        bool any_dec = false;
+        for (const Part& q : part) any_dec |= q.decreasing;
+        if (any_dec) return;
+        // Range (S_t, S_{t+1}]; thread 0 starts at the first row.
+        const uint64_t lo = t == 0 ? 0 : pt.sync;
+        if (t > 0 && lo == 0) return;  // no sync row in this chunk: the previous range covers it
+        uint64_t hi = nRows;
+        unsigned u = t + 1;
+        while (u < T && part[u].sync == 0) ++u;
+        if (u < T) hi = part[u].sync;
+        std::vector<uint32_t> longs;
+        for (unsigned k = t; k < T && (k == t || k <= u); ++k)
+            for (uint32_t r : part[k].longs)
+                if (r > lo && r <= hi) longs.push_back(r);
+        pt.ent.push_back(0);
+        if (write) pt.ent.reserve(64 + 2 * (hi - lo) / 16);
+        if (t > 0) {
+            // The state after the sync row lo, which the previous range processed.
+            const uint32_t len = rp[lo] - rp[lo - 1];
+            if (len >= static_cast<uint32_t>(kBlkSize))  // a row of its own (CSR-Vector / long row)
+                pt.st = BinState{lo, 0, 0};
+            else
+                pt.st = BinState{lo - 1, len, 1};
+        }
+        pt.stop = bin_rows(rp, nRows, lo + 1, hi, pt.st, write ? &pt.ent : nullptr, pt.total, longs);
+        pt.ran = true;
+    });
+
+    bool decreasing = false;
+    for (const Part& q : part) decreasing |= q.decreasing;
+    if (decreasing) {
+        if (reject_decreasing) return 0;
+        // Every row may stop the skip: bin_rows is then the sequential pass.
+        part.assign(1, Part{});
+        std::vector<uint32_t> all;
+        all.reserve(nRows);
+        for (uint32_t r = 1; r <= nRows; ++r) all.push_back(r);
+        part[0].ent.push_back(0);
+        part[0].stop = bin_rows(rp, nRows, 1, nRows, part[0].st, write ? &part[0].ent : nullptr, part[0].total, all);
+        part[0].ran = true;
+    }
+
+    uint64_t total = 1;  // rowBlocks[0]
+    uint64_t* w = out;
+    if (write) *w++ = 0;
+    const Part* last = nullptr;
+    for (const Part& q : part) {
+        if (!q.ran) continue;
+        total += q.total;
+        if (write) {
+            *(w - 1) |= q.ent[0];
+            std::copy(q.ent.begin() + 1, q.ent.end(), w);
+            w += q.ent.size() - 1;
+        }
+        last = &q;
+    }
 
     // Tail: the thread count uses i == nRows + 1 here, one more row than the
     // block holds (csr-meta.hpp L241-248).
+    const uint64_t i = last->stop;
+    const uint64_t last_i = last->st.last_i;
     if (write && (*(w - 1) >> (64 - kRowBits)) != static_cast<uint64_t>(nRows)) {
         *w = static_cast<uint64_t>(nRows) << (64 - kRowBits);
         if (static_cast<uint64_t>(nRows) - last_i > static_cast<uint64_t>(kRowsForVector))
@@ -238,11 +472,20 @@ extern "C" clsparseStatus clsparseCsrMetaCreate(clsparseCsrMatrix* csrMatx,
     CLS_HIP_TRY(hipStreamSynchronize(control->stream));
 
     // The kernels index vals/cols through this row pointer; a malformed one
-    // would send them out of bounds, so check it once here on the host.
-    bool decreasing = false;
-    for (uint32_t r = 0; r < n; ++r) decreasing |= rp[r + 1] < rp[r];
-    if (decreasing) return clsparseInvalidValue;
+    // would send them out of bounds, so check it here on the host: the end
+    // here, a decrease in the binning's pass (which then writes nothing).
     if (rp[n] > csrMatx->num_nonzeros) return clsparseInvalidValue;
+
+    // One writing pass into a buffer of twice the bound on the entries: the
+    // device array has the counting pass's size, two words per entry (the
+    // second half zero, as the reference's).
+    const uint64_t bound = 2 * row_blocks_bound(n, rp[n] >= rp[0] ? rp[n] - rp[0] : 0);
+    std::unique_ptr<uint64_t[]> host(new (std::nothrow) uint64_t[bound]);
+    if (!host) return clsparseOutOfHostMemory;
+    uint64_t size_words = 0;
+    const uint64_t created = compute_row_blocks(rp, n, host.get(), &size_words, true);
+    if (created == 0) return clsparseInvalidValue;  // the row pointer decreases
+    if (size_words > bound) return clsparseOutOfHostMemory;
 
     if (csrMatx->meta) {
         free_meta(static_cast<matrix_meta*>(csrMatx->meta));
@@ -254,22 +497,6 @@ extern "C" clsparseStatus clsparseCsrMetaCreate(clsparseCsrMatrix* csrMatx,
     m->device = control->device;
     m->num_rows = n;
     m->num_nonzeros = csrMatx->num_nonzeros;
-
-    // One writing pass into a buffer of twice the bound on the entries: the
-    // device array has the counting pass's size, two words per entry (the
-    // second half zero, as the reference's).
-    const uint64_t bound = 2 * row_blocks_bound(n, rp[n] - rp[0]);
-    std::unique_ptr<uint64_t[]> host(new (std::nothrow) uint64_t[bound]);
-    if (!host) {
-        delete m;
-        return clsparseOutOfHostMemory;
-    }
-    uint64_t size_words = 0;
-    const uint64_t created = compute_row_blocks(rp, n, host.get(), &size_words);
-    if (size_words > bound) {
-        delete m;
-        return clsparseOutOfHostMemory;
-    }
     std::fill(host.get() + created / 2, host.get() + size_words, uint64_t(0));
     m->rowBlockSize = static_cast<clsparseIdx_t>(created);
     m->entries = static_cast<uint32_t>(created / 2);

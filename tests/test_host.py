@@ -123,13 +123,30 @@ def test_library_binning_skips_match_oracle():
             np.concatenate([[0], np.cumsum(np.where(rng.random(300000) < 0.01, rng.integers(0, 3000, 300000),
                                                     0))]),
             np.concatenate([[0], np.cumsum(rng.choice([0, 1, 127, 128, 129, 1000, 1023, 1024, 1025], 400000,
-                                                      p=[.3, .3, .1, .1, .05, .05, .04, .03, .03]))])]
+                                                      p=[.3, .3, .1, .1, .05, .05, .04, .03, .03]))]),
+            # Large enough to be binned in several ranges on several threads,
+            # cut at long rows after short ones (lengths 129, 1024, 2000, 3073).
+            np.concatenate([[0], np.cumsum(np.tile([200, 40, 129, 31, 0, 0, 5, 128, 1024, 3073, 32, 33], 60000))]),
+            np.concatenate([[0], np.cumsum(np.tile([5, 1024, 3, 2000, 10, 129, 0, 1024, 7, 7, 7], 90000))]),
+            np.concatenate([[0], np.cumsum(rng.choice([0, 3, 9, 31, 129, 300, 1024, 5000], 1500000,
+                                                      p=[.2, .3, .3, .1, .04, .03, .02, .01]))])]
     for rp in pats:
         rp = np.ascontiguousarray(rp, dtype=np.uint32)
         e1, s1 = cls.row_blocks_host(rp)
         e2, s2 = oracle.row_blocks(rp)
         assert s1 == s2
         np.testing.assert_array_equal(e1, e2)
+        assert cls.row_blocks_host(rp, count_only=True) == oracle.row_blocks(rp, count_only=True)
+
+
+def test_library_binning_counts_a_decreasing_row_pointer_as_the_reference():
+    # clsparseCsrMetaSize counts any row pointer as the reference's pass does;
+    # a decreasing one takes the sequential pass (the search needs it sorted).
+    rng = np.random.default_rng(5)
+    small = np.array([0, 5, 10, 8, 20, 20, 400, 399, 2000, 2001], dtype=np.uint32)
+    big = np.concatenate([[0], np.cumsum(rng.integers(0, 40, 600000))]).astype(np.uint32)
+    big[300001] = big[300000] - 3
+    for rp in (small, big):
         assert cls.row_blocks_host(rp, count_only=True) == oracle.row_blocks(rp, count_only=True)
 
 
