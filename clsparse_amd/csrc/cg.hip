@@ -1426,6 +1426,7 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     FsaiDia dia{};
     bool have_dia = false;
     if (fsai) {
+        CLS_TRY(fsai_prepare(fsai, s));
         if constexpr (kMixed) {
             // The factor's values in BF16 (CLSPARSE_AMD_FSAI_BF16=0: FP32):
             // M = G G^T stays symmetric positive definite for any rounding of

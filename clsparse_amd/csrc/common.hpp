@@ -190,6 +190,8 @@ clsparseStatus launch_csrmm_rows(const clsparseCsrMatrix* A, uint32_t ncols, con
                                  uint32_t ldc, const T* alpha, const T* beta, int num_cus, hipStream_t stream);
 
 // Learned FSAI factor (fsai.hip): G and G^T in FP32 / FP64 with metadata.
+// Brings the factor's solver value copies up to date (after training steps).
+clsparseStatus fsai_prepare(clsparseAmdFsai f, hipStream_t stream);
 const clsparseCsrMatrix* fsai_G32(clsparseAmdFsai f);
 const clsparseCsrMatrix* fsai_GT32(clsparseAmdFsai f);
 const clsparseCsrMatrix* fsai_G64(clsparseAmdFsai f);

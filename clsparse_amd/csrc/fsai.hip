@@ -765,6 +765,11 @@ struct _clsparseAmdFsai
     bool meta_ready = false;
     clsparseCsrMatrix G16{}, GT16{};  // share G32's / GT32's metadata (not owned)
     bool valid = true;
+    // A training step assembles only the FP64 values (no host sync): the
+    // solver's FP32 / BF16 / diagonal copies and `valid` are then brought up
+    // to date from the stored network output before the factor is used.
+    bool solver_stale = false;
+    bool have_o = false;  // o holds the network output G was last assembled from
     // Device memory of the arrays above: two allocations (the n-sized
     // arrays, then the nnz(G)-sized ones), carved in 256-byte pieces.
     void* arena_n = nullptr;
@@ -785,7 +790,9 @@ hipError_t dmalloc(T** p, size_t count)
     return hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T));
 }
 
-clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_t st, int num_cus = 0);
+enum class Assemble { Full, Training, Refresh };
+clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_t st, int num_cus = 0,
+                             Assemble mode = Assemble::Full);
 
 // Diagonal scaling, row statistics and features from A's values.
 clsparseStatus compute_features(_clsparseAmdFsai* f, const clsparseCsrMatrix* A, hipStream_t st)
@@ -1296,13 +1303,18 @@ extern "C" clsparseStatus clsparseAmdFsaiDestroy(clsparseAmdFsai f)
 namespace {
 
 // num_cus: the device's compute units (sizes the forward kernel's grid).
-clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_t st, int num_cus)
+// Full: the network's output (or the Jacobi factor when net is null), every
+// value copy and `valid` (one host sync). Training: the network's output and
+// the FP64 values only (the training step needs nothing else). Refresh: the
+// value copies and `valid` from the stored output, after training steps.
+clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_t st, int num_cus, Assemble mode)
 {
     if (f->nnzG == 0) {
         f->valid = false;
+        f->solver_stale = false;
         return clsparseSuccess;
     }
-    if (net) {
+    if (net && mode != Assemble::Refresh) {
         // As many workgroups as the GPU holds at once (queried once), or one
         // per 4 tiles if fewer.
         static const int per_cu = [] {
@@ -1317,8 +1329,19 @@ clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_
         hipLaunchKernelGGL(mlp_forward_kernel, dim3(grid), dim3(64 * kMlpWaves), 0, st, net->dev(), f->X,
                            f->nnzG, f->o);
     }
+    if (mode != Assemble::Refresh) f->have_o = net != nullptr;
+    const float* o = f->have_o ? f->o : nullptr;
+    if (mode == Assemble::Training) {
+        hipLaunchKernelGGL((assemble_kernel<double>), dim3(blocks_for(f->nnzG)), dim3(256), 0, st, f->nnzG,
+                           f->g_row, f->g_ci, f->g_tpos, o, f->dhat, f->g64, f->gt64,
+                           static_cast<uint16_t*>(nullptr), static_cast<uint16_t*>(nullptr),
+                           static_cast<const uint32_t*>(nullptr), static_cast<uint16_t*>(nullptr), f->bad);
+        CLS_HIP_TRY(hipGetLastError());
+        f->solver_stale = true;
+        f->race = 0;
+        return clsparseSuccess;
+    }
     CLS_HIP_TRY(hipMemsetAsync(f->bad, 0, sizeof(int), st));
-    const float* o = net ? f->o : nullptr;
     hipLaunchKernelGGL((assemble_kernel<double>), dim3(blocks_for(f->nnzG)), dim3(256), 0, st, f->nnzG,
                        f->g_row, f->g_ci, f->g_tpos, o, f->dhat, f->g64, f->gt64,
                        static_cast<uint16_t*>(nullptr), static_cast<uint16_t*>(nullptr),
@@ -1332,6 +1355,7 @@ clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_
     CLS_HIP_TRY(hipStreamSynchronize(st));
     f->valid = (bad == 0) && f->full_diag;
     f->race = 0;
+    f->solver_stale = false;
     return clsparseSuccess;
 }
 
@@ -1341,6 +1365,8 @@ clsparseStatus ensure_train_scratch(_clsparseAmdFsai* f, uint32_t probes)
         hipError_t e = dmalloc(&f->gG, f->nnzG);
         if (e == hipSuccess) e = dmalloc(&f->sc, 8);
         if (e == hipSuccess) e = dmalloc(&f->gout, f->nnzG);
+        const double constants[2] = {1.0, 0.0};
+        if (e == hipSuccess) e = hipMemcpy(f->sc, constants, sizeof(constants), hipMemcpyHostToDevice);
         if (e != hipSuccess) return clsparseOutOfResources;
     }
     if (probes > f->probe_cap) {
@@ -1417,12 +1443,10 @@ extern "C" clsparseStatus clsparseAmdFsaiTrainStep(clsparseAmdFsai f, clsparseAm
     if (f->nnzG == 0) return clsparseInvalidValue;
     CLS_TRY(ensure_train_scratch(f, probes));
     hipStream_t st = control->stream;
-    CLS_TRY(fsai_assemble(f, net, st, control->num_cus));
+    CLS_TRY(fsai_assemble(f, net, st, control->num_cus, Assemble::Training));
     const uint32_t n = f->n, nnzG = f->nnzG;
-    double* one = f->sc;
-    double* zero = f->sc + 1;
-    hipLaunchKernelGGL((set_scalar<double>), dim3(1), dim3(64), 0, st, one, 1.0);
-    hipLaunchKernelGGL((set_scalar<double>), dim3(1), dim3(64), 0, st, zero, 0.0);
+    const double* one = f->sc;  // constants 1 and 0, written with the scratch
+    const double* zero = f->sc + 1;
     // All probes at once: each product is one sparse x dense (n x probes)
     // product, reading G, G^T and A once for every probe.
     const size_t np = probes, elems = static_cast<size_t>(n) * np;
@@ -1468,7 +1492,13 @@ extern "C" clsparseStatus clsparseAmdFsaiTrainStep(clsparseAmdFsai f, clsparseAm
     return clsparseSuccess;
 }
 
-// Accessors for the solver (cg.hip).
+// Accessors for the solver (cg.hip). fsai_prepare first: the value copies
+// the solver reads are brought up to date after training steps.
+clsparseStatus fsai_prepare(clsparseAmdFsai f, hipStream_t st)
+{
+    if (!f->solver_stale) return clsparseSuccess;
+    return fsai_assemble(f, nullptr, st, 0, Assemble::Refresh);
+}
 const clsparseCsrMatrix* fsai_G32(clsparseAmdFsai f) { return &f->G32; }
 const clsparseCsrMatrix* fsai_GT32(clsparseAmdFsai f) { return &f->GT32; }
 const clsparseCsrMatrix* fsai_G64(clsparseAmdFsai f) { return &f->G64; }
