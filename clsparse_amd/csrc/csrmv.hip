@@ -235,7 +235,22 @@ __global__ __launch_bounds__(64 * kWavesPerWg) void csrmv_adaptive_kernel(
 #pragma unroll
             for (int k = 0; k < kStreamPerLane / 4; ++k) {
                 const uint32_t i0 = 256 * k + 4 * lane;
-                if (i0 + 3 < cnt) {
+                if (!ROWWISE && i0 + 3 < cnt) {
+                    // Read once: non-temporal, so the stream leaves the
+                    // caches to the x gathers (the row-wise variant, for
+                    // stencils, is faster with ordinary loads).
+                    typedef uint32_t u32x4a __attribute__((ext_vector_type(4), aligned(4)));
+                    const u32x4a cc = __builtin_nontemporal_load(reinterpret_cast<const u32x4a*>(cols + base + i0));
+                    for (int u = 0; u < 4; ++u) c[4 * k + u] = cc[u];
+                    if constexpr (sizeof(TV) == 8) {
+                        const u32x4a v0 = __builtin_nontemporal_load(reinterpret_cast<const u32x4a*>(vals + base + i0));
+                        const u32x4a v1 = __builtin_nontemporal_load(reinterpret_cast<const u32x4a*>(vals + base + i0 + 2));
+                        __builtin_memcpy(&v[4 * k], &v0, 16);
+                        __builtin_memcpy(&v[4 * k + 2], &v1, 16);
+                    } else {
+                        __builtin_memcpy(&v[4 * k], vals + base + i0, 4 * sizeof(TV));
+                    }
+                } else if (i0 + 3 < cnt) {
                     __builtin_memcpy(&v[4 * k], vals + base + i0, 4 * sizeof(TV));
                     __builtin_memcpy(&c[4 * k], cols + base + i0, 4 * sizeof(uint32_t));
                 } else {
@@ -624,27 +639,46 @@ __global__ __launch_bounds__(256) void sum_ax_kernel(uint64_t nnz, const T* __re
                                                      const uint32_t* __restrict__ cols,
                                                      const T* __restrict__ x, double* __restrict__ part)
 {
+    // Each lane reads groups of 4 consecutive values and columns with 16-byte
+    // non-temporal loads (as the SpMV's stream path does), kSumAxUnroll / 4
+    // groups in flight, then gathers their 4 x values.
+    typedef uint32_t u32x4a __attribute__((ext_vector_type(4), aligned(4)));
+    constexpr int G = kSumAxUnroll / 4;
     __shared__ double sh[4];
+    const uint64_t ngroups = nnz / 4;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256;
     double acc = 0.0;
-    for (uint64_t j0 = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x; j0 < nnz;
-         j0 += kSumAxUnroll * stride) {
-        T v[kSumAxUnroll];
-        uint32_t c[kSumAxUnroll];
+    for (uint64_t g0 = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x; g0 < ngroups; g0 += G * stride) {
+        T v[G][4];
+        uint32_t c[G][4];
 #pragma unroll
-        for (int u = 0; u < kSumAxUnroll; ++u) {
-            const uint64_t j = j0 + u * stride;
-            const bool in = j < nnz;
-            v[u] = in ? __builtin_nontemporal_load(vals + j) : T(0);
-            c[u] = in ? __builtin_nontemporal_load(cols + j) : 0u;
+        for (int u = 0; u < G; ++u) {
+            const uint64_t g = g0 + u * stride;
+            if (g < ngroups) {
+                const u32x4a cc = __builtin_nontemporal_load(reinterpret_cast<const u32x4a*>(cols + 4 * g));
+                for (int e = 0; e < 4; ++e) c[u][e] = cc[e];
+                u32x4a vv[sizeof(T) / 4];
+                for (int h = 0; h < static_cast<int>(sizeof(T) / 4); ++h)
+                    vv[h] = __builtin_nontemporal_load(reinterpret_cast<const u32x4a*>(vals + 4 * g) + h);
+                __builtin_memcpy(&v[u][0], &vv[0], 4 * sizeof(T));
+            } else {
+                for (int e = 0; e < 4; ++e) {
+                    c[u][e] = 0u;
+                    v[u][e] = T(0);
+                }
+            }
         }
-        T xv[kSumAxUnroll];
+        T xv[G][4];
 #pragma unroll
-        for (int u = 0; u < kSumAxUnroll; ++u) xv[u] = (j0 + u * stride < nnz) ? x[c[u]] : T(0);
+        for (int u = 0; u < G; ++u)
+            for (int e = 0; e < 4; ++e) xv[u][e] = (g0 + u * stride < ngroups) ? x[c[u][e]] : T(0);
 #pragma unroll
-        for (int u = 0; u < kSumAxUnroll; ++u)
-            acc += static_cast<double>(v[u]) * static_cast<double>(xv[u]);
+        for (int u = 0; u < G; ++u)
+            for (int e = 0; e < 4; ++e) acc += static_cast<double>(v[u][e]) * static_cast<double>(xv[u][e]);
     }
+    // The last nnz % 4 entries.
+    const uint64_t t = 4 * ngroups + static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (t < nnz) acc += static_cast<double>(vals[t]) * static_cast<double>(x[cols[t]]);
     acc = block_sum256(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
