@@ -58,7 +58,9 @@ __device__ __forceinline__ void write_row(T* __restrict__ y, uint32_t row, const
                                           T beta)
 {
     // beta == 0 never reads y (csrmv_adaptive.cl L481-483, L505-507, L557-559).
-    y[row] = (beta != T(0)) ? a.with_beta(beta, y[row]) : a.value();
+    // Non-temporal: y is not read again by the kernel (the lower triangle of
+    // the 160^3 stencil in FP64: 70 -> 59 us; the others unchanged).
+    __builtin_nontemporal_store((beta != T(0)) ? a.with_beta(beta, y[row]) : a.value(), y + row);
 }
 
 // x gathers. FP32: through a buffer resource with 32-bit byte offsets (less
