@@ -276,7 +276,7 @@ uint64_t bin_rows(const uint32_t* rp, uint32_t nRows, uint64_t i_begin, uint64_t
 // needs a non-decreasing row pointer; a decreasing one is binned by the
 // reference's sequential pass instead (clsparseCsrMetaSize counts it).
 uint64_t compute_row_blocks(const uint32_t* rp, uint32_t nRows, uint64_t* out, uint64_t* counted = nullptr,
-                            bool reject_decreasing = false)
+                            bool reject_decreasing = false, bool* failed = nullptr)
 {
     const bool write = out != nullptr;
     HostPool& pool = HostPool::get();
@@ -292,6 +292,7 @@ uint64_t compute_row_blocks(const uint32_t* rp, uint32_t nRows, uint64_t* out, u
         uint64_t total = 0, stop = 0;
         BinState st;
         bool ran = false;
+        bool failed = false;  // out of host memory
     };
     std::vector<Part> part(T);
     std::atomic<unsigned> arrived{0};
@@ -299,56 +300,75 @@ uint64_t compute_row_blocks(const uint32_t* rp, uint32_t nRows, uint64_t* out, u
         Part& pt = part[t];
         const uint64_t a = 1 + static_cast<uint64_t>(nRows) * t / T, b = 1 + static_cast<uint64_t>(nRows) * (t + 1) / T;
         bool dec = false;
-        for (uint64_t r = a; r < b; ++r) {
-            dec |= rp[r] < rp[r - 1];
-            if (rp[r] - rp[r - 1] > 128u) {
-                pt.longs.push_back(static_cast<uint32_t>(r));
-                if (pt.sync == 0 && r >= 2 && rp[r - 1] - rp[r - 2] < 32u) pt.sync = r;
+        try {
+            for (uint64_t r = a; r < b; ++r) {
+                dec |= rp[r] < rp[r - 1];
+                if (rp[r] - rp[r - 1] > 128u) {
+                    pt.longs.push_back(static_cast<uint32_t>(r));
+                    if (pt.sync == 0 && r >= 2 && rp[r - 1] - rp[r - 2] < 32u) pt.sync = r;
+                }
             }
+        } catch (...) {
+            pt.failed = true;
         }
         pt.decreasing = dec;
-        arrived.fetch_add(1);
+        arrived.fetch_add(1);  // every thread arrives, or the others would wait for ever
         while (arrived.load() < T) std::this_thread::yield();
-        bool any_dec = false;
-        for (const Part& q : part) any_dec |= q.decreasing;
-        if (any_dec) return;
-        // Range (S_t, S_{t+1}]; thread 0 starts at the first row.
-        const uint64_t lo = t == 0 ? 0 : pt.sync;
-        if (t > 0 && lo == 0) return;  // no sync row in this chunk: the previous range covers it
-        uint64_t hi = nRows;
-        unsigned u = t + 1;
-        while (u < T && part[u].sync == 0) ++u;
-        if (u < T) hi = part[u].sync;
-        std::vector<uint32_t> longs;
-        for (unsigned k = t; k < T && (k == t || k <= u); ++k)
-            for (uint32_t r : part[k].longs)
-                if (r > lo && r <= hi) longs.push_back(r);
-        pt.ent.push_back(0);
-        if (write) pt.ent.reserve(64 + 2 * (hi - lo) / 16);
-        if (t > 0) {
-            // The state after the sync row lo, which the previous range processed.
-            const uint32_t len = rp[lo] - rp[lo - 1];
-            if (len >= static_cast<uint32_t>(kBlkSize))  // a row of its own (CSR-Vector / long row)
-                pt.st = BinState{lo, 0, 0};
-            else
-                pt.st = BinState{lo - 1, len, 1};
+        bool stop = false;
+        for (const Part& q : part) stop |= q.decreasing || q.failed;
+        if (stop) return;
+        try {
+            // Range (S_t, S_{t+1}]; thread 0 starts at the first row.
+            const uint64_t lo = t == 0 ? 0 : pt.sync;
+            if (t > 0 && lo == 0) return;  // no sync row in this chunk: the previous range covers it
+            uint64_t hi = nRows;
+            unsigned u = t + 1;
+            while (u < T && part[u].sync == 0) ++u;
+            if (u < T) hi = part[u].sync;
+            std::vector<uint32_t> longs;
+            for (unsigned k = t; k < T && (k == t || k <= u); ++k)
+                for (uint32_t r : part[k].longs)
+                    if (r > lo && r <= hi) longs.push_back(r);
+            pt.ent.push_back(0);
+            if (write) pt.ent.reserve(64 + 2 * (hi - lo) / 16);
+            if (t > 0) {
+                // The state after the sync row lo, which the previous range processed.
+                const uint32_t len = rp[lo] - rp[lo - 1];
+                if (len >= static_cast<uint32_t>(kBlkSize))  // a row of its own (CSR-Vector / long row)
+                    pt.st = BinState{lo, 0, 0};
+                else
+                    pt.st = BinState{lo - 1, len, 1};
+            }
+            pt.stop = bin_rows(rp, nRows, lo + 1, hi, pt.st, write ? &pt.ent : nullptr, pt.total, longs);
+            pt.ran = true;
+        } catch (...) {
+            pt.failed = true;
         }
-        pt.stop = bin_rows(rp, nRows, lo + 1, hi, pt.st, write ? &pt.ent : nullptr, pt.total, longs);
-        pt.ran = true;
     });
+    for (const Part& q : part)
+        if (q.failed) {
+            if (failed) *failed = true;
+            return 0;
+        }
 
     bool decreasing = false;
     for (const Part& q : part) decreasing |= q.decreasing;
     if (decreasing) {
         if (reject_decreasing) return 0;
         // Every row may stop the skip: bin_rows is then the sequential pass.
-        part.assign(1, Part{});
-        std::vector<uint32_t> all;
-        all.reserve(nRows);
-        for (uint32_t r = 1; r <= nRows; ++r) all.push_back(r);
-        part[0].ent.push_back(0);
-        part[0].stop = bin_rows(rp, nRows, 1, nRows, part[0].st, write ? &part[0].ent : nullptr, part[0].total, all);
-        part[0].ran = true;
+        try {
+            part.assign(1, Part{});
+            std::vector<uint32_t> all;
+            all.reserve(nRows);
+            for (uint32_t r = 1; r <= nRows; ++r) all.push_back(r);
+            part[0].ent.push_back(0);
+            part[0].stop =
+                bin_rows(rp, nRows, 1, nRows, part[0].st, write ? &part[0].ent : nullptr, part[0].total, all);
+            part[0].ran = true;
+        } catch (...) {
+            if (failed) *failed = true;
+            return 0;
+        }
     }
 
     uint64_t total = 1;  // rowBlocks[0]
@@ -449,7 +469,10 @@ extern "C" clsparseMetaSizeResult clsparseCsrMetaSize(clsparseCsrMatrix* csrMatx
     std::vector<uint32_t> rp;
     r.status = read_row_pointer(csrMatx, control, rp);
     if (r.status != clsparseSuccess) return r;
-    r.metaSize = static_cast<clsparseIdx_t>(compute_row_blocks(rp.data(), csrMatx->num_rows, nullptr));
+    bool failed = false;
+    r.metaSize = static_cast<clsparseIdx_t>(compute_row_blocks(rp.data(), csrMatx->num_rows, nullptr, nullptr,
+                                                               false, &failed));
+    if (failed) r.status = clsparseOutOfHostMemory;
     return r;
 }
 
@@ -483,7 +506,9 @@ extern "C" clsparseStatus clsparseCsrMetaCreate(clsparseCsrMatrix* csrMatx,
     std::unique_ptr<uint64_t[]> host(new (std::nothrow) uint64_t[bound]);
     if (!host) return clsparseOutOfHostMemory;
     uint64_t size_words = 0;
-    const uint64_t created = compute_row_blocks(rp, n, host.get(), &size_words, true);
+    bool failed = false;
+    const uint64_t created = compute_row_blocks(rp, n, host.get(), &size_words, true, &failed);
+    if (failed) return clsparseOutOfHostMemory;
     if (created == 0) return clsparseInvalidValue;  // the row pointer decreases
     if (size_words > bound) return clsparseOutOfHostMemory;
 
