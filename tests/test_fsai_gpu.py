@@ -269,6 +269,34 @@ def test_training_lowers_loss_and_beats_the_untrained_method(ctl, monkeypatch):
         net.release()
 
 
+@pytest.mark.parametrize("dia", ["1", "0"])
+def test_solve_after_a_training_step_uses_its_network_output(ctl, monkeypatch, dia):
+    """A training step assembles only the FP64 factor; the next solve brings
+    the FP32 / BF16 / diagonal copies up to date from the step's network
+    output, so it runs exactly as after apply_net with the same network."""
+    monkeypatch.setenv("CLSPARSE_AMD_FSAI_POLICY", "always")
+    monkeypatch.setenv("CLSPARSE_AMD_FSAI_DIA", dia)
+    rp, ci, v, m, A = _sample_matrix(ctl, 24, 24, seed=31)
+    b = cls.gen_uniform(m, -1, 1, seed=32)
+    net = cls.Mlp(seed=9)
+    f1, f2 = cls.Fsai(A, ctl), cls.Fsai(A, ctl)
+    try:
+        net.set_params(_random_params(33, scale=0.05))
+        say(f"fsai: solve after a training step (diagonal storage {dia})")
+        f1.train_step(net, A, ctl, probes=4, seed=7, lr=0.0)  # lr 0: the network stays
+        st1, x1 = _solve_fsai(ctl, A, f1, b)
+        f2.apply_net(net, ctl)
+        st2, x2 = _solve_fsai(ctl, A, f2, b)
+        assert st1.status_detail == 0 and st1.preconditioner == 2
+        assert st1.iterations == st2.iterations
+        np.testing.assert_array_equal(x1, x2)
+        np.testing.assert_array_equal(f1.values()[2], f2.values()[2])
+    finally:
+        f1.release()
+        f2.release()
+        net.release()
+
+
 def test_pilot_race_picks_a_preconditioner_and_converges(ctl, monkeypatch):
     monkeypatch.delenv("CLSPARSE_AMD_FSAI_POLICY", raising=False)
     rp, ci, v, m, A = _sample_matrix(ctl, 64, 64, seed=14)
