@@ -23,6 +23,8 @@
 #include <thread>
 #include <vector>
 
+#include <unistd.h>
+
 #include "common.hpp"
 
 namespace {
@@ -39,14 +41,20 @@ inline uint64_t threads_for_reduction(uint64_t num_rows)
 
 // A small pool of host threads kept for the life of the process (starting
 // threads costs about as much as the binning itself), used for one parallel
-// pass at a time.
+// pass at a time. A child of fork() inherits none of the threads: it runs its
+// passes on the calling thread alone.
 class HostPool
 {
 public:
     static HostPool& get()
     {
-        static HostPool pool;
-        return pool;
+        // On the heap, and deleted at exit only by the process that started
+        // its threads: in a child of fork() the condition variables may still
+        // count the parent's waiting workers, and destroying them would wait
+        // for ever.
+        static HostPool* pool = new HostPool;
+        static const Reaper reaper{pool};
+        return *pool;
     }
     // Threads the pass may use (the calling thread included).
     static unsigned width()
@@ -61,6 +69,7 @@ public:
     unsigned start(unsigned T)
     {
         std::lock_guard<std::mutex> lk(mu_);
+        if (getpid() != owner_) return 1;  // a child of fork(): the workers are the parent's
         try {
             while (workers_.size() + 1 < T) {
                 const unsigned idx = static_cast<unsigned>(workers_.size()) + 1;
@@ -99,6 +108,14 @@ public:
     }
 
 private:
+    struct Reaper
+    {
+        HostPool* pool;
+        ~Reaper()
+        {
+            if (getpid() == pool->owner_) delete pool;
+        }
+    };
     void loop(unsigned idx)
     {
         uint64_t seen = 0;
@@ -124,6 +141,7 @@ private:
     unsigned job_T_ = 0, pending_ = 0;
     uint64_t gen_ = 0;
     bool stop_ = false;
+    const pid_t owner_ = getpid();  // the process whose threads workers_ lists
 };
 
 // The binning state between rows: the current block starts after row

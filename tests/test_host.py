@@ -8,6 +8,7 @@ import json
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -148,6 +149,34 @@ def test_library_binning_counts_a_decreasing_row_pointer_as_the_reference():
     big[300001] = big[300000] - 3
     for rp in (small, big):
         assert cls.row_blocks_host(rp, count_only=True) == oracle.row_blocks(rp, count_only=True)
+
+
+_FORK_SCRIPT = r"""
+import os, sys
+import numpy as np
+import clsparse_amd as cls
+rng = np.random.default_rng(9)
+rp = np.concatenate([[0], np.cumsum(rng.integers(0, 300, 1 << 20))]).astype(np.uint32)
+ref, size = cls.row_blocks_host(rp)  # the parent's threads start
+pid = os.fork()
+if pid == 0:
+    e, s = cls.row_blocks_host(rp)
+    sys.exit(0 if (s == size and np.array_equal(e, ref)) else 3)  # a normal exit: destructors run
+_, status = os.waitpid(pid, 0)
+assert os.WIFEXITED(status) and os.WEXITSTATUS(status) == 0, status
+e, s = cls.row_blocks_host(rp)
+assert s == size and np.array_equal(e, ref)
+print("ok")
+"""
+
+
+def test_library_binning_in_a_forked_child():
+    # The binning's host threads start on first use; a child of fork()
+    # inherits none of them and must bin on its own thread (and exit) rather
+    # than wait for them. Run in a process of its own, with a time limit.
+    r = subprocess.run([sys.executable, "-c", _FORK_SCRIPT], cwd=ROOT, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", (r.returncode, r.stdout[-2000:], r.stderr[-2000:])
 
 
 def test_library_binning_config_patterns_match_oracle():
