@@ -8,9 +8,11 @@
 #include <cstdlib>
 #include <cstring>
 #include <thread>
+#include <new>
 #include <vector>
 
 #include "common.hpp"
+#include "host_threads.hpp"
 
 namespace {
 
@@ -58,21 +60,18 @@ unsigned num_threads()
     return t;
 }
 
+// f(i) for i = 0 .. n-1 in t contiguous chunks, one thread each (f must not
+// throw: it only fills preallocated arrays).
 template <typename F>
 void parallel_rows(uint64_t n, F&& f)
 {
     const unsigned t = num_threads();
-    std::vector<std::thread> pool;
     const uint64_t chunk = (n + t - 1) / t;
-    for (unsigned k = 0; k < t; ++k) {
+    run_on_threads(t, [&](unsigned k) {
         const uint64_t lo = k * chunk;
         const uint64_t hi = std::min<uint64_t>(n, lo + chunk);
-        if (lo >= hi) break;
-        pool.emplace_back([=, &f]() {
-            for (uint64_t i = lo; i < hi; ++i) f(i);
-        });
-    }
-    for (auto& th : pool) th.join();
+        for (uint64_t i = lo; i < hi; ++i) f(i);
+    });
 }
 
 bool alloc_csr(clsparseAmdHostCsr* out, uint32_t nrows, uint32_t ncols, uint64_t nnz)
@@ -151,10 +150,8 @@ extern "C" clsparseStatus clsparseAmdGenPoisson2D(cl_uint nx, cl_uint ny, clspar
     return clsparseSuccess;
 }
 
-extern "C" clsparseStatus clsparseAmdGenPowerLaw(cl_uint n, cl_double mean_len, cl_double sigma,
-                                                 cl_uint max_len, cl_uint n_dense,
-                                                 cl_uint dense_len, cl_ulong seed,
-                                                 clsparseAmdHostCsr* out)
+static clsparseStatus gen_power_law(cl_uint n, cl_double mean_len, cl_double sigma, cl_uint max_len,
+                                    cl_uint n_dense, cl_uint dense_len, cl_ulong seed, clsparseAmdHostCsr* out)
 {
     if (out == nullptr || n == 0 || mean_len <= 0 || max_len == 0) return clsparseInvalidValue;
     const uint32_t cap = std::min<uint32_t>(max_len, n);
@@ -218,6 +215,18 @@ extern "C" clsparseStatus clsparseAmdGenPowerLaw(cl_uint n, cl_double mean_len, 
     return clsparseSuccess;
 }
 
+extern "C" clsparseStatus clsparseAmdGenPowerLaw(cl_uint n, cl_double mean_len, cl_double sigma,
+                                                 cl_uint max_len, cl_uint n_dense,
+                                                 cl_uint dense_len, cl_ulong seed,
+                                                 clsparseAmdHostCsr* out)
+{
+    try {
+        return gen_power_law(n, mean_len, sigma, max_len, n_dense, dense_len, seed, out);
+    } catch (const std::bad_alloc&) {
+        return clsparseOutOfHostMemory;  // out is untouched: its arrays come after every throwing allocation
+    }
+}
+
 // Q1 (bilinear) element stiffness on the unit square for a constant tensor
 // K = [[k11, k12], [k12, k22]]; local nodes (0,0), (1,0), (0,1), (1,1).
 // Exact integrals of grad(phi_a)^T K grad(phi_b): SPSD by construction.
@@ -240,10 +249,9 @@ static void q1_stiffness(double k11, double k12, double k22, double ke[4][4])
             ke[a][b] = k11 * Sxx[a][b] + k22 * Syy[a][b] + k12 * (cx[a] * cy[b] + cx[b] * cy[a]);
 }
 
-extern "C" clsparseStatus clsparseAmdGenAnisoDiffusion2D(cl_uint nx, cl_uint ny, cl_double corr_len,
-                                                         cl_double sigma, cl_double ratio,
-                                                         cl_double theta, cl_ulong seed,
-                                                         clsparseAmdHostCsr* out)
+static clsparseStatus gen_aniso_diffusion2d(cl_uint nx, cl_uint ny, cl_double corr_len, cl_double sigma,
+                                            cl_double ratio, cl_double theta, cl_ulong seed,
+                                            clsparseAmdHostCsr* out)
 {
     if (out == nullptr || nx == 0 || ny == 0 || ratio <= 0 || corr_len <= 0)
         return clsparseInvalidValue;
@@ -317,10 +325,20 @@ extern "C" clsparseStatus clsparseAmdGenAnisoDiffusion2D(cl_uint nx, cl_uint ny,
     return clsparseSuccess;
 }
 
-extern "C" clsparseStatus clsparseAmdGenDiffusion3D(cl_uint nx, cl_uint ny, cl_uint nz,
-                                                    cl_double sigma, cl_double ax, cl_double ay,
-                                                    cl_double az, cl_ulong seed,
-                                                    clsparseAmdHostCsr* out)
+extern "C" clsparseStatus clsparseAmdGenAnisoDiffusion2D(cl_uint nx, cl_uint ny, cl_double corr_len,
+                                                         cl_double sigma, cl_double ratio,
+                                                         cl_double theta, cl_ulong seed,
+                                                         clsparseAmdHostCsr* out)
+{
+    try {
+        return gen_aniso_diffusion2d(nx, ny, corr_len, sigma, ratio, theta, seed, out);
+    } catch (const std::bad_alloc&) {
+        return clsparseOutOfHostMemory;  // out is untouched: its arrays come after every throwing allocation
+    }
+}
+
+static clsparseStatus gen_diffusion3d(cl_uint nx, cl_uint ny, cl_uint nz, cl_double sigma, cl_double ax,
+                                      cl_double ay, cl_double az, cl_ulong seed, clsparseAmdHostCsr* out)
 {
     if (out == nullptr || nx == 0 || ny == 0 || nz == 0) return clsparseInvalidValue;
     const uint64_t n = static_cast<uint64_t>(nx) * ny * nz;
@@ -381,4 +399,16 @@ extern "C" clsparseStatus clsparseAmdGenDiffusion3D(cl_uint nx, cl_uint ny, cl_u
         }
     });
     return clsparseSuccess;
+}
+
+extern "C" clsparseStatus clsparseAmdGenDiffusion3D(cl_uint nx, cl_uint ny, cl_uint nz,
+                                                    cl_double sigma, cl_double ax, cl_double ay,
+                                                    cl_double az, cl_ulong seed,
+                                                    clsparseAmdHostCsr* out)
+{
+    try {
+        return gen_diffusion3d(nx, ny, nz, sigma, ax, ay, az, seed, out);
+    } catch (const std::bad_alloc&) {
+        return clsparseOutOfHostMemory;  // out is untouched: its arrays come after every throwing allocation
+    }
 }

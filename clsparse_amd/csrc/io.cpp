@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 #include <string>
 #include <string_view>
 #include <thread>
@@ -38,6 +39,7 @@
 #include <unistd.h>
 
 #include "common.hpp"
+#include "host_threads.hpp"
 
 namespace {
 
@@ -196,6 +198,14 @@ struct Raw
     T v;
 };
 
+// Capacity to reserve for the entries of the text [p, end): at most `want`,
+// and no more than the text can hold (an entry takes at least four bytes,
+// "1 1" and a line end), whatever the header claims.
+size_t entry_bound(uint64_t want, const char* p, const char* end)
+{
+    return static_cast<size_t>(std::min<uint64_t>(want, static_cast<uint64_t>(end - p) / 4 + 1));
+}
+
 // Entries of [p, end) until `want` are read or the text ends; *crossed is set
 // when an entry runs past `end` (it continues beyond this chunk). Returns
 // false on a malformed token.
@@ -272,15 +282,13 @@ clsparseStatus read_entries(const char* path, bool explicit_zeroes, MmHeader& h,
         }
         part.assign(nt, {});
         std::vector<char> ok(nt, 1), crossed(nt, 0);
-        std::vector<std::thread> th;
-        for (unsigned k = 0; k < nt; ++k)
-            th.emplace_back([&, k]() {
+        if (!run_on_threads(nt, [&](unsigned k) {
                 bool cr = false;
-                part[k].reserve(static_cast<size_t>(h.nnz / nt + 16));
+                part[k].reserve(entry_bound(h.nnz / nt + 16, cut[k], cut[k + 1]));
                 ok[k] = parse_range<T>(cut[k], cut[k + 1], text_end, pattern, h.nnz, part[k], &cr);
                 crossed[k] = cr;
-            });
-        for (auto& t : th) t.join();
+            }))
+            throw std::bad_alloc();
         // Usable when every chunk up to the nnz-th entry parsed cleanly and no
         // entry ran across a chunk boundary; otherwise parse sequentially.
         bool regular = true;
@@ -299,7 +307,7 @@ clsparseStatus read_entries(const char* path, bool explicit_zeroes, MmHeader& h,
     if (!done) {
         nt = 1;
         part.assign(1, {});
-        part[0].reserve(static_cast<size_t>(h.nnz));
+        part[0].reserve(entry_bound(h.nnz, body, text_end));
         bool cr = false;
         if (!parse_range<T>(body, text_end, text_end, pattern, h.nnz, part[0], &cr)) return clsparseInvalidFile;
         if (part[0].size() < h.nnz) return clsparseInvalidFile;
@@ -315,11 +323,10 @@ clsparseStatus read_entries(const char* path, bool explicit_zeroes, MmHeader& h,
     std::vector<std::vector<Entry<T>>> ent(nt);
     {
         std::vector<char> ok(nt, 1);
-        std::vector<std::thread> th;
-        for (unsigned k = 1; k < nt; ++k)
-            th.emplace_back([&, k]() { ok[k] = filter_entries<T>(part[k].data(), take[k], h, explicit_zeroes, ent[k]); });
-        ok[0] = filter_entries<T>(part[0].data(), take[0], h, explicit_zeroes, ent[0]);
-        for (auto& t : th) t.join();
+        if (!run_on_threads(nt, [&](unsigned k) {
+                ok[k] = filter_entries<T>(part[k].data(), take[k], h, explicit_zeroes, ent[k]);
+            }))
+            throw std::bad_alloc();
         for (unsigned k = 0; k < nt; ++k)
             if (!ok[k]) return clsparseInvalidFile;
     }
@@ -333,6 +340,7 @@ clsparseStatus read_entries(const char* path, bool explicit_zeroes, MmHeader& h,
         total += e.size();
         for (const auto& x : e) ++start[x.r + 1];
     }
+    if (total > 0xFFFFFFFFull) return clsparseInvalidFile;  // the symmetric expansion passed 32-bit counts
     for (uint64_t r = 0; r < h.rows; ++r) start[r + 1] += start[r];
     out.resize(total);
     {
@@ -360,8 +368,8 @@ void row_pointer(const std::vector<Entry<T>>& e, uint32_t rows, std::vector<uint
 }
 
 template <typename T>
-clsparseStatus coo_from_file(clsparseCooMatrix* coo, const char* path, clsparseControl control,
-                             cl_bool explicit_zeroes)
+clsparseStatus coo_from_file_impl(clsparseCooMatrix* coo, const char* path, clsparseControl control,
+                                  cl_bool explicit_zeroes)
 {
     CLS_TRY(check_init_control(control));
     if (coo == nullptr) return clsparseInvalidValue;
@@ -396,8 +404,8 @@ clsparseStatus coo_from_file(clsparseCooMatrix* coo, const char* path, clsparseC
 }
 
 template <typename T>
-clsparseStatus csr_from_file(clsparseCsrMatrix* csr, const char* path, clsparseControl control,
-                             cl_bool explicit_zeroes)
+clsparseStatus csr_from_file_impl(clsparseCsrMatrix* csr, const char* path, clsparseControl control,
+                                  cl_bool explicit_zeroes)
 {
     CLS_TRY(check_init_control(control));
     if (csr == nullptr) return clsparseInvalidValue;
@@ -431,14 +439,37 @@ clsparseStatus csr_from_file(clsparseCsrMatrix* csr, const char* path, clsparseC
     return clsparseSuccess;
 }
 
-}  // namespace
-
-// src/library/io/mm-reader.cpp L441-466
-extern "C" clsparseStatus clsparseHeaderfromFile(clsparseIdx_t* nnz, clsparseIdx_t* row,
-                                                 clsparseIdx_t* col, const char* filePath)
+// The C entry points' bodies: host memory running out (or a thread that
+// could not finish its part) gives clsparseOutOfHostMemory, never an
+// exception through the C ABI.
+template <typename F>
+clsparseStatus guarded(F&& body)
 {
-    if (nnz == nullptr || row == nullptr || col == nullptr) return clsparseInvalidValue;
-    if (!has_mtx_extension(filePath)) return clsparseInvalidFileFormat;
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return clsparseOutOfHostMemory;
+    } catch (...) {
+        return clsparseInvalidFile;
+    }
+}
+
+template <typename T>
+clsparseStatus coo_from_file(clsparseCooMatrix* coo, const char* path, clsparseControl control,
+                             cl_bool explicit_zeroes)
+{
+    return guarded([&] { return coo_from_file_impl<T>(coo, path, control, explicit_zeroes); });
+}
+
+template <typename T>
+clsparseStatus csr_from_file(clsparseCsrMatrix* csr, const char* path, clsparseControl control,
+                             cl_bool explicit_zeroes)
+{
+    return guarded([&] { return csr_from_file_impl<T>(csr, path, control, explicit_zeroes); });
+}
+
+clsparseStatus header_from_file(clsparseIdx_t* nnz, clsparseIdx_t* row, clsparseIdx_t* col, const char* filePath)
+{
     // The header only: read up to the size line.
     FILE* f = std::fopen(filePath, "rb");
     if (!f) return clsparseInvalidFile;
@@ -462,6 +493,43 @@ extern "C" clsparseStatus clsparseHeaderfromFile(clsparseIdx_t* nnz, clsparseIdx
     *row = static_cast<clsparseIdx_t>(h.rows);
     *col = static_cast<clsparseIdx_t>(h.cols);
     return clsparseSuccess;
+}
+
+clsparseStatus csr_from_file_host(const char* filePath, cl_bool read_explicit_zeroes, clsparseAmdHostCsr* out)
+{
+    MmHeader h;
+    std::vector<Entry<double>> e;
+    CLS_TRY(read_entries<double>(filePath, read_explicit_zeroes != 0, h, e));
+    std::vector<uint32_t> rp;
+    row_pointer(e, static_cast<uint32_t>(h.rows), rp);
+    const size_t nnz = e.size();
+    out->num_rows = static_cast<clsparseIdx_t>(h.rows);
+    out->num_cols = static_cast<clsparseIdx_t>(h.cols);
+    out->num_nonzeros = static_cast<clsparseIdx_t>(nnz);
+    out->row_pointer = static_cast<cl_uint*>(std::malloc(rp.size() * sizeof(cl_uint)));
+    out->col_indices = static_cast<cl_uint*>(std::malloc(std::max<size_t>(nnz, 1) * sizeof(cl_uint)));
+    out->values = static_cast<cl_double*>(std::malloc(std::max<size_t>(nnz, 1) * sizeof(cl_double)));
+    if (!out->row_pointer || !out->col_indices || !out->values) {
+        clsparseAmdHostCsrFree(out);
+        return clsparseOutOfHostMemory;
+    }
+    std::memcpy(out->row_pointer, rp.data(), rp.size() * sizeof(cl_uint));
+    for (size_t i = 0; i < nnz; ++i) {
+        out->col_indices[i] = e[i].c;
+        out->values[i] = e[i].v;
+    }
+    return clsparseSuccess;
+}
+
+}  // namespace
+
+// src/library/io/mm-reader.cpp L441-466
+extern "C" clsparseStatus clsparseHeaderfromFile(clsparseIdx_t* nnz, clsparseIdx_t* row,
+                                                 clsparseIdx_t* col, const char* filePath)
+{
+    if (nnz == nullptr || row == nullptr || col == nullptr) return clsparseInvalidValue;
+    if (!has_mtx_extension(filePath)) return clsparseInvalidFileFormat;
+    return guarded([&] { return header_from_file(nnz, row, col, filePath); });
 }
 
 // mm-reader.cpp L471-519
@@ -499,26 +567,5 @@ extern "C" clsparseStatus clsparseAmdCsrMatrixfromFileHost(const char* filePath,
 {
     if (out == nullptr) return clsparseInvalidValue;
     if (!has_mtx_extension(filePath)) return clsparseInvalidFileFormat;
-    MmHeader h;
-    std::vector<Entry<double>> e;
-    CLS_TRY(read_entries<double>(filePath, read_explicit_zeroes != 0, h, e));
-    std::vector<uint32_t> rp;
-    row_pointer(e, static_cast<uint32_t>(h.rows), rp);
-    const size_t nnz = e.size();
-    out->num_rows = static_cast<clsparseIdx_t>(h.rows);
-    out->num_cols = static_cast<clsparseIdx_t>(h.cols);
-    out->num_nonzeros = static_cast<clsparseIdx_t>(nnz);
-    out->row_pointer = static_cast<cl_uint*>(std::malloc(rp.size() * sizeof(cl_uint)));
-    out->col_indices = static_cast<cl_uint*>(std::malloc(std::max<size_t>(nnz, 1) * sizeof(cl_uint)));
-    out->values = static_cast<cl_double*>(std::malloc(std::max<size_t>(nnz, 1) * sizeof(cl_double)));
-    if (!out->row_pointer || !out->col_indices || !out->values) {
-        clsparseAmdHostCsrFree(out);
-        return clsparseOutOfHostMemory;
-    }
-    std::memcpy(out->row_pointer, rp.data(), rp.size() * sizeof(cl_uint));
-    for (size_t i = 0; i < nnz; ++i) {
-        out->col_indices[i] = e[i].c;
-        out->values[i] = e[i].v;
-    }
-    return clsparseSuccess;
+    return guarded([&] { return csr_from_file_host(filePath, read_explicit_zeroes, out); });
 }

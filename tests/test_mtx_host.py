@@ -99,6 +99,24 @@ def test_bad_files_are_rejected(tmp_path):
         assert st == cls.clsparseInvalidFile, name
 
 
+def test_header_claiming_billions_of_entries_is_rejected_without_reserving_them(tmp_path):
+    # Four billion entries announced, two present: the reader reserves what
+    # the text can hold (24 bytes an entry would be 96 GB here) and reports a
+    # short file, in the small-file and the threaded (>= 1 MiB) parse alike.
+    import ctypes
+    import resource
+    lib = cls.load()
+    m = cls.clsparseAmdHostCsr()
+    small = "%%MatrixMarket matrix coordinate real general\n4 4 4000000000\n1 1 1.0\n2 2 2.0\n"
+    big = small + "%" + "x" * (1 << 21) + "\n"
+    before = resource.getrusage(resource.RUSAGE_SELF).ru_maxrss
+    for name, text in [("claims.mtx", small), ("claims_big.mtx", big)]:
+        st = lib.clsparseAmdCsrMatrixfromFileHost(_write(tmp_path, name, text).encode(), 0, ctypes.byref(m))
+        assert st == cls.clsparseInvalidFile, name
+    grown_kib = resource.getrusage(resource.RUSAGE_SELF).ru_maxrss - before
+    assert grown_kib < 256 * 1024, grown_kib
+
+
 def test_integer_and_pattern_files(tmp_path):
     ints = "%%MatrixMarket matrix coordinate integer general\n3 3 3\n3 3 -4\n1 2 7\n2 2 0\n"
     rp, ci, v, _ = cls.read_mtx_host(_write(tmp_path, "i.mtx", ints))
