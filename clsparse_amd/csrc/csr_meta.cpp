@@ -167,6 +167,7 @@ uint64_t bin_rows(const uint32_t* rp, uint32_t nRows, uint64_t i_begin, uint64_t
     uint64_t last_i = st.last_i, sum = st.sum;
     uint32_t long_run = st.long_run;
     size_t lp = std::lower_bound(longs.begin(), longs.end(), static_cast<uint32_t>(i_begin)) - longs.begin();
+    uint64_t skip_hint = 0;  // rows the last skip passed over
 
     auto close_block = [&](uint64_t end_row) {
         // New entry starts at end_row; the finished block [last_i, end_row)
@@ -189,12 +190,33 @@ uint64_t bin_rows(const uint32_t* rp, uint32_t nRows, uint64_t i_begin, uint64_t
             const uint64_t lim = lp < longs.size() ? longs[lp] : static_cast<uint64_t>(nRows) + 1;
             uint64_t r = i;
             if (r < lim && rp[r] - base < blk) {
-                uint64_t lo = r, step = 1;
-                while (lo + step < lim && rp[lo + step] - base < blk) {
-                    lo += step;
-                    step *= 2;
+                // lo: a row whose end stays under the budget; hi: the first
+                // row whose end reaches it, or lim. The search starts from the
+                // previous skip's length (a regular matrix fills every block
+                // with about as many rows), galloping down or up from there.
+                uint64_t lo = r, hi = lim;
+                const uint64_t g = r + skip_hint;
+                if (skip_hint > 1 && g < lim) {
+                    if (rp[g] - base < blk) {
+                        lo = g;
+                    } else {
+                        hi = g;
+                        uint64_t step = 1;
+                        while (hi - lo > step && rp[hi - step] - base >= blk) {
+                            hi -= step;
+                            step *= 2;
+                        }
+                        if (hi - lo > step) lo = hi - step;  // rp[hi - step] - base < blk
+                    }
                 }
-                uint64_t hi = std::min(lo + step, lim);  // rp[hi] - base >= blk, or hi == lim
+                if (hi == lim) {
+                    uint64_t step = 1;
+                    while (lo + step < lim && rp[lo + step] - base < blk) {
+                        lo += step;
+                        step *= 2;
+                    }
+                    hi = std::min(lo + step, lim);
+                }
                 while (hi - lo > 1) {
                     const uint64_t mid = lo + (hi - lo) / 2;
                     if (rp[mid] - base < blk)
@@ -204,6 +226,7 @@ uint64_t bin_rows(const uint32_t* rp, uint32_t nRows, uint64_t i_begin, uint64_t
                 }
                 r = hi;
             }
+            skip_hint = r - i;
             if (r > nRows) {
                 sum = rp[nRows] - base;
                 i = static_cast<uint64_t>(nRows) + 1;

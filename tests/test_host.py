@@ -140,6 +140,28 @@ def test_library_binning_skips_match_oracle():
         assert cls.row_blocks_host(rp, count_only=True) == oracle.row_blocks(rp, count_only=True)
 
 
+def test_library_binning_regime_changes_match_oracle():
+    # Runs of equal-length rows whose length changes abruptly (empty rows,
+    # stencil-like rows, rows near the 128 / 1024 limits, long rows): a block
+    # after a change holds a different number of rows than the one before,
+    # which the skip's search starts from.
+    rng = np.random.default_rng(17)
+    choices = np.array([0, 1, 3, 7, 31, 32, 64, 127, 128, 129, 300, 1023, 1024, 1025, 5000])
+    for trial in range(6):
+        runs = []
+        while sum(len(r) for r in runs) < 400000:
+            runs.append(np.full(int(rng.integers(1, 3000)), int(rng.choice(choices))))
+            if rng.random() < 0.3:
+                runs.append(rng.integers(0, 20, int(rng.integers(1, 500))))
+        lens = np.concatenate(runs)[:400000]
+        rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+        e1, s1 = cls.row_blocks_host(rp)
+        e2, s2 = oracle.row_blocks(rp)
+        assert s1 == s2, trial
+        np.testing.assert_array_equal(e1, e2)
+        assert cls.row_blocks_host(rp, count_only=True) == oracle.row_blocks(rp, count_only=True)
+
+
 def test_library_binning_counts_a_decreasing_row_pointer_as_the_reference():
     # clsparseCsrMetaSize counts any row pointer as the reference's pass does;
     # a decreasing one takes the sequential pass (the search needs it sorted).
