@@ -1134,7 +1134,10 @@ clsparseStatus build_pattern(_clsparseAmdFsai* f, const uint32_t* arp, const uin
     // Scratch for G^T: the sort's keys and indices, the column counts, the
     // scan's and the sort's temporary storage, a flag (the workspace is
     // taken again: nothing of the first scratch is needed any more).
-    int end_bit = 32;
+    // The sort orders the column bits of the (column, row) keys only: it is
+    // stable and G's entries come in row order, so the rows within a column
+    // stay ascending (3 passes of 8 bits at n = 4.1 M instead of 7).
+    int end_bit = 33;
     while (end_bit < 64 && (uint64_t(1) << (end_bit - 32)) < n) ++end_bit;
     uint64_t *key = nullptr, *key2 = nullptr;
     uint32_t *idx = nullptr, *idx2 = nullptr, *tcnt = nullptr;
@@ -1143,7 +1146,7 @@ clsparseStatus build_pattern(_clsparseAmdFsai* f, const uint32_t* arp, const uin
                                                  static_cast<int>(n), st));
     CLS_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb3, static_cast<uint64_t*>(nullptr),
                                                    static_cast<uint64_t*>(nullptr), static_cast<uint32_t*>(nullptr),
-                                                   static_cast<uint32_t*>(nullptr), static_cast<int>(nnzG), 0,
+                                                   static_cast<uint32_t*>(nullptr), static_cast<int>(nnzG), 32,
                                                    end_bit, st));
     void* tstore2 = nullptr;
     {
@@ -1172,7 +1175,7 @@ clsparseStatus build_pattern(_clsparseAmdFsai* f, const uint32_t* arp, const uin
     CLS_HIP_TRY(hipMemsetAsync(f->gt_rp, 0, sizeof(uint32_t), st));
     CLS_HIP_TRY(hipcub::DeviceScan::InclusiveSum(tstore2, tb2, tcnt, f->gt_rp + 1, static_cast<int>(n), st));
     if (nnzG > 0) {
-        CLS_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tstore2, tb3, key, key2, idx, idx2, static_cast<int>(nnzG), 0,
+        CLS_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tstore2, tb3, key, key2, idx, idx2, static_cast<int>(nnzG), 32,
                                                        end_bit, st));
         hipLaunchKernelGGL(transpose_fill_kernel, dim3(blocks256(nnzG)), dim3(256), 0, st, nnzG, key2, idx2,
                            f->gt_ci, f->g_tpos);
