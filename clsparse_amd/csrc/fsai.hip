@@ -965,20 +965,22 @@ __global__ __launch_bounds__(256) void tril_fill_kernel(uint32_t n, const uint32
     }
 }
 
-__global__ __launch_bounds__(256) void column_count_kernel(uint32_t nnz, const uint32_t* __restrict__ gci,
-                                                           uint32_t* __restrict__ tcnt)
-{
-    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
-    if (k < nnz) atomicAdd(&tcnt[gci[k]], 1u);
-}
-
 // Sorted (column, row) keys -> G^T's columns and the position of every G entry in G^T.
-__global__ __launch_bounds__(256) void transpose_fill_kernel(uint32_t nnz, const uint64_t* __restrict__ key,
+// Thread p of 0 .. nnz: G^T's column and G's position of sorted entry p, and
+// the starts of the G^T rows c in (column of p - 1, column of p] (every row
+// from the first to n is written once; empty rows start where the next
+// non-empty one does).
+__global__ __launch_bounds__(256) void transpose_fill_kernel(uint32_t nnz, uint32_t n, const uint64_t* __restrict__ key,
                                                              const uint32_t* __restrict__ idx,
-                                                             uint32_t* __restrict__ tci, uint32_t* __restrict__ tpos)
+                                                             uint32_t* __restrict__ tci, uint32_t* __restrict__ tpos,
+                                                             uint32_t* __restrict__ trp)
 {
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= nnz) return;
+    if (p > nnz) return;
+    const int64_t prev = p == 0 ? -1 : static_cast<int64_t>(key[p - 1] >> 32);
+    const int64_t cur = p == nnz ? static_cast<int64_t>(n) : static_cast<int64_t>(key[p] >> 32);
+    for (int64_t c = prev + 1; c <= cur; ++c) trp[c] = p;
+    if (p == nnz) return;
     tci[p] = static_cast<uint32_t>(key[p] & 0xFFFFFFFFull);
     tpos[idx[p]] = p;
 }
@@ -1131,19 +1133,16 @@ clsparseStatus build_pattern(_clsparseAmdFsai* f, const uint32_t* arp, const uin
         f->dia_pos = at<uint32_t>(b, o_dpos);
         f->bad = at<int>(b, o_bad);
     }
-    // Scratch for G^T: the sort's keys and indices, the column counts, the
-    // scan's and the sort's temporary storage, a flag (the workspace is
-    // taken again: nothing of the first scratch is needed any more).
-    // The sort orders the column bits of the (column, row) keys only: it is
-    // stable and G's entries come in row order, so the rows within a column
-    // stay ascending (3 passes of 8 bits at n = 4.1 M instead of 7).
+    // Scratch for G^T: the sort's keys and indices, its temporary storage,
+    // flags (the workspace is taken again: nothing of the first scratch is
+    // needed any more). The sort orders the column bits of the (column, row)
+    // keys only: it is stable and G's entries come in row order, so the rows
+    // within a column stay ascending (3 passes of 8 bits at n = 4.1 M, not 7).
     int end_bit = 33;
     while (end_bit < 64 && (uint64_t(1) << (end_bit - 32)) < n) ++end_bit;
     uint64_t *key = nullptr, *key2 = nullptr;
-    uint32_t *idx = nullptr, *idx2 = nullptr, *tcnt = nullptr;
-    size_t tb2 = 0, tb3 = 0;
-    CLS_HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, tb2, static_cast<uint32_t*>(nullptr), f->gt_rp + 1,
-                                                 static_cast<int>(n), st));
+    uint32_t *idx = nullptr, *idx2 = nullptr;
+    size_t tb3 = 0;
     CLS_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb3, static_cast<uint64_t*>(nullptr),
                                                    static_cast<uint64_t*>(nullptr), static_cast<uint32_t*>(nullptr),
                                                    static_cast<uint32_t*>(nullptr), static_cast<int>(nnzG), 32,
@@ -1152,35 +1151,29 @@ clsparseStatus build_pattern(_clsparseAmdFsai* f, const uint32_t* arp, const uin
     {
         ByteLayout c;
         const size_t o_k = c.add<uint64_t>(nnzG), o_k2 = c.add<uint64_t>(nnzG);
-        const size_t o_i = c.add<uint32_t>(nnzG), o_i2 = c.add<uint32_t>(nnzG), o_tc = c.add<uint32_t>(n);
-        const size_t o_t = c.add<char>(std::max(tb2, tb3)), o_f = c.add<int>(4);
+        const size_t o_i = c.add<uint32_t>(nnzG), o_i2 = c.add<uint32_t>(nnzG);
+        const size_t o_t = c.add<char>(tb3), o_f = c.add<int>(4);
         void* work = nullptr;
         CLS_TRY(control_workspace(control, c.bytes, &work));
         key = at<uint64_t>(work, o_k);
         key2 = at<uint64_t>(work, o_k2);
         idx = at<uint32_t>(work, o_i);
         idx2 = at<uint32_t>(work, o_i2);
-        tcnt = at<uint32_t>(work, o_tc);
         tstore2 = at<void>(work, o_t);
         flags = at<int>(work, o_f);
     }
     CLS_HIP_TRY(hipMemsetAsync(flags, 0, 4 * sizeof(int), st));
     hipLaunchKernelGGL(tril_fill_kernel, dim3(blocks256(n)), dim3(256), 0, st, n, arp, aci, f->g_rp, f->g_ci,
                        f->g_row, f->g_src, key, idx);
-    // G^T: row pointer from the column counts, columns and positions from a
-    // radix sort of the (column, row) keys (rows ascend within each column).
-    CLS_HIP_TRY(hipMemsetAsync(tcnt, 0, static_cast<size_t>(n) * sizeof(uint32_t), st));
-    hipLaunchKernelGGL(column_count_kernel, dim3(blocks256(nnzG)), dim3(256), 0, st, nnzG, f->g_ci, tcnt);
-    CLS_HIP_TRY(hipGetLastError());
-    CLS_HIP_TRY(hipMemsetAsync(f->gt_rp, 0, sizeof(uint32_t), st));
-    CLS_HIP_TRY(hipcub::DeviceScan::InclusiveSum(tstore2, tb2, tcnt, f->gt_rp + 1, static_cast<int>(n), st));
-    if (nnzG > 0) {
+    // G^T: a radix sort of the (column, row) keys (rows ascend within each
+    // column); its columns, G's positions and its row pointer from the
+    // sorted keys.
+    if (nnzG > 0)
         CLS_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tstore2, tb3, key, key2, idx, idx2, static_cast<int>(nnzG), 32,
                                                        end_bit, st));
-        hipLaunchKernelGGL(transpose_fill_kernel, dim3(blocks256(nnzG)), dim3(256), 0, st, nnzG, key2, idx2,
-                           f->gt_ci, f->g_tpos);
-        CLS_HIP_TRY(hipGetLastError());
-    }
+    hipLaunchKernelGGL(transpose_fill_kernel, dim3(blocks256(static_cast<uint64_t>(nnzG) + 1)), dim3(256), 0, st,
+                       nnzG, n, key2, idx2, f->gt_ci, f->g_tpos, f->gt_rp);
+    CLS_HIP_TRY(hipGetLastError());
 
     // Diagonal storage of the BF16 values.
     clsparseCsrMatrix Gm;

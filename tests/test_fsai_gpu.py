@@ -472,6 +472,46 @@ def test_first_iteration_applies_the_bf16_factor(ctl, monkeypatch, dia):
         net.release()
 
 
+def test_first_iteration_applies_an_irregular_factor(ctl, monkeypatch):
+    """An irregular pattern (columns of G from 1 to dozens of entries) keeps
+    the CSR SpMVs of G and G^T: one PCG iteration from x = 0 is x = alpha
+    G (G^T b) with the BF16-rounded factor, so G^T's row pointer, columns and
+    value positions (from the device sort of G's entries) are right."""
+    monkeypatch.setenv("CLSPARSE_AMD_FSAI_POLICY", "always")
+    from clsparse_amd import fsai as fs
+    params, _ = fs.load_weights()
+    rng = np.random.default_rng(11)
+    a = sp.random(3000, 3000, density=0.004, random_state=rng, format="csr")
+    a = a + a.T
+    a = a + sp.diags(np.asarray(abs(a).sum(axis=1)).ravel() + 1.0)
+    a = sp.csr_matrix(a)
+    a.sort_indices()
+    rp, ci, v, m = a.indptr.astype(np.uint32), a.indices.astype(np.uint32), a.data, 3000
+    A = cls.CsrMatrix(rp, ci, v, m)
+    A.meta_create(ctl)
+    net = cls.Mlp(seed=1)
+    net.set_params(params)
+    f = cls.Fsai(A, ctl)
+    try:
+        f.apply_net(net, ctl)
+        assert f.diagonals() == []
+        grp, gci, gva = f.values()
+        counts = np.bincount(gci.astype(np.int64), minlength=m)
+        assert counts.min() >= 1 and counts.max() >= 10
+        b = cls.gen_uniform(m, -1, 1, seed=13)
+        say("fsai: first iteration, irregular CSR factor")
+        st, x = _solve_fsai(ctl, A, f, b, max_iters=1)
+        G = sp.csr_matrix((of.bf16_round(gva.astype(np.float32)).astype(np.float64), gci, grp),
+                          shape=(m, m))
+        z0 = G @ (G.T @ b.astype(np.float32).astype(np.float64))
+        err = np.linalg.norm(x / np.linalg.norm(x) - z0 / np.linalg.norm(z0))
+        assert err <= 1e-5, err
+        assert st.preconditioner == 2 and st.iterations == 1
+    finally:
+        f.release()
+        net.release()
+
+
 @pytest.mark.parametrize("dims", [(24, 24, 24), (64, 64, 16)])
 def test_diagonal_storage_solves_like_the_csr_factor(ctl, monkeypatch, dims):
     """The learned factor on a 3-D anisotropic stencil: the solve from
