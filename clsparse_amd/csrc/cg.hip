@@ -591,6 +591,98 @@ void launch_dia_iteration(size_t n, uint32_t g, const DiaOffsets& off, const uin
                        part_rz, part_rz_prev, part_nr, g, st);
 }
 
+constexpr int kFuseRows = 2;  // rows per lane in flight in the fused update (registers)
+
+// The x / r update of cg_update_kernel (learned FSAI: no z) fused with
+// w = G^T r_new, the ||r_new|| partials of the convergence test and the
+// ||w||^2 partials (<r, z>). G^T's row i reads r_new at rows i - off[d]; this
+// workgroup forms r_new there from r and y exactly as the row's owner forms
+// it, so r_new goes to the other buffer (r_out) and r stays whole for the
+// other workgroups; w goes to a buffer of its own for the same reason (y is
+// read at other rows). Partials are indexed by the XCD-remapped workgroup,
+// whose rows are those of cg_update_kernel's workgroup of that index, so
+// the sums are those of the separate kernels.
+template <int K, typename TX, int R = kFuseRows>
+__global__ __launch_bounds__(256) void fsai_update_gt_dia_kernel(
+    size_t n, DiaOffsets off, const uint16_t* __restrict__ vals, TX* __restrict__ x,
+    const float* __restrict__ r, float* __restrict__ r_out, const float* __restrict__ p,
+    const float* __restrict__ y, float* __restrict__ w, const double* __restrict__ part_yp,
+    const double* __restrict__ part_rz_prev, double* __restrict__ part_rz, double* __restrict__ part_nr,
+    uint32_t g, CgDevState* __restrict__ st)
+{
+    __shared__ double sh[4];
+    const int done = st->done;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->stop = done;
+    if (done) return;
+    const double yp = sum_partials(part_yp, g, sh);
+    const double rz = sum_partials(part_rz_prev, g, sh);
+    const float alpha = static_cast<float>(rz / yp);
+    const TX alpha_x = static_cast<TX>(rz / yp);
+    const int crit = st->crit;
+    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+    double nr = 0.0, a = 0.0;
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i0 = static_cast<size_t>(blk) * 256 + threadIdx.x; i0 < n; i0 += R * stride) {
+        uint16_t h[R][K];
+        float rv[R][K], yv[R][K], pv[R];
+        TX xv[R];
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const size_t i = i0 + u * stride;
+#pragma unroll
+            for (int d = 0; d < K; ++d) {
+                const size_t j = i - static_cast<int64_t>(off.o[d]);
+                const bool in = i < n && j < n;
+                h[u][d] = in ? vals[static_cast<size_t>(d) * n + j] : uint16_t(0);
+                rv[u][d] = in ? r[j] : 0.f;
+                yv[u][d] = in ? y[j] : 0.f;
+            }
+            pv[u] = i < n ? p[i] : 0.f;
+            xv[u] = i < n ? x[i] : TX(0);
+        }
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const size_t i = i0 + u * stride;
+            if (i >= n) continue;
+            x[i] = xv[u] + alpha_x * static_cast<TX>(pv[u]);
+            double sum = 0.0;  // products summed in FP64, as the CSR SpMV sums them
+#pragma unroll
+            for (int d = K - 1; d >= 0; --d) {
+                const float rn = rv[u][d] - alpha * yv[u][d];  // as cg_update_kernel forms r
+                if (d == K - 1) {  // off[K - 1] = 0: row i itself
+                    r_out[i] = rn;
+                    nr += norm_term(crit, static_cast<double>(rn));
+                }
+                if (h[u][d] != 0) sum += static_cast<double>(mul_rounded_f(bf16_to_f32(h[u][d]), rn));
+            }
+            const float acc = static_cast<float>(sum);
+            w[i] = acc;
+            a += static_cast<double>(acc) * static_cast<double>(acc);
+        }
+    }
+    nr = block_sum256(nr, sh);
+    a = block_sum256(a, sh);
+    if (threadIdx.x == 0) {
+        part_nr[blk] = nr;
+        part_rz[blk] = a;
+    }
+}
+
+// One learned-FSAI iteration after y = A p: the fused x / r update and
+// w = G^T r_new (r from r_in to r_out), then p = G w + beta p and the
+// convergence test.
+template <int K, typename TX>
+void launch_dia_update_iteration(size_t n, uint32_t g, const DiaOffsets& off, const uint16_t* vals, TX* x,
+                                 const float* r_in, float* r_out, const float* p_in, const float* y, float* w,
+                                 float* p, const double* part_yp, double* part_rz, const double* part_rz_prev,
+                                 double* part_nr, CgDevState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL((fsai_update_gt_dia_kernel<K, TX>), dim3(g), dim3(256), 0, s, n, off, vals, x, r_in, r_out,
+                       p_in, y, w, part_yp, part_rz_prev, part_rz, part_nr, g, st);
+    hipLaunchKernelGGL((fsai_g_pupdate_dia_kernel<K>), dim3(g), dim3(256), 0, s, n, off, vals, w, p, part_rz,
+                       part_rz_prev, part_nr, g, st);
+}
+
 template <int K>
 void launch_dia_apply(size_t n, uint32_t g, const DiaOffsets& off, const uint16_t* vals, const float* r,
                       float* tmp, float* out, double* part, const int* done, hipStream_t s)
@@ -691,12 +783,15 @@ public:
     // G, GT: the factor of M = G G^T and its transpose (kPrecFsai only);
     // g_bf16: their values are BF16 bit patterns (FP32 engines only);
     // dia: the BF16 factor in diagonal storage, used instead of G and GT.
+    // r_alt: n values the engine may keep r in between iterations (the
+    // learned factor in diagonal storage fuses the x / r update into G^T r
+    // with it); their contents are not kept.
     CgEngine(const clsparseCsrMatrix* A, TX* x, const T* b, size_t n, int prec, int crit,
              CgBuffers<T>& buf, hipStream_t s, bool ext, const clsparseCsrMatrix* G = nullptr,
              const clsparseCsrMatrix* GT = nullptr, bool g_bf16 = false,
-             const FsaiDia* dia = nullptr, const DiaMatrix<T>* adia = nullptr)
+             const FsaiDia* dia = nullptr, const DiaMatrix<T>* adia = nullptr, T* r_alt = nullptr)
         : A_(A), x_(x), b_(b), n_(n), prec_(prec), crit_(crit), buf_(buf), s_(s), ext_(ext),
-          g_(grid_for(n)), G_(G), GT_(GT), g_bf16_(g_bf16)
+          g_(grid_for(n)), G_(G), GT_(GT), g_bf16_(g_bf16), r_alt_(r_alt)
     {
         if (adia != nullptr && adia->vals != nullptr) adia_ = *adia;
         // FP32 only: in FP64 the fused SpMV (p and z at every stencil offset)
@@ -712,6 +807,8 @@ public:
             dia_ = *dia;
             for (int d = 0; d < kFsaiMaxDiags; ++d) dia_off_.o[d] = dia->off[d];
         }
+        // CLSPARSE_AMD_CG_FUSE=0 keeps the separate x / r update (the same iterates).
+        fuse_gt_ = prec_ == kPrecFsai && use_dia_ && r_alt_ != nullptr && cg_fuse_enabled();
     }
     ~CgEngine()
     {
@@ -809,6 +906,15 @@ public:
     int prec() const { return prec_; }
     bool fused() const { return fused_; }
 
+    // After a chunk that ran `ran` iterations: r back in buf_.r when the
+    // fused learned-FSAI iteration left it in r_alt (an odd count).
+    clsparseStatus settle_r(int ran)
+    {
+        if (!fuse_gt_ || (ran & 1) == 0) return clsparseSuccess;
+        CLS_HIP_TRY(hipMemcpyAsync(buf_.r, r_alt_, n_ * sizeof(T), hipMemcpyDeviceToDevice, s_));
+        return clsparseSuccess;
+    }
+
     // The fused iteration leaves p one update behind z: bring it up to date
     // (before a residual replacement re-bases it). ran: iterations the last
     // chunk executed.
@@ -865,6 +971,7 @@ private:
             hipLaunchKernelGGL((cg_dot_kernel<T>), dim3(g_), dim3(256), 0, s_, n_, buf_.y, buf_.p,
                                buf_.part_yp, buf_.st);
         }
+        if (fuse_gt_) return dia_update_iteration(par);
         hipLaunchKernelGGL((cg_update_kernel<T, TX>), dim3(g_), dim3(256), 0, s_, n_, x_, buf_.r,
                            buf_.z, buf_.p, buf_.y, buf_.invD, prec_, buf_.part_yp,
                            buf_.part_rz[par ^ 1], buf_.part_rz[par], buf_.part_nr[par], g_,
@@ -901,6 +1008,41 @@ private:
             default: return clsparseInvalidValue;
             }
 #undef CLS_DIA_ITER
+            CLS_HIP_TRY(hipGetLastError());
+            return clsparseSuccess;
+        }
+        return clsparseInvalidValue;
+    }
+
+    // The learned-FSAI iteration after y = A p with the x / r update fused
+    // into w = G^T r: iteration k of a chunk reads r from r_[k & 1] and
+    // writes it to the other buffer (chunks are of even length, so a whole
+    // chunk ends with r in buf_.r; settle_r moves it back after a short
+    // one); w lives in z, which this engine keeps in registers otherwise.
+    clsparseStatus dia_update_iteration(int par)
+    {
+        if constexpr (std::is_same_v<T, float>) {
+            const uint16_t* v = dia_.vals;
+            const T* r_in = par ? r_alt_ : buf_.r;
+            T* r_out = par ? buf_.r : r_alt_;
+            double* rz = buf_.part_rz[par];
+            const double* rz0 = buf_.part_rz[par ^ 1];
+            double* nr = buf_.part_nr[par];
+#define CLS_DIA_UPD(K)                                                                                          \
+    launch_dia_update_iteration<K, TX>(n_, g_, dia_off_, v, x_, r_in, r_out, buf_.p, buf_.y, buf_.z, buf_.p,    \
+                                       buf_.part_yp, rz, rz0, nr, buf_.st, s_)
+            switch (dia_.ndiag) {
+            case 1: CLS_DIA_UPD(1); break;
+            case 2: CLS_DIA_UPD(2); break;
+            case 3: CLS_DIA_UPD(3); break;
+            case 4: CLS_DIA_UPD(4); break;
+            case 5: CLS_DIA_UPD(5); break;
+            case 6: CLS_DIA_UPD(6); break;
+            case 7: CLS_DIA_UPD(7); break;
+            case 8: CLS_DIA_UPD(8); break;
+            default: return clsparseInvalidValue;
+            }
+#undef CLS_DIA_UPD
             CLS_HIP_TRY(hipGetLastError());
             return clsparseSuccess;
         }
@@ -964,6 +1106,8 @@ private:
     const clsparseCsrMatrix* G_;
     const clsparseCsrMatrix* GT_;
     bool g_bf16_;
+    T* r_alt_;
+    bool fuse_gt_ = false;  // learned factor in diagonal storage: x / r update fused into G^T r
     bool use_dia_ = false;
     FsaiDia dia_{};
     DiaMatrix<T> adia_{};
@@ -1516,8 +1660,10 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
         else
             return xp;
     }();
+    // rI holds the inner right-hand side for each set-up only: the learned
+    // factor's fused iteration may keep r in it between iterations.
     Engine eng_main(&AI, xe, rI, n, prec, kCritL2, buf, s, false, G, GT, g_bf16, have_dia ? &dia : nullptr,
-                    &adia);
+                    &adia, rI);
     Engine eng_jac(&AI, xe, rI, n, kPrecJacobi, kCritL2, buf, s, false, nullptr, nullptr, false, nullptr,
                    &adia);
     Engine* eng = &eng_main;
@@ -1599,6 +1745,7 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
         for (;;) {
             CLS_TRY(eng->run_chunk(hs));
             const int ran = hs->iter - iter_at_chunk;
+            CLS_TRY(eng->settle_r(ran));
             iter_at_chunk = hs->iter;
             iters = hs->iter;
             const double cur = hs->cur_res;

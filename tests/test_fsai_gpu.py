@@ -472,6 +472,44 @@ def test_first_iteration_applies_the_bf16_factor(ctl, monkeypatch, dia):
         net.release()
 
 
+@pytest.mark.parametrize("x64", ["0", "1"])
+def test_fused_update_iteration_matches_the_separate_kernels(ctl, monkeypatch, x64):
+    """With the factor in diagonal storage the x / r update is fused into
+    w = G^T r (r alternating between two buffers): the iterates, iteration
+    and replacement counts equal those of the separate kernels bit for bit,
+    through residual replacements after chunks cut short by convergence
+    (after an odd count the residual is moved back to its buffer before the
+    replacement reads it), with the solution as an FP32 increment and in
+    FP64."""
+    monkeypatch.setenv("CLSPARSE_AMD_FSAI_POLICY", "always")
+    monkeypatch.setenv("CLSPARSE_AMD_MIXED_X64", x64)
+    from clsparse_amd import fsai as fs
+    params, _ = fs.load_weights()
+    rp, ci, v, m = cls.gen_diffusion3d(28, 26, 24, sigma=1.0, ax=1.0, ay=1.0, az=100.0, seed=6)
+    A = cls.CsrMatrix(rp, ci, v, m)
+    A.meta_create(ctl)
+    net = cls.Mlp(seed=1)
+    net.set_params(params)
+    f = cls.Fsai(A, ctl)
+    try:
+        f.apply_net(net, ctl)
+        assert len(f.diagonals()) == 4
+        b = cls.gen_uniform(m, -1, 1, seed=31)
+        out = {}
+        for fuse in ("1", "0"):
+            monkeypatch.setenv("CLSPARSE_AMD_CG_FUSE", fuse)
+            say(f"fsai: fused update {fuse}, x64 {x64}")
+            out[fuse] = [_solve_fsai(ctl, A, f, b, tol=tol) for tol in (1e-10, 3e-9, 1e-9, 3e-10)]
+        for (s1, x1), (s0, x0) in zip(out["1"], out["0"]):
+            assert s1.preconditioner == 2 and s0.preconditioner == 2
+            assert (s1.iterations, s1.outer_iterations) == (s0.iterations, s0.outer_iterations)
+            assert s1.outer_iterations >= 2
+            np.testing.assert_array_equal(x1, x0)
+    finally:
+        f.release()
+        net.release()
+
+
 def test_first_iteration_applies_an_irregular_factor(ctl, monkeypatch):
     """An irregular pattern (columns of G from 1 to dozens of entries) keeps
     the CSR SpMVs of G and G^T: one PCG iteration from x = 0 is x = alpha
