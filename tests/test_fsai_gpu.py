@@ -472,8 +472,8 @@ def test_first_iteration_applies_the_bf16_factor(ctl, monkeypatch, dia):
         net.release()
 
 
-@pytest.mark.parametrize("x64", ["0", "1"])
-def test_fused_update_iteration_matches_the_separate_kernels(ctl, monkeypatch, x64):
+@pytest.mark.parametrize("x64,kind", [("0", "3d"), ("1", "3d"), ("0", "2d")])
+def test_fused_update_iteration_matches_the_separate_kernels(ctl, monkeypatch, x64, kind):
     """With the factor in diagonal storage the x / r update is fused into
     w = G^T r (r alternating between two buffers): the iterates, iteration
     and replacement counts equal those of the separate kernels bit for bit,
@@ -485,7 +485,10 @@ def test_fused_update_iteration_matches_the_separate_kernels(ctl, monkeypatch, x
     monkeypatch.setenv("CLSPARSE_AMD_MIXED_X64", x64)
     from clsparse_amd import fsai as fs
     params, _ = fs.load_weights()
-    rp, ci, v, m = cls.gen_diffusion3d(28, 26, 24, sigma=1.0, ax=1.0, ay=1.0, az=100.0, seed=6)
+    if kind == "3d":
+        rp, ci, v, m = cls.gen_diffusion3d(28, 26, 24, sigma=1.0, ax=1.0, ay=1.0, az=100.0, seed=6)
+    else:
+        rp, ci, v, m = _aniso(130, 120, seed=6)
     A = cls.CsrMatrix(rp, ci, v, m)
     A.meta_create(ctl)
     net = cls.Mlp(seed=1)
@@ -493,12 +496,12 @@ def test_fused_update_iteration_matches_the_separate_kernels(ctl, monkeypatch, x
     f = cls.Fsai(A, ctl)
     try:
         f.apply_net(net, ctl)
-        assert len(f.diagonals()) == 4
+        assert len(f.diagonals()) == (4 if kind == "3d" else 5)
         b = cls.gen_uniform(m, -1, 1, seed=31)
         out = {}
         for fuse in ("1", "0"):
             monkeypatch.setenv("CLSPARSE_AMD_CG_FUSE", fuse)
-            say(f"fsai: fused update {fuse}, x64 {x64}")
+            say(f"fsai: fused update {fuse}, x64 {x64}, {kind}")
             out[fuse] = [_solve_fsai(ctl, A, f, b, tol=tol) for tol in (1e-10, 3e-9, 1e-9, 3e-10)]
         for (s1, x1), (s0, x0) in zip(out["1"], out["0"]):
             assert s1.preconditioner == 2 and s0.preconditioner == 2
