@@ -93,7 +93,8 @@ class clsparseAmdSolverStats(Structure):
                 ("relative_tolerance", c_double), ("absolute_tolerance", c_double),
                 ("initial_residual", c_double), ("current_residual", c_double),
                 ("pilot_rate_fsai", c_double), ("pilot_rate_jacobi", c_double),
-                ("a_diagonals", c_int32), ("g_diagonals", c_int32)]
+                ("a_diagonals", c_int32), ("g_diagonals", c_int32),
+                ("loss_fsai", c_double), ("loss_jacobi", c_double)]
 
 
 class clsparseAmdHostCsr(Structure):

@@ -1379,11 +1379,128 @@ double race_margin()
 }
 
 // CLSPARSE_AMD_FSAI_POLICY=always keeps the learned factor unless it breaks
-// down or stagnates; the default ("race") also runs the pilot race.
+// down or stagnates; "race" runs the pilot race on every new factor; the
+// default runs it only when a loss estimate leaves the factor's lead over
+// Jacobi in doubt (estimate_enabled, factor_losses).
 bool race_enabled()
 {
     const char* e = std::getenv("CLSPARSE_AMD_FSAI_POLICY");
     return !(e && std::strcmp(e, "always") == 0);
+}
+bool estimate_enabled()
+{
+    const char* e = std::getenv("CLSPARSE_AMD_FSAI_POLICY");
+    return !(e && (std::strcmp(e, "always") == 0 || std::strcmp(e, "race") == 0));
+}
+// A new factor whose estimated ||G^T A G - I||_F^2 is at most this fraction
+// of Jacobi's (G = D^-1/2) is kept without the pilot race. Measured ratios:
+// 0.25-0.44 for the trained network on the held-out 3-D problems, 0.33 on
+// the 2-D training family; 1 for the zero network, 8-140 for a network with
+// a wrong output layer.
+constexpr double kLossRatioKeep = 0.5;
+constexpr int kLossProbes = 2;
+
+// z_i = +-1 from a hash of (seed, i): a Rademacher probe.
+__global__ __launch_bounds__(256) void probe_kernel(size_t n, uint64_t seed, float* __restrict__ z)
+{
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
+        uint64_t x = (seed + 1) * 0x9E3779B97F4A7C15ull ^ (static_cast<uint64_t>(i) * 0xD1B54A32D192ED03ull);
+        x ^= x >> 31;
+        x *= 0xBF58476D1CE4E5B9ull;
+        x ^= x >> 29;
+        z[i] = (x & 1) ? 1.f : -1.f;
+    }
+}
+
+// out = D^-1/2 in (invD = 1 / a_ii), and the partials of ||out||^2 (one
+// per workgroup) when part is given.
+__global__ __launch_bounds__(256) void jacobi_factor_kernel(size_t n, const float* __restrict__ invD,
+                                                            const float* __restrict__ in, float* __restrict__ out,
+                                                            double* __restrict__ part)
+{
+    __shared__ double sh[4];
+    double a = 0.0;
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
+        const float v = sqrtf(invD[i]) * in[i];
+        out[i] = v;
+        a += static_cast<double>(v) * static_cast<double>(v);
+    }
+    a = block_sum256(a, sh);
+    if (part != nullptr && threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
+// Hutchinson estimates, over `probes` Rademacher vectors z, of
+// ||F^T A F - I||_F^2 / n for the learned factor F = G (diagonal storage,
+// as the solver applies it) and for Jacobi's F = D^-1/2, with A in diagonal
+// storage: t = F z, y = A t with <y, t>, w = F^T y with ||w||^2, and
+// ||F^T A F z - z||^2 = ||w||^2 - 2 <y, t> + n. z, t, y, w: n values each;
+// part_a, part_b: one double per workgroup of grid_for(n).
+clsparseStatus factor_losses(const FsaiDia& dia, const DiaMatrix<float>& adia, const float* invD, size_t n,
+                             int probes, float* z, float* t, float* y, float* w, double* part_a, double* part_b,
+                             hipStream_t s, double* loss_fsai, double* loss_jacobi)
+{
+    const uint32_t g = grid_for(n);
+    DiaOffsets off{};
+    for (int d = 0; d < kFsaiMaxDiags; ++d) off.o[d] = dia.off[d];
+    std::vector<double> ha(g), hb(g);
+    double acc[2] = {0.0, 0.0};
+    for (int k = 0; k < probes; ++k) {
+        hipLaunchKernelGGL(probe_kernel, dim3(g), dim3(256), 0, s, n, static_cast<uint64_t>(k), z);
+        for (int f = 0; f < 2; ++f) {
+            if (f == 0) {
+                switch (dia.ndiag) {
+#define CLS_G(K) hipLaunchKernelGGL((fsai_g_dia_kernel<K>), dim3(g), dim3(256), 0, s, n, off, dia.vals, z, t, nullptr)
+                case 1: CLS_G(1); break;
+                case 2: CLS_G(2); break;
+                case 3: CLS_G(3); break;
+                case 4: CLS_G(4); break;
+                case 5: CLS_G(5); break;
+                case 6: CLS_G(6); break;
+                case 7: CLS_G(7); break;
+                case 8: CLS_G(8); break;
+                default: return clsparseInvalidValue;
+#undef CLS_G
+                }
+            } else {
+                hipLaunchKernelGGL(jacobi_factor_kernel, dim3(g), dim3(256), 0, s, n, invD, z, t,
+                                   static_cast<double*>(nullptr));
+            }
+            CLS_TRY(launch_dia_spmv(adia, n, g, t, y, t, part_a, nullptr, s));
+            if (f == 0) {
+                switch (dia.ndiag) {
+#define CLS_GT(K) \
+    hipLaunchKernelGGL((fsai_gt_dia_kernel<K>), dim3(g), dim3(256), 0, s, n, off, dia.vals, y, w, part_b, nullptr)
+                case 1: CLS_GT(1); break;
+                case 2: CLS_GT(2); break;
+                case 3: CLS_GT(3); break;
+                case 4: CLS_GT(4); break;
+                case 5: CLS_GT(5); break;
+                case 6: CLS_GT(6); break;
+                case 7: CLS_GT(7); break;
+                case 8: CLS_GT(8); break;
+                default: return clsparseInvalidValue;
+#undef CLS_GT
+                }
+            } else {
+                hipLaunchKernelGGL(jacobi_factor_kernel, dim3(g), dim3(256), 0, s, n, invD, y, w, part_b);
+            }
+            CLS_HIP_TRY(hipGetLastError());
+            CLS_HIP_TRY(hipMemcpyAsync(ha.data(), part_a, g * sizeof(double), hipMemcpyDeviceToHost, s));
+            CLS_HIP_TRY(hipMemcpyAsync(hb.data(), part_b, g * sizeof(double), hipMemcpyDeviceToHost, s));
+            CLS_HIP_TRY(hipStreamSynchronize(s));
+            double yt = 0.0, ww = 0.0;
+            for (uint32_t i = 0; i < g; ++i) {
+                yt += ha[i];
+                ww += hb[i];
+            }
+            acc[f] += (ww - 2.0 * yt + static_cast<double>(n)) / static_cast<double>(n);
+        }
+    }
+    *loss_fsai = acc[0] / probes;
+    *loss_jacobi = acc[1] / probes;
+    return clsparseSuccess;
 }
 
 // r = (T) r64
@@ -1698,11 +1815,27 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     int detail = 1;
     int iters = 0;
     double rate_fsai_out = 0.0, rate_jac_out = 0.0;
+    double loss_fsai = 0.0, loss_jac = 0.0;
     if (rel <= sc->relativeTolerance) {
         detail = 0;
     } else {
         if constexpr (!kIncr)
             CLS_HIP_TRY(hipMemcpyAsync(xsave, xp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        // A new factor clearly ahead of Jacobi by the loss estimate is kept
+        // without the pilot race (the factor keeps that outcome as a race's);
+        // otherwise the race decides. The iteration buffers are free here.
+        if constexpr (kMixed) {
+            if (eng->prec() == kPrecFsai && race_enabled() && race_known == 0 && estimate_enabled() && have_dia &&
+                adia.vals != nullptr) {
+                CLS_TRY(factor_losses(dia, adia, buf.invD, n, kLossProbes, buf.z, buf.p, buf.y, buf.r, buf.part_yp,
+                                      buf.part_rz[0], s, &loss_fsai, &loss_jac));
+                if (std::isfinite(loss_fsai) && loss_fsai <= kLossRatioKeep * loss_jac) {
+                    race_known = 1;
+                    known_rf = known_rj = 0.0;  // no race was run
+                    fsai_set_race(fsai, 1, 0.0, 0.0);
+                }
+            }
+        }
         CLS_TRY(restart());
         double res_at_replace = rel;
         double best_res = rel;
@@ -1860,6 +1993,8 @@ clsparseStatus reliable_solve(cldenseVector* x, const clsparseCsrMatrix* A, cons
     sc->stats.preconditioner = used;
     sc->stats.pilot_rate_fsai = rate_fsai_out;
     sc->stats.pilot_rate_jacobi = rate_jac_out;
+    sc->stats.loss_fsai = loss_fsai;
+    sc->stats.loss_jacobi = loss_jac;
     sc->stats.a_diagonals = a_ndiag;
     sc->stats.g_diagonals = have_dia ? dia.ndiag : 0;
     return clsparseSuccess;

@@ -1334,7 +1334,7 @@ clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_
                            static_cast<const uint32_t*>(nullptr), static_cast<uint16_t*>(nullptr), f->bad);
         CLS_HIP_TRY(hipGetLastError());
         f->solver_stale = true;
-        f->race = 0;
+        fsai_set_race(f, 0, 0.0, 0.0);  // a new factor races (or is judged) afresh
         return clsparseSuccess;
     }
     CLS_HIP_TRY(hipMemsetAsync(f->bad, 0, sizeof(int), st));
@@ -1350,7 +1350,7 @@ clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_
     CLS_HIP_TRY(hipMemcpyAsync(&bad, f->bad, sizeof(int), hipMemcpyDeviceToHost, st));
     CLS_HIP_TRY(hipStreamSynchronize(st));
     f->valid = (bad == 0) && f->full_diag;
-    f->race = 0;
+    fsai_set_race(f, 0, 0.0, 0.0);
     f->solver_stale = false;
     return clsparseSuccess;
 }

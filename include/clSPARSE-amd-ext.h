@@ -61,6 +61,13 @@ typedef struct clsparseAmdSolverStats_
      * factor's storage (0: CSR SpMVs or no factor). */
     cl_int a_diagonals;
     cl_int g_diagonals;
+    /* Learned-FSAI mixed solver, factor in diagonal storage: Hutchinson
+     * estimates of ||F^T A F - I||_F^2 / n for the factor and for Jacobi
+     * (F = D^-1/2) taken before the first solve with a new factor (0 if not
+     * taken); the pilot race is skipped when the factor's is at most half of
+     * Jacobi's. */
+    cl_double loss_fsai;
+    cl_double loss_jacobi;
 } clsparseAmdSolverStats;
 
 CLSPARSE_EXPORT clsparseStatus

@@ -298,7 +298,7 @@ def test_solve_after_a_training_step_uses_its_network_output(ctl, monkeypatch, d
 
 
 def test_pilot_race_picks_a_preconditioner_and_converges(ctl, monkeypatch):
-    monkeypatch.delenv("CLSPARSE_AMD_FSAI_POLICY", raising=False)
+    monkeypatch.setenv("CLSPARSE_AMD_FSAI_POLICY", "race")
     rp, ci, v, m, A = _sample_matrix(ctl, 64, 64, seed=14)
     b = cls.gen_uniform(m, -1, 1, seed=7)
     f = cls.Fsai(A, ctl)
@@ -330,12 +330,66 @@ def test_pilot_race_picks_a_preconditioner_and_converges(ctl, monkeypatch):
         f.release()
 
 
+def test_loss_estimate_decides_whether_to_race(ctl, monkeypatch):
+    """By default a new factor in diagonal storage is first judged by
+    Hutchinson estimates of ||F^T A F - I||_F^2 / n (two probes, the factor
+    as the solver applies it, and Jacobi's D^-1/2): the trained network's
+    factor, well under half of Jacobi's, is kept without the pilot race; the
+    zero network's (G = D^-1/2) races. The estimates match the training
+    step's FP64 Hutchinson loss over eight other probes."""
+    monkeypatch.delenv("CLSPARSE_AMD_FSAI_POLICY", raising=False)
+    from clsparse_amd import fsai as fs
+    params, _ = fs.load_weights()
+    rp, ci, v, m = cls.gen_diffusion3d(48, 44, 40, sigma=1.0, ax=1.0, ay=1.0, az=100.0, seed=9)
+    A = cls.CsrMatrix(rp, ci, v, m)
+    A.meta_create(ctl)
+    b = cls.gen_uniform(m, -1, 1, seed=19)
+    Acpu = sp.csr_matrix((v, ci, rp), shape=(m, m))
+    net, net0 = cls.Mlp(seed=1), cls.Mlp(seed=1)
+    net.set_params(params)
+    f = cls.Fsai(A, ctl)
+    try:
+        loss_l = f.train_step(net, A, ctl, probes=8, seed=5, lr=0.0)[0]
+        loss_0 = f.train_step(net0, A, ctl, probes=8, seed=5, lr=0.0)[0]
+        f.apply_net(net, ctl)
+        assert len(f.diagonals()) == 4
+        say("fsai: loss estimate, trained network")
+        st, x = _solve_fsai(ctl, A, f, b)
+        assert st.loss_fsai == pytest.approx(loss_l, rel=0.05), (st.loss_fsai, loss_l)
+        assert st.loss_jacobi == pytest.approx(loss_0, rel=0.05), (st.loss_jacobi, loss_0)
+        assert st.loss_fsai <= 0.5 * st.loss_jacobi
+        assert st.preconditioner == 2 and st.pilot_rate_fsai == 0 and st.pilot_rate_jacobi == 0
+        assert st.status_detail == 0 and np.linalg.norm(b - Acpu @ x) / np.linalg.norm(b) <= 1e-8
+        # the outcome stays with the factor: no estimate on the next solve
+        st2, _ = _solve_fsai(ctl, A, f, b)
+        assert st2.loss_fsai == 0 and st2.preconditioner == 2 and st2.iterations <= st.iterations
+        f.apply_net(None, ctl)  # G = D^-1/2: no lead over Jacobi, so the race runs
+        say("fsai: loss estimate, zero network")
+        st3, x3 = _solve_fsai(ctl, A, f, b)
+        assert st3.loss_fsai == pytest.approx(st3.loss_jacobi, rel=0.05)
+        assert st3.pilot_rate_fsai > 0 and st3.pilot_rate_jacobi > 0
+        assert st3.status_detail == 0 and np.linalg.norm(b - Acpu @ x3) / np.linalg.norm(b) <= 1e-8
+        # a new factor forgets the race it ran: judged afresh, no rates left over
+        f.apply_net(net, ctl)
+        st5, _ = _solve_fsai(ctl, A, f, b)
+        assert st5.loss_fsai > 0 and st5.pilot_rate_fsai == 0 and st5.pilot_rate_jacobi == 0
+        # CLSPARSE_AMD_FSAI_POLICY=race races whatever the estimate would say
+        monkeypatch.setenv("CLSPARSE_AMD_FSAI_POLICY", "race")
+        f.apply_net(net, ctl)
+        st4, _ = _solve_fsai(ctl, A, f, b)
+        assert st4.loss_fsai == 0 and st4.pilot_rate_fsai > 0 and st4.pilot_rate_jacobi > 0
+    finally:
+        f.release()
+        net.release()
+        net0.release()
+
+
 @pytest.mark.parametrize("margin,expect", [("1e-6", 3), ("1e6", 2)])
 def test_race_margin_decides_the_pilot_race(ctl, monkeypatch, margin, expect):
     """CLSPARSE_AMD_RACE_MARGIN scales Jacobi's required lead: a tiny margin
     hands any raced solve to Jacobi, a huge one keeps the learned factor;
     both converge."""
-    monkeypatch.delenv("CLSPARSE_AMD_FSAI_POLICY", raising=False)
+    monkeypatch.setenv("CLSPARSE_AMD_FSAI_POLICY", "race")
     monkeypatch.setenv("CLSPARSE_AMD_RACE_MARGIN", margin)
     rp, ci, v, m, A = _sample_matrix(ctl, 64, 64, seed=15)
     b = cls.gen_uniform(m, -1, 1, seed=8)
