@@ -540,7 +540,7 @@ def csr_only():
 def timed_solves(cls, torch, ctl, fn, A, bt, m, tol, reps=5, fsai=None, max_iters=50000):
     """reps + 1 solves from x = 0; the first is reported on its own (it
     captures the solver's graphs and, for a new learned factor, runs the
-    pilot race), the median and spread of the other reps are the figure."""
+    loss estimate or pilot race), the median and spread of the other reps are the figure."""
     rows = []
     for _ in range(reps + 1):
         x = torch.zeros(m, dtype=torch.float64, device="cuda")
@@ -775,7 +775,7 @@ def fsai_workload(cls, torch, ctl, n1d=160, tol=1e-8, weights=None, cpu=True, sw
     for name, fn, with_f, kv in runs:
         with env(**kv):
             if with_f:
-                f.apply_net(net, ctl)  # a fresh factor: the first solve runs its pilot race
+                f.apply_net(net, ctl)  # a fresh factor: the first solve judges it (estimate or race)
             out[name] = timed_solves(cls, torch, ctl, fn, A, bt, m, tol, fsai=f if with_f else None)
         log(f"{name}: {out[name]}")
     f.apply_net(None, ctl)  # zero network: the same solver with G = D^-1/2
@@ -799,7 +799,7 @@ def fsai_workload(cls, torch, ctl, n1d=160, tol=1e-8, weights=None, cpu=True, sw
         "setup_s": round(setup, 5),
         "what": "FP64 Jacobi-PCG median solve time over learned-FSAI mixed PCG median solve time; "
                 "first_solve_with_setup adds the factor's pattern build and network evaluation to "
-                "the learned solver's first solve (pilot race included)"}
+                "the learned solver's first solve (its loss estimate or pilot race included)"}
     out["iteration_ratio_learned_vs_untrained"] = round(
         out["learned_fsai_mixed"]["iterations"] / out["untrained_fsai_mixed"]["iterations"], 3)
     if cpu:
