@@ -769,6 +769,7 @@ struct _clsparseAmdFsai
     // solver's FP32 / BF16 / diagonal copies and `valid` are then brought up
     // to date from the stored network output before the factor is used.
     bool solver_stale = false;
+    bool g64_stale = false;  // the FP64 copies (g64, gt64) are not assembled yet
     bool have_o = false;  // o holds the network output G was last assembled from
     // Device memory of the arrays above: two allocations (the n-sized
     // arrays, then the nnz(G)-sized ones), carved in 256-byte pieces.
@@ -793,6 +794,7 @@ hipError_t dmalloc(T** p, size_t count)
 enum class Assemble { Full, Training, Refresh };
 clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_t st, int num_cus = 0,
                              Assemble mode = Assemble::Full);
+clsparseStatus defer_jacobi(_clsparseAmdFsai* f);
 
 // Diagonal scaling, row statistics and features from A's values.
 clsparseStatus compute_features(_clsparseAmdFsai* f, const clsparseCsrMatrix* A, hipStream_t st)
@@ -1266,7 +1268,7 @@ extern "C" clsparseStatus clsparseAmdFsaiCreate(clsparseAmdFsai* out, const clsp
     f->GT16.values = f->gt16;
 
     s = compute_features(f, A, st);
-    if (s == clsparseSuccess) s = fsai_assemble(f, nullptr, st);
+    if (s == clsparseSuccess) s = defer_jacobi(f);
     if (s != clsparseSuccess) {
         free_fsai(f);
         return s;
@@ -1286,7 +1288,7 @@ extern "C" clsparseStatus clsparseAmdFsaiUpdateValues(clsparseAmdFsai f, const c
     CLS_TRY(check_row_pointer(dev_ptr<uint32_t>(A->row_pointer, A->off_row_pointer), f->n, f->nnzA, f->bad,
                               control->stream));
     CLS_TRY(compute_features(f, A, control->stream));
-    return fsai_assemble(f, nullptr, control->stream);
+    return defer_jacobi(f);
 }
 
 extern "C" clsparseStatus clsparseAmdFsaiDestroy(clsparseAmdFsai f)
@@ -1334,6 +1336,7 @@ clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_
                            static_cast<const uint32_t*>(nullptr), static_cast<uint16_t*>(nullptr), f->bad);
         CLS_HIP_TRY(hipGetLastError());
         f->solver_stale = true;
+        f->g64_stale = false;
         fsai_set_race(f, 0, 0.0, 0.0);  // a new factor races (or is judged) afresh
         return clsparseSuccess;
     }
@@ -1352,6 +1355,22 @@ clsparseStatus fsai_assemble(_clsparseAmdFsai* f, clsparseAmdMlp net, hipStream_
     f->valid = (bad == 0) && f->full_diag;
     fsai_set_race(f, 0, 0.0, 0.0);
     f->solver_stale = false;
+    f->g64_stale = false;
+    return clsparseSuccess;
+}
+
+// G = D^-1/2 (no network output) left to be assembled where it is first
+// read (the solver's refresh, clsparseAmdFsaiGetValues): a caller usually
+// applies a network next, which assembles G itself. D^-1/2 is finite by
+// construction (row_diag_kernel), so the factor is valid with a full diagonal.
+clsparseStatus defer_jacobi(_clsparseAmdFsai* f)
+{
+    CLS_HIP_TRY(hipGetLastError());
+    f->have_o = false;
+    f->valid = f->nnzG > 0 && f->full_diag;
+    f->solver_stale = f->nnzG > 0;
+    f->g64_stale = f->nnzG > 0;
+    fsai_set_race(f, 0, 0.0, 0.0);
     return clsparseSuccess;
 }
 
@@ -1396,8 +1415,10 @@ extern "C" clsparseStatus clsparseAmdFsaiGetValues(clsparseAmdFsai f, cl_uint* g
         CLS_HIP_TRY(hipMemcpy(g_row_pointer, f->g_rp, (f->n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (g_col_indices)
         CLS_HIP_TRY(hipMemcpy(g_col_indices, f->g_ci, f->nnzG * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    if (g_values)
+    if (g_values) {
+        if (f->g64_stale) CLS_TRY(fsai_assemble(f, nullptr, nullptr, 0, Assemble::Refresh));
         CLS_HIP_TRY(hipMemcpy(g_values, f->g64, f->nnzG * sizeof(double), hipMemcpyDeviceToHost));
+    }
     return clsparseSuccess;
 }
 

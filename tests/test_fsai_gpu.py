@@ -66,12 +66,36 @@ def test_features_and_jacobi_factor_match_oracle(ctl):
         assert np.all(diff <= 2.0 ** -7 * np.maximum(np.abs(Xo), 1e-30) + 1e-30)
         assert (diff > 0).mean() < 0.01
         np.testing.assert_array_equal(X[:, 0:2], Xo[:, 0:2])
+        G = of.assemble_g(rp, ci, v)
+        # a new factor is Jacobi's, assembled where it is first read
+        np.testing.assert_array_equal(f.values()[2], G.data)
         f.apply_net(None, ctl)
         grp, gci, gv = f.values()
-        G = of.assemble_g(rp, ci, v)
         np.testing.assert_array_equal(grp, G.indptr)
         np.testing.assert_array_equal(gci, G.indices)
         np.testing.assert_array_equal(gv, G.data)  # 1/sqrt(a_ii): correctly rounded both sides
+    finally:
+        f.release()
+
+
+def test_update_values_gives_the_new_matrix_its_jacobi_factor(ctl):
+    """clsparseAmdFsaiUpdateValues on new values of the same pattern: the
+    features and G = D^-1/2 follow the new matrix (G assembled where it is
+    first read), and a solve with the factor converges on it."""
+    rp, ci, v, m, A = _sample_matrix(ctl, 30, 26, seed=12)
+    f = cls.Fsai(A, ctl)
+    try:
+        v2 = v * 3.0
+        A2 = cls.CsrMatrix(rp, ci, v2, m)
+        A2.meta_create(ctl)
+        say("fsai: update values")
+        f.update_values(A2, ctl)
+        np.testing.assert_array_equal(f.values()[2], of.assemble_g(rp, ci, v2).data)
+        b = cls.gen_uniform(m, -1, 1, seed=3)
+        st, x = _solve_fsai(ctl, A2, f, b)
+        Acpu = sp.csr_matrix((v2, ci, rp), shape=(m, m))
+        assert st.status_detail == 0
+        assert np.linalg.norm(b - Acpu @ x) / np.linalg.norm(b) <= 1e-8
     finally:
         f.release()
 
