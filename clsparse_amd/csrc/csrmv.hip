@@ -393,7 +393,10 @@ __global__ __launch_bounds__(64 * kWavesPerWg) void csrmv_adaptive_kernel(
     }
 }
 
-// y[row] = beta * y[row] + sum of the row's segment partials, in segment order.
+// y[row] = beta * y[row] + sum of the row's segment partials, in segment
+// order. One wave per long row: the lanes load 64 partials at a time into
+// LDS (loads in parallel, not one dependent load per segment), lane 0 adds
+// them in segment order.
 template <typename T, bool EXT>
 __global__ __launch_bounds__(64) void csrmv_longrow_combine(const LongRow* __restrict__ lr,
                                                            uint32_t numLong,
@@ -402,15 +405,31 @@ __global__ __launch_bounds__(64) void csrmv_longrow_combine(const LongRow* __res
                                                            const T* __restrict__ pBeta,
                                                            const int* __restrict__ done)
 {
+    __shared__ double ps[64];
+    __shared__ double pe[EXT ? 64 : 1];
     if (done != nullptr && *done != 0) return;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = blockIdx.x;
     if (i >= numLong) return;
     const LongRow L = lr[i];
+    const uint32_t lane = threadIdx.x;
     Acc<double, EXT> acc;
-    for (uint32_t w = 0; w < L.num_segments; ++w) {
-        acc.add(partials[2 * (L.first_entry + w)]);
-        if constexpr (EXT) acc.c += partials[2 * (L.first_entry + w) + 1];
+    for (uint32_t w0 = 0; w0 < L.num_segments; w0 += 64) {
+        const uint32_t w = w0 + lane;
+        if (w < L.num_segments) {
+            ps[lane] = partials[2 * (static_cast<size_t>(L.first_entry) + w)];
+            if constexpr (EXT) pe[lane] = partials[2 * (static_cast<size_t>(L.first_entry) + w) + 1];
+        }
+        __syncthreads();
+        if (lane == 0) {
+            const uint32_t cnt = min(64u, L.num_segments - w0);
+            for (uint32_t k = 0; k < cnt; ++k) {
+                acc.add(ps[k]);
+                if constexpr (EXT) acc.c += pe[k];
+            }
+        }
+        __syncthreads();
     }
+    if (lane != 0) return;
     const T beta = *pBeta;
     if (beta == T(0)) {
         y[L.row] = static_cast<T>(acc.value());
@@ -534,7 +553,7 @@ clsparseStatus launch_csrmv(const T* alpha, const clsparseCsrMatrix* A, const T*
         CLS_HIP_TRY(hipGetLastError());
     }
     if (m->num_long_rows > 0) {
-        const uint32_t g = (m->num_long_rows + 63) / 64;
+        const uint32_t g = m->num_long_rows;  // one wave per long row
         if (ext)
             hipLaunchKernelGGL((csrmv_longrow_combine<T, true>), dim3(g), dim3(64), 0, s,
                                m->d_long, m->num_long_rows, m->d_partials, y, beta, done);
@@ -567,7 +586,7 @@ clsparseStatus launch_csrmv_bf16(const float* alpha, const clsparseCsrMatrix* A,
         CLS_HIP_TRY(hipGetLastError());
     }
     if (m->num_long_rows > 0) {
-        const uint32_t g = (m->num_long_rows + 63) / 64;
+        const uint32_t g = m->num_long_rows;  // one wave per long row
         hipLaunchKernelGGL((csrmv_longrow_combine<float, false>), dim3(g), dim3(64), 0, s, m->d_long,
                            m->num_long_rows, m->d_partials, y, beta, done);
         CLS_HIP_TRY(hipGetLastError());
