@@ -356,7 +356,7 @@ def test_pilot_race_picks_a_preconditioner_and_converges(ctl, monkeypatch):
 
 def test_loss_estimate_decides_whether_to_race(ctl, monkeypatch):
     """By default a new factor in diagonal storage is first judged by
-    Hutchinson estimates of ||F^T A F - I||_F^2 / n (two probes, the factor
+    Hutchinson estimates of ||F^T A F - I||_F^2 / n (one probe, the factor
     as the solver applies it, and Jacobi's D^-1/2): the trained network's
     factor, well under half of Jacobi's, is kept without the pilot race; the
     zero network's (G = D^-1/2) races. The estimates match the training
